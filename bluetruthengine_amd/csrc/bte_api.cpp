@@ -1,0 +1,887 @@
+// bte_api.cpp — C ABI (include/bte_api.h) over the HIP kernels.
+//
+// Reference counterparts: include/render_kernel.h:96-211 (API),
+// src/host/scene_upload.cu (uploads), src/host/optix_setup.cu:215-292 (BVH
+// build), src/host/wavefront_host.cu:155-236 (bounce loop),
+// src/render_kernel.cu:33-183 (frame dispatch, reset, cleanup).
+//
+// Differences by design: status codes instead of exit(); the bounce loop keeps
+// every count on the device (no per-bounce readback); several frames are traced
+// concurrently as one wavefront pass; NEE contributions are applied by
+// deterministic per-light-class connect passes instead of float atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bte_bvh.h"
+#include "bte_kernels.h"
+
+namespace bte {
+namespace {
+
+char g_err[1024] = "";
+
+int fail(int code, const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(BTE_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                  \
+    } while (0)
+
+// Owning device allocation.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int ensure(size_t n) {
+        if (n <= bytes && p) return BTE_OK;
+        release();
+        if (n == 0) return BTE_OK;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(BTE_ERR_MEMORY, "hipMalloc(%zu) failed", n);
+        }
+        bytes = n;
+        return BTE_OK;
+    }
+    int upload(const void* src, size_t n) {
+        int rc = ensure(n);
+        if (rc) return rc;
+        if (n) HIPCHK(hipMemcpy(p, src, n, hipMemcpyHostToDevice));
+        return BTE_OK;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Context {
+    bool deviceReady = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // host copies (the BVH is built from triangles on the host)
+    std::vector<BteTriangle> tris;
+    std::vector<BteFloat3> materials;
+    std::vector<int> triMaterialIds;
+    std::vector<BteBsdf> bsdfsHost;
+    std::vector<int> triBsdfHost;
+    std::vector<BteEmitter> emittersHost;
+    std::vector<int> emTriIdxHost;
+    int emitterCount = 0, emTriIdxCount = 0, emTriCdfCount = 0;
+    int spotCount = 0;
+    bool haveCamera = false;
+    BteCamera camera{};
+    int sceneW = 0, sceneH = 0;
+
+    // device scene
+    DBuf dTris, dMaterials, dTriMat, dTriBsdf, dBsdfs, dEmission, dFlags, dEmitterOfTri;
+    DBuf dEmitters, dEmTriIdx, dEmTriCdf, dSceneCdf, dSpots;
+    DBuf dNodes, dLeafTris;
+    DBuf dEnv, dEnvMarg, dEnvCond, dEnvRowSums, dEnvTotal;
+    bool haveEmission = false, haveFlags = false;
+    bool bvhDirty = true;
+    int bvhNodes = 0, bvhDepth = 0;
+    int envW = 0, envH = 0;
+    float envIntensity = 0.0f, envRot = 0.0f, envTotal = 0.0f;
+    bool hasEnv = false;
+
+    // output
+    DBuf dAccum, dFb, dSum64;
+    uint32_t* externalFb = nullptr;
+    int accW = 0, accH = 0;
+    uint32_t frameIndex = 0;
+    bool accum64 = false;
+    int mode = BTE_MODE_WAVEFRONT;
+    bool denoiser = false;
+
+    // wavefront buffers
+    DBuf dRayO[2], dRayD[2], dThr[2], dRad[2], dSt[2], dHit, dFinal, dShO, dShD, dShC, dCounters, dHitCap, dOvf;
+    long long cap = 0;
+    int capKinds = 0;
+    int hitCaptureRec = 0;
+    long long lastPassPaths = 0;
+    int lastPassFrames = 0;
+
+    BteConfig cfg{200, 5, 0, 32768};
+    BteStats stats{};
+    int instr = 0;
+    DBuf dTrav;
+    std::vector<hipEvent_t> events;
+};
+
+Context& ctx() {
+    static Context c;
+    return c;
+}
+
+int ensureDevice() {
+    Context& c = ctx();
+    if (c.deviceReady) return BTE_OK;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return fail(BTE_ERR_DEVICE, "no HIP device available");
+    }
+    HIPCHK(hipGetDevice(&c.device));
+    HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&c.ev0));
+    HIPCHK(hipEventCreate(&c.ev1));
+    c.deviceReady = true;
+    return BTE_OK;
+}
+
+#define ENSURE_DEVICE()              \
+    do {                             \
+        int rc_ = ensureDevice();    \
+        if (rc_) return rc_;         \
+    } while (0)
+#define CHECK_RC(expr)               \
+    do {                             \
+        int rc_ = (expr);            \
+        if (rc_) return rc_;         \
+    } while (0)
+
+int rebuildEmitterOfTri() {
+    Context& c = ctx();
+    const int T = (int)c.tris.size();
+    if (c.emitterCount <= 0 || T == 0) {
+        c.dEmitterOfTri.release();
+        return BTE_OK;
+    }
+    std::vector<int> eot((size_t)T, -1);
+    for (int e = 0; e < c.emitterCount; ++e) {
+        const BteEmitter& E = c.emittersHost[e];
+        for (int j = 0; j < E.triCount; ++j) {
+            const int t = c.emTriIdxHost[E.triIndexOffset + j];
+            if (t >= 0 && t < T && eot[t] < 0) eot[t] = e;
+        }
+    }
+    return c.dEmitterOfTri.upload(eot.data(), eot.size() * sizeof(int));
+}
+
+int ensureAccel() {
+    Context& c = ctx();
+    if (!c.bvhDirty) return BTE_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    BvhBuild b = build_bvh2(c.tris.data(), (int)c.tris.size());
+    (void)t0;
+    CHECK_RC(c.dNodes.upload(b.nodes.data(), b.nodes.size() * sizeof(BvhHostNode)));
+    std::vector<float> lt = std::move(b.leafTris);
+    if (lt.empty()) lt.assign(12, 0.0f);
+    CHECK_RC(c.dLeafTris.upload(lt.data(), lt.size() * sizeof(float)));
+    c.bvhNodes = (int)b.nodes.size();
+    c.bvhDepth = b.depth;
+    c.bvhDirty = false;
+    return BTE_OK;
+}
+
+DevScene makeScene() {
+    Context& c = ctx();
+    DevScene s{};
+    s.nodes = c.dNodes.as<float4>();
+    s.leafTris = c.dLeafTris.as<float4>();
+    s.tris = c.dTris.as<BteTriangle>();
+    s.triBsdf = c.dTriBsdf.as<int>();
+    s.bsdfs = c.dBsdfs.as<float4>();
+    s.triEmission = c.haveEmission ? c.dEmission.as<float>() : nullptr;
+    s.triFlags = c.haveFlags ? c.dFlags.as<uint8_t>() : nullptr;
+    s.emitterOfTri = c.emitterCount > 0 ? c.dEmitterOfTri.as<int>() : nullptr;
+    s.emitters = c.dEmitters.as<BteEmitter>();
+    s.emTriIdx = c.dEmTriIdx.as<int>();
+    s.emTriCdf = c.dEmTriCdf.as<float>();
+    s.sceneCdf = c.dSceneCdf.as<float>();
+    s.spots = c.dSpots.as<BteSpotlight>();
+    s.env = c.dEnv.as<float4>();
+    s.envMarg = c.dEnvMarg.as<float>();
+    s.envCond = c.dEnvCond.as<float>();
+    s.cam = c.camera;
+    s.emitterCount = c.emitterCount;
+    s.spotCount = c.spotCount;
+    s.envW = c.envW;
+    s.envH = c.envH;
+    s.envIntensity = c.envIntensity;
+    s.envRot = c.envRot;
+    s.envTotal = c.envTotal;
+    s.hasEnv = c.hasEnv ? 1 : 0;
+    s.width = c.accW;
+    s.height = c.accH;
+    s.triCount = (int)c.tris.size();
+    return s;
+}
+
+int validateForRender() {
+    Context& c = ctx();
+    if (c.tris.empty()) return fail(BTE_ERR_STATE, "no triangles uploaded (bte_init_scene)");
+    if (!c.haveCamera) return fail(BTE_ERR_STATE, "no camera uploaded (bte_init_camera)");
+    if (c.bsdfsHost.empty() || c.triBsdfHost.size() != c.tris.size())
+        return fail(BTE_ERR_STATE, "BSDF table missing or triangle count mismatch (bte_init_bsdfs)");
+    if (c.haveEmission && c.dEmission.bytes < c.tris.size() * 12)
+        return fail(BTE_ERR_STATE, "triangle emission count does not match the triangles");
+    if (c.haveFlags && c.dFlags.bytes < c.tris.size())
+        return fail(BTE_ERR_STATE, "triangle emitter flags count does not match the triangles");
+    if (c.emitterCount > 0) {
+        for (int e = 0; e < c.emitterCount; ++e) {
+            const BteEmitter& E = c.emittersHost[e];
+            if (E.triCount < 0 || E.triIndexOffset < 0 || E.triIndexOffset + E.triCount > c.emTriIdxCount ||
+                E.cdfOffset < 0 || E.cdfOffset + E.triCount + 1 > c.emTriCdfCount)
+                return fail(BTE_ERR_INVALID, "emitter %d references data outside the emitter tables", e);
+        }
+        for (int i = 0; i < c.emTriIdxCount; ++i)
+            if (c.emTriIdxHost[i] < 0 || c.emTriIdxHost[i] >= (int)c.tris.size())
+                return fail(BTE_ERR_INVALID, "emitter triangle index %d out of range", c.emTriIdxHost[i]);
+    }
+    return BTE_OK;
+}
+
+int resetAccum(int w, int h) {
+    Context& c = ctx();
+    if (w <= 0 || h <= 0) return fail(BTE_ERR_INVALID, "invalid resolution %dx%d", w, h);
+    const size_t N = (size_t)w * (size_t)h;
+    CHECK_RC(c.dAccum.ensure(N * sizeof(float4)));
+    CHECK_RC(c.dFb.ensure(N * sizeof(uint32_t)));
+    HIPCHK(hipMemsetAsync(c.dAccum.p, 0, N * sizeof(float4), c.stream));
+    if (c.accum64) {
+        CHECK_RC(c.dSum64.ensure(N * 3 * sizeof(double)));
+        HIPCHK(hipMemsetAsync(c.dSum64.p, 0, N * 3 * sizeof(double), c.stream));
+    }
+    HIPCHK(hipStreamSynchronize(c.stream));
+    c.accW = w;
+    c.accH = h;
+    c.frameIndex = 0;
+    return BTE_OK;
+}
+
+int kindsInUse(int& kindArea, int& kindSpot0, int& kindEnv) {
+    Context& c = ctx();
+    int k = 0;
+    kindArea = c.emitterCount > 0 ? k++ : -1;
+    kindSpot0 = k;
+    k += c.spotCount;
+    kindEnv = c.hasEnv ? k++ : -1;
+    return k;
+}
+
+size_t bytesPerPath(int kinds) {
+    return 2 * (4 * sizeof(float4) + sizeof(uint2)) + 2 * sizeof(float4) + (size_t)kinds * 3 * sizeof(float4);
+}
+
+int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
+    Context& c = ctx();
+    const size_t P = (size_t)cap;
+    for (int i = 0; i < 2; ++i) {
+        CHECK_RC(c.dRayO[i].ensure(P * sizeof(float4)));
+        CHECK_RC(c.dRayD[i].ensure(P * sizeof(float4)));
+        CHECK_RC(c.dThr[i].ensure(P * sizeof(float4)));
+        CHECK_RC(c.dRad[i].ensure(P * sizeof(float4)));
+        CHECK_RC(c.dSt[i].ensure(P * sizeof(uint2)));
+    }
+    CHECK_RC(c.dHit.ensure(P * sizeof(float4)));
+    CHECK_RC(c.dFinal.ensure(P * sizeof(float4)));
+    const size_t K = (size_t)std::max(kinds, 1);
+    CHECK_RC(c.dShO.ensure(K * P * sizeof(float4)));
+    CHECK_RC(c.dShD.ensure(K * P * sizeof(float4)));
+    CHECK_RC(c.dShC.ensure(K * P * sizeof(float4)));
+    CHECK_RC(c.dCounters.ensure((size_t)kCtrStride * (size_t)(maxDepth + 1) * sizeof(int)));
+    const LaunchShape ls = query_launch_shape(c.device);
+    CHECK_RC(c.dOvf.ensure(((size_t)persistent_threads(ls) * (kStackSize - 32) + 64) * sizeof(int)));
+    if (c.hitCaptureRec > 0) CHECK_RC(c.dHitCap.ensure(P * (size_t)c.hitCaptureRec * sizeof(int)));
+    c.cap = cap;
+    c.capKinds = kinds;
+    for (int i = 0; i < 2; ++i) {
+        wb.rayO[i] = c.dRayO[i].as<float4>();
+        wb.rayD[i] = c.dRayD[i].as<float4>();
+        wb.thr[i] = c.dThr[i].as<float4>();
+        wb.rad[i] = c.dRad[i].as<float4>();
+        wb.st[i] = c.dSt[i].as<uint2>();
+    }
+    wb.hit = c.dHit.as<float4>();
+    wb.finalRad = c.dFinal.as<float4>();
+    wb.shO = c.dShO.as<float4>();
+    wb.shD = c.dShD.as<float4>();
+    wb.shC = c.dShC.as<float4>();
+    wb.counters = c.dCounters.as<int>();
+    wb.hitCapture = c.hitCaptureRec > 0 ? c.dHitCap.as<int>() : nullptr;
+    wb.ovfStack = c.dOvf.as<int>() + 64;
+    wb.cap = cap;
+    wb.maxRec = c.hitCaptureRec;
+    return BTE_OK;
+}
+
+int renderWavefront(int nframes) {
+    Context& c = ctx();
+    const int N = c.accW * c.accH;
+    int kindArea, kindSpot0, kindEnv;
+    const int kinds = kindsInUse(kindArea, kindSpot0, kindEnv);
+    const int maxDepth = std::max(1, c.cfg.maxDepth);
+    int F = c.cfg.framesInFlight;
+    if (F <= 0) {
+        const double budget = (double)(c.cfg.memoryBudgetMB > 0 ? c.cfg.memoryBudgetMB : 32768) * 1048576.0;
+        F = (int)std::max(1.0, std::floor(budget / ((double)N * (double)bytesPerPath(kinds))));
+    }
+    F = std::max(1, std::min(F, nframes));
+    if ((long long)N * F >= (1ll << 31) - 4096) F = std::max(1, (int)(((1ll << 31) - 4096) / N));
+    WaveBuffers wb{};
+    CHECK_RC(ensureWaveBuffers((long long)N * F, kinds, maxDepth, wb));
+    wb.kinds = kinds;
+    wb.travCtr = nullptr;
+    if (c.instr & BTE_INSTR_TRAVERSAL_COUNTS) {
+        CHECK_RC(c.dTrav.ensure(4 * sizeof(unsigned long long)));
+        HIPCHK(hipMemsetAsync(c.dTrav.p, 0, 4 * sizeof(unsigned long long), c.stream));
+        wb.travCtr = c.dTrav.as<unsigned long long>();
+    }
+    const bool timed = (c.instr & BTE_INSTR_KERNEL_TIMES) != 0;
+    if (timed && c.events.size() < (size_t)4 * maxDepth) {
+        for (size_t i = c.events.size(); i < (size_t)4 * maxDepth; ++i) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            c.events.push_back(e);
+        }
+    }
+    wb.kindArea = kindArea;
+    wb.kindSpot0 = kindSpot0;
+    wb.kindEnv = kindEnv;
+    const DevScene sc = makeScene();
+    const LaunchShape ls = query_launch_shape(c.device);
+    uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
+    double* sum64 = c.accum64 ? c.dSum64.as<double>() : nullptr;
+    const size_t ctrInts = (size_t)kCtrStride * (size_t)(maxDepth + 1);
+    std::vector<int> ctrHost(ctrInts);
+
+    c.stats = BteStats{};
+    c.stats.framesInFlight = F;
+    c.stats.bvhNodes = c.bvhNodes;
+    c.stats.bvhDepth = c.bvhDepth;
+    HIPCHK(hipEventRecord(c.ev0, c.stream));
+    int done = 0;
+    while (done < nframes) {
+        const int frames = std::min(F, nframes - done);
+        const long long paths = (long long)N * frames;
+        launch_fill_i32(wb.counters, (long long)ctrInts, 0, c.stream);
+        if (wb.hitCapture) launch_fill_i32(wb.hitCapture, paths * wb.maxRec, -2, c.stream);
+        launch_generate(sc, wb, paths, N, (int)c.frameIndex, c.stream);
+        int b = 0;
+        for (; b < maxDepth; ++b) {
+            launch_bounce(sc, wb, b, maxDepth, ls, c.stream, timed ? &c.events[4 * (size_t)b] : nullptr);
+            if (maxDepth > 16 && (b % 16) == 15 && b + 1 < maxDepth) {
+                int next = 0;
+                HIPCHK(hipMemcpyAsync(&next, wb.counters + (size_t)(b + 1) * kCtrStride + kCtrRays, sizeof(int),
+                                      hipMemcpyDeviceToHost, c.stream));
+                HIPCHK(hipStreamSynchronize(c.stream));
+                if (next == 0) { ++b; break; }
+            }
+        }
+        launch_present(sc, wb, N, frames, (int)c.frameIndex, c.dAccum.as<float4>(), sum64, fb, c.stream);
+        HIPCHK(hipMemcpyAsync(ctrHost.data(), wb.counters, ctrInts * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        HIPCHK(hipGetLastError());
+        for (int i = 0; i < b; ++i) {
+            const int* cb = ctrHost.data() + (size_t)i * kCtrStride;
+            c.stats.extensionRays += cb[kCtrRays];
+            c.stats.hits += cb[kCtrHits];
+            for (int k = 0; k < kinds; ++k) c.stats.shadowRays += cb[kCtrShadow + k];
+            if (i < 64) c.stats.raysPerBounce[i] += cb[kCtrRays];
+        }
+        c.stats.bounces = std::max(c.stats.bounces, b);
+        if (timed) {
+            for (int i = 0; i < b; ++i) {
+                float e = 0.0f, sh = 0.0f, cn = 0.0f;
+                HIPCHK(hipEventElapsedTime(&e, c.events[4 * i], c.events[4 * i + 1]));
+                HIPCHK(hipEventElapsedTime(&sh, c.events[4 * i + 1], c.events[4 * i + 2]));
+                HIPCHK(hipEventElapsedTime(&cn, c.events[4 * i + 2], c.events[4 * i + 3]));
+                c.stats.extendMs += e;
+                c.stats.shadeMs += sh;
+                c.stats.connectMs += cn;
+            }
+            c.stats.extendLaunches += b;
+            c.stats.connectLaunches += b * kinds;
+        }
+        c.stats.pathsTraced += paths;
+        c.frameIndex += (uint32_t)frames;
+        c.lastPassPaths = paths;
+        c.lastPassFrames = frames;
+        done += frames;
+    }
+    HIPCHK(hipEventRecord(c.ev1, c.stream));
+    HIPCHK(hipEventSynchronize(c.ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    c.stats.renderMs = ms;
+    if (wb.travCtr) {
+        unsigned long long tc[4];
+        HIPCHK(hipMemcpy(tc, c.dTrav.p, sizeof(tc), hipMemcpyDeviceToHost));
+        c.stats.extendNodes = (long long)tc[0];
+        c.stats.extendTris = (long long)tc[1];
+        c.stats.connectNodes = (long long)tc[2];
+        c.stats.connectTris = (long long)tc[3];
+    }
+    return BTE_OK;
+}
+
+int renderMegakernel(int nframes) {
+    Context& c = ctx();
+    const DevScene sc = makeScene();
+    const LaunchShape ls = query_launch_shape(c.device);
+    CHECK_RC(c.dOvf.ensure(((size_t)persistent_threads(ls) * (kStackSize - 32) + 64) * sizeof(int)));
+    uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
+    double* sum64 = c.accum64 ? c.dSum64.as<double>() : nullptr;
+    const int spp = std::max(1, c.cfg.megakernelSpp);
+    c.stats = BteStats{};
+    c.stats.bvhNodes = c.bvhNodes;
+    c.stats.bvhDepth = c.bvhDepth;
+    HIPCHK(hipEventRecord(c.ev0, c.stream));
+    for (int f = 0; f < nframes; ++f) {
+        launch_megakernel(sc, spp, std::max(1, c.cfg.maxDepth), (int)c.frameIndex, c.dAccum.as<float4>(), sum64, fb,
+                          c.dOvf.as<int>(), c.stream);
+        c.frameIndex++;
+    }
+    HIPCHK(hipEventRecord(c.ev1, c.stream));
+    HIPCHK(hipEventSynchronize(c.ev1));
+    HIPCHK(hipGetLastError());
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    c.stats.renderMs = ms;
+    c.stats.pathsTraced = (long long)c.accW * c.accH * spp * nframes;
+    return BTE_OK;
+}
+
+int renderFrames(int w, int h, int nframes) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (nframes <= 0) return fail(BTE_ERR_INVALID, "nframes must be positive");
+    CHECK_RC(validateForRender());
+    if (!c.dAccum.p || c.accW != w || c.accH != h) CHECK_RC(resetAccum(w, h));
+    CHECK_RC(ensureAccel());
+    if (c.mode == BTE_MODE_MEGAKERNEL) return renderMegakernel(nframes);
+    return renderWavefront(nframes);
+}
+
+}  // namespace
+}  // namespace bte
+
+using namespace bte;
+
+extern "C" {
+
+const char* bte_last_error(void) { return g_err; }
+
+int bte_init_scene(const BteTriangle* triangles, int triangleCount, const BteFloat3* materials, int materialCount,
+                   const int* triangleMaterialIds, int imageWidth, int imageHeight) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (triangleCount < 0 || (triangleCount > 0 && !triangles))
+        return fail(BTE_ERR_INVALID, "bte_init_scene: invalid triangle array");
+    if (triangleCount >= (1 << 28)) return fail(BTE_ERR_INVALID, "bte_init_scene: at most 2^28 triangles");
+    c.tris.assign(triangles, triangles + triangleCount);
+    c.materials.assign(materials ? materials : nullptr, materials ? materials + std::max(0, materialCount) : nullptr);
+    if (triangleMaterialIds) c.triMaterialIds.assign(triangleMaterialIds, triangleMaterialIds + triangleCount);
+    else c.triMaterialIds.assign((size_t)triangleCount, 0);
+    CHECK_RC(c.dTris.upload(c.tris.data(), c.tris.size() * sizeof(BteTriangle)));
+    CHECK_RC(c.dMaterials.upload(c.materials.data(), c.materials.size() * sizeof(BteFloat3)));
+    CHECK_RC(c.dTriMat.upload(c.triMaterialIds.data(), c.triMaterialIds.size() * sizeof(int)));
+    c.sceneW = imageWidth;
+    c.sceneH = imageHeight;
+    c.bvhDirty = true;
+    return rebuildEmitterOfTri();
+}
+
+int bte_init(const BteTriangle* tri, BteFloat3 color, int imageWidth, int imageHeight) {
+    if (!tri) return fail(BTE_ERR_INVALID, "bte_init: null triangle");
+    const int matId = 0;
+    return bte_init_scene(tri, 1, &color, 1, &matId, imageWidth, imageHeight);
+}
+
+int bte_init_camera(const BteCamera* camera) {
+    if (!camera) return fail(BTE_ERR_INVALID, "bte_init_camera: null camera");
+    ctx().camera = *camera;
+    ctx().haveCamera = true;
+    return BTE_OK;
+}
+
+int bte_init_triangle_emission(const BteFloat3* triangleEmission, int triangleCount) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (!triangleEmission || triangleCount <= 0) {
+        c.haveEmission = false;
+        c.dEmission.release();
+        return BTE_OK;
+    }
+    CHECK_RC(c.dEmission.upload(triangleEmission, (size_t)triangleCount * sizeof(BteFloat3)));
+    c.haveEmission = true;
+    return BTE_OK;
+}
+
+int bte_init_triangle_emitter_flags(const uint8_t* flags, int triangleCount) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (!flags || triangleCount <= 0) {
+        c.haveFlags = false;
+        c.dFlags.release();
+        return BTE_OK;
+    }
+    CHECK_RC(c.dFlags.upload(flags, (size_t)triangleCount));
+    c.haveFlags = true;
+    return BTE_OK;
+}
+
+int bte_init_emitter_table(const BteEmitter* emitters, int emitterCount, const int* emitterTriIndices,
+                           int emitterTriIndexCount, const float* emitterTriCdf, int emitterTriCdfCount,
+                           const float* sceneEmitterCdf) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (!emitters || emitterCount <= 0) {
+        c.emitterCount = 0;
+        c.emittersHost.clear();
+        c.emTriIdxHost.clear();
+        c.emTriIdxCount = c.emTriCdfCount = 0;
+        c.dEmitters.release(); c.dEmTriIdx.release(); c.dEmTriCdf.release(); c.dSceneCdf.release();
+        c.dEmitterOfTri.release();
+        return BTE_OK;
+    }
+    if (!emitterTriIndices || !emitterTriCdf || !sceneEmitterCdf || emitterTriIndexCount <= 0 || emitterTriCdfCount <= 0)
+        return fail(BTE_ERR_INVALID, "bte_init_emitter_table: missing emitter tables");
+    c.emittersHost.assign(emitters, emitters + emitterCount);
+    c.emTriIdxHost.assign(emitterTriIndices, emitterTriIndices + emitterTriIndexCount);
+    c.emitterCount = emitterCount;
+    c.emTriIdxCount = emitterTriIndexCount;
+    c.emTriCdfCount = emitterTriCdfCount;
+    CHECK_RC(c.dEmitters.upload(emitters, (size_t)emitterCount * sizeof(BteEmitter)));
+    CHECK_RC(c.dEmTriIdx.upload(emitterTriIndices, (size_t)emitterTriIndexCount * sizeof(int)));
+    CHECK_RC(c.dEmTriCdf.upload(emitterTriCdf, (size_t)emitterTriCdfCount * sizeof(float)));
+    CHECK_RC(c.dSceneCdf.upload(sceneEmitterCdf, (size_t)(emitterCount + 1) * sizeof(float)));
+    return rebuildEmitterOfTri();
+}
+
+int bte_init_bsdfs(const BteBsdf* bsdfs, int bsdfCount, const int* triangleBsdfIds, int triangleCount) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (!bsdfs || bsdfCount <= 0 || !triangleBsdfIds || triangleCount <= 0) {
+        c.bsdfsHost.clear();
+        c.triBsdfHost.clear();
+        c.dBsdfs.release();
+        c.dTriBsdf.release();
+        return BTE_OK;
+    }
+    for (int i = 0; i < bsdfCount; ++i)
+        if (bsdfs[i].type < BTE_BSDF_DIFFUSE || bsdfs[i].type > BTE_BSDF_DISNEY)
+            return fail(BTE_ERR_INVALID, "bte_init_bsdfs: unsupported bsdf type %d at %d", bsdfs[i].type, i);
+    for (int i = 0; i < triangleCount; ++i)
+        if (triangleBsdfIds[i] < 0 || triangleBsdfIds[i] >= bsdfCount)
+            return fail(BTE_ERR_INVALID, "bte_init_bsdfs: triangle %d references bsdf %d of %d", i, triangleBsdfIds[i],
+                        bsdfCount);
+    c.bsdfsHost.assign(bsdfs, bsdfs + bsdfCount);
+    c.triBsdfHost.assign(triangleBsdfIds, triangleBsdfIds + triangleCount);
+    std::vector<float> packed((size_t)bsdfCount * 16);
+    for (int i = 0; i < bsdfCount; ++i) {
+        const BteBsdf& b = bsdfs[i];
+        float* p = &packed[(size_t)i * 16];
+        std::memcpy(p + 0, &b.p0, 16);
+        std::memcpy(p + 4, &b.p1, 16);
+        std::memcpy(p + 8, &b.p2, 16);
+        std::memcpy(p + 12, &b.p3, 12);
+        std::memcpy(p + 15, &b.type, 4);
+    }
+    CHECK_RC(c.dBsdfs.upload(packed.data(), packed.size() * sizeof(float)));
+    CHECK_RC(c.dTriBsdf.upload(triangleBsdfIds, (size_t)triangleCount * sizeof(int)));
+    return BTE_OK;
+}
+
+int bte_init_spotlights(const BteSpotlight* spotlights, int spotlightCount) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (!spotlights || spotlightCount <= 0) {
+        c.spotCount = 0;
+        c.dSpots.release();
+        return BTE_OK;
+    }
+    if (spotlightCount > kMaxShadowKinds - 2)
+        return fail(BTE_ERR_UNSUPPORTED, "bte_init_spotlights: at most %d spotlights", kMaxShadowKinds - 2);
+    CHECK_RC(c.dSpots.upload(spotlights, (size_t)spotlightCount * sizeof(BteSpotlight)));
+    c.spotCount = spotlightCount;
+    return BTE_OK;
+}
+
+static int initTextureLayer(const uint8_t* const* pixels, int textureCount) {
+    if (!pixels || textureCount <= 0) return BTE_OK;
+    for (int i = 0; i < textureCount; ++i)
+        if (pixels[i]) return fail(BTE_ERR_UNSUPPORTED, "texture maps are not supported by this build");
+    return BTE_OK;
+}
+int bte_init_textures(const uint8_t* const* pixels, const int* widths, const int* heights, int textureCount) {
+    (void)widths; (void)heights;
+    return initTextureLayer(pixels, textureCount);
+}
+int bte_init_mr_textures(const uint8_t* const* pixels, const int* widths, const int* heights, int textureCount) {
+    (void)widths; (void)heights;
+    return initTextureLayer(pixels, textureCount);
+}
+
+int bte_init_env_map(const float* hdriPixelsRGBA, int width, int height, float intensity, float rotationDeg) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    c.hasEnv = false;
+    c.dEnv.release(); c.dEnvMarg.release(); c.dEnvCond.release(); c.dEnvRowSums.release(); c.dEnvTotal.release();
+    c.envW = c.envH = 0;
+    c.envTotal = 0.0f;
+    if (!hdriPixelsRGBA || width <= 0 || height <= 0) return BTE_OK;
+    const size_t texels = (size_t)width * (size_t)height;
+    CHECK_RC(c.dEnv.upload(hdriPixelsRGBA, texels * sizeof(float4)));
+    CHECK_RC(c.dEnvMarg.ensure((size_t)(height + 1) * sizeof(float)));
+    CHECK_RC(c.dEnvCond.ensure((size_t)height * (size_t)(width + 1) * sizeof(float)));
+    CHECK_RC(c.dEnvRowSums.ensure((size_t)height * sizeof(double)));
+    CHECK_RC(c.dEnvTotal.ensure(sizeof(float)));
+    launch_env_cdf(c.dEnv.as<float4>(), width, height, c.dEnvMarg.as<float>(), c.dEnvCond.as<float>(),
+                   c.dEnvRowSums.as<double>(), c.dEnvTotal.as<float>(), c.stream);
+    float total = 0.0f;
+    HIPCHK(hipMemcpyAsync(&total, c.dEnvTotal.p, sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipGetLastError());
+    c.envW = width;
+    c.envH = height;
+    c.envIntensity = intensity;
+    c.envRot = rotationDeg * (3.14159265358979323846f / 180.0f);
+    c.envTotal = total;
+    c.hasEnv = true;
+    return BTE_OK;
+}
+
+int bte_build_accel(void) {
+    ENSURE_DEVICE();
+    ctx().bvhDirty = true;
+    return ensureAccel();
+}
+
+int bte_register_framebuffer(uint32_t* deviceRGBA8) {
+    ctx().externalFb = deviceRGBA8;
+    return BTE_OK;
+}
+
+int bte_reset_accumulation(int imageWidth, int imageHeight) {
+    ENSURE_DEVICE();
+    return resetAccum(imageWidth, imageHeight);
+}
+
+int bte_render(int imageWidth, int imageHeight) { return renderFrames(imageWidth, imageHeight, 1); }
+
+int bte_render_frames(int imageWidth, int imageHeight, int nframes) {
+    return renderFrames(imageWidth, imageHeight, nframes);
+}
+
+int bte_toggle_denoiser(void) {
+    return fail(BTE_ERR_UNSUPPORTED, "the learned denoiser is not part of this build");
+}
+
+int bte_set_render_mode(int mode) {
+    if (mode != BTE_MODE_MEGAKERNEL && mode != BTE_MODE_WAVEFRONT) return fail(BTE_ERR_INVALID, "unknown mode %d", mode);
+    Context& c = ctx();
+    if (mode == c.mode) return BTE_OK;
+    c.mode = mode;
+    if (c.accW > 0) return bte_reset_accumulation(c.accW, c.accH);   // render_kernel.cu:171-176
+    return BTE_OK;
+}
+int bte_get_render_mode(void) { return ctx().mode; }
+
+int bte_cleanup(void) {
+    Context& c = ctx();
+    if (c.deviceReady) (void)hipStreamSynchronize(c.stream);
+    DBuf* bufs[] = {&c.dTris, &c.dMaterials, &c.dTriMat, &c.dTriBsdf, &c.dBsdfs, &c.dEmission, &c.dFlags,
+                    &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes,
+                    &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dAccum,
+                    &c.dFb, &c.dSum64, &c.dHit, &c.dFinal, &c.dShO, &c.dShD, &c.dShC, &c.dCounters, &c.dHitCap,
+                    &c.dOvf};
+    for (DBuf* b : bufs) b->release();
+    c.dTrav.release();
+    for (int i = 0; i < 2; ++i) {
+        c.dRayO[i].release(); c.dRayD[i].release(); c.dThr[i].release(); c.dRad[i].release(); c.dSt[i].release();
+    }
+    c.tris.clear(); c.materials.clear(); c.triMaterialIds.clear(); c.bsdfsHost.clear(); c.triBsdfHost.clear();
+    c.emittersHost.clear(); c.emTriIdxHost.clear();
+    c.emitterCount = c.spotCount = 0;
+    c.haveEmission = c.haveFlags = c.haveCamera = c.hasEnv = false;
+    c.bvhDirty = true;
+    c.accW = c.accH = 0;
+    c.frameIndex = 0;
+    c.cap = 0;
+    c.externalFb = nullptr;
+    return BTE_OK;
+}
+
+int bte_set_config(const BteConfig* config) {
+    if (!config) return fail(BTE_ERR_INVALID, "null config");
+    if (config->maxDepth <= 0 || config->maxDepth > 4096) return fail(BTE_ERR_INVALID, "maxDepth out of range");
+    if (config->megakernelSpp <= 0) return fail(BTE_ERR_INVALID, "megakernelSpp must be positive");
+    if (config->framesInFlight < 0 || config->memoryBudgetMB < 0) return fail(BTE_ERR_INVALID, "negative config");
+    ctx().cfg = *config;
+    return BTE_OK;
+}
+int bte_get_config(BteConfig* config) {
+    if (!config) return fail(BTE_ERR_INVALID, "null config");
+    *config = ctx().cfg;
+    return BTE_OK;
+}
+int bte_get_stats(BteStats* stats) {
+    if (!stats) return fail(BTE_ERR_INVALID, "null stats");
+    *stats = ctx().stats;
+    return BTE_OK;
+}
+int bte_frame_index(void) { return (int)ctx().frameIndex; }
+int bte_set_instrumentation(int flags) {
+    if (flags < 0 || flags > 3) return fail(BTE_ERR_INVALID, "unknown instrumentation flags %d", flags);
+    ctx().instr = flags;
+    return BTE_OK;
+}
+
+int bte_read_framebuffer(uint32_t* hostRGBA8, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    const size_t N = (size_t)c.accW * c.accH;
+    if (!hostRGBA8 || count < N) return fail(BTE_ERR_INVALID, "framebuffer copy needs %zu pixels", N);
+    const void* src = c.externalFb ? (const void*)c.externalFb : c.dFb.p;
+    if (!src) return fail(BTE_ERR_STATE, "nothing rendered yet");
+    HIPCHK(hipMemcpy(hostRGBA8, src, N * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return BTE_OK;
+}
+
+int bte_read_accum(float* hostRGB, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    const size_t N = (size_t)c.accW * c.accH;
+    if (!hostRGB || count < 3 * N) return fail(BTE_ERR_INVALID, "accum copy needs %zu floats", 3 * N);
+    if (!c.dAccum.p) return fail(BTE_ERR_STATE, "no accumulation buffer");
+    std::vector<float> tmp(4 * N);
+    HIPCHK(hipMemcpy(tmp.data(), c.dAccum.p, N * sizeof(float4), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < N; ++i) {
+        hostRGB[3 * i] = tmp[4 * i];
+        hostRGB[3 * i + 1] = tmp[4 * i + 1];
+        hostRGB[3 * i + 2] = tmp[4 * i + 2];
+    }
+    return BTE_OK;
+}
+
+int bte_enable_accum64(int enable) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    c.accum64 = enable != 0;
+    if (c.accum64 && c.accW > 0) {
+        const size_t N = (size_t)c.accW * c.accH;
+        CHECK_RC(c.dSum64.ensure(N * 3 * sizeof(double)));
+        HIPCHK(hipMemset(c.dSum64.p, 0, N * 3 * sizeof(double)));
+    }
+    return BTE_OK;
+}
+
+int bte_read_accum64(double* hostRGB, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    const size_t N = (size_t)c.accW * c.accH;
+    if (!hostRGB || count < 3 * N) return fail(BTE_ERR_INVALID, "accum64 copy needs %zu doubles", 3 * N);
+    if (!c.accum64 || !c.dSum64.p) return fail(BTE_ERR_STATE, "binary64 accumulation not enabled");
+    HIPCHK(hipMemcpy(hostRGB, c.dSum64.p, 3 * N * sizeof(double), hipMemcpyDeviceToHost));
+    return BTE_OK;
+}
+
+int bte_accum_device_ptr(void** devicePtr) {
+    if (!devicePtr) return fail(BTE_ERR_INVALID, "null pointer");
+    *devicePtr = ctx().dAccum.p;
+    return BTE_OK;
+}
+
+int bte_read_sample_radiance(float* hostRGB, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    const size_t P = (size_t)c.lastPassPaths;
+    if (!hostRGB || count < 3 * P) return fail(BTE_ERR_INVALID, "sample copy needs %zu floats", 3 * P);
+    if (P == 0) return fail(BTE_ERR_STATE, "no wavefront pass recorded");
+    std::vector<float> tmp(4 * P);
+    HIPCHK(hipMemcpy(tmp.data(), c.dFinal.p, P * sizeof(float4), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < P; ++i) {
+        hostRGB[3 * i] = tmp[4 * i];
+        hostRGB[3 * i + 1] = tmp[4 * i + 1];
+        hostRGB[3 * i + 2] = tmp[4 * i + 2];
+    }
+    return BTE_OK;
+}
+
+int bte_set_hit_capture(int maxRecord) {
+    if (maxRecord < 0 || maxRecord > 64) return fail(BTE_ERR_INVALID, "maxRecord must be in [0, 64]");
+    ctx().hitCaptureRec = maxRecord;
+    return BTE_OK;
+}
+
+int bte_read_hit_capture(int* hostTris, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    const size_t n = (size_t)c.lastPassPaths * (size_t)c.hitCaptureRec;
+    if (!hostTris || count < n) return fail(BTE_ERR_INVALID, "hit capture copy needs %zu ints", n);
+    if (n == 0 || !c.dHitCap.p) return fail(BTE_ERR_STATE, "no hit capture recorded");
+    HIPCHK(hipMemcpy(hostTris, c.dHitCap.p, n * sizeof(int), hipMemcpyDeviceToHost));
+    return BTE_OK;
+}
+
+int bte_read_env_cdf(float* marginal, float* conditional, float* totalPower) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (!c.hasEnv) return fail(BTE_ERR_STATE, "no environment map");
+    if (marginal) HIPCHK(hipMemcpy(marginal, c.dEnvMarg.p, (size_t)(c.envH + 1) * sizeof(float), hipMemcpyDeviceToHost));
+    if (conditional)
+        HIPCHK(hipMemcpy(conditional, c.dEnvCond.p, (size_t)c.envH * (size_t)(c.envW + 1) * sizeof(float),
+                         hipMemcpyDeviceToHost));
+    if (totalPower) *totalPower = c.envTotal;
+    return BTE_OK;
+}
+
+int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, float tmin, float tmax, int anyHit,
+                   int* tri, float* t, float* u, float* v) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (n <= 0) return BTE_OK;
+    if (!rays || !tri || (!anyHit && (!t || !u || !v))) return fail(BTE_ERR_INVALID, "bte_trace_rays: null buffer");
+    if (n >= (1ll << 31) - 128) return fail(BTE_ERR_INVALID, "bte_trace_rays: too many rays");
+    if (c.tris.empty()) return fail(BTE_ERR_STATE, "no triangles uploaded");
+    CHECK_RC(ensureAccel());
+    if (c.accW <= 0) { c.accW = std::max(1, c.sceneW); c.accH = std::max(1, c.sceneH); }
+    const LaunchShape ls = query_launch_shape(c.device);
+    CHECK_RC(c.dOvf.ensure(((size_t)persistent_threads(ls) * (kStackSize - 32) + 64) * sizeof(int)));
+    DBuf dr, dtm, dtri, dt, du, dv;
+    CHECK_RC(dr.upload(rays, (size_t)n * 6 * sizeof(float)));
+    if (anyHit && tmaxPerRay) CHECK_RC(dtm.upload(tmaxPerRay, (size_t)n * sizeof(float)));
+    CHECK_RC(dtri.ensure((size_t)n * sizeof(int)));
+    CHECK_RC(dt.ensure((size_t)n * sizeof(float)));
+    CHECK_RC(du.ensure((size_t)n * sizeof(float)));
+    CHECK_RC(dv.ensure((size_t)n * sizeof(float)));
+    const DevScene sc = makeScene();
+    launch_trace_rays(sc, dr.as<float>(), (anyHit && tmaxPerRay) ? dtm.as<float>() : nullptr, n, tmin, tmax, anyHit,
+                      dtri.as<int>(), dt.as<float>(), du.as<float>(), dv.as<float>(), c.dOvf.as<int>(), c.stream);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(tri, dtri.p, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    if (!anyHit) {
+        HIPCHK(hipMemcpy(t, dt.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(u, du.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(v, dv.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return BTE_OK;
+}
+
+}  // extern "C"

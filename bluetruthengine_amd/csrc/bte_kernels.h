@@ -1,0 +1,95 @@
+// bte_kernels.h — host-visible interface of the HIP kernels (bte_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "bte_types.h"
+
+namespace bte {
+
+// Read-only scene view passed by value to every kernel.
+struct DevScene {
+    const float4* nodes;          // BVH2, 4 float4 per node
+    const float4* leafTris;       // 3 float4 per leaf-ordered triangle (v0.w = triangle id bits)
+    const BteTriangle* tris;      // original order, 96 B each
+    const int* triBsdf;
+    const float4* bsdfs;          // 4 float4 per BSDF, type in p3.w bits
+    const float* triEmission;     // 3 floats per triangle, may be null
+    const uint8_t* triFlags;      // may be null
+    const int* emitterOfTri;      // first emitter listing the triangle, -1 if none; may be null
+    const BteEmitter* emitters;
+    const int* emTriIdx;
+    const float* emTriCdf;
+    const float* sceneCdf;
+    const BteSpotlight* spots;
+    const float4* env;            // envW*envH texels (RGBA)
+    const float* envMarg;
+    const float* envCond;
+    BteCamera cam;
+    int emitterCount;
+    int spotCount;
+    int envW, envH;
+    float envIntensity, envRot, envTotal;
+    int hasEnv;
+    int width, height;
+    int triCount;
+};
+
+// Counter block layout: kCtrStride ints per bounce iteration.
+constexpr int kCtrRays = 0;       // rays entering this bounce
+constexpr int kCtrExtendWork = 1;
+constexpr int kCtrShadeWork = 2;
+constexpr int kCtrHits = 3;
+constexpr int kCtrShadow = 4;                       // + kind: shadow rays of kind
+constexpr int kCtrConnectWork = 4 + kMaxShadowKinds; // + kind
+constexpr int kCtrStride = 4 + 2 * kMaxShadowKinds;
+
+// Wavefront buffers.  Path state lives in queue order and is compacted into
+// the other half of each double buffer by the shade kernel.
+struct WaveBuffers {
+    float4* rayO[2];      // origin xyz, w = brdfPdf of the last scatter
+    float4* rayD[2];      // direction xyz, w = eta
+    float4* thr[2];       // throughput xyz
+    float4* rad[2];       // radiance xyz
+    uint2* st[2];         // x = rng state, y = path id | specular << 31
+    float4* hit;          // (triangle id bits, t, u, v) per queue slot
+    float4* finalRad;     // per path id
+    float4* shO;          // [kinds * cap]: origin xyz, w = tMax
+    float4* shD;          // [kinds * cap]: direction xyz, w = destination bits
+    float4* shC;          // [kinds * cap]: contribution xyz
+    int* counters;        // kCtrStride * (maxDepth + 1)
+    int* hitCapture;      // optional [P * maxRec]
+    int* ovfStack;        // traversal stack overflow, per persistent thread
+    unsigned long long* travCtr;  // optional: extend nodes, extend tris, connect nodes, connect tris
+    long long cap;        // slots per buffer (paths in flight)
+    int maxRec;
+    int kinds;            // shadow kinds in use
+    int kindArea, kindSpot0, kindEnv;   // kind index of each light class (-1 absent)
+};
+
+struct LaunchShape {
+    int extendBlocks, shadeBlocks, connectBlocks;
+};
+
+// Launchers (all asynchronous on `stream`).
+void launch_fill_i32(int* p, long long n, int value, hipStream_t s);
+void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths, int pixels, int frame0,
+                     hipStream_t s);
+// ev (optional): 4 events recorded before extend, after extend, after shade,
+// after the connect passes.
+void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                   hipStream_t s, hipEvent_t* ev);
+void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, int frameBase,
+                    float4* accum, double* sum64, uint32_t* fb, hipStream_t s);
+void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex, float4* accum, double* sum64,
+                       uint32_t* fb, int* ovfStack, hipStream_t s);
+void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxPerRay, long long n, float tmin,
+                       float tmax, int anyHit, int* tri, float* t, float* u, float* v, int* ovfStack,
+                       hipStream_t s);
+void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
+                    hipStream_t s);
+LaunchShape query_launch_shape(int device);
+int persistent_threads(const LaunchShape& ls);
+
+}  // namespace bte

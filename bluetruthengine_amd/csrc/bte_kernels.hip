@@ -1,0 +1,1042 @@
+// bte_kernels.hip — CDNA4 (gfx950) kernels of the wavefront path tracer.
+//
+// Pipeline per wavefront pass (frames [f0, f0+F) of every pixel in flight):
+//   generate -> maxDepth x { extend, shade, connect[kind 0..K-1] } -> present
+// All counts stay on the device: every kernel is persistent (grid sized to the
+// machine) and pulls 64-item chunks with one atomic per wave, so the host never
+// reads a counter back between stages.
+//
+// Reference counterparts: src/wavefront/ray_generate.cu:42-80 (generate),
+// ray_extend.cu:49-72 + src/optix_programs.cu:110-148 (extend), ray_shade.cu:103-183
+// (shade), ray_connect.cu:52-72 (connect), ray_present.cu:43-62 (present),
+// src/optix_programs.cu:54-108 (megakernel), src/host/scene_upload.cu:428-459
+// (env-map CDF).
+#include <algorithm>
+
+#include "bte_kernels.h"
+#include "bte_device.h"
+
+namespace bte {
+
+constexpr int kBlock = 256;
+constexpr int kLdsStack = 32;                 // stack entries kept in LDS per lane
+constexpr int kOvfStack = kStackSize - kLdsStack;
+constexpr uint32_t kSpecBit = 0x80000000u;
+constexpr uint32_t kFinalBit = 0x80000000u;
+
+// ---------------------------------------------------------------------------
+// wave helpers
+// ---------------------------------------------------------------------------
+BTE_DEV int lane_id() { return (int)(threadIdx.x & 63u); }
+
+// Wave-aggregated append: one atomic per wave, returns this lane's slot when
+// pred is true.  Must be reached by every active lane of the wave.
+BTE_DEV int wave_append(int* ctr, bool pred) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return -1;
+    const int total = __popcll(mask);
+    const int leader = __ffsll((long long)mask) - 1;
+    int base = 0;
+    if (lane_id() == leader) base = atomicAdd(ctr, total);
+    base = __shfl(base, leader, 64);
+    const unsigned lo = (unsigned)(mask & 0xffffffffull), hi = (unsigned)(mask >> 32);
+    const int prefix = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    return base + prefix;
+}
+
+BTE_DEV void wave_add_u64(unsigned long long* ctr, uint32_t v) {
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane_id() == 0 && x) atomicAdd(ctr, x);
+}
+
+BTE_DEV int wave_grab(int* ctr) {
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(ctr, 64);
+    return __shfl(base, 0, 64);
+}
+
+// ---------------------------------------------------------------------------
+// BVH2 traversal (replaces optixTrace: rt_optix_shading.cuh:57-95)
+// Watertight ray/triangle test of Woop, Benthin & Wald (JCGT 2013); ties on t
+// resolve to the lower triangle index so the result is independent of the
+// tree.  Stack: kLdsStack entries per lane in LDS (stride kBlock, conflict
+// free), the rest in a per-thread global overflow slice.
+// ---------------------------------------------------------------------------
+struct WtRay {
+    V3 o, d;
+    float inv[3];
+    float Sx, Sy, Sz;
+    int kx, ky, kz;
+    bool valid;
+};
+BTE_DEV float comp(V3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+BTE_DEV WtRay makeWtRay(V3 o, V3 d) {
+    WtRay r;
+    r.o = o;
+    r.d = d;
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    r.valid = (ax > 0.0f || ay > 0.0f || az > 0.0f);
+    int kz = 0;
+    float best = ax;
+    if (ay > best) { kz = 1; best = ay; }
+    if (az > best) { kz = 2; }
+    int kx = kz + 1; if (kx == 3) kx = 0;
+    int ky = kx + 1; if (ky == 3) ky = 0;
+    const float dz = comp(d, kz);
+    if (dz < 0.0f) { const int t = kx; kx = ky; ky = t; }
+    r.kx = kx; r.ky = ky; r.kz = kz;
+    r.Sx = comp(d, kx) / dz;
+    r.Sy = comp(d, ky) / dz;
+    r.Sz = 1.0f / dz;
+    r.inv[0] = 1.0f / d.x;
+    r.inv[1] = 1.0f / d.y;
+    r.inv[2] = 1.0f / d.z;
+    return r;
+}
+
+// Returns true with (t, u, v) when the ray hits within [tmin, tmax].
+BTE_DEV bool hitTriangle(const WtRay& r, float4 a, float4 b, float4 c, float tmin, float tmax,
+                         float& tOut, float& uOut, float& vOut) {
+    const V3 A = sub3(xyz(a), r.o);
+    const V3 B = sub3(xyz(b), r.o);
+    const V3 C = sub3(xyz(c), r.o);
+    const float Az_ = comp(A, r.kz), Bz_ = comp(B, r.kz), Cz_ = comp(C, r.kz);
+    const float Ax = comp(A, r.kx) - r.Sx * Az_;
+    const float Ay = comp(A, r.ky) - r.Sy * Az_;
+    const float Bx = comp(B, r.kx) - r.Sx * Bz_;
+    const float By = comp(B, r.ky) - r.Sy * Bz_;
+    const float Cx = comp(C, r.kx) - r.Sx * Cz_;
+    const float Cy = comp(C, r.ky) - r.Sy * Cz_;
+    const float U = Cx * By - Cy * Bx;
+    const float V = Ax * Cy - Ay * Cx;
+    const float W = Bx * Ay - By * Ax;
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = r.Sz * Az_;
+    const float Bz = r.Sz * Bz_;
+    const float Cz = r.Sz * Cz_;
+    const float T = U * Az + V * Bz + W * Cz;
+    const float t = T / det;
+    if (!(t >= tmin && t <= tmax)) return false;
+    tOut = t;
+    uOut = V / det;
+    vOut = W / det;
+    return true;
+}
+
+// Conservative slab interval of one child box (far distance inflated).
+BTE_DEV float boxEnter(const WtRay& r, float lx, float hx, float ly, float hy, float lz, float hz, float tmin,
+                       float tmax, float& tfar) {
+    const float ax = (lx - r.o.x) * r.inv[0], bx = (hx - r.o.x) * r.inv[0];
+    const float ay = (ly - r.o.y) * r.inv[1], by = (hy - r.o.y) * r.inv[1];
+    const float az = (lz - r.o.z) * r.inv[2], bz = (hz - r.o.z) * r.inv[2];
+    const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
+    const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)) * 1.0000005f;
+    tfar = fminf(t1, tmax);
+    return t0;
+}
+
+struct StackRef {
+    int* lds;      // &lds[threadIdx.x]
+    int* ovf;      // per-thread global slice (kOvfStack entries)
+};
+BTE_DEV void push(const StackRef& s, int& sp, int v) {
+    if (sp < kLdsStack) s.lds[sp * kBlock] = v;
+    else s.ovf[sp - kLdsStack] = v;
+    ++sp;
+}
+BTE_DEV int pop(const StackRef& s, int& sp) {
+    --sp;
+    return (sp < kLdsStack) ? s.lds[sp * kBlock] : s.ovf[sp - kLdsStack];
+}
+
+// ANY = false: closest hit, returns triangle id (or -1) with t,u,v.
+// ANY = true : returns 1 if any triangle is hit within [tmin, tmax], else 0.
+template <bool ANY, bool COUNT = false>
+BTE_DEV int traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, const StackRef& stk, float& tOut,
+                     float& uOut, float& vOut, uint32_t* nodeCount = nullptr, uint32_t* triCount = nullptr) {
+    const WtRay r = makeWtRay(o, d);
+    if (!r.valid || sc.triCount == 0) return ANY ? 0 : -1;
+    float best = tmax;
+    int bestTri = -1;
+    int node = 0;
+    int sp = 0;
+    for (;;) {
+        if (node >= 0) {
+            if (COUNT) ++*nodeCount;
+            const float4* np = sc.nodes + 4 * (size_t)node;
+            const float4 n0 = np[0], n1 = np[1], n2 = np[2];
+            const int4 ch = *reinterpret_cast<const int4*>(np + 3);
+            float f0, f1;
+            const float e0 = boxEnter(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, tmin, best, f0);
+            const float e1 = boxEnter(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, tmin, best, f1);
+            const bool h0 = e0 <= f0, h1 = e1 <= f1;
+            if (h0 && h1) {
+                const bool first0 = e0 <= e1;
+                node = first0 ? ch.x : ch.y;
+                push(stk, sp, first0 ? ch.y : ch.x);
+                continue;
+            }
+            if (h0) { node = ch.x; continue; }
+            if (h1) { node = ch.y; continue; }
+        } else {
+            const int code = ~node;
+            const int start = code >> kLeafCountBits;
+            const int cnt = (code & (kMaxLeafTris - 1)) + 1;
+            for (int i = 0; i < cnt; ++i) {
+                const float4* tp = sc.leafTris + 3 * (size_t)(start + i);
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                if (COUNT) ++*triCount;
+                float t, u, v;
+                if (hitTriangle(r, a, b, c, tmin, best, t, u, v)) {
+                    if (ANY) return 1;
+                    const int id = __float_as_int(a.w);
+                    if (t < best || bestTri < 0 || id < bestTri) {
+                        best = t; bestTri = id; tOut = t; uOut = u; vOut = v;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        node = pop(stk, sp);
+    }
+    return ANY ? 0 : bestTri;
+}
+
+// ---------------------------------------------------------------------------
+// Scene helpers
+// ---------------------------------------------------------------------------
+BTE_DEV Bsdf loadBsdf(const DevScene& sc, int tri) {            // rt_material.cuh:15-24
+    const int id = sc.triBsdf[tri];
+    const float4* p = sc.bsdfs + 4 * (size_t)id;
+    Bsdf b;
+    b.p0 = p[0]; b.p1 = p[1]; b.p2 = p[2]; b.p3 = p[3];
+    b.type = __float_as_int(b.p3.w);
+    return b;
+}
+
+// Hit attributes — src/optix_programs.cu:127-141 (hit point O + tD, normalized
+// barycentric shading normal, not face-forwarded).
+struct Its { V3 p, n; int tri; };
+BTE_DEV Its makeIts(const DevScene& sc, V3 o, V3 d, float4 h) {
+    Its its;
+    its.tri = __float_as_int(h.x);
+    const float t = h.y, u = h.z, v = h.w;
+    its.p = {o.x + t * d.x, o.y + t * d.y, o.z + t * d.z};
+    const float* tn = reinterpret_cast<const float*>(sc.tris + its.tri) + 9;
+    const float w0 = 1.0f - u - v;
+    its.n = normalize3({w0 * tn[0] + u * tn[3] + v * tn[6], w0 * tn[1] + u * tn[4] + v * tn[7],
+                        w0 * tn[2] + u * tn[5] + v * tn[8]});
+    return its;
+}
+
+// Env map — rt_env_map.cuh:36-199 with exact binary32 bilinear filtering.
+BTE_DEV V3 envTexel(const DevScene& sc, int ix, int iy) {
+    const float4 t = sc.env[(size_t)iy * (size_t)sc.envW + (size_t)ix];
+    return {t.x, t.y, t.z};
+}
+BTE_DEV V3 envBilinear(const DevScene& sc, float u, float v) {
+    const int W = sc.envW, H = sc.envH;
+    float x = u * (float)W - 0.5f;
+    float y = v * (float)H - 0.5f;
+    if (!(x == x)) x = 0.0f;
+    if (!(y == y)) y = 0.0f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    const int ix = (int)fx, iy = (int)fy;
+    int ix0 = ix % W; if (ix0 < 0) ix0 += W;
+    int ix1 = ix0 + 1; if (ix1 == W) ix1 = 0;
+    const int iy0 = min(max(iy, 0), H - 1);
+    const int iy1 = min(max(iy + 1, 0), H - 1);
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    const V3 t00 = envTexel(sc, ix0, iy0), t10 = envTexel(sc, ix1, iy0);
+    const V3 t01 = envTexel(sc, ix0, iy1), t11 = envTexel(sc, ix1, iy1);
+    return {w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
+            w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
+            w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z};
+}
+BTE_DEV void dirToEquirect(V3 dir, float rot, float& u, float& v) {
+    float phi = datan2(dir.z, dir.x) + rot;
+    const float theta = dacos(fminf(fmaxf(dir.y, -1.0f), 1.0f));
+    phi = fmodf(phi + m::PI, 2.0f * m::PI);
+    if (phi < 0.0f) phi += 2.0f * m::PI;
+    phi -= m::PI;
+    u = (phi + m::PI) / (2.0f * m::PI);
+    v = theta / m::PI;
+}
+BTE_DEV V3 envEval(const DevScene& sc, V3 dir) {
+    float u, v;
+    dirToEquirect(dir, sc.envRot, u, v);
+    const V3 t = envBilinear(sc, u, v);
+    const float k = sc.envIntensity;
+    return {t.x * k, t.y * k, t.z * k};
+}
+BTE_DEV int cdfSearch(const float* cdf, int count, float x) {
+    int lo = 0, hi = count;
+    while (lo + 1 < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+BTE_DEV float envPdf(const DevScene& sc, V3 dir) {
+    if (sc.envTotal <= 0.0f) return 0.0f;
+    const int W = sc.envW, H = sc.envH;
+    float u, v;
+    dirToEquirect(dir, sc.envRot, u, v);
+    const float fx = u * (float)W;
+    const float fy = v * (float)H;
+    const int ix = min(max((int)fminf(fmaxf(fx, 0.0f), (float)W), 0), W - 1);
+    const int iy = min(max((int)fminf(fmaxf(fy, 0.0f), (float)H), 0), H - 1);
+    const float ms = sc.envMarg[H];
+    const float rw = sc.envMarg[iy + 1] - sc.envMarg[iy];
+    if (ms <= 0.0f || rw <= 0.0f) return 0.0f;
+    const float pdfRow = rw / ms;
+    const float* row = sc.envCond + (size_t)iy * (size_t)(W + 1);
+    const float rs = row[W];
+    const float cw = row[ix + 1] - row[ix];
+    if (rs <= 0.0f || cw <= 0.0f) return 0.0f;
+    const float pdfCol = cw / rs;
+    const float theta = v * m::PI;
+    const float st = fmaxf(dsin(theta), 1e-6f);
+    return (pdfRow * pdfCol * (float)W * (float)H) / (2.0f * m::PI * m::PI * st);
+}
+
+// ---------------------------------------------------------------------------
+// Direct lighting — rt_direct_lighting.cuh, rt_emitter_sampling.cuh
+// ---------------------------------------------------------------------------
+struct NeeRec { V3 d; float tMax; V3 c; bool ok; };
+
+BTE_DEV int cdfReuse(const float* cdf, int count, float& u, float& pdf) {   // :33-65
+    const float sum = cdf[count];
+    if (sum <= 0.0f || count <= 0) { pdf = 0.0f; u = 0.0f; return 0; }
+    const float x = u * sum;
+    int lo = 0, hi = count;
+    while (lo + 1 < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= x) lo = mid; else hi = mid;
+    }
+    const float c0 = cdf[lo], c1 = cdf[lo + 1];
+    const float den = c1 - c0;
+    if (den <= 0.0f) { u = 0.0f; pdf = 0.0f; return lo; }
+    u = (x - c0) / den;
+    pdf = den / sum;
+    return lo;
+}
+BTE_DEV float areaToSolidAngle(float pdfArea, V3 hitPoint, V3 origin, V3 hitNormal, V3 dirToLight) {
+    const V3 diff = sub3(hitPoint, origin);
+    const float dist2 = dot3(diff, diff);
+    const V3 negL = {-dirToLight.x, -dirToLight.y, -dirToLight.z};
+    const float den = fmaxf(fabsf(dot3(hitNormal, negL)), 1e-3f);
+    return pdfArea * dist2 / den;
+}
+
+// Emitter hit with BSDF-side MIS (:60-99).  rad += Le * throughput * w.
+BTE_DEV void emitterHit(const DevScene& sc, V3 o, V3 d, bool spec, float brdfPdf, V3 thr, const Its& its, V3& rad) {
+    V3 Le = {0, 0, 0};
+    if (sc.triEmission) {
+        const float* e = sc.triEmission + 3 * (size_t)its.tri;
+        Le = {e[0], e[1], e[2]};
+    }
+    float w = 1.0f;
+    if (!spec) {
+        const int eIdx = sc.emitterOfTri ? sc.emitterOfTri[its.tri] : -1;
+        if (eIdx >= 0) {
+            const float areaSum = sc.emitters[eIdx].areaSum;
+            const float areaPdf = (areaSum <= 0.0f) ? 0.0f : 1.0f / areaSum;
+            float sel = 0.0f;
+            const float sum = sc.sceneCdf[sc.emitterCount];
+            if (sum > 0.0f) sel = (sc.sceneCdf[eIdx + 1] - sc.sceneCdf[eIdx]) / sum;
+            const float lp = areaToSolidAngle(areaPdf * sel, its.p, o, its.n, d);
+            const float den = brdfPdf + lp;
+            w = (den <= 0.0f) ? 0.0f : brdfPdf / den;
+        }
+    }
+    if (dot3(its.n, V3{-d.x, -d.y, -d.z}) <= 0.0f) Le = {0, 0, 0};   // checkRadiance (:112-125)
+    rad = add3(rad, mul3(mul3(Le, thr), w));
+}
+
+BTE_DEV NeeRec areaNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Bsdf& bsdf, float u0, float u1) {
+    NeeRec rec;
+    rec.ok = false;
+    if (sc.emitterCount <= 0) return rec;
+    float epdf = 0.0f;
+    const int eIdx = cdfReuse(sc.sceneCdf, sc.emitterCount, u0, epdf);
+    const BteEmitter e = sc.emitters[eIdx];
+    V3 pos = {0, 0, 0}, nrm = {0, 0, 0};
+    float meshPdf = 0.0f;
+    if (e.triCount > 0) {
+        const float* triCdf = sc.emTriCdf + e.cdfOffset;
+        float pick = 0.0f;
+        const int li = cdfReuse(triCdf, e.triCount, u0, pick);
+        const BteTriangle& T = sc.tris[sc.emTriIdx[e.triIndexOffset + li]];
+        const V3 v0 = {T.v0.x, T.v0.y, T.v0.z}, v1 = {T.v1.x, T.v1.y, T.v1.z}, v2 = {T.v2.x, T.v2.y, T.v2.z};
+        const float root = sqrtf(fmaxf(0.0f, 1.0f - u0));
+        const float alpha = 1.0f - root;
+        const float beta = u1 * root;
+        const float gamma = 1.0f - alpha - beta;
+        pos = add3(add3(mul3(v0, alpha), mul3(v1, beta)), mul3(v2, gamma));
+        nrm = normalize3(cross3(sub3(v1, v0), sub3(v2, v0)));
+        const float total = triCdf[e.triCount];
+        if (total > 0.0f) meshPdf = 1.0f / total;
+    }
+    const V3 L = normalize3(sub3(pos, its.p));
+    V3 le = {e.radiance.x, e.radiance.y, e.radiance.z};
+    if (dot3(nrm, V3{-L.x, -L.y, -L.z}) <= 0.0f) le = {0, 0, 0};
+    if ((le.x <= 0.0f && le.y <= 0.0f && le.z <= 0.0f) || epdf <= 0.0f || meshPdf <= 0.0f) return rec;
+    const float dist = length3(sub3(pos, its.p));
+    const float geo = fabsf(dot3(its.n, L)) * fabsf(dot3(nrm, mul3(L, -1.0f))) / (dist * dist);
+    const Frame f = makeFrame(its.n);
+    BRec br = makeRec();
+    br.wi = toLocal(f, mul3(dIn, -1.0f));
+    br.wo = toLocal(f, L);
+    br.measure = 1;
+    const V3 fr = bsdfEval(bsdf, br);
+    const float lpa = epdf * meshPdf;
+    const float lps = areaToSolidAngle(lpa, pos, its.p, nrm, L);
+    const float bp = bsdfPdf(bsdf, br);
+    const float den = bp + lps;
+    if (den <= 0.0f) return rec;
+    const float lw = lps / den;
+    V3 c = mul3(fr, le);
+    c = mul3(c, lw);
+    c = mul3(c, geo);
+    c = div3(c, lpa);
+    rec.d = L;
+    rec.tMax = dist - RT_EPS;
+    rec.c = mul3(c, thr);
+    rec.ok = true;
+    return rec;
+}
+
+BTE_DEV NeeRec spotNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Bsdf& bsdf, int si) {  // :175-230
+    NeeRec rec;
+    rec.ok = false;
+    const BteSpotlight sp = sc.spots[si];
+    const V3 pos = {sp.position.x, sp.position.y, sp.position.z};
+    const V3 toL = sub3(pos, its.p);
+    const float sq = dot3(toL, toL);
+    if (sq < RT_EPS * RT_EPS) return rec;
+    const float dist = sqrtf(sq);
+    const V3 L = div3(toL, dist);
+    const float lc = dot3(mul3(L, -1.0f), V3{sp.direction.x, sp.direction.y, sp.direction.z});
+    if (lc <= sp.outerConeCosine) return rec;
+    const V3 srad = {sp.radiance.x, sp.radiance.y, sp.radiance.z};
+    V3 le;
+    if (lc >= sp.innerConeCosine) {
+        le = mul3(srad, sp.intensity / sq);
+    } else {
+        const float fo = (lc - sp.outerConeCosine) / (sp.innerConeCosine - sp.outerConeCosine);
+        const float sm = (fo * fo) * (3.0f - (2.0f * fo));
+        le = mul3(srad, (sp.intensity * sm) / sq);
+    }
+    const Frame f = makeFrame(its.n);
+    BRec br = makeRec();
+    br.wi = toLocal(f, mul3(dIn, -1.0f));
+    br.wo = toLocal(f, L);
+    br.measure = 1;
+    const V3 fr = bsdfEval(bsdf, br);
+    const float cs = fabsf(dot3(its.n, L));
+    rec.d = L;
+    rec.tMax = dist - RT_EPS;
+    rec.c = mul3(mul3(mul3(fr, le), cs), thr);
+    rec.ok = true;
+    return rec;
+}
+
+BTE_DEV NeeRec envNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Bsdf& bsdf, float u0, float u1) {
+    NeeRec rec;
+    rec.ok = false;
+    if (sc.envTotal <= 0.0f) return rec;
+    const int W = sc.envW, H = sc.envH;
+    const float ms = sc.envMarg[H];
+    if (ms <= 0.0f) return rec;
+    const float xr = u0 * ms;
+    const int row = cdfSearch(sc.envMarg, H, xr);
+    const float c0r = sc.envMarg[row], c1r = sc.envMarg[row + 1];
+    const float dRow = c1r - c0r;
+    const float fRow = (dRow > 0.0f) ? (xr - c0r) / dRow : 0.0f;
+    const float* rc = sc.envCond + (size_t)row * (size_t)(W + 1);
+    const float rs = rc[W];
+    if (rs <= 0.0f) return rec;
+    const float xc = u1 * rs;
+    const int col = cdfSearch(rc, W, xc);
+    const float c0c = rc[col], c1c = rc[col + 1];
+    const float dCol = c1c - c0c;
+    const float fCol = (dCol > 0.0f) ? (xc - c0c) / dCol : 0.0f;
+    const float u = ((float)col + fCol) / (float)W;
+    const float v = ((float)row + fRow) / (float)H;
+    // equirectToDir (:56-67)
+    const float phi = u * 2.0f * m::PI - m::PI - sc.envRot;
+    const float theta = v * m::PI;
+    float st, ct, sp, cp;
+    dsincos(theta, &st, &ct);
+    dsincos(phi, &sp, &cp);
+    const V3 dir = normalize3({st * cp, ct, st * sp});
+    const V3 tx = envBilinear(sc, u, v);
+    const float k = sc.envIntensity;
+    const V3 Le = {tx.x * k, tx.y * k, tx.z * k};
+    const float pdfRow = dRow / ms;
+    const float pdfCol = dCol / rs;
+    const float s2 = fmaxf(dsin(theta), 1e-6f);
+    const float pdf = (pdfRow * pdfCol * (float)W * (float)H) / (2.0f * m::PI * m::PI * s2);
+    if (pdf <= 0.0f) return rec;
+    if (Le.x <= 0.0f && Le.y <= 0.0f && Le.z <= 0.0f) return rec;
+    const float ca = dot3(its.n, dir);
+    if (ca <= 0.0f) return rec;
+    const Frame f = makeFrame(its.n);
+    BRec br = makeRec();
+    br.wi = toLocal(f, mul3(dIn, -1.0f));
+    br.wo = toLocal(f, dir);
+    br.measure = 1;
+    const V3 fr = bsdfEval(bsdf, br);
+    const float bp = bsdfPdf(bsdf, br);
+    const float sum = bp + pdf;
+    if (sum <= 0.0f) return rec;
+    const float lw = pdf / sum;
+    V3 c = mul3(fr, Le);
+    c = mul3(c, ca);
+    c = div3(c, pdf);
+    c = mul3(c, lw);
+    rec.d = dir;
+    rec.tMax = 1e30f - RT_EPS;
+    rec.c = mul3(c, thr);
+    rec.ok = true;
+    return rec;
+}
+
+BTE_DEV void envHit(const DevScene& sc, V3 d, bool spec, float brdfPdf, V3 thr, V3& rad) {   // :237-259
+    if (!sc.hasEnv) return;
+    const V3 Le = envEval(sc, d);
+    float w = 1.0f;
+    if (!spec) {
+        const float ep = envPdf(sc, d);
+        const float sum = brdfPdf + ep;
+        w = (sum > 0.0f) ? brdfPdf / sum : 0.0f;
+    }
+    rad = add3(rad, mul3(mul3(Le, thr), w));
+}
+
+// scatterPath (rt_path_integrator.cuh:25-47); writes the next ray, throughput,
+// eta and brdfPdf.
+BTE_DEV void scatter(const Its& its, const Bsdf& bsdf, uint32_t& rng, V3& o, V3& d, V3& thr, float& eta,
+                     float& brdfPdf) {
+    const Frame f = makeFrame(its.n);
+    BRec br = makeRec();
+    br.wi = toLocal(f, mul3(d, -1.0f));
+    br.rndExtra = rngNext(rng);
+    const float u0 = rngNext(rng);
+    const float u1 = rngNext(rng);
+    const V3 w = bsdfSample(bsdf, br, u0, u1);
+    brdfPdf = bsdfPdf(bsdf, br);
+    const V3 nd = normalize3(toWorld(makeFrame(its.n), br.wo));
+    o = add3(its.p, mul3(nd, RT_EPS));
+    d = nd;
+    thr = mul3(thr, w);
+    eta *= br.eta;
+}
+// russianRoulette (:54-65)
+BTE_DEV bool roulette(int bounce, uint32_t& rng, V3& thr, float eta) {
+    if (bounce > 3) {
+        const float q = fminf(maxCoeff3(thr) * eta * eta, 0.99f);
+        if (rngNext(rng) >= q) return false;
+        thr = div3(thr, q);
+    }
+    return true;
+}
+
+BTE_DEV V3 primaryDir(const BteCamera& cam, int px, int py, int W, int H, float jx, float jy) {  // rt_camera.cuh:15-37
+    const float fx = ((float)px + 0.5f + jx) / (float)W;
+    const float fy = ((float)py + 0.5f + jy) / (float)H;
+    const float ndcX = 2.0f * fx - 1.0f;
+    const float ndcY = 2.0f * fy - 1.0f;
+    const float th = dtan(0.5f * cam.fovYRadians);
+    const float sx = ndcX * cam.aspect * th;
+    const float sy = ndcY * th;
+    const V3 fw = {cam.forward.x, cam.forward.y, cam.forward.z};
+    const V3 rt = {cam.right.x, cam.right.y, cam.right.z};
+    const V3 up = {cam.up.x, cam.up.y, cam.up.z};
+    return normalize3(add3(fw, add3(mul3(rt, sx), mul3(up, sy))));
+}
+
+BTE_DEV uint32_t packGamma(V3 lin) {                               // rt_path_integrator.cuh:93-105
+    const float g = 1.0f / 2.2f;
+    const float dr = dpow(fmaxf(0.0f, fminf(lin.x, 1.0f)), g);
+    const float dg = dpow(fmaxf(0.0f, fminf(lin.y, 1.0f)), g);
+    const float db = dpow(fmaxf(0.0f, fminf(lin.z, 1.0f)), g);
+    const uint32_t r = (uint32_t)(uint8_t)(dr * 255.0f);
+    const uint32_t gg = (uint32_t)(uint8_t)(dg * 255.0f);
+    const uint32_t b = (uint32_t)(uint8_t)(db * 255.0f);
+    return (255u << 24) | (b << 16) | (gg << 8) | r;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void k_fill_i32(int* p, long long n, int v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb, long long paths, int pixels,
+                                                      int frame0) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= paths) return;
+    const int pixel = (int)(i % pixels);
+    const int frame = frame0 + (int)(i / pixels);
+    const int px = pixel % sc.width, py = pixel / sc.width;
+    uint32_t rng = seedRng(hashUint((uint32_t)pixel ^ ((uint32_t)frame * 0x9e3779b9u)));
+    const float jx = rngNext(rng) - 0.5f;
+    const float jy = rngNext(rng) - 0.5f;
+    const V3 d = primaryDir(sc.cam, px, py, sc.width, sc.height, jx, jy);
+    wb.rayO[0][i] = make_float4(sc.cam.origin.x, sc.cam.origin.y, sc.cam.origin.z, 0.0f);
+    wb.rayD[0][i] = make_float4(d.x, d.y, d.z, 1.0f);
+    wb.thr[0][i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    wb.rad[0][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    wb.st[0][i] = make_uint2(rng, (uint32_t)i | kSpecBit);
+    if (i == 0) wb.counters[kCtrRays] = (int)paths;
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, int bounce) {
+    __shared__ int lds[kLdsStack * kBlock];
+    const int cur = bounce & 1;
+    int* ctr = wb.counters + (size_t)bounce * kCtrStride;
+    const int count = ctr[kCtrRays];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    const StackRef stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
+    uint32_t nodes = 0, tris = 0;
+    for (;;) {
+        const int base = wave_grab(ctr + kCtrExtendWork);
+        if (base >= count) break;
+        const int qi = base + lane_id();
+        int tri = -1;
+        if (qi < count) {
+            const float4 o = wb.rayO[cur][qi];
+            const float4 d = wb.rayD[cur][qi];
+            float t = 0.0f, u = 0.0f, v = 0.0f;
+            tri = traverse<false, COUNT>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, &nodes, &tris);
+            wb.hit[qi] = make_float4(__int_as_float(tri), t, u, v);
+        }
+        const unsigned long long hm = __ballot(qi < count && tri >= 0);
+        if (lane_id() == 0 && hm) atomicAdd(ctr + kCtrHits, __popcll(hm));
+    }
+    if (COUNT) {
+        wave_add_u64(wb.travCtr + 0, nodes);
+        wave_add_u64(wb.travCtr + 1, tris);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, int bounce, int maxDepth) {
+    const int cur = bounce & 1, nxt = cur ^ 1;
+    int* ctr = wb.counters + (size_t)bounce * kCtrStride;
+    int* ctrNext = wb.counters + (size_t)(bounce + 1) * kCtrStride;
+    const int count = ctr[kCtrRays];
+    for (;;) {
+        const int base = wave_grab(ctr + kCtrShadeWork);
+        if (base >= count) break;
+        const int qi = base + lane_id();
+        const bool active = qi < count;
+        // ---- load path state
+        V3 o = {0, 0, 0}, d = {0, 0, 0}, thr = {0, 0, 0}, rad = {0, 0, 0};
+        float brdfPdf = 0.0f, eta = 0.0f;
+        uint32_t rng = 0, pid = 0;
+        bool spec = false, hitSurface = false;
+        float4 h = make_float4(0, 0, 0, 0);
+        if (active) {
+            const float4 ro = wb.rayO[cur][qi], rd = wb.rayD[cur][qi], th = wb.thr[cur][qi], ra = wb.rad[cur][qi];
+            const uint2 s = wb.st[cur][qi];
+            h = wb.hit[qi];
+            o = xyz(ro); brdfPdf = ro.w;
+            d = xyz(rd); eta = rd.w;
+            thr = xyz(th);
+            rad = xyz(ra);
+            rng = s.x;
+            pid = s.y & ~kSpecBit;
+            spec = (s.y & kSpecBit) != 0;
+            hitSurface = __float_as_int(h.x) >= 0;
+            if (wb.hitCapture && bounce < wb.maxRec) wb.hitCapture[(size_t)pid * wb.maxRec + bounce] = __float_as_int(h.x);
+        }
+        // ---- miss: environment with MIS, path ends
+        if (active && !hitSurface) envHit(sc, d, spec, brdfPdf, thr, rad);
+        // ---- hit: emitter MIS, then scatter + RR (RNG advanced past the NEE draws)
+        Its its;
+        its.tri = -1; its.p = {0, 0, 0}; its.n = {0, 0, 0};
+        Bsdf bsdf;
+        bsdf.type = 0;
+        const V3 dIn = d, thrIn = thr;
+        uint32_t rngNee = rng;
+        bool survive = false, diffuseType = false;
+        if (active && hitSurface) {
+            its = makeIts(sc, o, d, h);
+            bsdf = loadBsdf(sc, its.tri);
+            if (sc.triFlags && sc.triFlags[its.tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
+            diffuseType = bsdfIsDiffuse(bsdf.type);
+            spec = !diffuseType;
+            if (diffuseType) {
+                rngNext(rng); rngNext(rng);                 // area NEE draws (always)
+                if (sc.hasEnv) { rngNext(rng); rngNext(rng); }   // env NEE draws
+            }
+            scatter(its, bsdf, rng, o, d, thr, eta, brdfPdf);
+            survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
+        }
+        // ---- compaction of survivors into the next queue
+        const int slot = wave_append(ctrNext + kCtrRays, survive);
+        if (survive) {
+            wb.rayO[nxt][slot] = make_float4(o.x, o.y, o.z, brdfPdf);
+            wb.rayD[nxt][slot] = make_float4(d.x, d.y, d.z, eta);
+            wb.thr[nxt][slot] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+            wb.rad[nxt][slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+            wb.st[nxt][slot] = make_uint2(rng, pid | (spec ? kSpecBit : 0u));
+        } else if (active) {
+            wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+        }
+        // ---- NEE records, one kind at a time in the reference's order
+        const uint32_t dest = survive ? (uint32_t)slot : (pid | kFinalBit);
+        if (wb.kinds > 0) {
+            float a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f;
+            if (diffuseType) {
+                a0 = rngNext(rngNee); a1 = rngNext(rngNee);
+                if (sc.hasEnv) { e0 = rngNext(rngNee); e1 = rngNext(rngNee); }
+            }
+            for (int k = 0; k < wb.kinds; ++k) {
+                NeeRec rec;
+                rec.ok = false;
+                if (diffuseType) {
+                    if (k == wb.kindArea) rec = areaNEE(sc, dIn, thrIn, its, bsdf, a0, a1);
+                    else if (k == wb.kindEnv) rec = envNEE(sc, dIn, thrIn, its, bsdf, e0, e1);
+                    else rec = spotNEE(sc, dIn, thrIn, its, bsdf, k - wb.kindSpot0);
+                }
+                const int s = wave_append(ctr + kCtrShadow + k, rec.ok);
+                if (rec.ok) {
+                    const size_t idx = (size_t)k * (size_t)wb.cap + (size_t)s;
+                    wb.shO[idx] = make_float4(its.p.x, its.p.y, its.p.z, rec.tMax);
+                    wb.shD[idx] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(dest));
+                    wb.shC[idx] = make_float4(rec.c.x, rec.c.y, rec.c.z, 0.0f);
+                }
+            }
+        }
+    }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb, int bounce, int kind) {
+    __shared__ int lds[kLdsStack * kBlock];
+    const int nxt = (bounce & 1) ^ 1;
+    int* ctr = wb.counters + (size_t)bounce * kCtrStride;
+    const int count = ctr[kCtrShadow + kind];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    const StackRef stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
+    const size_t off = (size_t)kind * (size_t)wb.cap;
+    uint32_t nodes = 0, tris = 0;
+    for (;;) {
+        const int base = wave_grab(ctr + kCtrConnectWork + kind);
+        if (base >= count) break;
+        const int si = base + lane_id();
+        if (si < count) {
+            const float4 so = wb.shO[off + si];
+            const float4 sd = wb.shD[off + si];
+            float t, u, v;
+            const int occ = traverse<true, COUNT>(sc, xyz(so), xyz(sd), RT_EPS, so.w, stk, t, u, v, &nodes, &tris);
+            if (!occ) {
+                const float4 c = wb.shC[off + si];
+                const uint32_t dest = __float_as_uint(sd.w);
+                float4* r = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.rad[nxt] + dest);
+                float4 cur = *r;
+                cur.x = cur.x + c.x;
+                cur.y = cur.y + c.y;
+                cur.z = cur.z + c.z;
+                *r = cur;
+            }
+        }
+    }
+    if (COUNT) {
+        wave_add_u64(wb.travCtr + 2, nodes);
+        wave_add_u64(wb.travCtr + 3, tris);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_present(WaveBuffers wb, int pixels, int frames, int frameBase,
+                                                     float4* accum, double* sum64, uint32_t* fb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    float4 mean = accum[p];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int f = 0; f < frames; ++f) {
+        const float4 x = wb.finalRad[(size_t)f * pixels + p];
+        const float n = (float)((uint32_t)(frameBase + f) + 1u);
+        mean.x = mean.x + (x.x - mean.x) / n;
+        mean.y = mean.y + (x.y - mean.y) / n;
+        mean.z = mean.z + (x.z - mean.z) / n;
+        sx += (double)x.x; sy += (double)x.y; sz += (double)x.z;
+    }
+    accum[p] = mean;
+    if (sum64) {
+        sum64[3 * (size_t)p] += sx;
+        sum64[3 * (size_t)p + 1] += sy;
+        sum64[3 * (size_t)p + 2] += sz;
+    }
+    if (fb) fb[p] = packGamma(V3{mean.x, mean.y, mean.z});
+}
+
+// Megakernel (src/optix_programs.cu:54-108): one pixel per lane, spp paths
+// sharing one RNG stream, inline visibility.
+__global__ __launch_bounds__(kBlock) void k_megakernel(DevScene sc, int spp, int maxDepth, int frameIndex,
+                                                        float4* accum, double* sum64, uint32_t* fb, int* ovf,
+                                                        int* work) {
+    __shared__ int lds[kLdsStack * kBlock];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    const StackRef stk{lds + threadIdx.x, ovf + (size_t)gtid * kOvfStack};
+    const int pixels = sc.width * sc.height;
+    for (;;) {
+        const int base = wave_grab(work);
+        if (base >= pixels) break;
+        const int p = base + lane_id();
+        if (p >= pixels) continue;
+        const int px = p % sc.width, py = p / sc.width;
+        uint32_t rng = seedRng(hashUint((uint32_t)p ^ ((uint32_t)frameIndex * 0x9e3779b9u)));
+        V3 fin = {0, 0, 0};
+        for (int i = 0; i < spp; ++i) {
+            const float jx = rngNext(rng) - 0.5f;
+            const float jy = rngNext(rng) - 0.5f;
+            V3 o = {sc.cam.origin.x, sc.cam.origin.y, sc.cam.origin.z};
+            V3 d = primaryDir(sc.cam, px, py, sc.width, sc.height, jx, jy);
+            V3 thr = {1, 1, 1}, rad = {0, 0, 0};
+            float eta = 1.0f, brdfPdf = 0.0f;
+            bool spec = true;
+            for (int bounce = 0; bounce < maxDepth; ++bounce) {
+                float t = 0.0f, u = 0.0f, v = 0.0f;
+                const int tri = traverse<false>(sc, o, d, RT_EPS, 1e30f, stk, t, u, v);
+                if (tri < 0) { envHit(sc, d, spec, brdfPdf, thr, rad); break; }
+                const Its its = makeIts(sc, o, d, make_float4(__int_as_float(tri), t, u, v));
+                const Bsdf bsdf = loadBsdf(sc, tri);
+                if (sc.triFlags && sc.triFlags[tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
+                if (bsdfIsDiffuse(bsdf.type)) {
+                    spec = false;
+                    const float a0 = rngNext(rng), a1 = rngNext(rng);
+                    NeeRec r = areaNEE(sc, d, thr, its, bsdf, a0, a1);
+                    if (r.ok && !traverse<true>(sc, its.p, r.d, RT_EPS, r.tMax, stk, t, u, v)) rad = add3(rad, r.c);
+                    for (int s = 0; s < sc.spotCount; ++s) {
+                        r = spotNEE(sc, d, thr, its, bsdf, s);
+                        if (r.ok && !traverse<true>(sc, its.p, r.d, RT_EPS, r.tMax, stk, t, u, v)) rad = add3(rad, r.c);
+                    }
+                    if (sc.hasEnv) {
+                        const float e0 = rngNext(rng), e1 = rngNext(rng);
+                        r = envNEE(sc, d, thr, its, bsdf, e0, e1);
+                        if (r.ok && !traverse<true>(sc, its.p, r.d, RT_EPS, r.tMax, stk, t, u, v)) rad = add3(rad, r.c);
+                    }
+                } else {
+                    spec = true;
+                }
+                scatter(its, bsdf, rng, o, d, thr, eta, brdfPdf);
+                if (!roulette(bounce, rng, thr, eta)) break;
+            }
+            fin = add3(fin, rad);
+        }
+        fin = div3(fin, (float)spp);
+        float4 mean = accum[p];
+        const float n = (float)((uint32_t)frameIndex + 1u);
+        mean.x = mean.x + (fin.x - mean.x) / n;
+        mean.y = mean.y + (fin.y - mean.y) / n;
+        mean.z = mean.z + (fin.z - mean.z) / n;
+        accum[p] = mean;
+        if (sum64) {
+            sum64[3 * (size_t)p] += (double)fin.x;
+            sum64[3 * (size_t)p + 1] += (double)fin.y;
+            sum64[3 * (size_t)p + 2] += (double)fin.z;
+        }
+        if (fb) fb[p] = packGamma(V3{mean.x, mean.y, mean.z});
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene sc, const float* rays, const float* tmaxPer,
+                                                        long long n, float tmin, float tmax, int anyHit, int* tri,
+                                                        float* tt, float* uu, float* vv, int* ovf, int* work) {
+    __shared__ int lds[kLdsStack * kBlock];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    const StackRef stk{lds + threadIdx.x, ovf + (size_t)gtid * kOvfStack};
+    for (;;) {
+        const int base = wave_grab(work);
+        if (base >= n) break;
+        const long long i = base + lane_id();
+        if (i >= n) continue;
+        const float* r = rays + 6 * i;
+        const V3 o = {r[0], r[1], r[2]}, d = {r[3], r[4], r[5]};
+        float t = 0.0f, u = 0.0f, v = 0.0f;
+        if (anyHit) {
+            tri[i] = traverse<true>(sc, o, d, tmin, tmaxPer ? tmaxPer[i] : tmax, stk, t, u, v);
+        } else {
+            tri[i] = traverse<false>(sc, o, d, tmin, tmax, stk, t, u, v);
+            tt[i] = t; uu[i] = u; vv[i] = v;
+        }
+    }
+}
+
+// Env-map CDF (src/host/scene_upload.cu:428-459) in binary64: one block per
+// row scans lum(texel) * sin(theta_row), then one block scans the row sums.
+constexpr int kScanBlock = 256;
+__device__ double block_exclusive_scan(double v, double* sh, double& total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+        const double a = (t >= off) ? sh[t - off] : 0.0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    total = sh[kScanBlock - 1];
+    const double incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+__global__ __launch_bounds__(kScanBlock) void k_env_rows(const float4* env, int W, int H, float* cond,
+                                                          double* rowSums) {
+    __shared__ double sh[kScanBlock];
+    const int iy = blockIdx.x;
+    const double theta = 3.14159265358979323846 * ((double)iy + 0.5) / (double)H;
+    const double st = sin(theta);
+    const int per = (W + kScanBlock - 1) / kScanBlock;
+    const int x0 = threadIdx.x * per;
+    double local = 0.0;
+    for (int k = 0; k < per; ++k) {
+        const int ix = x0 + k;
+        if (ix < W) {
+            const float4 p = env[(size_t)iy * W + ix];
+            local += (0.2126 * (double)p.x + 0.7152 * (double)p.y + 0.0722 * (double)p.z) * st;
+        }
+    }
+    double total;
+    double run = block_exclusive_scan(local, sh, total);
+    float* row = cond + (size_t)iy * (size_t)(W + 1);
+    if (threadIdx.x == 0) row[0] = 0.0f;
+    for (int k = 0; k < per; ++k) {
+        const int ix = x0 + k;
+        if (ix < W) {
+            const float4 p = env[(size_t)iy * W + ix];
+            run += (0.2126 * (double)p.x + 0.7152 * (double)p.y + 0.0722 * (double)p.z) * st;
+            row[ix + 1] = (float)run;
+        }
+    }
+    if (threadIdx.x == 0) rowSums[iy] = total;
+}
+__global__ __launch_bounds__(kScanBlock) void k_env_marginal(const double* rowSums, int H, float* marg,
+                                                              float* total) {
+    __shared__ double sh[kScanBlock];
+    const int per = (H + kScanBlock - 1) / kScanBlock;
+    const int y0 = threadIdx.x * per;
+    double local = 0.0;
+    for (int k = 0; k < per; ++k)
+        if (y0 + k < H) local += rowSums[y0 + k];
+    double tot;
+    double run = block_exclusive_scan(local, sh, tot);
+    if (threadIdx.x == 0) marg[0] = 0.0f;
+    for (int k = 0; k < per; ++k) {
+        if (y0 + k < H) {
+            run += rowSums[y0 + k];
+            marg[y0 + k + 1] = (float)run;
+        }
+    }
+    if (threadIdx.x == 0) *total = (float)tot;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static int blocksFor(long long n, int b) { return (int)((n + b - 1) / b); }
+
+void launch_fill_i32(int* p, long long n, int value, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocksFor(n, 256)), dim3(256), 0, s, p, n, value);
+}
+
+void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths, int pixels, int frame0,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_generate, dim3(blocksFor(paths, kBlock)), dim3(kBlock), 0, s, sc, wb, paths, pixels, frame0);
+}
+
+void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                   hipStream_t s, hipEvent_t* ev) {
+    const bool count = wb.travCtr != nullptr;
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if (count) hipLaunchKernelGGL(k_extend<true>, dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    else hipLaunchKernelGGL(k_extend<false>, dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    hipLaunchKernelGGL(k_shade, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce, maxDepth);
+    if (ev) (void)hipEventRecord(ev[2], s);
+    for (int k = 0; k < wb.kinds; ++k) {
+        if (count) hipLaunchKernelGGL(k_connect<true>, dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+        else hipLaunchKernelGGL(k_connect<false>, dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+    }
+    if (ev) (void)hipEventRecord(ev[3], s);
+}
+
+void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, int frameBase,
+                    float4* accum, double* sum64, uint32_t* fb, hipStream_t s) {
+    (void)sc;
+    hipLaunchKernelGGL(k_present, dim3(blocksFor(pixels, kBlock)), dim3(kBlock), 0, s, wb, pixels, frames, frameBase,
+                       accum, sum64, fb);
+}
+
+void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex, float4* accum, double* sum64,
+                       uint32_t* fb, int* ovfStack, hipStream_t s) {
+    // work counter lives in the last int of the overflow area's header
+    int* work = ovfStack;
+    int* ovf = ovfStack + 64;
+    launch_fill_i32(work, 1, 0, s);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const LaunchShape ls = query_launch_shape(dev);
+    hipLaunchKernelGGL(k_megakernel, dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, spp, maxDepth, frameIndex, accum,
+                       sum64, fb, ovf, work);
+}
+
+void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxPerRay, long long n, float tmin,
+                       float tmax, int anyHit, int* tri, float* t, float* u, float* v, int* ovfStack, hipStream_t s) {
+    int* work = ovfStack;
+    int* ovf = ovfStack + 64;
+    launch_fill_i32(work, 1, 0, s);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const LaunchShape ls = query_launch_shape(dev);
+    hipLaunchKernelGGL(k_trace_rays, dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, rays, tmaxPerRay, n, tmin, tmax,
+                       anyHit, tri, t, u, v, ovf, work);
+}
+
+void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_env_rows, dim3(H), dim3(kScanBlock), 0, s, env, W, H, cond, rowSums);
+    hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(kScanBlock), 0, s, rowSums, H, marg, total);
+}
+
+LaunchShape query_launch_shape(int device) {
+    static int cachedDev = -1;
+    static LaunchShape cached{};
+    if (device == cachedDev) return cached;
+    hipDeviceProp_t prop;
+    (void)hipGetDeviceProperties(&prop, device);
+    const int cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
+    int occE = 0, occS = 0, occC = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occE, k_extend<false>, kBlock, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_shade, kBlock, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occC, k_connect<false>, kBlock, 0);
+    int occM = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occM, k_megakernel, kBlock, 0);
+    int occT = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_trace_rays, kBlock, 0);
+    const int occTrav = std::max(1, std::min(std::min(occE, occC), std::min(occM, occT)));
+    cached.extendBlocks = cus * occTrav;
+    cached.connectBlocks = cus * occTrav;
+    cached.shadeBlocks = cus * std::max(1, occS);
+    cachedDev = device;
+    return cached;
+}
+
+int persistent_threads(const LaunchShape& ls) {
+    return std::max(ls.extendBlocks, ls.connectBlocks) * kBlock;
+}
+
+}  // namespace bte

@@ -1,0 +1,33 @@
+// bte_types.h — host/device shared layouts of the wavefront path tracer.
+#pragma once
+
+#include <cstdint>
+
+#include "../../include/bte_api.h"
+
+namespace bte {
+
+// Device BVH2 node, 64 B = 4 x float4 (two child boxes per node):
+//   n0 = (c0.lo.x, c0.hi.x, c0.lo.y, c0.hi.y)
+//   n1 = (c1.lo.x, c1.hi.x, c1.lo.y, c1.hi.y)
+//   n2 = (c0.lo.z, c0.hi.z, c1.lo.z, c1.hi.z)
+//   n3 = (child0, child1, 0, 0) as int bits
+// child >= 0: inner node index; child < 0: leaf, ~child = start << 3 | (count - 1)
+// where [start, start+count) indexes the leaf-ordered triangle array.
+constexpr int kLeafCountBits = 3;
+constexpr int kMaxLeafTris = 1 << kLeafCountBits;
+constexpr int kMaxBvhDepth = 60;      // builder guarantee; traversal stack holds kStackSize entries
+constexpr int kStackSize = 64;
+
+struct BvhHostNode {
+    float n[12];
+    int child[2];
+    int pad[2];
+};
+static_assert(sizeof(BvhHostNode) == 64, "BVH node must be 64 B");
+
+// Shadow-slot kinds, traced in this order so contributions add in the
+// reference's order (area, spotlights, environment).
+constexpr int kMaxShadowKinds = 2 + 14;   // area + env + up to 14 spotlights
+
+}  // namespace bte

@@ -1,0 +1,318 @@
+"""Python mirror of the C ABI in ``include/bte_api.h`` (libbte.so).
+
+The functions keep the reference's names and argument meaning
+(``include/render_kernel.h:96-211``: cudaInitScene, cudaInitCamera, ...,
+cudaRender); each raises ``BteError`` on a non-zero status instead of exiting,
+so callers and tests can observe failures.  ``Renderer`` uploads a flattened
+``scenes.SceneData`` in the reference's call order (``src/main.cpp:846-935``).
+
+The library is loaded from ``bluetruthengine_amd/_lib/libbte.so``; there is no
+CPU fallback — a missing library raises immediately.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import pod
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libbte.so")
+
+BTE_MODE_MEGAKERNEL = 0
+BTE_MODE_WAVEFRONT = 1
+
+_p = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_ll = ctypes.c_longlong
+_sz = ctypes.c_size_t
+
+
+class BteError(RuntimeError):
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn} failed with status {code}: {msg}")
+        self.code = code
+
+
+class BteFloat3(ctypes.Structure):
+    _fields_ = [("x", _f), ("y", _f), ("z", _f)]
+
+
+class BteConfig(ctypes.Structure):
+    _fields_ = [("maxDepth", _i), ("megakernelSpp", _i), ("framesInFlight", _i), ("memoryBudgetMB", _i)]
+
+
+class BteStats(ctypes.Structure):
+    _fields_ = [("renderMs", ctypes.c_double), ("extensionRays", _ll), ("hits", _ll), ("shadowRays", _ll),
+                ("pathsTraced", _ll), ("bounces", _i), ("framesInFlight", _i), ("bvhNodes", _i), ("bvhDepth", _i),
+                ("raysPerBounce", _ll * 64),
+                ("extendMs", ctypes.c_double), ("shadeMs", ctypes.c_double), ("connectMs", ctypes.c_double),
+                ("extendLaunches", _i), ("connectLaunches", _i),
+                ("extendNodes", _ll), ("extendTris", _ll), ("connectNodes", _ll), ("connectTris", _ll)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "raysPerBounce"}
+        d["raysPerBounce"] = [int(x) for x in self.raysPerBounce[: max(1, self.bounces)]]
+        return d
+
+
+# (name, argtypes) for every entry point of include/bte_api.h
+API = {
+    "bte_init_scene": [_p, _i, _p, _i, _p, _i, _i],
+    "bte_init": [_p, BteFloat3, _i, _i],
+    "bte_init_camera": [_p],
+    "bte_init_triangle_emission": [_p, _i],
+    "bte_init_triangle_emitter_flags": [_p, _i],
+    "bte_init_emitter_table": [_p, _i, _p, _i, _p, _i, _p],
+    "bte_init_bsdfs": [_p, _i, _p, _i],
+    "bte_init_spotlights": [_p, _i],
+    "bte_init_textures": [_p, _p, _p, _i],
+    "bte_init_mr_textures": [_p, _p, _p, _i],
+    "bte_init_env_map": [_p, _i, _i, _f, _f],
+    "bte_build_accel": [],
+    "bte_register_framebuffer": [_p],
+    "bte_reset_accumulation": [_i, _i],
+    "bte_render": [_i, _i],
+    "bte_render_frames": [_i, _i, _i],
+    "bte_toggle_denoiser": [],
+    "bte_set_render_mode": [_i],
+    "bte_get_render_mode": [],
+    "bte_cleanup": [],
+    "bte_set_config": [_p],
+    "bte_get_config": [_p],
+    "bte_get_stats": [_p],
+    "bte_set_instrumentation": [_i],
+    "bte_last_error": [],
+    "bte_frame_index": [],
+    "bte_read_framebuffer": [_p, _sz],
+    "bte_read_accum": [_p, _sz],
+    "bte_enable_accum64": [_i],
+    "bte_read_accum64": [_p, _sz],
+    "bte_accum_device_ptr": [_p],
+    "bte_read_sample_radiance": [_p, _sz],
+    "bte_set_hit_capture": [_i],
+    "bte_read_hit_capture": [_p, _sz],
+    "bte_read_env_cdf": [_p, _p, _p],
+    "bte_trace_rays": [_p, _p, _ll, _f, _f, _i, _p, _p, _p, _p],
+}
+
+_lib = None
+
+
+def lib():
+    """Load libbte.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in API.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "bte_last_error" else _i
+        _lib = L
+    return _lib
+
+
+def _call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc < 0:
+        raise BteError(name, rc, lib().bte_last_error().decode(errors="replace"))
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---- reference-named wrappers (include/render_kernel.h:103-211) -------------
+def cudaInitScene(triangles, materials, triangle_material_ids, width, height):
+    tris = np.ascontiguousarray(triangles, pod.TRIANGLE)
+    mats = np.ascontiguousarray(materials, np.float32).reshape(-1, 3)
+    ids = np.ascontiguousarray(triangle_material_ids, np.int32)
+    return _call("bte_init_scene", _ptr(tris), tris.shape[0], _ptr(mats), mats.shape[0], _ptr(ids), width, height)
+
+
+def cudaInitCamera(camera):
+    cam = np.ascontiguousarray(np.asarray(camera, pod.CAMERA).reshape(()))
+    return _call("bte_init_camera", _ptr(cam))
+
+
+def cudaInitTriangleEmission(emission):
+    if emission is None:
+        return _call("bte_init_triangle_emission", None, 0)
+    e = np.ascontiguousarray(emission, np.float32).reshape(-1, 3)
+    return _call("bte_init_triangle_emission", _ptr(e), e.shape[0])
+
+
+def cudaInitTriangleEmitterFlags(flags):
+    if flags is None:
+        return _call("bte_init_triangle_emitter_flags", None, 0)
+    f = np.ascontiguousarray(flags, np.uint8)
+    return _call("bte_init_triangle_emitter_flags", _ptr(f), f.size)
+
+
+def cudaInitEmitterTable(emitters, tri_indices, tri_cdf, scene_cdf):
+    if emitters is None or len(emitters) == 0:
+        return _call("bte_init_emitter_table", None, 0, None, 0, None, 0, None)
+    e = np.ascontiguousarray(emitters, pod.EMITTER)
+    ti = np.ascontiguousarray(tri_indices, np.int32)
+    tc = np.ascontiguousarray(tri_cdf, np.float32)
+    sc = np.ascontiguousarray(scene_cdf, np.float32)
+    return _call("bte_init_emitter_table", _ptr(e), e.shape[0], _ptr(ti), ti.size, _ptr(tc), tc.size, _ptr(sc))
+
+
+def cudaInitBsdfs(bsdfs, tri_bsdf_ids):
+    b = np.ascontiguousarray(bsdfs, pod.BSDF)
+    ids = np.ascontiguousarray(tri_bsdf_ids, np.int32)
+    return _call("bte_init_bsdfs", _ptr(b), b.shape[0], _ptr(ids), ids.size)
+
+
+def cudaInitSpotlights(spotlights):
+    if spotlights is None or len(spotlights) == 0:
+        return _call("bte_init_spotlights", None, 0)
+    s = np.ascontiguousarray(spotlights, pod.SPOTLIGHT)
+    return _call("bte_init_spotlights", _ptr(s), s.shape[0])
+
+
+def cudaInitEnvMap(rgba, intensity=1.0, rotation_deg=0.0):
+    if rgba is None:
+        return _call("bte_init_env_map", None, 0, 0, 0.0, 0.0)
+    e = np.ascontiguousarray(rgba, np.float32)
+    H, W = e.shape[:2]
+    return _call("bte_init_env_map", _ptr(e), W, H, float(intensity), float(rotation_deg))
+
+
+def cudaInitOptix():
+    return _call("bte_build_accel")
+
+
+def cudaResetAccumulation(width, height):
+    return _call("bte_reset_accumulation", width, height)
+
+
+def cudaRender(width, height):
+    return _call("bte_render", width, height)
+
+
+def cudaSetRenderMode(mode):
+    return _call("bte_set_render_mode", int(mode))
+
+
+def cudaGetRenderMode():
+    return _call("bte_get_render_mode")
+
+
+def cudaToggleDenoiser():
+    return _call("bte_toggle_denoiser")
+
+
+def cudaCleanup():
+    return _call("bte_cleanup")
+
+
+# ---- headless renderer -----------------------------------------------------------
+class Renderer:
+    """Uploads a SceneData through the C ABI and renders into device buffers."""
+
+    def __init__(self, scene, max_depth=None, frames_in_flight=0, memory_budget_mb=0, mode=BTE_MODE_WAVEFRONT,
+                 megakernel_spp=5):
+        self.scene = scene
+        lib()
+        cudaCleanup()
+        cudaInitScene(scene.triangles, np.zeros((1, 3), np.float32), scene.tri_material_ids, scene.width,
+                      scene.height)
+        cudaInitCamera(scene.camera)
+        cudaInitTriangleEmission(scene.tri_emission)
+        cudaInitTriangleEmitterFlags(scene.tri_emitter_flags)
+        cudaInitEmitterTable(scene.emitters, scene.emitter_tri_indices, scene.emitter_tri_cdf,
+                             scene.scene_emitter_cdf)
+        cudaInitBsdfs(scene.bsdfs, scene.tri_bsdf_ids)
+        cudaInitSpotlights(scene.spotlights)
+        cudaInitEnvMap(scene.env_rgba, scene.env_intensity, scene.env_rotation_deg)
+        self.set_config(max_depth if max_depth is not None else scene.max_depth, frames_in_flight,
+                        memory_budget_mb, megakernel_spp)
+        cudaSetRenderMode(mode)
+        cudaResetAccumulation(scene.width, scene.height)
+
+    def set_config(self, max_depth, frames_in_flight=0, memory_budget_mb=0, megakernel_spp=5):
+        cfg = BteConfig(int(max_depth), int(megakernel_spp), int(frames_in_flight), int(memory_budget_mb))
+        _call("bte_set_config", ctypes.byref(cfg))
+
+    def build_accel(self):
+        cudaInitOptix()
+
+    def reset(self):
+        cudaResetAccumulation(self.scene.width, self.scene.height)
+
+    def render(self, frames=1):
+        _call("bte_render_frames", self.scene.width, self.scene.height, int(frames))
+
+    def set_instrumentation(self, flags):
+        _call("bte_set_instrumentation", int(flags))
+
+    def stats(self):
+        s = BteStats()
+        _call("bte_get_stats", ctypes.byref(s))
+        return s
+
+    def frame_index(self):
+        return _call("bte_frame_index")
+
+    def accum(self):
+        W, H = self.scene.width, self.scene.height
+        out = np.empty((H, W, 3), np.float32)
+        _call("bte_read_accum", _ptr(out), out.size)
+        return out
+
+    def framebuffer(self):
+        W, H = self.scene.width, self.scene.height
+        out = np.empty((H, W), np.uint32)
+        _call("bte_read_framebuffer", _ptr(out), out.size)
+        return out
+
+    def enable_accum64(self, on=True):
+        _call("bte_enable_accum64", int(on))
+
+    def accum64(self):
+        W, H = self.scene.width, self.scene.height
+        out = np.empty((H, W, 3), np.float64)
+        _call("bte_read_accum64", _ptr(out), out.size)
+        return out
+
+    def sample_radiance(self, frames):
+        W, H = self.scene.width, self.scene.height
+        out = np.empty((frames, H, W, 3), np.float32)
+        _call("bte_read_sample_radiance", _ptr(out), out.size)
+        return out
+
+    def set_hit_capture(self, max_rec):
+        _call("bte_set_hit_capture", int(max_rec))
+
+    def hit_capture(self, frames, max_rec):
+        W, H = self.scene.width, self.scene.height
+        out = np.empty((frames, H * W, max_rec), np.int32)
+        _call("bte_read_hit_capture", _ptr(out), out.size)
+        return out
+
+    def env_cdf(self):
+        H, W = self.scene.env_rgba.shape[:2]
+        marg = np.empty(H + 1, np.float32)
+        cond = np.empty(H * (W + 1), np.float32)
+        total = ctypes.c_float(0)
+        _call("bte_read_env_cdf", _ptr(marg), _ptr(cond), ctypes.byref(total))
+        return marg, cond, np.float32(total.value)
+
+
+def trace_rays(rays, tmin=1e-4, tmax=1e30, any_hit=False, tmax_per_ray=None):
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    tri = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    u = np.empty(n, np.float32)
+    v = np.empty(n, np.float32)
+    tm = None if tmax_per_ray is None else np.ascontiguousarray(tmax_per_ray, np.float32)
+    _call("bte_trace_rays", _ptr(rays), _ptr(tm), n, float(tmin), float(tmax), int(any_hit), _ptr(tri), _ptr(t),
+          _ptr(u), _ptr(v))
+    return (tri, t, u, v) if not any_hit else tri

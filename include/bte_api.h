@@ -1,0 +1,204 @@
+/* bte_api.h — C ABI of the MI355X wavefront path tracer (libbte.so).
+ *
+ * Drop-in boundary for the reference's public renderer API
+ * (include/render_kernel.h:96-211 of Anil-Parthasarathi/blueTruthEngine).
+ * Every entry point keeps the reference's argument list and meaning; the
+ * differences are the ones a headless, non-exiting C ABI needs:
+ *   - functions return an int status (BTE_OK = 0, negative = error class)
+ *     instead of calling exit(); bte_last_error() describes the last failure;
+ *   - rendering targets a device framebuffer owned by the library (or one the
+ *     caller registers) instead of an OpenGL PBO;
+ *   - the OptiX acceleration-structure build becomes bte_build_accel().
+ * All scene uploads copy synchronously from caller memory, as the reference
+ * does.  The API is single-threaded; calls return after the GPU work they
+ * issue has completed.
+ */
+#ifndef BTE_API_H
+#define BTE_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- POD types: byte-identical to include/render_kernel.h:9-94 ---------- */
+typedef struct { float x, y; } BteFloat2;
+typedef struct { float x, y, z; } BteFloat3;
+typedef struct { float x, y, z, w; } BteFloat4;
+
+typedef struct {                       /* TriangleData, render_kernel.h:22-26 (96 B) */
+    BteFloat3 v0, v1, v2;
+    BteFloat3 n0, n1, n2;
+    BteFloat2 uv0, uv1, uv2;
+} BteTriangle;
+
+typedef struct {                       /* CameraData, render_kernel.h:33-40 (56 B) */
+    BteFloat3 origin, forward, right, up;
+    float fovYRadians, aspect;
+} BteCamera;
+
+typedef struct {                       /* EmitterData, render_kernel.h:44-51 (32 B) */
+    int triIndexOffset, triCount, cdfOffset;
+    BteFloat3 radiance;
+    float areaSum, powerWeight;
+} BteEmitter;
+
+typedef struct {                       /* SpotlightData, render_kernel.h:56-63 (48 B) */
+    BteFloat3 position, direction, radiance;
+    float intensity, innerConeCosine, outerConeCosine;
+} BteSpotlight;
+
+typedef struct {                       /* BsdfData, render_kernel.h:88-94 (68 B) */
+    int type;                          /* BteBsdfType */
+    BteFloat4 p0, p1, p2, p3;
+} BteBsdf;
+
+enum BteBsdfType {                     /* BsdfType, render_kernel.h:79-85 */
+    BTE_BSDF_DIFFUSE = 1,
+    BTE_BSDF_DIELECTRIC = 2,
+    BTE_BSDF_MIRROR = 3,
+    BTE_BSDF_MICROFACET = 4,
+    BTE_BSDF_DISNEY = 5
+};
+
+enum BteRenderMode {                   /* RenderMode, render_kernel.h:199-202 */
+    BTE_MODE_MEGAKERNEL = 0,
+    BTE_MODE_WAVEFRONT = 1
+};
+
+enum BteStatus {
+    BTE_OK = 0,
+    BTE_ERR_INVALID = -1,              /* bad argument / call order */
+    BTE_ERR_DEVICE = -2,               /* HIP runtime failure */
+    BTE_ERR_MEMORY = -3,               /* device allocation failed */
+    BTE_ERR_STATE = -4,                /* required scene data missing */
+    BTE_ERR_UNSUPPORTED = -5           /* feature not available in this build */
+};
+
+/* Renderer configuration (the reference's compile-time constants,
+ * rt_constants.cuh:11-19, made runtime). */
+typedef struct {
+    int maxDepth;            /* bounce cap, reference MAX_BOUNCES = 200 */
+    int megakernelSpp;       /* paths per pixel per frame in megakernel mode (reference 5) */
+    int framesInFlight;      /* wavefront: frames traced concurrently (0 = auto) */
+    int memoryBudgetMB;      /* cap for wavefront path state (0 = default 32768) */
+} BteConfig;
+
+/* Counters of the last bte_render / bte_render_frames call. */
+typedef struct {
+    double renderMs;         /* wall time of the GPU work, HIP events */
+    long long extensionRays; /* closest-hit rays traced */
+    long long hits;          /* extension rays that hit a triangle */
+    long long shadowRays;    /* any-hit rays traced */
+    long long pathsTraced;   /* primary paths (pixel samples) */
+    int bounces;             /* bounce iterations executed */
+    int framesInFlight;      /* frames per wavefront pass */
+    int bvhNodes;            /* nodes of the current BVH2 */
+    int bvhDepth;            /* depth of the current BVH2 */
+    long long raysPerBounce[64]; /* extension rays per bounce iteration (first 64) */
+    /* filled when bte_set_instrumentation enables them: */
+    double extendMs, shadeMs, connectMs;   /* summed kernel time (HIP events) */
+    int extendLaunches, connectLaunches;
+    long long extendNodes, extendTris;     /* BVH nodes visited / triangles tested */
+    long long connectNodes, connectTris;
+} BteStats;
+
+enum BteInstrument {
+    BTE_INSTR_KERNEL_TIMES = 1,        /* HIP events around every extend/shade/connect launch */
+    BTE_INSTR_TRAVERSAL_COUNTS = 2     /* count BVH nodes and triangle tests (slower kernels) */
+};
+
+/* ---- scene upload: render_kernel.h:103-173 ------------------------------ */
+/* ≙ cudaInitScene (render_kernel.h:111).  materials are stored but, as in the
+ * reference (scene_upload.cu:48-54), never read by the integrator. */
+int bte_init_scene(const BteTriangle* triangles, int triangleCount,
+                   const BteFloat3* materials, int materialCount,
+                   const int* triangleMaterialIds, int imageWidth, int imageHeight);
+/* ≙ cudaInit (render_kernel.h:117): single-triangle convenience form. */
+int bte_init(const BteTriangle* tri, BteFloat3 color, int imageWidth, int imageHeight);
+/* ≙ cudaInitCamera (render_kernel.h:121). */
+int bte_init_camera(const BteCamera* camera);
+/* ≙ cudaInitTriangleEmission (render_kernel.h:125); null/0 clears. */
+int bte_init_triangle_emission(const BteFloat3* triangleEmission, int triangleCount);
+/* ≙ cudaInitTriangleEmitterFlags (render_kernel.h:129); null/0 clears. */
+int bte_init_triangle_emitter_flags(const uint8_t* flags, int triangleCount);
+/* ≙ cudaInitEmitterTable (render_kernel.h:137); null/0 clears. */
+int bte_init_emitter_table(const BteEmitter* emitters, int emitterCount,
+                           const int* emitterTriIndices, int emitterTriIndexCount,
+                           const float* emitterTriCdf, int emitterTriCdfCount,
+                           const float* sceneEmitterCdf);
+/* ≙ cudaInitBsdfs (render_kernel.h:143). */
+int bte_init_bsdfs(const BteBsdf* bsdfs, int bsdfCount, const int* triangleBsdfIds, int triangleCount);
+/* ≙ cudaInitSpotlights (render_kernel.h:148); null/0 clears (at most 14). */
+int bte_init_spotlights(const BteSpotlight* spotlights, int spotlightCount);
+/* ≙ cudaInitTextures / cudaInitMrTextures (render_kernel.h:155,163).
+ * Null/0 clears; texture data returns BTE_ERR_UNSUPPORTED in this build. */
+int bte_init_textures(const uint8_t* const* pixels, const int* widths, const int* heights, int textureCount);
+int bte_init_mr_textures(const uint8_t* const* pixels, const int* widths, const int* heights, int textureCount);
+/* ≙ cudaInitEnvMap (render_kernel.h:173).  The marginal/conditional CDFs are
+ * built on the GPU in binary64 and stored as binary32.  null/0 clears. */
+int bte_init_env_map(const float* hdriPixelsRGBA, int width, int height, float intensity, float rotationDeg);
+/* ≙ cudaInitOptix (render_kernel.h:103): builds the BVH2 over the uploaded
+ * triangles.  Rendering builds it implicitly when the triangles changed. */
+int bte_build_accel(void);
+
+/* ---- framebuffer / accumulation / rendering: render_kernel.h:177-211 ---- */
+/* ≙ cudaRegisterPBO (render_kernel.h:177): a caller-owned device buffer of
+ * imageWidth*imageHeight RGBA8 pixels (ABGR packed); null = library-owned. */
+int bte_register_framebuffer(uint32_t* deviceRGBA8);
+/* ≙ cudaResetAccumulation (render_kernel.h:183). */
+int bte_reset_accumulation(int imageWidth, int imageHeight);
+/* ≙ cudaRender (render_kernel.h:187): one frame (wavefront: 1 sample per
+ * pixel; megakernel: megakernelSpp paths per pixel). */
+int bte_render(int imageWidth, int imageHeight);
+/* nframes consecutive frames in as few wavefront passes as memory allows;
+ * identical result to nframes calls of bte_render. */
+int bte_render_frames(int imageWidth, int imageHeight, int nframes);
+/* ≙ cudaToggleDenoiser (render_kernel.h:192): returns the new state (0/1) or
+ * BTE_ERR_UNSUPPORTED while no denoiser is built. */
+int bte_toggle_denoiser(void);
+/* ≙ cudaSetRenderMode / cudaGetRenderMode (render_kernel.h:205,208). */
+int bte_set_render_mode(int mode);
+int bte_get_render_mode(void);
+/* ≙ cudaCleanup (render_kernel.h:211). */
+int bte_cleanup(void);
+
+/* ---- headless extensions ------------------------------------------------ */
+int bte_set_config(const BteConfig* config);
+int bte_get_config(BteConfig* config);
+int bte_get_stats(BteStats* stats);
+int bte_set_instrumentation(int flags);
+const char* bte_last_error(void);
+/* Current frame index (samples accumulated since the last reset). */
+int bte_frame_index(void);
+/* Copies of the display buffer (W*H uint32) and the binary32 running mean
+ * (W*H*3 floats, row-major, row 0 = bottom image row as in the reference). */
+int bte_read_framebuffer(uint32_t* hostRGBA8, size_t count);
+int bte_read_accum(float* hostRGB, size_t count);
+/* Binary64 accumulation of plain sample sums (oracle images). */
+int bte_enable_accum64(int enable);
+int bte_read_accum64(double* hostRGB, size_t count);
+/* Device pointer of the binary32 running mean (float4 per pixel, w unused). */
+int bte_accum_device_ptr(void** devicePtr);
+/* Per-sample radiance of the last wavefront pass, indexed by
+ * (frame - firstFrameOfPass) * W*H + pixel; 3 floats each. */
+int bte_read_sample_radiance(float* hostRGB, size_t count);
+/* Record the hit triangle of the first maxRecord bounces of every path of the
+ * next wavefront pass (-1 miss, -2 not traced); 0 disables. */
+int bte_set_hit_capture(int maxRecord);
+int bte_read_hit_capture(int* hostTris, size_t count);
+/* Env-map CDF arrays as built on the GPU (marginal H+1, conditional H*(W+1)). */
+int bte_read_env_cdf(float* marginal, float* conditional, float* totalPower);
+/* Trace caller rays with the BVH2 kernels: rays = n x (ox,oy,oz,dx,dy,dz).
+ * anyHit = 0: closest hit in [tmin, tmax] -> tri (-1 miss), t, u, v.
+ * anyHit = 1: occlusion in [tmin, tmaxPerRay[i]] -> tri = 1 occluded / 0. */
+int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, float tmin, float tmax,
+                   int anyHit, int* tri, float* t, float* u, float* v);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BTE_API_H */

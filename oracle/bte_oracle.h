@@ -1,0 +1,84 @@
+// bte_oracle.h — ORACLE (test infrastructure only).
+//
+// CPU restatement of the reference path integrator, written from the
+// reference's behaviour (operation order included) so that the product's HIP
+// kernels can be checked against it.  Only tests/, __graft_entry__.smoke() and
+// bench.py's CPU-baseline leg may load this library; the product never does.
+//
+// Reference files restated (paths relative to the reference repository):
+//   include/render_kernel.h:9-94        POD scene types (layouts kept byte-exact)
+//   include/rt_rng.cuh:8-46             hash-seeded xorshift32
+//   include/rt_camera.cuh:15-37         primary ray
+//   include/rt_cuda_math.cuh:49-123     vector ops, shading frame
+//   include/rt_bsdf*.cuh                5 BSDFs + dispatch
+//   include/rt_emitter_sampling.cuh     emitter CDF sampling
+//   include/rt_env_map.cuh              equirect env eval / pdf / sample
+//   include/rt_direct_lighting.cuh      NEE + MIS, emitter / env hits
+//   include/rt_path_integrator.cuh      scatter, Russian roulette, Welford, gamma
+//   include/rt_optix_shading.cuh:213-248 per-bounce order
+//   src/optix_programs.cu:54-141        seeding, closest-hit attributes
+//   src/host/scene_upload.cu:428-459    env-map CDF construction
+//
+// Deliberate restatement choices (the reference's own results depend on
+// closed NVIDIA code and cannot be reproduced):
+//   * normalize uses 1/sqrt (IEEE) instead of rsqrtf;
+//   * transcendental functions follow ora_math.h;
+//   * triangle intersection is the watertight test of Woop, Benthin and Wald
+//     (JCGT 2013) with ties on t broken by the lower triangle index;
+//   * texture filtering is exact binary32 bilinear (wrap u, clamp v);
+//   * M_PI is taken as the binary32 constant (rt_env_map.cuh:28-30).
+#pragma once
+
+#include <cstdint>
+
+extern "C" {
+
+struct OraFloat2 { float x, y; };
+struct OraFloat3 { float x, y, z; };
+struct OraFloat4 { float x, y, z, w; };
+
+struct OraTriangle {            // include/render_kernel.h:22-26 (96 B)
+    OraFloat3 v0, v1, v2;
+    OraFloat3 n0, n1, n2;
+    OraFloat2 uv0, uv1, uv2;
+};
+struct OraCamera {              // include/render_kernel.h:33-40 (56 B)
+    OraFloat3 origin, forward, right, up;
+    float fovYRadians, aspect;
+};
+struct OraEmitter {             // include/render_kernel.h:44-51 (32 B)
+    int triIndexOffset, triCount, cdfOffset;
+    OraFloat3 radiance;
+    float areaSum, powerWeight;
+};
+struct OraSpotlight {           // include/render_kernel.h:56-63 (48 B)
+    OraFloat3 position, direction, radiance;
+    float intensity, innerConeCosine, outerConeCosine;
+};
+struct OraBsdf {                // include/render_kernel.h:88-94 (68 B)
+    int type;
+    OraFloat4 p0, p1, p2, p3;
+};
+
+// Scene description handed to ora_scene_create.  Every pointer is borrowed
+// for the lifetime of the scene object (the oracle copies nothing large).
+struct OraSceneDesc {
+    const OraTriangle* triangles; int triangleCount;
+    const OraBsdf* bsdfs; int bsdfCount;
+    const int* triangleBsdfIds;
+    const OraFloat3* triangleEmission;      // may be null (no emission)
+    const uint8_t* triangleEmitterFlags;    // may be null
+    const OraEmitter* emitters; int emitterCount;
+    const int* emitterTriIndices; int emitterTriIndexCount;
+    const float* emitterTriCdf; int emitterTriCdfCount;
+    const float* sceneEmitterCdf;            // emitterCount+1 entries
+    const OraSpotlight* spotlights; int spotlightCount;
+    const float* envRGBA; int envWidth, envHeight;   // null = no env map
+    float envIntensity; float envRotationDeg;
+    const float* envMarginalCdf;             // height+1   (required with env)
+    const float* envConditionalCdf;          // height*(width+1)
+    float envTotalPower;
+    OraCamera camera;
+};
+
+}  // extern "C"

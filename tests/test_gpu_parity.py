@@ -1,0 +1,175 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bar (north_star): hit triangle IDs identical; per-sample radiance within 1e-4
+relative in binary32 (the kernels and the oracle perform the same binary32
+operation sequence, so the observed difference is expected to be zero and the
+bit-exact fraction is asserted as well).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from bluetruthengine_amd import renderer, scenes
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4
+ABS_FLOOR = 1e-6
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _compare(gpu, cpu, what):
+    gpu = np.asarray(gpu, np.float32).reshape(-1, 3)
+    cpu = np.asarray(cpu, np.float32).reshape(-1, 3)
+    both_nan = np.isnan(gpu) & np.isnan(cpu)
+    diff = np.where(both_nan, 0.0, np.abs(gpu.astype(np.float64) - cpu.astype(np.float64)))
+    rel = diff / np.maximum(np.abs(cpu.astype(np.float64)), ABS_FLOOR)
+    exact = np.all((gpu == cpu) | both_nan, axis=1)
+    _say(f"{what}: {gpu.shape[0]} samples, bit-exact {exact.mean():.6f}, max rel err {rel.max():.3g}")
+    bad = np.where(rel.max(axis=1) > REL_TOL)[0]
+    assert bad.size == 0, f"{what}: {bad.size} samples exceed {REL_TOL} (first {bad[:5]}: gpu {gpu[bad[:3]]} cpu {cpu[bad[:3]]})"
+    return exact.mean()
+
+
+def _random_rays(n, seed, lo=(-0.95, 0.05, -0.95), hi=(0.95, 1.95, 0.95)):
+    rs = np.random.default_rng(seed)
+    o = rs.uniform(lo, hi, (n, 3)).astype(np.float32)
+    d = rs.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([o, d.astype(np.float32)], 1)
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return scenes.cornell_c1(width=96, height=96, spp=3, max_depth=8)
+
+
+def test_closest_hit_matches_oracle(c1):
+    _say("case: closest-hit rays, Cornell C1 geometry, 200k rays seed 11")
+    renderer.Renderer(c1)
+    rays = _random_rays(200_000, 11)
+    tri, t, u, v = renderer.trace_rays(rays)
+    ref = oracle.OracleScene(c1)
+    rtri, rt, ru, rv = ref.trace_closest(rays)
+    btri, bt, _, _ = ref.trace_closest(rays[:20000], brute=True)
+    assert np.array_equal(rtri[:20000], btri) and np.array_equal(rt[:20000], bt)
+    mism = np.where(tri != rtri)[0]
+    _say(f"hit-id mismatches {mism.size} / {rays.shape[0]}; misses {(tri < 0).mean():.3f}")
+    assert mism.size == 0, f"first mismatches {mism[:10]}: gpu {tri[mism[:5]]} cpu {rtri[mism[:5]]}"
+    hit = tri >= 0
+    assert np.array_equal(t[hit], rt[hit]) and np.array_equal(u[hit], ru[hit]) and np.array_equal(v[hit], rv[hit])
+
+
+def test_any_hit_matches_oracle(c1):
+    _say("case: any-hit rays, Cornell C1 geometry, 100k rays seed 12")
+    renderer.Renderer(c1)
+    rays = _random_rays(100_000, 12)
+    tmax = np.random.default_rng(13).uniform(0.05, 3.0, rays.shape[0]).astype(np.float32)
+    occ = renderer.trace_rays(rays, any_hit=True, tmax_per_ray=tmax)
+    ref = oracle.OracleScene(c1).trace_occluded(rays, tmax)
+    assert np.array_equal(occ.astype(np.uint8), ref)
+
+
+def _sample_parity(sd, frames, max_rec=4, env_from_gpu=True):
+    r = renderer.Renderer(sd)
+    r.set_hit_capture(max_rec)
+    r.render(frames)
+    gpu = r.sample_radiance(frames)
+    hits = r.hit_capture(frames, max_rec)
+    r.set_hit_capture(0)
+    env_cdf = r.env_cdf() if (env_from_gpu and sd.env_rgba is not None) else None
+    ref = oracle.OracleScene(sd, env_cdf=env_cdf)
+    W, H = sd.width, sd.height
+    pixels = np.arange(W * H, dtype=np.int32)
+    exact = []
+    for f in range(frames):
+        cpu, tris = ref.trace_samples(pixels, W, H, f, sd.max_depth, max_rec=max_rec)
+        mism = np.where(np.any(hits[f] != tris, axis=1))[0]
+        _say(f"{sd.name} frame {f}: hit-id mismatching paths {mism.size} / {pixels.size}")
+        assert mism.size == 0, f"paths {mism[:5]}: gpu {hits[f][mism[:3]]} cpu {tris[mism[:3]]}"
+        exact.append(_compare(gpu[f], cpu, f"{sd.name} frame {f}"))
+    return r, ref, exact
+
+
+def test_cornell_samples_match_oracle(c1):
+    _say("case: Cornell C1 96x96, 3 frames, depth 8")
+    _, _, exact = _sample_parity(c1, 3)
+    assert min(exact) == 1.0
+
+
+def test_envlit_samples_match_oracle():
+    _say("case: reduced env-lit scene 128x72, 2 frames, depth 12")
+    sd = scenes.small_envlit()
+    _, _, exact = _sample_parity(sd, 2)
+    assert min(exact) == 1.0
+
+
+def test_terrain_disney_samples_match_oracle():
+    _say("case: terrain freq 20 (8000 tris) + 64 Disney slots + 4 lights, 96x54, 2 frames, depth 10")
+    sd = scenes.terrain_c2(width=96, height=54, spp=2, max_depth=10, freq=20)
+    _, _, exact = _sample_parity(sd, 2)
+    assert min(exact) == 1.0
+
+
+def test_env_cdf_binary64_matches_oracle():
+    _say("case: env CDF 2048x1024 sky, GPU binary64 scan vs oracle sequential binary64")
+    sd = scenes.small_envlit(env_size=(2048, 1024))
+    r = renderer.Renderer(sd)
+    marg, cond, total = r.env_cdf()
+    om, oc, ot = oracle.env_build_cdf(sd.env_rgba, 1)
+    for a, b, name in ((marg, om, "marginal"), (cond, oc, "conditional")):
+        ulps = np.abs(a.view(np.int32).astype(np.int64) - b.view(np.int32).astype(np.int64))
+        _say(f"{name}: max ulp diff {ulps.max()}, differing entries {(ulps > 0).sum()} / {a.size}")
+        assert ulps.max() <= 1
+    assert abs(float(total) - float(ot)) <= 1e-6 * float(ot)
+
+
+def test_accumulation_and_display_match_oracle():
+    _say("case: Cornell 64x64, 4 frames: Welford mean and gamma-packed display")
+    sd = scenes.cornell_c1(width=64, height=64, spp=4, max_depth=8)
+    r = renderer.Renderer(sd)
+    r.render(4)
+    acc = r.accum()
+    fb = r.framebuffer()
+    ref = oracle.OracleScene(sd)
+    racc, rfb = ref.render(64, 64, 0, 4, sd.max_depth, rgba8=True)
+    _compare(acc, racc, "accum")
+    assert np.array_equal(acc, racc)
+    assert np.array_equal(fb, rfb)
+    assert r.frame_index() == 4
+
+
+def test_frames_in_flight_do_not_change_result():
+    _say("case: Cornell 48x48, 4 frames with 1, 3 and 4 frames in flight")
+    sd = scenes.cornell_c1(width=48, height=48, spp=4, max_depth=8)
+    outs = []
+    for fif in (1, 3, 4):
+        r = renderer.Renderer(sd, frames_in_flight=fif)
+        r.render(4)
+        outs.append(r.accum())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+
+
+def test_megakernel_one_spp_equals_wavefront():
+    _say("case: Cornell 48x48 megakernel (1 path/frame) vs wavefront, 2 frames")
+    sd = scenes.cornell_c1(width=48, height=48, spp=2, max_depth=8)
+    w = renderer.Renderer(sd)
+    w.render(2)
+    a = w.accum()
+    m = renderer.Renderer(sd, mode=renderer.BTE_MODE_MEGAKERNEL, megakernel_spp=1)
+    m.render(2)
+    b = m.accum()
+    _compare(b, a, "megakernel vs wavefront")
+    assert np.array_equal(a, b)
+
+
+def test_error_paths_are_reported():
+    _say("case: C-ABI error reporting")
+    renderer.cudaCleanup()
+    with pytest.raises(renderer.BteError):
+        renderer.cudaRender(16, 16)
+    with pytest.raises(renderer.BteError):
+        renderer.cudaToggleDenoiser()
