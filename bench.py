@@ -45,11 +45,11 @@ def cpu_baseline(sd, seconds):
     threads = oracle.lib().ora_num_threads()
     W, H = sd.width, sd.height
     rs = np.random.default_rng(1234)
-    n = 4096
+    n = 65536
     t0 = time.perf_counter()
     ref.trace_samples(rs.integers(0, W * H, n).astype(np.int32), W, H, 0, sd.max_depth)
     dt = max(time.perf_counter() - t0, 1e-3)
-    n2 = int(min(2_000_000, max(n, n * (seconds - dt) / dt)))
+    n2 = int(min(60_000_000, max(n, n * seconds / dt)))
     pix = rs.integers(0, W * H, n2).astype(np.int32)
     t0 = time.perf_counter()
     ref.trace_samples(pix, W, H, 0, sd.max_depth)
@@ -68,7 +68,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-instrumented", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 times) to calibrate rocprof byte counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * 1024 * 1024 // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "repeats": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the path shards by sample index across GPUs with one final reduce, which is not run here",

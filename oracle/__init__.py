@@ -73,6 +73,7 @@ def lib():
         L.ora_sample_emitter.argtypes = [_p, _p, _f, _f, _p]
         L.ora_render.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p, _p, _i]
         L.ora_trace_samples.argtypes = [_p, _p, _l, _i, _i, _i, _i, _p, _p, _i]
+        L.ora_render_megakernel.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p]
         L.ora_pack_gamma.restype = _u32
         L.ora_pack_gamma.argtypes = [_f, _f, _f]
         L.ora_num_threads.restype = _i
@@ -270,6 +271,12 @@ class OracleScene:
         lib().ora_render(self.handle, W, H, frame0, nframes, max_depth, _ptr(accum), _ptr(sum64),
                          _ptr(out8), threads)
         return (accum, out8) if rgba8 else accum
+
+    def render_megakernel(self, W, H, frame0, nframes, spp, max_depth, accum=None):
+        if accum is None:
+            accum = np.zeros((H, W, 3), np.float32)
+        lib().ora_render_megakernel(self.handle, W, H, frame0, nframes, spp, max_depth, _ptr(accum))
+        return accum
 
     def trace_samples(self, pixels, W, H, frame, max_depth, max_rec=0):
         pixels = np.ascontiguousarray(pixels, np.int32)

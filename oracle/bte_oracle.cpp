@@ -1248,10 +1248,9 @@ static inline F3 primaryRayDir(const OraCamera& cam, int px, int py, int W, int 
     return normalize3(dir);
 }
 
-static F3 traceSample(const Scene& s, int px, int py, int W, int H, uint32_t frameIndex, int maxDepth,
-                      SampleTrace* dbg) {
-    const uint32_t pixel = (uint32_t)py * (uint32_t)W + (uint32_t)px;
-    Rng rng = makeRng(hashUint(pixel ^ (frameIndex * 0x9e3779b9u)));
+// One path from pixel (px, py) continuing the caller's RNG stream
+// (src/optix_programs.cu:69-98 loop body).
+static F3 tracePath(const Scene& s, int px, int py, int W, int H, Rng& rng, int maxDepth, SampleTrace* dbg) {
     const float jx = rngNextFloat01(rng) - 0.5f;
     const float jy = rngNextFloat01(rng) - 0.5f;
     PathState ps;
@@ -1292,6 +1291,14 @@ static F3 traceSample(const Scene& s, int px, int py, int W, int H, uint32_t fra
         ps.bounce++;
     }
     return ps.accum;
+}
+
+// Wavefront semantics: one path per (pixel, frameIndex) with its own seed.
+static F3 traceSample(const Scene& s, int px, int py, int W, int H, uint32_t frameIndex, int maxDepth,
+                      SampleTrace* dbg) {
+    const uint32_t pixel = (uint32_t)py * (uint32_t)W + (uint32_t)px;
+    Rng rng = makeRng(hashUint(pixel ^ (frameIndex * 0x9e3779b9u)));
+    return tracePath(s, px, py, W, H, rng, maxDepth, dbg);
 }
 
 static inline uint32_t packPixelGamma(const F3& lin) {               // rt_path_integrator.cuh:93-105
@@ -1545,6 +1552,29 @@ void ora_trace_samples(void* p, const int* pixels, long n, int W, int H, int fra
         SampleTrace tr{maxRec, tris ? tris + i * maxRec : nullptr};
         const F3 r = traceSample(s, px, py, W, H, (uint32_t)frame, maxDepth, tris ? &tr : nullptr);
         rad3[3 * i] = r.x; rad3[3 * i + 1] = r.y; rad3[3 * i + 2] = r.z;
+    }
+}
+
+// Megakernel frames (src/optix_programs.cu:54-108): spp paths per pixel and
+// frame sharing one RNG stream seeded from (pixel, frameIndex), averaged with
+// div3, then the Welford update of frame f uses count f+1.
+void ora_render_megakernel(void* p, int W, int H, int frame0, int nframes, int spp, int maxDepth, float* accum) {
+    const Scene& s = *static_cast<Scene*>(p);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int py = 0; py < H; ++py) {
+        for (int px = 0; px < W; ++px) {
+            const size_t pix = (size_t)py * W + px;
+            F3 mean = {accum[3 * pix], accum[3 * pix + 1], accum[3 * pix + 2]};
+            for (int f = frame0; f < frame0 + nframes; ++f) {
+                Rng rng = makeRng(hashUint((uint32_t)pix ^ ((uint32_t)f * 0x9e3779b9u)));
+                F3 fin = {0.0f, 0.0f, 0.0f};
+                for (int i = 0; i < spp; ++i) fin = add3(fin, tracePath(s, px, py, W, H, rng, maxDepth, nullptr));
+                fin = div3(fin, (float)spp);
+                const float nn = (float)((uint32_t)f + 1u);
+                mean = {mean.x + (fin.x - mean.x) / nn, mean.y + (fin.y - mean.y) / nn, mean.z + (fin.z - mean.z) / nn};
+            }
+            accum[3 * pix] = mean.x; accum[3 * pix + 1] = mean.y; accum[3 * pix + 2] = mean.z;
+        }
     }
 }
 

@@ -166,6 +166,16 @@ def test_megakernel_one_spp_equals_wavefront():
     assert np.array_equal(a, b)
 
 
+def test_megakernel_five_paths_per_frame_matches_oracle():
+    _say("case: Cornell 48x48 megakernel mode, 5 paths/pixel/frame sharing one RNG stream, 2 frames")
+    sd = scenes.cornell_c1(width=48, height=48, spp=2, max_depth=8)
+    m = renderer.Renderer(sd, mode=renderer.BTE_MODE_MEGAKERNEL, megakernel_spp=5)
+    m.render(2)
+    ref = oracle.OracleScene(sd).render_megakernel(48, 48, 0, 2, 5, sd.max_depth)
+    _compare(m.accum(), ref, "megakernel spp5")
+    assert np.array_equal(m.accum(), ref)
+
+
 def test_error_paths_are_reported():
     _say("case: C-ABI error reporting")
     renderer.cudaCleanup()
