@@ -68,6 +68,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-instrumented", action="store_true")
+    ap.add_argument("--traversal-stack", type=int, default=0, help="LDS stack entries per lane (8/16/32)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) to calibrate rocprof byte counters")
     args = ap.parse_args()
@@ -92,7 +93,7 @@ def main():
     sd = scenes.envlit_c3(spp=args.spp)
     log(f"scene: {sd.triangles.shape[0]} triangles, {sd.bsdfs.shape[0]} bsdfs, generated in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    r = renderer.Renderer(sd)
+    r = renderer.Renderer(sd, traversal_stack=args.traversal_stack)
     r.build_accel()
     log(f"upload + env CDF + BVH build: {time.perf_counter() - t0:.1f} s")
     W, H, spp = sd.width, sd.height, sd.spp
@@ -133,10 +134,12 @@ def main():
         bytes_per_launch = bytes_total / launches
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         traffic = None
-        tp = os.path.join(ROOT, "profiles", "extend_c3_hbm_traffic.json")
+        tp = os.path.join(ROOT, "profiles", "extend_c3_hbm_traffic.txt")
         if os.path.exists(tp):
             with open(tp) as fh:
-                traffic = json.load(fh).get("bytes_per_launch")
+                for line in fh:
+                    if line.startswith("bytes_per_launch:"):
+                        traffic = int(line.split(":")[1])
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                     "kernel": "k_extend (BVH2 closest hit)", "launches": launches,
@@ -158,7 +161,7 @@ def main():
         "config": {"workload": f"env-lit 1920x1080, {spp} spp, depth 12, 500 displaced spheres "
                                f"({sd.triangles.shape[0]} tris), 2048x1024 HDR sky, NEE+MIS",
                    "width": W, "height": H, "spp": spp, "max_depth": sd.max_depth,
-                   "frames_in_flight": st["framesInFlight"]},
+                   "frames_in_flight": st["framesInFlight"], "traversal_stack": args.traversal_stack or 16},
         "mrays_per_s": round(rays * args.steps / elapsed / 1e6, 1) if rays else None,
         "rays_per_step": {"extension": st["extensionRays"], "shadow": st["shadowRays"], "hits": st["hits"]},
         "roofline": roofline, "kernels": kernel, "cpu_baseline": cpu,

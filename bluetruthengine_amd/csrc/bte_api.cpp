@@ -124,7 +124,7 @@ struct Context {
     long long lastPassPaths = 0;
     int lastPassFrames = 0;
 
-    BteConfig cfg{200, 5, 0, 32768};
+    BteConfig cfg{200, 5, 0, 32768, 16};
     BteStats stats{};
     int instr = 0;
     DBuf dTrav;
@@ -304,8 +304,8 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     CHECK_RC(c.dShD.ensure(K * P * sizeof(float4)));
     CHECK_RC(c.dShC.ensure(K * P * sizeof(float4)));
     CHECK_RC(c.dCounters.ensure((size_t)kCtrStride * (size_t)(maxDepth + 1) * sizeof(int)));
-    const LaunchShape ls = query_launch_shape(c.device);
-    CHECK_RC(c.dOvf.ensure(((size_t)persistent_threads(ls) * (kStackSize - 32) + 64) * sizeof(int)));
+    const LaunchShape ls = query_launch_shape(c.device, 8);
+    CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     if (c.hitCaptureRec > 0) CHECK_RC(c.dHitCap.ensure(P * (size_t)c.hitCaptureRec * sizeof(int)));
     c.cap = cap;
     c.capKinds = kinds;
@@ -363,7 +363,7 @@ int renderWavefront(int nframes) {
     wb.kindSpot0 = kindSpot0;
     wb.kindEnv = kindEnv;
     const DevScene sc = makeScene();
-    const LaunchShape ls = query_launch_shape(c.device);
+    const LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16);
     uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
     double* sum64 = c.accum64 ? c.dSum64.as<double>() : nullptr;
     const size_t ctrInts = (size_t)kCtrStride * (size_t)(maxDepth + 1);
@@ -442,8 +442,8 @@ int renderWavefront(int nframes) {
 int renderMegakernel(int nframes) {
     Context& c = ctx();
     const DevScene sc = makeScene();
-    const LaunchShape ls = query_launch_shape(c.device);
-    CHECK_RC(c.dOvf.ensure(((size_t)persistent_threads(ls) * (kStackSize - 32) + 64) * sizeof(int)));
+    const LaunchShape ls = query_launch_shape(c.device, 8);
+    CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
     double* sum64 = c.accum64 ? c.dSum64.as<double>() : nullptr;
     const int spp = std::max(1, c.cfg.megakernelSpp);
@@ -732,6 +732,9 @@ int bte_set_config(const BteConfig* config) {
     if (config->maxDepth <= 0 || config->maxDepth > 4096) return fail(BTE_ERR_INVALID, "maxDepth out of range");
     if (config->megakernelSpp <= 0) return fail(BTE_ERR_INVALID, "megakernelSpp must be positive");
     if (config->framesInFlight < 0 || config->memoryBudgetMB < 0) return fail(BTE_ERR_INVALID, "negative config");
+    if (config->traversalStack != 0 && config->traversalStack != 8 && config->traversalStack != 16 &&
+        config->traversalStack != 32)
+        return fail(BTE_ERR_INVALID, "traversalStack must be 0, 8, 16 or 32");
     ctx().cfg = *config;
     return BTE_OK;
 }
@@ -861,8 +864,8 @@ int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, floa
     if (c.tris.empty()) return fail(BTE_ERR_STATE, "no triangles uploaded");
     CHECK_RC(ensureAccel());
     if (c.accW <= 0) { c.accW = std::max(1, c.sceneW); c.accH = std::max(1, c.sceneH); }
-    const LaunchShape ls = query_launch_shape(c.device);
-    CHECK_RC(c.dOvf.ensure(((size_t)persistent_threads(ls) * (kStackSize - 32) + 64) * sizeof(int)));
+    const LaunchShape ls = query_launch_shape(c.device, 8);
+    CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     DBuf dr, dtm, dtri, dt, du, dv;
     CHECK_RC(dr.upload(rays, (size_t)n * 6 * sizeof(float)));
     if (anyHit && tmaxPerRay) CHECK_RC(dtm.upload(tmaxPerRay, (size_t)n * sizeof(float)));

@@ -591,11 +591,16 @@ BTE_DEV V3 evalDisney(const Bsdf& b, const BRec& r) {
     }
     const float minMetal = (1.0f - b.p1.x);
     const float minSpecT = (1.0f - specT);
-    const V3 fd = mul3(evalDiffuseLobe(b, r), minMetal * minSpecT);
-    const V3 fs = mul3(evalSheenLobe(b, r), minMetal * b.p2.x);
-    const V3 fm = mul3(evalMetalLobe(b, r, ax, ay), (1.0f - (specT * minMetal)));
-    const V3 fc = mul3(evalClearcoatLobe(b, r), 0.25f * b.p3.x);
-    const V3 fg = mul3(evalGlassLobe(b, r, ax, ay), minMetal * specT);
+    // Every lobe is finite when wi.z > 0, so a lobe whose weight is exactly
+    // zero contributes a signed zero to the sum; it is skipped unevaluated.
+    const float wD = minMetal * minSpecT, wS = minMetal * b.p2.x, wM = (1.0f - (specT * minMetal));
+    const float wC = 0.25f * b.p3.x, wG = minMetal * specT;
+    const V3 z = {0.0f, 0.0f, 0.0f};
+    const V3 fd = (wD != 0.0f) ? mul3(evalDiffuseLobe(b, r), wD) : z;
+    const V3 fs = (wS != 0.0f) ? mul3(evalSheenLobe(b, r), wS) : z;
+    const V3 fm = (wM != 0.0f) ? mul3(evalMetalLobe(b, r, ax, ay), wM) : z;
+    const V3 fc = (wC != 0.0f) ? mul3(evalClearcoatLobe(b, r), wC) : z;
+    const V3 fg = (wG != 0.0f) ? mul3(evalGlassLobe(b, r, ax, ay), wG) : z;
     return add3(add3(add3(add3(fd, fm), fg), fc), fs);
 }
 BTE_DEV float pdfDisney(const Bsdf& b, const BRec& r) {
@@ -613,10 +618,11 @@ BTE_DEV float pdfDisney(const Bsdf& b, const BRec& r) {
         if (specT <= 0.0f) return 0.0f;
         return pdfGlassLobe(b, r, ax, ay);
     }
-    const float pD = pdfDiffuseLobe(r);
-    const float pM = pdfMetalLobe(r, ax, ay);
-    const float pG = pdfGlassLobe(b, r, ax, ay);
-    const float pC = pdfClearcoatLobe(b, r);
+    // zero-weight lobes skipped as in evalDisney (their pdfs are finite)
+    const float pD = (dW != 0.0f) ? pdfDiffuseLobe(r) : 0.0f;
+    const float pM = (mW != 0.0f) ? pdfMetalLobe(r, ax, ay) : 0.0f;
+    const float pG = (gW != 0.0f) ? pdfGlassLobe(b, r, ax, ay) : 0.0f;
+    const float pC = (cW != 0.0f) ? pdfClearcoatLobe(b, r) : 0.0f;
     return (dW * pD + gW * pG + mW * pM + cW * pC) / wSum;
 }
 BTE_DEV V3 sampleDisney(const Bsdf& b, BRec& r, float u1, float u2) {

@@ -69,7 +69,8 @@ struct WaveBuffers {
 };
 
 struct LaunchShape {
-    int extendBlocks, shadeBlocks, connectBlocks;
+    int extendBlocks, shadeBlocks, connectBlocks, auxBlocks;
+    int ldsStack;         // traversal LDS stack entries per lane: 8, 16 or 32
 };
 
 // Launchers (all asynchronous on `stream`).
@@ -89,7 +90,8 @@ void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxP
                        hipStream_t s);
 void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
                     hipStream_t s);
-LaunchShape query_launch_shape(int device);
+LaunchShape query_launch_shape(int device, int ldsStack);
 int persistent_threads(const LaunchShape& ls);
+int overflow_stack_ints(const LaunchShape& ls);   // size of the overflow-stack buffer (ints)
 
 }  // namespace bte

@@ -19,9 +19,14 @@
 namespace bte {
 
 constexpr int kBlock = 256;
-constexpr int kLdsStack = 32;                 // stack entries kept in LDS per lane
-constexpr int kOvfStack = kStackSize - kLdsStack;
+constexpr int kLdsStack = 32;                 // LDS stack entries per lane (megakernel, trace_rays)
+constexpr int kMinLdsStack = 8;               // smallest LDS stack of any traversal kernel variant
+constexpr int kOvfStack = kStackSize - kMinLdsStack;   // global overflow entries per thread
 constexpr uint32_t kSpecBit = 0x80000000u;
+constexpr int kTraversalDone = 0x7fffffff;   // sentinel: no node left to visit
+#ifndef BTE_TRAVERSAL_WHILE_WHILE
+#define BTE_TRAVERSAL_WHILE_WHILE 0
+#endif
 constexpr uint32_t kFinalBit = 0x80000000u;
 
 // ---------------------------------------------------------------------------
@@ -139,24 +144,29 @@ BTE_DEV float boxEnter(const WtRay& r, float lx, float hx, float ly, float hy, f
     return t0;
 }
 
+// Traversal stack: the first L entries of each lane live in LDS (stride kBlock,
+// conflict free), deeper entries in the lane's global overflow slice.
+template <int L>
 struct StackRef {
     int* lds;      // &lds[threadIdx.x]
     int* ovf;      // per-thread global slice (kOvfStack entries)
 };
-BTE_DEV void push(const StackRef& s, int& sp, int v) {
-    if (sp < kLdsStack) s.lds[sp * kBlock] = v;
-    else s.ovf[sp - kLdsStack] = v;
+template <int L>
+BTE_DEV void push(const StackRef<L>& s, int& sp, int v) {
+    if (sp < L) s.lds[sp * kBlock] = v;
+    else s.ovf[sp - L] = v;
     ++sp;
 }
-BTE_DEV int pop(const StackRef& s, int& sp) {
+template <int L>
+BTE_DEV int pop(const StackRef<L>& s, int& sp) {
     --sp;
-    return (sp < kLdsStack) ? s.lds[sp * kBlock] : s.ovf[sp - kLdsStack];
+    return (sp < L) ? s.lds[sp * kBlock] : s.ovf[sp - L];
 }
 
 // ANY = false: closest hit, returns triangle id (or -1) with t,u,v.
 // ANY = true : returns 1 if any triangle is hit within [tmin, tmax], else 0.
-template <bool ANY, bool COUNT = false>
-BTE_DEV int traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, const StackRef& stk, float& tOut,
+template <bool ANY, bool COUNT = false, int L = kLdsStack>
+BTE_DEV int traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, const StackRef<L>& stk, float& tOut,
                      float& uOut, float& vOut, uint32_t* nodeCount = nullptr, uint32_t* triCount = nullptr) {
     const WtRay r = makeWtRay(o, d);
     if (!r.valid || sc.triCount == 0) return ANY ? 0 : -1;
@@ -164,6 +174,55 @@ BTE_DEV int traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, con
     int bestTri = -1;
     int node = 0;
     int sp = 0;
+#if BTE_TRAVERSAL_WHILE_WHILE
+    for (;;) {
+        // descend through inner nodes until this lane holds a leaf
+        while (node >= 0) {
+            if (COUNT) ++*nodeCount;
+            const float4* np = sc.nodes + 4 * (size_t)node;
+            const float4 n0 = np[0], n1 = np[1], n2 = np[2];
+            const int4 ch = *reinterpret_cast<const int4*>(np + 3);
+            float f0, f1;
+            const float e0 = boxEnter(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, tmin, best, f0);
+            const float e1 = boxEnter(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, tmin, best, f1);
+            const bool h0 = e0 <= f0, h1 = e1 <= f1;
+            if (h0 && h1) {
+                const bool first0 = e0 <= e1;
+                node = first0 ? ch.x : ch.y;
+                push(stk, sp, first0 ? ch.y : ch.x);
+            } else if (h0) {
+                node = ch.x;
+            } else if (h1) {
+                node = ch.y;
+            } else {
+                if (sp == 0) { node = kTraversalDone; break; }
+                node = pop(stk, sp);
+            }
+        }
+        if (node == kTraversalDone) break;
+        {
+            const int code = ~node;
+            const int start = code >> kLeafCountBits;
+            const int cnt = (code & (kMaxLeafTris - 1)) + 1;
+            for (int i = 0; i < cnt; ++i) {
+                const float4* tp = sc.leafTris + 3 * (size_t)(start + i);
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                if (COUNT) ++*triCount;
+                float t, u, v;
+                if (hitTriangle(r, a, b, c, tmin, best, t, u, v)) {
+                    if (ANY) return 1;
+                    const int id = __float_as_int(a.w);
+                    if (t < best || bestTri < 0 || id < bestTri) {
+                        best = t; bestTri = id; tOut = t; uOut = u; vOut = v;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        node = pop(stk, sp);
+    }
+    return ANY ? 0 : bestTri;
+#else
     for (;;) {
         if (node >= 0) {
             if (COUNT) ++*nodeCount;
@@ -204,6 +263,7 @@ BTE_DEV int traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, con
         node = pop(stk, sp);
     }
     return ANY ? 0 : bestTri;
+#endif
 }
 
 // ---------------------------------------------------------------------------
@@ -282,11 +342,9 @@ BTE_DEV int cdfSearch(const float* cdf, int count, float x) {
     }
     return lo;
 }
-BTE_DEV float envPdf(const DevScene& sc, V3 dir) {
+BTE_DEV float envPdfUV(const DevScene& sc, float u, float v) {
     if (sc.envTotal <= 0.0f) return 0.0f;
     const int W = sc.envW, H = sc.envH;
-    float u, v;
-    dirToEquirect(dir, sc.envRot, u, v);
     const float fx = u * (float)W;
     const float fy = v * (float)H;
     const int ix = min(max((int)fminf(fmaxf(fx, 0.0f), (float)W), 0), W - 1);
@@ -359,7 +417,8 @@ BTE_DEV void emitterHit(const DevScene& sc, V3 o, V3 d, bool spec, float brdfPdf
     rad = add3(rad, mul3(mul3(Le, thr), w));
 }
 
-BTE_DEV NeeRec areaNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Bsdf& bsdf, float u0, float u1) {
+BTE_DEV NeeRec areaNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Frame& f, const Bsdf& bsdf, float u0,
+                       float u1) {
     NeeRec rec;
     rec.ok = false;
     if (sc.emitterCount <= 0) return rec;
@@ -389,7 +448,6 @@ BTE_DEV NeeRec areaNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const
     if ((le.x <= 0.0f && le.y <= 0.0f && le.z <= 0.0f) || epdf <= 0.0f || meshPdf <= 0.0f) return rec;
     const float dist = length3(sub3(pos, its.p));
     const float geo = fabsf(dot3(its.n, L)) * fabsf(dot3(nrm, mul3(L, -1.0f))) / (dist * dist);
-    const Frame f = makeFrame(its.n);
     BRec br = makeRec();
     br.wi = toLocal(f, mul3(dIn, -1.0f));
     br.wo = toLocal(f, L);
@@ -412,7 +470,8 @@ BTE_DEV NeeRec areaNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const
     return rec;
 }
 
-BTE_DEV NeeRec spotNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Bsdf& bsdf, int si) {  // :175-230
+BTE_DEV NeeRec spotNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Frame& f, const Bsdf& bsdf,
+                       int si) {  // :175-230
     NeeRec rec;
     rec.ok = false;
     const BteSpotlight sp = sc.spots[si];
@@ -433,7 +492,6 @@ BTE_DEV NeeRec spotNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const
         const float sm = (fo * fo) * (3.0f - (2.0f * fo));
         le = mul3(srad, (sp.intensity * sm) / sq);
     }
-    const Frame f = makeFrame(its.n);
     BRec br = makeRec();
     br.wi = toLocal(f, mul3(dIn, -1.0f));
     br.wo = toLocal(f, L);
@@ -447,7 +505,8 @@ BTE_DEV NeeRec spotNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const
     return rec;
 }
 
-BTE_DEV NeeRec envNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Bsdf& bsdf, float u0, float u1) {
+BTE_DEV NeeRec envNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Frame& f, const Bsdf& bsdf, float u0,
+                      float u1) {
     NeeRec rec;
     rec.ok = false;
     if (sc.envTotal <= 0.0f) return rec;
@@ -487,7 +546,6 @@ BTE_DEV NeeRec envNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const 
     if (Le.x <= 0.0f && Le.y <= 0.0f && Le.z <= 0.0f) return rec;
     const float ca = dot3(its.n, dir);
     if (ca <= 0.0f) return rec;
-    const Frame f = makeFrame(its.n);
     BRec br = makeRec();
     br.wi = toLocal(f, mul3(dIn, -1.0f));
     br.wo = toLocal(f, dir);
@@ -510,10 +568,13 @@ BTE_DEV NeeRec envNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const 
 
 BTE_DEV void envHit(const DevScene& sc, V3 d, bool spec, float brdfPdf, V3 thr, V3& rad) {   // :237-259
     if (!sc.hasEnv) return;
-    const V3 Le = envEval(sc, d);
+    float u, v;
+    dirToEquirect(d, sc.envRot, u, v);       // shared by eval and pdf (same inputs, same result)
+    const V3 tx = envBilinear(sc, u, v);
+    const V3 Le = {tx.x * sc.envIntensity, tx.y * sc.envIntensity, tx.z * sc.envIntensity};
     float w = 1.0f;
     if (!spec) {
-        const float ep = envPdf(sc, d);
+        const float ep = envPdfUV(sc, u, v);
         const float sum = brdfPdf + ep;
         w = (sum > 0.0f) ? brdfPdf / sum : 0.0f;
     }
@@ -522,9 +583,8 @@ BTE_DEV void envHit(const DevScene& sc, V3 d, bool spec, float brdfPdf, V3 thr, 
 
 // scatterPath (rt_path_integrator.cuh:25-47); writes the next ray, throughput,
 // eta and brdfPdf.
-BTE_DEV void scatter(const Its& its, const Bsdf& bsdf, uint32_t& rng, V3& o, V3& d, V3& thr, float& eta,
-                     float& brdfPdf) {
-    const Frame f = makeFrame(its.n);
+BTE_DEV void scatter(const Its& its, const Frame& f, const Bsdf& bsdf, uint32_t& rng, V3& o, V3& d, V3& thr,
+                     float& eta, float& brdfPdf) {
     BRec br = makeRec();
     br.wi = toLocal(f, mul3(d, -1.0f));
     br.rndExtra = rngNext(rng);
@@ -532,7 +592,7 @@ BTE_DEV void scatter(const Its& its, const Bsdf& bsdf, uint32_t& rng, V3& o, V3&
     const float u1 = rngNext(rng);
     const V3 w = bsdfSample(bsdf, br, u0, u1);
     brdfPdf = bsdfPdf(bsdf, br);
-    const V3 nd = normalize3(toWorld(makeFrame(its.n), br.wo));
+    const V3 nd = normalize3(toWorld(f, br.wo));
     o = add3(its.p, mul3(nd, RT_EPS));
     d = nd;
     thr = mul3(thr, w);
@@ -600,14 +660,14 @@ __global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb
     if (i == 0) wb.counters[kCtrRays] = (int)paths;
 }
 
-template <bool COUNT>
+template <bool COUNT, int L>
 __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, int bounce) {
-    __shared__ int lds[kLdsStack * kBlock];
+    __shared__ int lds[L * kBlock];
     const int cur = bounce & 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
     const int count = ctr[kCtrRays];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    const StackRef stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
+    const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     uint32_t nodes = 0, tris = 0;
     for (;;) {
         const int base = wave_grab(ctr + kCtrExtendWork);
@@ -618,7 +678,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
             const float4 o = wb.rayO[cur][qi];
             const float4 d = wb.rayD[cur][qi];
             float t = 0.0f, u = 0.0f, v = 0.0f;
-            tri = traverse<false, COUNT>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, &nodes, &tris);
+            tri = traverse<false, COUNT, L>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, &nodes, &tris);
             wb.hit[qi] = make_float4(__int_as_float(tri), t, u, v);
         }
         const unsigned long long hm = __ballot(qi < count && tri >= 0);
@@ -667,6 +727,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
         its.tri = -1; its.p = {0, 0, 0}; its.n = {0, 0, 0};
         Bsdf bsdf;
         bsdf.type = 0;
+        Frame frame;
+        frame.t = frame.b = frame.n = V3{0, 0, 0};
         const V3 dIn = d, thrIn = thr;
         uint32_t rngNee = rng;
         bool survive = false, diffuseType = false;
@@ -680,7 +742,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
                 rngNext(rng); rngNext(rng);                 // area NEE draws (always)
                 if (sc.hasEnv) { rngNext(rng); rngNext(rng); }   // env NEE draws
             }
-            scatter(its, bsdf, rng, o, d, thr, eta, brdfPdf);
+            frame = makeFrame(its.n);
+            scatter(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
             survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
         }
         // ---- compaction of survivors into the next queue
@@ -706,9 +769,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
                 NeeRec rec;
                 rec.ok = false;
                 if (diffuseType) {
-                    if (k == wb.kindArea) rec = areaNEE(sc, dIn, thrIn, its, bsdf, a0, a1);
-                    else if (k == wb.kindEnv) rec = envNEE(sc, dIn, thrIn, its, bsdf, e0, e1);
-                    else rec = spotNEE(sc, dIn, thrIn, its, bsdf, k - wb.kindSpot0);
+                    if (k == wb.kindArea) rec = areaNEE(sc, dIn, thrIn, its, frame, bsdf, a0, a1);
+                    else if (k == wb.kindEnv) rec = envNEE(sc, dIn, thrIn, its, frame, bsdf, e0, e1);
+                    else rec = spotNEE(sc, dIn, thrIn, its, frame, bsdf, k - wb.kindSpot0);
                 }
                 const int s = wave_append(ctr + kCtrShadow + k, rec.ok);
                 if (rec.ok) {
@@ -722,14 +785,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
     }
 }
 
-template <bool COUNT>
+template <bool COUNT, int L>
 __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb, int bounce, int kind) {
-    __shared__ int lds[kLdsStack * kBlock];
+    __shared__ int lds[L * kBlock];
     const int nxt = (bounce & 1) ^ 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
     const int count = ctr[kCtrShadow + kind];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    const StackRef stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
+    const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     const size_t off = (size_t)kind * (size_t)wb.cap;
     uint32_t nodes = 0, tris = 0;
     for (;;) {
@@ -740,7 +803,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
             const float4 so = wb.shO[off + si];
             const float4 sd = wb.shD[off + si];
             float t, u, v;
-            const int occ = traverse<true, COUNT>(sc, xyz(so), xyz(sd), RT_EPS, so.w, stk, t, u, v, &nodes, &tris);
+            const int occ = traverse<true, COUNT, L>(sc, xyz(so), xyz(sd), RT_EPS, so.w, stk, t, u, v, &nodes, &tris);
             if (!occ) {
                 const float4 c = wb.shC[off + si];
                 const uint32_t dest = __float_as_uint(sd.w);
@@ -789,7 +852,7 @@ __global__ __launch_bounds__(kBlock) void k_megakernel(DevScene sc, int spp, int
                                                         int* work) {
     __shared__ int lds[kLdsStack * kBlock];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    const StackRef stk{lds + threadIdx.x, ovf + (size_t)gtid * kOvfStack};
+    const StackRef<kLdsStack> stk{lds + threadIdx.x, ovf + (size_t)gtid * kOvfStack};
     const int pixels = sc.width * sc.height;
     for (;;) {
         const int base = wave_grab(work);
@@ -814,24 +877,25 @@ __global__ __launch_bounds__(kBlock) void k_megakernel(DevScene sc, int spp, int
                 const Its its = makeIts(sc, o, d, make_float4(__int_as_float(tri), t, u, v));
                 const Bsdf bsdf = loadBsdf(sc, tri);
                 if (sc.triFlags && sc.triFlags[tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
+                const Frame frame = makeFrame(its.n);
                 if (bsdfIsDiffuse(bsdf.type)) {
                     spec = false;
                     const float a0 = rngNext(rng), a1 = rngNext(rng);
-                    NeeRec r = areaNEE(sc, d, thr, its, bsdf, a0, a1);
+                    NeeRec r = areaNEE(sc, d, thr, its, frame, bsdf, a0, a1);
                     if (r.ok && !traverse<true>(sc, its.p, r.d, RT_EPS, r.tMax, stk, t, u, v)) rad = add3(rad, r.c);
                     for (int s = 0; s < sc.spotCount; ++s) {
-                        r = spotNEE(sc, d, thr, its, bsdf, s);
+                        r = spotNEE(sc, d, thr, its, frame, bsdf, s);
                         if (r.ok && !traverse<true>(sc, its.p, r.d, RT_EPS, r.tMax, stk, t, u, v)) rad = add3(rad, r.c);
                     }
                     if (sc.hasEnv) {
                         const float e0 = rngNext(rng), e1 = rngNext(rng);
-                        r = envNEE(sc, d, thr, its, bsdf, e0, e1);
+                        r = envNEE(sc, d, thr, its, frame, bsdf, e0, e1);
                         if (r.ok && !traverse<true>(sc, its.p, r.d, RT_EPS, r.tMax, stk, t, u, v)) rad = add3(rad, r.c);
                     }
                 } else {
                     spec = true;
                 }
-                scatter(its, bsdf, rng, o, d, thr, eta, brdfPdf);
+                scatter(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
                 if (!roulette(bounce, rng, thr, eta)) break;
             }
             fin = add3(fin, rad);
@@ -857,7 +921,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene sc, const float*
                                                         float* tt, float* uu, float* vv, int* ovf, int* work) {
     __shared__ int lds[kLdsStack * kBlock];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    const StackRef stk{lds + threadIdx.x, ovf + (size_t)gtid * kOvfStack};
+    const StackRef<kLdsStack> stk{lds + threadIdx.x, ovf + (size_t)gtid * kOvfStack};
     for (;;) {
         const int base = wave_grab(work);
         if (base >= n) break;
@@ -958,20 +1022,32 @@ void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths,
     hipLaunchKernelGGL(k_generate, dim3(blocksFor(paths, kBlock)), dim3(kBlock), 0, s, sc, wb, paths, pixels, frame0);
 }
 
-void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
-                   hipStream_t s, hipEvent_t* ev) {
+template <int L>
+static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth,
+                            const LaunchShape& ls, hipStream_t s, hipEvent_t* ev) {
     const bool count = wb.travCtr != nullptr;
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (count) hipLaunchKernelGGL(k_extend<true>, dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
-    else hipLaunchKernelGGL(k_extend<false>, dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    if (count) hipLaunchKernelGGL((k_extend<true, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    else hipLaunchKernelGGL((k_extend<false, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
     if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(k_shade, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce, maxDepth);
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int k = 0; k < wb.kinds; ++k) {
-        if (count) hipLaunchKernelGGL(k_connect<true>, dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
-        else hipLaunchKernelGGL(k_connect<false>, dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+        if (count)
+            hipLaunchKernelGGL((k_connect<true, L>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+        else
+            hipLaunchKernelGGL((k_connect<false, L>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
     }
     if (ev) (void)hipEventRecord(ev[3], s);
+}
+
+void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                   hipStream_t s, hipEvent_t* ev) {
+    switch (ls.ldsStack) {
+        case 8: launch_bounce_l<8>(sc, wb, bounce, maxDepth, ls, s, ev); break;
+        case 16: launch_bounce_l<16>(sc, wb, bounce, maxDepth, ls, s, ev); break;
+        default: launch_bounce_l<32>(sc, wb, bounce, maxDepth, ls, s, ev); break;
+    }
 }
 
 void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, int frameBase,
@@ -989,8 +1065,8 @@ void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex
     launch_fill_i32(work, 1, 0, s);
     int dev = 0;
     (void)hipGetDevice(&dev);
-    const LaunchShape ls = query_launch_shape(dev);
-    hipLaunchKernelGGL(k_megakernel, dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, spp, maxDepth, frameIndex, accum,
+    const LaunchShape ls = query_launch_shape(dev, 32);
+    hipLaunchKernelGGL(k_megakernel, dim3(ls.auxBlocks), dim3(kBlock), 0, s, sc, spp, maxDepth, frameIndex, accum,
                        sum64, fb, ovf, work);
 }
 
@@ -1001,8 +1077,8 @@ void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxP
     launch_fill_i32(work, 1, 0, s);
     int dev = 0;
     (void)hipGetDevice(&dev);
-    const LaunchShape ls = query_launch_shape(dev);
-    hipLaunchKernelGGL(k_trace_rays, dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, rays, tmaxPerRay, n, tmin, tmax,
+    const LaunchShape ls = query_launch_shape(dev, 32);
+    hipLaunchKernelGGL(k_trace_rays, dim3(ls.auxBlocks), dim3(kBlock), 0, s, sc, rays, tmaxPerRay, n, tmin, tmax,
                        anyHit, tri, t, u, v, ovf, work);
 }
 
@@ -1012,31 +1088,47 @@ void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, d
     hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(kScanBlock), 0, s, rowSums, H, marg, total);
 }
 
-LaunchShape query_launch_shape(int device) {
+template <int L>
+static int travOccupancy() {
+    int a = 0, b = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false, L>, kBlock, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_connect<false, L>, kBlock, 0);
+    return std::max(1, std::min(a, b));
+}
+
+LaunchShape query_launch_shape(int device, int ldsStack) {
     static int cachedDev = -1;
-    static LaunchShape cached{};
-    if (device == cachedDev) return cached;
-    hipDeviceProp_t prop;
-    (void)hipGetDeviceProperties(&prop, device);
-    const int cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
-    int occE = 0, occS = 0, occC = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occE, k_extend<false>, kBlock, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occS, k_shade, kBlock, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occC, k_connect<false>, kBlock, 0);
-    int occM = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occM, k_megakernel, kBlock, 0);
-    int occT = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&occT, k_trace_rays, kBlock, 0);
-    const int occTrav = std::max(1, std::min(std::min(occE, occC), std::min(occM, occT)));
-    cached.extendBlocks = cus * occTrav;
-    cached.connectBlocks = cus * occTrav;
-    cached.shadeBlocks = cus * std::max(1, occS);
-    cachedDev = device;
-    return cached;
+    static int cus = 1, occS = 1, occMT = 1, occL[3] = {1, 1, 1};
+    if (device != cachedDev) {
+        hipDeviceProp_t prop;
+        (void)hipGetDeviceProperties(&prop, device);
+        cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
+        int o = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_shade, kBlock, 0);
+        occS = std::max(1, o);
+        int m = 0, t = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_megakernel, kBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&t, k_trace_rays, kBlock, 0);
+        occMT = std::max(1, std::min(m, t));
+        occL[0] = travOccupancy<8>();
+        occL[1] = travOccupancy<16>();
+        occL[2] = travOccupancy<32>();
+        cachedDev = device;
+    }
+    LaunchShape ls{};
+    ls.ldsStack = (ldsStack == 8 || ldsStack == 16) ? ldsStack : 32;
+    const int occ = occL[ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
+    ls.extendBlocks = cus * occ;
+    ls.connectBlocks = cus * occ;
+    ls.shadeBlocks = cus * occS;
+    ls.auxBlocks = cus * occMT;
+    return ls;
 }
 
 int persistent_threads(const LaunchShape& ls) {
-    return std::max(ls.extendBlocks, ls.connectBlocks) * kBlock;
+    return std::max(std::max(ls.extendBlocks, ls.connectBlocks), ls.auxBlocks) * kBlock;
 }
+
+int overflow_stack_ints(const LaunchShape& ls) { return persistent_threads(ls) * kOvfStack + 64; }
 
 }  // namespace bte

@@ -40,7 +40,8 @@ class BteFloat3(ctypes.Structure):
 
 
 class BteConfig(ctypes.Structure):
-    _fields_ = [("maxDepth", _i), ("megakernelSpp", _i), ("framesInFlight", _i), ("memoryBudgetMB", _i)]
+    _fields_ = [("maxDepth", _i), ("megakernelSpp", _i), ("framesInFlight", _i), ("memoryBudgetMB", _i),
+                ("traversalStack", _i)]
 
 
 class BteStats(ctypes.Structure):
@@ -217,7 +218,7 @@ class Renderer:
     """Uploads a SceneData through the C ABI and renders into device buffers."""
 
     def __init__(self, scene, max_depth=None, frames_in_flight=0, memory_budget_mb=0, mode=BTE_MODE_WAVEFRONT,
-                 megakernel_spp=5):
+                 megakernel_spp=5, traversal_stack=0):
         self.scene = scene
         lib()
         cudaCleanup()
@@ -232,12 +233,13 @@ class Renderer:
         cudaInitSpotlights(scene.spotlights)
         cudaInitEnvMap(scene.env_rgba, scene.env_intensity, scene.env_rotation_deg)
         self.set_config(max_depth if max_depth is not None else scene.max_depth, frames_in_flight,
-                        memory_budget_mb, megakernel_spp)
+                        memory_budget_mb, megakernel_spp, traversal_stack)
         cudaSetRenderMode(mode)
         cudaResetAccumulation(scene.width, scene.height)
 
-    def set_config(self, max_depth, frames_in_flight=0, memory_budget_mb=0, megakernel_spp=5):
-        cfg = BteConfig(int(max_depth), int(megakernel_spp), int(frames_in_flight), int(memory_budget_mb))
+    def set_config(self, max_depth, frames_in_flight=0, memory_budget_mb=0, megakernel_spp=5, traversal_stack=0):
+        cfg = BteConfig(int(max_depth), int(megakernel_spp), int(frames_in_flight), int(memory_budget_mb),
+                        int(traversal_stack))
         _call("bte_set_config", ctypes.byref(cfg))
 
     def build_accel(self):

@@ -84,6 +84,7 @@ typedef struct {
     int megakernelSpp;       /* paths per pixel per frame in megakernel mode (reference 5) */
     int framesInFlight;      /* wavefront: frames traced concurrently (0 = auto) */
     int memoryBudgetMB;      /* cap for wavefront path state (0 = default 32768) */
+    int traversalStack;      /* BVH stack entries kept in LDS per lane: 8, 16 or 32 (0 = default 16) */
 } BteConfig;
 
 /* Counters of the last bte_render / bte_render_frames call. */

@@ -114,6 +114,37 @@ def test_terrain_disney_samples_match_oracle():
     assert min(exact) == 1.0
 
 
+def _spot_scene():
+    """Cornell room + ceiling light + two spotlights (cone falloff) + a microfacet
+    sphere: exercises spotlight NEE (rt_direct_lighting.cuh:175-230) and three
+    shadow-ray classes in one path (area, spot x2)."""
+    from bluetruthengine_amd import pod
+    sd = scenes.cornell_c1(width=80, height=80, spp=2, max_depth=8)
+    sp = np.zeros(2, pod.SPOTLIGHT)
+    sp[0]["position"] = (-0.5, 1.8, 0.5)
+    sp[0]["direction"] = np.array([0.3, -1.0, -0.3]) / np.linalg.norm([0.3, -1.0, -0.3])
+    sp[0]["radiance"] = (1.0, 0.8, 0.6)
+    sp[0]["intensity"] = 4.0
+    sp[0]["innerConeCosine"] = np.cos(np.radians(15.0))
+    sp[0]["outerConeCosine"] = np.cos(np.radians(35.0))
+    sp[1]["position"] = (0.6, 1.5, 0.8)
+    sp[1]["direction"] = (0.0, -1.0, 0.0)
+    sp[1]["radiance"] = (0.5, 0.7, 1.0)
+    sp[1]["intensity"] = 2.0
+    sp[1]["innerConeCosine"] = np.cos(np.radians(5.0))
+    sp[1]["outerConeCosine"] = np.cos(np.radians(60.0))
+    sd.spotlights = sp
+    sd.bsdfs[0] = pod.bsdf_microfacet((0.5, 0.4, 0.3), alpha=0.2)   # walls: microfacet
+    sd.name = "cornell_spot"
+    return sd
+
+
+def test_spotlight_samples_match_oracle():
+    _say("case: Cornell + 2 spotlights + microfacet walls 80x80, 2 frames, depth 8")
+    _, _, exact = _sample_parity(_spot_scene(), 2)
+    assert min(exact) == 1.0
+
+
 def test_env_cdf_binary64_matches_oracle():
     _say("case: env CDF 2048x1024 sky, GPU binary64 scan vs oracle sequential binary64")
     sd = scenes.small_envlit(env_size=(2048, 1024))

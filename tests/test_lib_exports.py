@@ -30,6 +30,6 @@ def test_library_loads_and_reports_errors_without_device():
     L = renderer.lib()
     assert L.bte_get_render_mode() in (0, 1)
     import ctypes
-    cfg = renderer.BteConfig(0, 5, 0, 0)
+    cfg = renderer.BteConfig(0, 5, 0, 0, 0)
     assert L.bte_set_config(ctypes.byref(cfg)) == -1        # invalid maxDepth
     assert b"maxDepth" in L.bte_last_error()
