@@ -99,7 +99,8 @@ struct Context {
     DBuf dTris, dMaterials, dTriMat, dTriBsdf, dBsdfs, dEmission, dFlags, dEmitterOfTri;
     DBuf dEmitters, dEmTriIdx, dEmTriCdf, dSceneCdf, dSpots;
     DBuf dNodes, dLeafTris;
-    DBuf dEnv, dEnvMarg, dEnvCond, dEnvRowSums, dEnvTotal;
+    DBuf dEnv, dEnvMarg, dEnvCond, dEnvRowSums, dEnvTotal, dEnvMargGuide, dEnvCondGuide, dEnvMono;
+    bool envGuides = false;
     bool haveEmission = false, haveFlags = false;
     bool bvhDirty = true;
     int bvhNodes = 0, bvhDepth = 0;
@@ -117,7 +118,7 @@ struct Context {
     bool denoiser = false;
 
     // wavefront buffers
-    DBuf dRayO[2], dRayD[2], dThr[2], dRad[2], dSt[2], dHit, dFinal, dShO, dShD, dShC, dCounters, dHitCap, dOvf;
+    DBuf dRayO[2], dRayD[2], dThr[2], dRad[2], dSt[2], dHit, dCatQ, dCat, dFinal, dShO, dShD, dShC, dCounters, dHitCap, dOvf;
     long long cap = 0;
     int capKinds = 0;
     int hitCaptureRec = 0;
@@ -216,6 +217,8 @@ DevScene makeScene() {
     s.env = c.dEnv.as<float4>();
     s.envMarg = c.dEnvMarg.as<float>();
     s.envCond = c.dEnvCond.as<float>();
+    s.envMargGuide = c.envGuides ? c.dEnvMargGuide.as<int>() : nullptr;
+    s.envCondGuide = c.envGuides ? c.dEnvCondGuide.as<int>() : nullptr;
     s.cam = c.camera;
     s.emitterCount = c.emitterCount;
     s.spotCount = c.spotCount;
@@ -284,7 +287,8 @@ int kindsInUse(int& kindArea, int& kindSpot0, int& kindEnv) {
 }
 
 size_t bytesPerPath(int kinds) {
-    return 2 * (4 * sizeof(float4) + sizeof(uint2)) + 2 * sizeof(float4) + (size_t)kinds * 3 * sizeof(float4);
+    return 2 * (4 * sizeof(float4) + sizeof(uint2)) + 2 * sizeof(float4) + kNumCats * sizeof(int) + 1 +
+           (size_t)kinds * 3 * sizeof(float4);
 }
 
 int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
@@ -298,6 +302,8 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
         CHECK_RC(c.dSt[i].ensure(P * sizeof(uint2)));
     }
     CHECK_RC(c.dHit.ensure(P * sizeof(float4)));
+    CHECK_RC(c.dCatQ.ensure((size_t)kNumCats * P * sizeof(int)));
+    CHECK_RC(c.dCat.ensure(P));
     CHECK_RC(c.dFinal.ensure(P * sizeof(float4)));
     const size_t K = (size_t)std::max(kinds, 1);
     CHECK_RC(c.dShO.ensure(K * P * sizeof(float4)));
@@ -317,6 +323,8 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
         wb.st[i] = c.dSt[i].as<uint2>();
     }
     wb.hit = c.dHit.as<float4>();
+    wb.catQ = c.dCatQ.as<int>();
+    wb.cat = c.dCat.as<uint8_t>();
     wb.finalRad = c.dFinal.as<float4>();
     wb.shO = c.dShO.as<float4>();
     wb.shD = c.dShD.as<float4>();
@@ -359,6 +367,8 @@ int renderWavefront(int nframes) {
             c.events.push_back(e);
         }
     }
+    wb.catMask = 0;
+    for (const BteBsdf& b : c.bsdfsHost) wb.catMask |= 1 << b.type;
     wb.kindArea = kindArea;
     wb.kindSpot0 = kindSpot0;
     wb.kindEnv = kindEnv;
@@ -642,6 +652,8 @@ int bte_init_env_map(const float* hdriPixelsRGBA, int width, int height, float i
     Context& c = ctx();
     c.hasEnv = false;
     c.dEnv.release(); c.dEnvMarg.release(); c.dEnvCond.release(); c.dEnvRowSums.release(); c.dEnvTotal.release();
+    c.dEnvMargGuide.release(); c.dEnvCondGuide.release(); c.dEnvMono.release();
+    c.envGuides = false;
     c.envW = c.envH = 0;
     c.envTotal = 0.0f;
     if (!hdriPixelsRGBA || width <= 0 || height <= 0) return BTE_OK;
@@ -651,11 +663,18 @@ int bte_init_env_map(const float* hdriPixelsRGBA, int width, int height, float i
     CHECK_RC(c.dEnvCond.ensure((size_t)height * (size_t)(width + 1) * sizeof(float)));
     CHECK_RC(c.dEnvRowSums.ensure((size_t)height * sizeof(double)));
     CHECK_RC(c.dEnvTotal.ensure(sizeof(float)));
+    CHECK_RC(c.dEnvMargGuide.ensure((size_t)(height + 1) * sizeof(int)));
+    CHECK_RC(c.dEnvCondGuide.ensure((size_t)height * (size_t)(width + 1) * sizeof(int)));
+    CHECK_RC(c.dEnvMono.ensure(sizeof(int)));
     launch_env_cdf(c.dEnv.as<float4>(), width, height, c.dEnvMarg.as<float>(), c.dEnvCond.as<float>(),
-                   c.dEnvRowSums.as<double>(), c.dEnvTotal.as<float>(), c.stream);
+                   c.dEnvRowSums.as<double>(), c.dEnvTotal.as<float>(), c.dEnvMargGuide.as<int>(),
+                   c.dEnvCondGuide.as<int>(), c.dEnvMono.as<int>(), c.stream);
     float total = 0.0f;
+    int monotone = 0;
     HIPCHK(hipMemcpyAsync(&total, c.dEnvTotal.p, sizeof(float), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(&monotone, c.dEnvMono.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
+    c.envGuides = monotone != 0;
     HIPCHK(hipGetLastError());
     c.envW = width;
     c.envH = height;
@@ -707,8 +726,9 @@ int bte_cleanup(void) {
     if (c.deviceReady) (void)hipStreamSynchronize(c.stream);
     DBuf* bufs[] = {&c.dTris, &c.dMaterials, &c.dTriMat, &c.dTriBsdf, &c.dBsdfs, &c.dEmission, &c.dFlags,
                     &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes,
-                    &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dAccum,
-                    &c.dFb, &c.dSum64, &c.dHit, &c.dFinal, &c.dShO, &c.dShD, &c.dShC, &c.dCounters, &c.dHitCap,
+                    &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dEnvMargGuide,
+                    &c.dEnvCondGuide, &c.dEnvMono, &c.dAccum,
+                    &c.dFb, &c.dSum64, &c.dHit, &c.dCatQ, &c.dCat, &c.dFinal, &c.dShO, &c.dShD, &c.dShC, &c.dCounters, &c.dHitCap,
                     &c.dOvf};
     for (DBuf* b : bufs) b->release();
     c.dTrav.release();

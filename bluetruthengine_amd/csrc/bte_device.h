@@ -625,14 +625,11 @@ BTE_DEV float pdfDisney(const Bsdf& b, const BRec& r) {
     const float pC = (cW != 0.0f) ? pdfClearcoatLobe(b, r) : 0.0f;
     return (dW * pD + gW * pG + mW * pM + cW * pC) / wSum;
 }
-BTE_DEV V3 sampleDisney(const Bsdf& b, BRec& r, float u1, float u2) {
+// Lobe choice and lobe sampling of bsdfSampleDisney for wi.z > 0
+// (rt_bsdf_disney.cuh:843-880).
+BTE_DEV void sampleDisneyLobeChoice(const Bsdf& b, BRec& r, float u1, float u2, float ax, float ay) {
     const float specT = b.p1.z;
-    float ax, ay;
-    disneyAlpha(b, ax, ay);
-    if (r.wi.z <= 0.0f) {
-        if (specT <= 0.0f) return {0, 0, 0};
-        sampleGlassLobe(b, r, u1, u2, ax, ay);
-    } else {
+    {
         const float minMetal = (1.0f - b.p1.x);
         const float dW = minMetal * (1.0f - specT);
         const float sW = minMetal * specT;
@@ -669,6 +666,17 @@ BTE_DEV V3 sampleDisney(const Bsdf& b, BRec& r, float u1, float u2) {
             r.wo = sub3(mul3(h, 2.0f * dot3(r.wi, h)), r.wi);
             r.eta = 1.0f;
         }
+    }
+}
+BTE_DEV V3 sampleDisney(const Bsdf& b, BRec& r, float u1, float u2) {
+    const float specT = b.p1.z;
+    float ax, ay;
+    disneyAlpha(b, ax, ay);
+    if (r.wi.z <= 0.0f) {
+        if (specT <= 0.0f) return {0, 0, 0};
+        sampleGlassLobe(b, r, u1, u2, ax, ay);
+    } else {
+        sampleDisneyLobeChoice(b, r, u1, u2, ax, ay);
     }
     const float pdf = pdfDisney(b, r);
     if (pdf < 1e-7f) return {0, 0, 0};

@@ -26,6 +26,8 @@ struct DevScene {
     const float4* env;            // envW*envH texels (RGBA)
     const float* envMarg;
     const float* envCond;
+    const int* envMargGuide;      // guide tables of the CDF inversion, null = plain binary search
+    const int* envCondGuide;
     BteCamera cam;
     int emitterCount;
     int spotCount;
@@ -39,11 +41,14 @@ struct DevScene {
 // Counter block layout: kCtrStride ints per bounce iteration.
 constexpr int kCtrRays = 0;       // rays entering this bounce
 constexpr int kCtrExtendWork = 1;
-constexpr int kCtrShadeWork = 2;
 constexpr int kCtrHits = 3;
 constexpr int kCtrShadow = 4;                       // + kind: shadow rays of kind
 constexpr int kCtrConnectWork = 4 + kMaxShadowKinds; // + kind
-constexpr int kCtrStride = 4 + 2 * kMaxShadowKinds;
+constexpr int kNumCats = 6;                          // shading categories: miss + the 5 BsdfTypes
+constexpr int kCatMiss = 0;
+constexpr int kCtrCat = 4 + 2 * kMaxShadowKinds;     // + category: rays queued for that category
+constexpr int kCtrShadeWork = kCtrCat + kNumCats;    // + category
+constexpr int kCtrStride = kCtrShadeWork + kNumCats;
 
 // Wavefront buffers.  Path state lives in queue order and is compacted into
 // the other half of each double buffer by the shade kernel.
@@ -54,6 +59,8 @@ struct WaveBuffers {
     float4* rad[2];       // radiance xyz
     uint2* st[2];         // x = rng state, y = path id | specular << 31
     float4* hit;          // (triangle id bits, t, u, v) per queue slot
+    int* catQ;            // [kNumCats * cap]: queue slots grouped by shading category
+    uint8_t* cat;         // [cap]: shading category of each queue slot
     float4* finalRad;     // per path id
     float4* shO;          // [kinds * cap]: origin xyz, w = tMax
     float4* shD;          // [kinds * cap]: direction xyz, w = destination bits
@@ -66,6 +73,7 @@ struct WaveBuffers {
     int maxRec;
     int kinds;            // shadow kinds in use
     int kindArea, kindSpot0, kindEnv;   // kind index of each light class (-1 absent)
+    int catMask;          // bit t set: the scene has BSDF type t (its shading kernel runs)
 };
 
 struct LaunchShape {
@@ -89,7 +97,7 @@ void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxP
                        float tmax, int anyHit, int* tri, float* t, float* u, float* v, int* ovfStack,
                        hipStream_t s);
 void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
-                    hipStream_t s);
+                    int* margGuide, int* condGuide, int* monotone, hipStream_t s);
 LaunchShape query_launch_shape(int device, int ldsStack);
 int persistent_threads(const LaunchShape& ls);
 int overflow_stack_ints(const LaunchShape& ls);   // size of the overflow-stack buffer (ints)

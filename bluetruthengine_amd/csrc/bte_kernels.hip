@@ -55,11 +55,12 @@ BTE_DEV void wave_add_u64(unsigned long long* ctr, uint32_t v) {
     if (lane_id() == 0 && x) atomicAdd(ctr, x);
 }
 
-BTE_DEV int wave_grab(int* ctr) {
+BTE_DEV int wave_grab(int* ctr, int n = 64) {
     int base = 0;
-    if (lane_id() == 0) base = atomicAdd(ctr, 64);
+    if (lane_id() == 0) base = atomicAdd(ctr, n);
     return __shfl(base, 0, 64);
 }
+constexpr int kGrabRows = 4;   // persistent waves take 4 x 64 queue slots per atomic
 
 // ---------------------------------------------------------------------------
 // BVH2 traversal (replaces optixTrace: rt_optix_shading.cuh:57-95)
@@ -342,6 +343,23 @@ BTE_DEV int cdfSearch(const float* cdf, int count, float x) {
     }
     return lo;
 }
+// Same result as cdfSearch on a non-decreasing CDF (the index i with
+// cdf[i] <= x < cdf[i+1] is unique): a guide table gives a bracket
+// [guide[g], guide[g+1]] for g = floor(u*G), which is checked against x before
+// the binary search runs inside it (full range if the check fails).
+BTE_DEV int cdfSearchGuided(const float* cdf, int count, float x, const int* guide, int G, float u) {
+    if (!guide) return cdfSearch(cdf, count, x);
+    const int g = min(max((int)fminf(fmaxf(u * (float)G, 0.0f), (float)(G - 1)), 0), G - 1);
+    int lo = guide[g];
+    int hi = guide[g + 1] + 1;
+    if (!(cdf[lo] <= x)) lo = 0;
+    if (hi < count && !(cdf[hi] > x)) hi = count;
+    while (lo + 1 < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cdf[mid] <= x) lo = mid; else hi = mid;
+    }
+    return lo;
+}
 BTE_DEV float envPdfUV(const DevScene& sc, float u, float v) {
     if (sc.envTotal <= 0.0f) return 0.0f;
     const int W = sc.envW, H = sc.envH;
@@ -417,6 +435,53 @@ BTE_DEV void emitterHit(const DevScene& sc, V3 o, V3 d, bool spec, float brdfPdf
     rad = add3(rad, mul3(mul3(Le, thr), w));
 }
 
+// BSDF dispatch specialised by a compile-time type (T = 0: runtime dispatch on b.type).
+template <int T>
+BTE_DEV V3 bEval(const Bsdf& b, const BRec& r) {
+    if constexpr (T == BTE_BSDF_DIFFUSE) return evalDiffuse(b, r);
+    else if constexpr (T == BTE_BSDF_MICROFACET) return evalMicrofacet(b, r);
+    else if constexpr (T == BTE_BSDF_DISNEY) return evalDisney(b, r);
+    else if constexpr (T == BTE_BSDF_DIELECTRIC || T == BTE_BSDF_MIRROR) return V3{0, 0, 0};
+    else return bsdfEval(b, r);
+}
+template <int T>
+BTE_DEV float bPdf(const Bsdf& b, const BRec& r) {
+    if constexpr (T == BTE_BSDF_DIFFUSE) return pdfDiffuse(r);
+    else if constexpr (T == BTE_BSDF_MICROFACET) return pdfMicrofacet(b, r);
+    else if constexpr (T == BTE_BSDF_DISNEY) return pdfDisney(b, r);
+    else if constexpr (T == BTE_BSDF_DIELECTRIC || T == BTE_BSDF_MIRROR) return 0.0f;
+    else return bsdfPdf(b, r);
+}
+// Sample and the pdf a separate bsdfPdf call on the sampled record returns
+// (rt_path_integrator.cuh:37-39); where the sampler already evaluated that pdf
+// on the final record the value is reused (same inputs, same result).
+template <int T>
+BTE_DEV V3 bSample(const Bsdf& b, BRec& r, float u1, float u2, float& pdf) {
+    if constexpr (T == BTE_BSDF_DISNEY) {
+        const float specT = b.p1.z;
+        float ax, ay;
+        disneyAlpha(b, ax, ay);
+        if (r.wi.z <= 0.0f) {
+            if (specT <= 0.0f) { pdf = 0.0f; return {0, 0, 0}; }
+            sampleGlassLobe(b, r, u1, u2, ax, ay);
+        } else {
+            sampleDisneyLobeChoice(b, r, u1, u2, ax, ay);
+        }
+        pdf = pdfDisney(b, r);
+        if (pdf < 1e-7f) return {0, 0, 0};
+        return div3(evalDisney(b, r), pdf);
+    } else if constexpr (T == BTE_BSDF_MICROFACET) {
+        const V3 w = sampleMicrofacet(b, r, u1, u2);
+        pdf = pdfMicrofacet(b, r);
+        return w;
+    } else {
+        const V3 w = bsdfSample(b, r, u1, u2);
+        pdf = bPdf<T>(b, r);
+        return w;
+    }
+}
+
+template <int T>
 BTE_DEV NeeRec areaNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Frame& f, const Bsdf& bsdf, float u0,
                        float u1) {
     NeeRec rec;
@@ -431,8 +496,8 @@ BTE_DEV NeeRec areaNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const
         const float* triCdf = sc.emTriCdf + e.cdfOffset;
         float pick = 0.0f;
         const int li = cdfReuse(triCdf, e.triCount, u0, pick);
-        const BteTriangle& T = sc.tris[sc.emTriIdx[e.triIndexOffset + li]];
-        const V3 v0 = {T.v0.x, T.v0.y, T.v0.z}, v1 = {T.v1.x, T.v1.y, T.v1.z}, v2 = {T.v2.x, T.v2.y, T.v2.z};
+        const BteTriangle& tri = sc.tris[sc.emTriIdx[e.triIndexOffset + li]];
+        const V3 v0 = {tri.v0.x, tri.v0.y, tri.v0.z}, v1 = {tri.v1.x, tri.v1.y, tri.v1.z}, v2 = {tri.v2.x, tri.v2.y, tri.v2.z};
         const float root = sqrtf(fmaxf(0.0f, 1.0f - u0));
         const float alpha = 1.0f - root;
         const float beta = u1 * root;
@@ -452,10 +517,10 @@ BTE_DEV NeeRec areaNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const
     br.wi = toLocal(f, mul3(dIn, -1.0f));
     br.wo = toLocal(f, L);
     br.measure = 1;
-    const V3 fr = bsdfEval(bsdf, br);
+    const V3 fr = bEval<T>(bsdf, br);
     const float lpa = epdf * meshPdf;
     const float lps = areaToSolidAngle(lpa, pos, its.p, nrm, L);
-    const float bp = bsdfPdf(bsdf, br);
+    const float bp = bPdf<T>(bsdf, br);
     const float den = bp + lps;
     if (den <= 0.0f) return rec;
     const float lw = lps / den;
@@ -470,6 +535,7 @@ BTE_DEV NeeRec areaNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const
     return rec;
 }
 
+template <int T>
 BTE_DEV NeeRec spotNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Frame& f, const Bsdf& bsdf,
                        int si) {  // :175-230
     NeeRec rec;
@@ -496,7 +562,7 @@ BTE_DEV NeeRec spotNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const
     br.wi = toLocal(f, mul3(dIn, -1.0f));
     br.wo = toLocal(f, L);
     br.measure = 1;
-    const V3 fr = bsdfEval(bsdf, br);
+    const V3 fr = bEval<T>(bsdf, br);
     const float cs = fabsf(dot3(its.n, L));
     rec.d = L;
     rec.tMax = dist - RT_EPS;
@@ -505,6 +571,7 @@ BTE_DEV NeeRec spotNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const
     return rec;
 }
 
+template <int T>
 BTE_DEV NeeRec envNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const Frame& f, const Bsdf& bsdf, float u0,
                       float u1) {
     NeeRec rec;
@@ -514,7 +581,7 @@ BTE_DEV NeeRec envNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const 
     const float ms = sc.envMarg[H];
     if (ms <= 0.0f) return rec;
     const float xr = u0 * ms;
-    const int row = cdfSearch(sc.envMarg, H, xr);
+    const int row = cdfSearchGuided(sc.envMarg, H, xr, sc.envMargGuide, H, u0);
     const float c0r = sc.envMarg[row], c1r = sc.envMarg[row + 1];
     const float dRow = c1r - c0r;
     const float fRow = (dRow > 0.0f) ? (xr - c0r) / dRow : 0.0f;
@@ -522,7 +589,8 @@ BTE_DEV NeeRec envNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const 
     const float rs = rc[W];
     if (rs <= 0.0f) return rec;
     const float xc = u1 * rs;
-    const int col = cdfSearch(rc, W, xc);
+    const int col = cdfSearchGuided(rc, W, xc, sc.envCondGuide ? sc.envCondGuide + (size_t)row * (size_t)(W + 1) : nullptr,
+                                    W, u1);
     const float c0c = rc[col], c1c = rc[col + 1];
     const float dCol = c1c - c0c;
     const float fCol = (dCol > 0.0f) ? (xc - c0c) / dCol : 0.0f;
@@ -550,8 +618,8 @@ BTE_DEV NeeRec envNEE(const DevScene& sc, V3 dIn, V3 thr, const Its& its, const 
     br.wi = toLocal(f, mul3(dIn, -1.0f));
     br.wo = toLocal(f, dir);
     br.measure = 1;
-    const V3 fr = bsdfEval(bsdf, br);
-    const float bp = bsdfPdf(bsdf, br);
+    const V3 fr = bEval<T>(bsdf, br);
+    const float bp = bPdf<T>(bsdf, br);
     const float sum = bp + pdf;
     if (sum <= 0.0f) return rec;
     const float lw = pdf / sum;
@@ -583,6 +651,7 @@ BTE_DEV void envHit(const DevScene& sc, V3 d, bool spec, float brdfPdf, V3 thr, 
 
 // scatterPath (rt_path_integrator.cuh:25-47); writes the next ray, throughput,
 // eta and brdfPdf.
+template <int T>
 BTE_DEV void scatter(const Its& its, const Frame& f, const Bsdf& bsdf, uint32_t& rng, V3& o, V3& d, V3& thr,
                      float& eta, float& brdfPdf) {
     BRec br = makeRec();
@@ -590,8 +659,7 @@ BTE_DEV void scatter(const Its& its, const Frame& f, const Bsdf& bsdf, uint32_t&
     br.rndExtra = rngNext(rng);
     const float u0 = rngNext(rng);
     const float u1 = rngNext(rng);
-    const V3 w = bsdfSample(bsdf, br, u0, u1);
-    brdfPdf = bsdfPdf(bsdf, br);
+    const V3 w = bSample<T>(bsdf, br, u0, u1, brdfPdf);
     const V3 nd = normalize3(toWorld(f, br.wo));
     o = add3(its.p, mul3(nd, RT_EPS));
     d = nd;
@@ -660,6 +728,15 @@ __global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb
     if (i == 0) wb.counters[kCtrRays] = (int)paths;
 }
 
+// Shading category of a finished extension ray: 0 = miss, else the BsdfType.
+BTE_DEV int shadeCategory(const DevScene& sc, int tri) {
+    if (tri < 0) return kCatMiss;
+    return __float_as_int(sc.bsdfs[4 * (size_t)sc.triBsdf[tri] + 3].w);
+}
+
+// Closest hit of every queued ray (src/wavefront/ray_extend.cu:49-72).  Waves
+// take 64 consecutive queue slots per atomic: consecutive slots hold rays that
+// were adjacent before compaction, so a wave's lanes share BVH cache lines.
 template <bool COUNT, int L>
 __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, int bounce) {
     __shared__ int lds[L * kBlock];
@@ -670,19 +747,19 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
     const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     uint32_t nodes = 0, tris = 0;
     for (;;) {
-        const int base = wave_grab(ctr + kCtrExtendWork);
-        if (base >= count) break;
-        const int qi = base + lane_id();
-        int tri = -1;
+        const int chunk = wave_grab(ctr + kCtrExtendWork, 64 * kGrabRows);
+        if (chunk >= count) break;
+        for (int row = 0; row < kGrabRows; ++row) {
+        const int qi = chunk + row * 64 + lane_id();
         if (qi < count) {
             const float4 o = wb.rayO[cur][qi];
             const float4 d = wb.rayD[cur][qi];
             float t = 0.0f, u = 0.0f, v = 0.0f;
-            tri = traverse<false, COUNT, L>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, &nodes, &tris);
+            const int tri = traverse<false, COUNT, L>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, &nodes, &tris);
             wb.hit[qi] = make_float4(__int_as_float(tri), t, u, v);
+            wb.cat[qi] = (uint8_t)shadeCategory(sc, tri);
         }
-        const unsigned long long hm = __ballot(qi < count && tri >= 0);
-        if (lane_id() == 0 && hm) atomicAdd(ctr + kCtrHits, __popcll(hm));
+        }
     }
     if (COUNT) {
         wave_add_u64(wb.travCtr + 0, nodes);
@@ -690,63 +767,119 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
     }
 }
 
+// Groups the queue slots of one bounce by shading category, keeping queue
+// order within each category.  A block handles 1024 consecutive slots per
+// iteration: ballot ranks within each wave, an LDS scan over the block, and one
+// global atomic per non-empty category (device-wide atomics on a hot counter
+// serialise, so they are amortised over the block).
+constexpr int kCatSub = 4;   // 64-slot rows per wave per iteration
+__global__ __launch_bounds__(kBlock) void k_categorize(WaveBuffers wb, int bounce) {
+    __shared__ int sCnt[kCatSub][kBlock / 64][kNumCats];
+    __shared__ int sBase[kNumCats];
+    int* ctr = wb.counters + (size_t)bounce * kCtrStride;
+    const int count = ctr[kCtrRays];
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    const int per = kCatSub * kBlock;
+    for (int base = blockIdx.x * per; base < count; base += gridDim.x * per) {
+        int c[kCatSub];
+        for (int j = 0; j < kCatSub; ++j) {
+            const int i = base + j * kBlock + wave * 64 + lane;
+            c[j] = (i < count) ? (int)wb.cat[i] : -1;
+            for (int k = 0; k < kNumCats; ++k) {
+                const unsigned long long m = __ballot(c[j] == k);
+                if (lane == 0) sCnt[j][wave][k] = __popcll(m);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < kNumCats) {
+            const int k = threadIdx.x;
+            int run = 0;
+            for (int j = 0; j < kCatSub; ++j)
+                for (int w = 0; w < kBlock / 64; ++w) {
+                    const int n = sCnt[j][w][k];
+                    sCnt[j][w][k] = run;
+                    run += n;
+                }
+            sBase[k] = run ? atomicAdd(ctr + kCtrCat + k, run) : 0;
+            if (k != kCatMiss && run) atomicAdd(ctr + kCtrHits, run);
+        }
+        __syncthreads();
+        for (int j = 0; j < kCatSub; ++j) {
+            const int i = base + j * kBlock + wave * 64 + lane;
+            for (int k = 0; k < kNumCats; ++k) {
+                const unsigned long long m = __ballot(c[j] == k);
+                if (c[j] == k) {
+                    const unsigned lo = (unsigned)(m & 0xffffffffull), hi = (unsigned)(m >> 32);
+                    const int rank = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+                    wb.catQ[(size_t)k * (size_t)wb.cap + (size_t)(sBase[k] + sCnt[j][wave][k] + rank)] = i;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Shade one category of hits (T = BsdfType) or the misses (T = kCatMiss).
+template <int T>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, int bounce, int maxDepth) {
     const int cur = bounce & 1, nxt = cur ^ 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
     int* ctrNext = wb.counters + (size_t)(bounce + 1) * kCtrStride;
-    const int count = ctr[kCtrRays];
+    const int count = ctr[kCtrCat + T];
+    const int* list = wb.catQ + (size_t)T * (size_t)wb.cap;
+    constexpr bool kMiss = (T == kCatMiss);
+    constexpr bool kDiffuseType = (T == BTE_BSDF_DIFFUSE || T == BTE_BSDF_MICROFACET || T == BTE_BSDF_DISNEY);
     for (;;) {
-        const int base = wave_grab(ctr + kCtrShadeWork);
-        if (base >= count) break;
-        const int qi = base + lane_id();
-        const bool active = qi < count;
-        // ---- load path state
+        const int chunk = wave_grab(ctr + kCtrShadeWork + T, 64 * kGrabRows);
+        if (chunk >= count) break;
+        for (int row = 0; row < kGrabRows; ++row) {
+        const int li = chunk + row * 64 + lane_id();
+        const bool active = li < count;
         V3 o = {0, 0, 0}, d = {0, 0, 0}, thr = {0, 0, 0}, rad = {0, 0, 0};
         float brdfPdf = 0.0f, eta = 0.0f;
         uint32_t rng = 0, pid = 0;
-        bool spec = false, hitSurface = false;
+        bool spec = false;
         float4 h = make_float4(0, 0, 0, 0);
         if (active) {
+            const int qi = list[li];
             const float4 ro = wb.rayO[cur][qi], rd = wb.rayD[cur][qi], th = wb.thr[cur][qi], ra = wb.rad[cur][qi];
-            const uint2 s = wb.st[cur][qi];
+            const uint2 st = wb.st[cur][qi];
             h = wb.hit[qi];
             o = xyz(ro); brdfPdf = ro.w;
             d = xyz(rd); eta = rd.w;
             thr = xyz(th);
             rad = xyz(ra);
-            rng = s.x;
-            pid = s.y & ~kSpecBit;
-            spec = (s.y & kSpecBit) != 0;
-            hitSurface = __float_as_int(h.x) >= 0;
+            rng = st.x;
+            pid = st.y & ~kSpecBit;
+            spec = (st.y & kSpecBit) != 0;
             if (wb.hitCapture && bounce < wb.maxRec) wb.hitCapture[(size_t)pid * wb.maxRec + bounce] = __float_as_int(h.x);
         }
-        // ---- miss: environment with MIS, path ends
-        if (active && !hitSurface) envHit(sc, d, spec, brdfPdf, thr, rad);
-        // ---- hit: emitter MIS, then scatter + RR (RNG advanced past the NEE draws)
+        bool survive = false;
         Its its;
         its.tri = -1; its.p = {0, 0, 0}; its.n = {0, 0, 0};
         Bsdf bsdf;
-        bsdf.type = 0;
+        bsdf.type = T;
         Frame frame;
         frame.t = frame.b = frame.n = V3{0, 0, 0};
         const V3 dIn = d, thrIn = thr;
-        uint32_t rngNee = rng;
-        bool survive = false, diffuseType = false;
-        if (active && hitSurface) {
-            its = makeIts(sc, o, d, h);
-            bsdf = loadBsdf(sc, its.tri);
-            if (sc.triFlags && sc.triFlags[its.tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
-            diffuseType = bsdfIsDiffuse(bsdf.type);
-            spec = !diffuseType;
-            if (diffuseType) {
-                rngNext(rng); rngNext(rng);                 // area NEE draws (always)
-                if (sc.hasEnv) { rngNext(rng); rngNext(rng); }   // env NEE draws
+        const uint32_t rngNee = rng;
+        if constexpr (kMiss) {
+            if (active) envHit(sc, d, spec, brdfPdf, thr, rad);       // path ends
+        } else {
+            if (active) {
+                its = makeIts(sc, o, d, h);
+                bsdf = loadBsdf(sc, its.tri);
+                if (sc.triFlags && sc.triFlags[its.tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
+                spec = !kDiffuseType;
+                if (kDiffuseType) {
+                    rngNext(rng); rngNext(rng);                       // area NEE draws (always)
+                    if (sc.hasEnv) { rngNext(rng); rngNext(rng); }    // env NEE draws
+                }
+                frame = makeFrame(its.n);
+                scatter<T>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
+                survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
             }
-            frame = makeFrame(its.n);
-            scatter(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
-            survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
         }
-        // ---- compaction of survivors into the next queue
         const int slot = wave_append(ctrNext + kCtrRays, survive);
         if (survive) {
             wb.rayO[nxt][slot] = make_float4(o.x, o.y, o.z, brdfPdf);
@@ -757,34 +890,38 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
         } else if (active) {
             wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
         }
-        // ---- NEE records, one kind at a time in the reference's order
-        const uint32_t dest = survive ? (uint32_t)slot : (pid | kFinalBit);
-        if (wb.kinds > 0) {
+        if constexpr (kDiffuseType) {
+            // NEE records, one light class at a time in the reference's order
+            const uint32_t dest = survive ? (uint32_t)slot : (pid | kFinalBit);
+            uint32_t r2 = rngNee;
             float a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f;
-            if (diffuseType) {
-                a0 = rngNext(rngNee); a1 = rngNext(rngNee);
-                if (sc.hasEnv) { e0 = rngNext(rngNee); e1 = rngNext(rngNee); }
+            if (active) {
+                a0 = rngNext(r2); a1 = rngNext(r2);
+                if (sc.hasEnv) { e0 = rngNext(r2); e1 = rngNext(r2); }
             }
             for (int k = 0; k < wb.kinds; ++k) {
                 NeeRec rec;
                 rec.ok = false;
-                if (diffuseType) {
-                    if (k == wb.kindArea) rec = areaNEE(sc, dIn, thrIn, its, frame, bsdf, a0, a1);
-                    else if (k == wb.kindEnv) rec = envNEE(sc, dIn, thrIn, its, frame, bsdf, e0, e1);
-                    else rec = spotNEE(sc, dIn, thrIn, its, frame, bsdf, k - wb.kindSpot0);
+                if (active) {
+                    if (k == wb.kindArea) rec = areaNEE<T>(sc, dIn, thrIn, its, frame, bsdf, a0, a1);
+                    else if (k == wb.kindEnv) rec = envNEE<T>(sc, dIn, thrIn, its, frame, bsdf, e0, e1);
+                    else rec = spotNEE<T>(sc, dIn, thrIn, its, frame, bsdf, k - wb.kindSpot0);
                 }
-                const int s = wave_append(ctr + kCtrShadow + k, rec.ok);
+                const int sl = wave_append(ctr + kCtrShadow + k, rec.ok);
                 if (rec.ok) {
-                    const size_t idx = (size_t)k * (size_t)wb.cap + (size_t)s;
+                    const size_t idx = (size_t)k * (size_t)wb.cap + (size_t)sl;
                     wb.shO[idx] = make_float4(its.p.x, its.p.y, its.p.z, rec.tMax);
                     wb.shD[idx] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(dest));
                     wb.shC[idx] = make_float4(rec.c.x, rec.c.y, rec.c.z, 0.0f);
                 }
             }
         }
+        }
     }
 }
 
+// Any-hit visibility of one light class's shadow rays; the unoccluded ones add
+// their contribution to the destination path (src/wavefront/ray_connect.cu:52-72).
 template <bool COUNT, int L>
 __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb, int bounce, int kind) {
     __shared__ int lds[L * kBlock];
@@ -796,9 +933,10 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
     const size_t off = (size_t)kind * (size_t)wb.cap;
     uint32_t nodes = 0, tris = 0;
     for (;;) {
-        const int base = wave_grab(ctr + kCtrConnectWork + kind);
-        if (base >= count) break;
-        const int si = base + lane_id();
+        const int chunk = wave_grab(ctr + kCtrConnectWork + kind, 64 * kGrabRows);
+        if (chunk >= count) break;
+        for (int row = 0; row < kGrabRows; ++row) {
+        const int si = chunk + row * 64 + lane_id();
         if (si < count) {
             const float4 so = wb.shO[off + si];
             const float4 sd = wb.shD[off + si];
@@ -808,12 +946,13 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
                 const float4 c = wb.shC[off + si];
                 const uint32_t dest = __float_as_uint(sd.w);
                 float4* r = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.rad[nxt] + dest);
-                float4 cur = *r;
-                cur.x = cur.x + c.x;
-                cur.y = cur.y + c.y;
-                cur.z = cur.z + c.z;
-                *r = cur;
+                float4 v4 = *r;
+                v4.x = v4.x + c.x;
+                v4.y = v4.y + c.y;
+                v4.z = v4.z + c.z;
+                *r = v4;
             }
+        }
         }
     }
     if (COUNT) {
@@ -881,21 +1020,21 @@ __global__ __launch_bounds__(kBlock) void k_megakernel(DevScene sc, int spp, int
                 if (bsdfIsDiffuse(bsdf.type)) {
                     spec = false;
                     const float a0 = rngNext(rng), a1 = rngNext(rng);
-                    NeeRec r = areaNEE(sc, d, thr, its, frame, bsdf, a0, a1);
+                    NeeRec r = areaNEE<0>(sc, d, thr, its, frame, bsdf, a0, a1);
                     if (r.ok && !traverse<true>(sc, its.p, r.d, RT_EPS, r.tMax, stk, t, u, v)) rad = add3(rad, r.c);
                     for (int s = 0; s < sc.spotCount; ++s) {
-                        r = spotNEE(sc, d, thr, its, frame, bsdf, s);
+                        r = spotNEE<0>(sc, d, thr, its, frame, bsdf, s);
                         if (r.ok && !traverse<true>(sc, its.p, r.d, RT_EPS, r.tMax, stk, t, u, v)) rad = add3(rad, r.c);
                     }
                     if (sc.hasEnv) {
                         const float e0 = rngNext(rng), e1 = rngNext(rng);
-                        r = envNEE(sc, d, thr, its, frame, bsdf, e0, e1);
+                        r = envNEE<0>(sc, d, thr, its, frame, bsdf, e0, e1);
                         if (r.ok && !traverse<true>(sc, its.p, r.d, RT_EPS, r.tMax, stk, t, u, v)) rad = add3(rad, r.c);
                     }
                 } else {
                     spec = true;
                 }
-                scatter(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
+                scatter<0>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
                 if (!roulette(bounce, rng, thr, eta)) break;
             }
             fin = add3(fin, rad);
@@ -1007,6 +1146,25 @@ __global__ __launch_bounds__(kScanBlock) void k_env_marginal(const double* rowSu
     if (threadIdx.x == 0) *total = (float)tot;
 }
 
+// Guide tables for cdfSearchGuided: guide[g] = last index i < count with
+// cdf[i] <= sum * g / G, g = 0..G (G = count).  Clears *monotone when a CDF
+// decreases anywhere or holds a NaN (guides are then not used).
+__global__ __launch_bounds__(256) void k_env_guides(const float* marg, const float* cond, int W, int H,
+                                                     int* margGuide, int* condGuide, int* monotone) {
+    const int r = blockIdx.x;
+    const bool isMarg = (r == H);
+    const int count = isMarg ? H : W;
+    const float* cdf = isMarg ? marg : cond + (size_t)r * (size_t)(W + 1);
+    int* guide = isMarg ? margGuide : condGuide + (size_t)r * (size_t)(W + 1);
+    const float sum = cdf[count];
+    for (int g = threadIdx.x; g <= count; g += blockDim.x) {
+        const float thr = sum * ((float)g / (float)count);
+        guide[g] = cdfSearch(cdf, count, thr);
+    }
+    for (int i = threadIdx.x; i < count; i += blockDim.x)
+        if (!(cdf[i + 1] >= cdf[i])) atomicAnd(monotone, 0);   // NaN clears it too
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1029,8 +1187,25 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     if (ev) (void)hipEventRecord(ev[0], s);
     if (count) hipLaunchKernelGGL((k_extend<true, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
     else hipLaunchKernelGGL((k_extend<false, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    hipLaunchKernelGGL(k_categorize, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, wb, bounce);
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(k_shade, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce, maxDepth);
+    // misses are always shaded; a BSDF category only when the scene uses that type
+    hipLaunchKernelGGL(k_shade<kCatMiss>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce, maxDepth);
+    if (wb.catMask & (1 << BTE_BSDF_DIFFUSE))
+        hipLaunchKernelGGL(k_shade<BTE_BSDF_DIFFUSE>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+                           maxDepth);
+    if (wb.catMask & (1 << BTE_BSDF_DIELECTRIC))
+        hipLaunchKernelGGL(k_shade<BTE_BSDF_DIELECTRIC>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+                           maxDepth);
+    if (wb.catMask & (1 << BTE_BSDF_MIRROR))
+        hipLaunchKernelGGL(k_shade<BTE_BSDF_MIRROR>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+                           maxDepth);
+    if (wb.catMask & (1 << BTE_BSDF_MICROFACET))
+        hipLaunchKernelGGL(k_shade<BTE_BSDF_MICROFACET>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+                           maxDepth);
+    if (wb.catMask & (1 << BTE_BSDF_DISNEY))
+        hipLaunchKernelGGL(k_shade<BTE_BSDF_DISNEY>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+                           maxDepth);
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int k = 0; k < wb.kinds; ++k) {
         if (count)
@@ -1083,9 +1258,11 @@ void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxP
 }
 
 void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
-                    hipStream_t s) {
+                    int* margGuide, int* condGuide, int* monotone, hipStream_t s) {
     hipLaunchKernelGGL(k_env_rows, dim3(H), dim3(kScanBlock), 0, s, env, W, H, cond, rowSums);
     hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(kScanBlock), 0, s, rowSums, H, marg, total);
+    launch_fill_i32(monotone, 1, 1, s);
+    hipLaunchKernelGGL(k_env_guides, dim3(H + 1), dim3(256), 0, s, marg, cond, W, H, margGuide, condGuide, monotone);
 }
 
 template <int L>
@@ -1104,7 +1281,7 @@ LaunchShape query_launch_shape(int device, int ldsStack) {
         (void)hipGetDeviceProperties(&prop, device);
         cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 1;
         int o = 0;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_shade, kBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_shade<BTE_BSDF_DISNEY>, kBlock, 0);
         occS = std::max(1, o);
         int m = 0, t = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_megakernel, kBlock, 0);

@@ -17,7 +17,8 @@ import numpy as np
 from . import pod
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libbte.so")
+# BTE_LIB selects another build of the same library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("BTE_LIB") or os.path.join(_HERE, "_lib", "libbte.so")
 
 BTE_MODE_MEGAKERNEL = 0
 BTE_MODE_WAVEFRONT = 1
