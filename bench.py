@@ -69,6 +69,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-instrumented", action="store_true")
     ap.add_argument("--traversal-stack", type=int, default=0, help="LDS stack entries per lane (8/16/32)")
+    ap.add_argument("--work-chunk", type=int, default=0, help="64-slot rows per work grab (1/2/4/8)")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident traversal blocks per CU (0 = max)")
+    ap.add_argument("--no-graphs", action="store_true", help="plain launches instead of captured HIP graphs")
+    ap.add_argument("--no-autotune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) to calibrate rocprof byte counters")
     args = ap.parse_args()
@@ -93,10 +97,16 @@ def main():
     sd = scenes.envlit_c3(spp=args.spp)
     log(f"scene: {sd.triangles.shape[0]} triangles, {sd.bsdfs.shape[0]} bsdfs, generated in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    r = renderer.Renderer(sd, traversal_stack=args.traversal_stack)
+    r = renderer.Renderer(sd, traversal_stack=args.traversal_stack, graphs=0 if args.no_graphs else 1,
+                          work_chunk=args.work_chunk, traversal_blocks_per_cu=args.blocks_per_cu)
     r.build_accel()
     log(f"upload + env CDF + BVH build: {time.perf_counter() - t0:.1f} s")
     W, H, spp = sd.width, sd.height, sd.spp
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        best_ms, cfg = r.autotune(spp)
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): best {best_ms:.2f} ms with {cfg}")
+    cfg = r.config()
 
     def step():
         r.reset()
@@ -161,7 +171,9 @@ def main():
         "config": {"workload": f"env-lit 1920x1080, {spp} spp, depth 12, 500 displaced spheres "
                                f"({sd.triangles.shape[0]} tris), 2048x1024 HDR sky, NEE+MIS",
                    "width": W, "height": H, "spp": spp, "max_depth": sd.max_depth,
-                   "frames_in_flight": st["framesInFlight"], "traversal_stack": args.traversal_stack or 16},
+                   "frames_in_flight": st["framesInFlight"], "traversal_stack": cfg["traversalStack"],
+                   "work_chunk_rows": cfg["workChunk"], "traversal_blocks_per_cu": cfg["traversalBlocksPerCU"],
+                   "hip_graphs": bool(cfg["graphs"]), "autotuned": not args.no_autotune},
         "mrays_per_s": round(rays * args.steps / elapsed / 1e6, 1) if rays else None,
         "rays_per_step": {"extension": st["extensionRays"], "shadow": st["shadowRays"], "hits": st["hits"]},
         "roofline": roofline, "kernels": kernel, "cpu_baseline": cpu,

@@ -119,13 +119,24 @@ struct Context {
 
     // wavefront buffers
     DBuf dRayO[2], dRayD[2], dThr[2], dRad[2], dSt[2], dHit, dCatQ, dCat, dFinal, dShO, dShD, dShC, dCounters, dHitCap, dOvf;
+    DBuf dFrameBase;
+
+    // captured bounce-loop segments, keyed by every value their launches bake in
+    struct GraphEntry {
+        std::vector<unsigned char> key;
+        hipGraphExec_t exec = nullptr;
+        unsigned long long lastUse = 0;
+    };
+    std::vector<GraphEntry> graphs;
+    unsigned long long graphTick = 0;
+    long long graphLaunches = 0;
     long long cap = 0;
     int capKinds = 0;
     int hitCaptureRec = 0;
     long long lastPassPaths = 0;
     int lastPassFrames = 0;
 
-    BteConfig cfg{200, 5, 0, 32768, 16};
+    BteConfig cfg{200, 5, 0, 32768, 16, 1, 4, 0};
     BteStats stats{};
     int instr = 0;
     DBuf dTrav;
@@ -200,7 +211,8 @@ int ensureAccel() {
 
 DevScene makeScene() {
     Context& c = ctx();
-    DevScene s{};
+    DevScene s;
+    std::memset(&s, 0, sizeof s);   // padding too: scenes are compared bytewise as graph keys
     s.nodes = c.dNodes.as<float4>();
     s.leafTris = c.dLeafTris.as<float4>();
     s.tris = c.dTris.as<BteTriangle>();
@@ -310,7 +322,8 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     CHECK_RC(c.dShD.ensure(K * P * sizeof(float4)));
     CHECK_RC(c.dShC.ensure(K * P * sizeof(float4)));
     CHECK_RC(c.dCounters.ensure((size_t)kCtrStride * (size_t)(maxDepth + 1) * sizeof(int)));
-    const LaunchShape ls = query_launch_shape(c.device, 8);
+    CHECK_RC(c.dFrameBase.ensure(sizeof(int)));
+    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4);
     CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     if (c.hitCaptureRec > 0) CHECK_RC(c.dHitCap.ensure(P * (size_t)c.hitCaptureRec * sizeof(int)));
     c.cap = cap;
@@ -330,10 +343,73 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     wb.shD = c.dShD.as<float4>();
     wb.shC = c.dShC.as<float4>();
     wb.counters = c.dCounters.as<int>();
+    wb.frameBase = c.dFrameBase.as<int>();
     wb.hitCapture = c.hitCaptureRec > 0 ? c.dHitCap.as<int>() : nullptr;
     wb.ovfStack = c.dOvf.as<int>() + 64;
     wb.cap = cap;
     wb.maxRec = c.hitCaptureRec;
+    return BTE_OK;
+}
+
+void dropGraphs() {
+    Context& c = ctx();
+    for (auto& g : c.graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    c.graphs.clear();
+}
+
+template <class T>
+void appendKey(std::vector<unsigned char>& key, const T& v) {
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&v);
+    key.insert(key.end(), b, b + sizeof(T));
+}
+
+// Runs `emit` (a sequence of kernel launches on the context stream) either
+// directly or as a captured HIP graph that is cached under `key` and replayed
+// whenever the same key comes back.  Capture records kernel nodes only (fills
+// are kernels, never memset nodes).
+template <class F>
+int runSegment(bool useGraph, const std::vector<unsigned char>& key, F&& emit) {
+    Context& c = ctx();
+    if (!useGraph) {
+        emit();
+        return BTE_OK;
+    }
+    for (auto& g : c.graphs) {
+        if (g.key == key) {
+            g.lastUse = ++c.graphTick;
+            HIPCHK(hipGraphLaunch(g.exec, c.stream));
+            ++c.graphLaunches;
+            return BTE_OK;
+        }
+    }
+    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    emit();
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(c.stream, &graph);
+    if (ce != hipSuccess || !graph) {
+        (void)hipGetLastError();
+        return fail(BTE_ERR_DEVICE, "stream capture of the bounce loop failed: %s", hipGetErrorString(ce));
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(BTE_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ie));
+    }
+    constexpr size_t kMaxGraphs = 32;
+    if (c.graphs.size() >= kMaxGraphs) {
+        auto oldest = std::min_element(c.graphs.begin(), c.graphs.end(),
+                                       [](const Context::GraphEntry& x, const Context::GraphEntry& y) {
+                                           return x.lastUse < y.lastUse;
+                                       });
+        (void)hipGraphExecDestroy(oldest->exec);
+        c.graphs.erase(oldest);
+    }
+    c.graphs.push_back(Context::GraphEntry{key, exec, ++c.graphTick});
+    HIPCHK(hipGraphLaunch(exec, c.stream));
+    ++c.graphLaunches;
     return BTE_OK;
 }
 
@@ -350,7 +426,8 @@ int renderWavefront(int nframes) {
     }
     F = std::max(1, std::min(F, nframes));
     if ((long long)N * F >= (1ll << 31) - 4096) F = std::max(1, (int)(((1ll << 31) - 4096) / N));
-    WaveBuffers wb{};
+    WaveBuffers wb;
+    std::memset(&wb, 0, sizeof wb);   // padding too: compared bytewise as graph keys
     CHECK_RC(ensureWaveBuffers((long long)N * F, kinds, maxDepth, wb));
     wb.kinds = kinds;
     wb.travCtr = nullptr;
@@ -373,7 +450,11 @@ int renderWavefront(int nframes) {
     wb.kindSpot0 = kindSpot0;
     wb.kindEnv = kindEnv;
     const DevScene sc = makeScene();
-    const LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16);
+    const LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16,
+                                              c.cfg.traversalBlocksPerCU, c.cfg.workChunk);
+    wb.grabRows = ls.grabRows;
+    // instrumented runs record events between launches: those stay uncaptured
+    const bool useGraph = c.cfg.graphs != 0 && c.instr == 0;
     uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
     double* sum64 = c.accum64 ? c.dSum64.as<double>() : nullptr;
     const size_t ctrInts = (size_t)kCtrStride * (size_t)(maxDepth + 1);
@@ -384,25 +465,53 @@ int renderWavefront(int nframes) {
     c.stats.bvhNodes = c.bvhNodes;
     c.stats.bvhDepth = c.bvhDepth;
     HIPCHK(hipEventRecord(c.ev0, c.stream));
+    const long long graphLaunches0 = c.graphLaunches;
     int done = 0;
     while (done < nframes) {
         const int frames = std::min(F, nframes - done);
         const long long paths = (long long)N * frames;
-        launch_fill_i32(wb.counters, (long long)ctrInts, 0, c.stream);
-        if (wb.hitCapture) launch_fill_i32(wb.hitCapture, paths * wb.maxRec, -2, c.stream);
-        launch_generate(sc, wb, paths, N, (int)c.frameIndex, c.stream);
+        // the batch's first frame index lives in device memory, so captured
+        // segments replay unchanged from frame to frame
+        launch_fill_i32(c.dFrameBase.as<int>(), 1, (int)c.frameIndex, c.stream);
+        // Bounces run in segments of kSegment; between segments the host reads
+        // the next bounce's ray count and stops early once no path is left.
+        constexpr int kSegment = 16;
         int b = 0;
-        for (; b < maxDepth; ++b) {
-            launch_bounce(sc, wb, b, maxDepth, ls, c.stream, timed ? &c.events[4 * (size_t)b] : nullptr);
-            if (maxDepth > 16 && (b % 16) == 15 && b + 1 < maxDepth) {
-                int next = 0;
-                HIPCHK(hipMemcpyAsync(&next, wb.counters + (size_t)(b + 1) * kCtrStride + kCtrRays, sizeof(int),
-                                      hipMemcpyDeviceToHost, c.stream));
-                HIPCHK(hipStreamSynchronize(c.stream));
-                if (next == 0) { ++b; break; }
+        bool presented = false;
+        for (int b0 = 0; b0 < maxDepth; b0 += kSegment) {
+            const int b1 = std::min(b0 + kSegment, maxDepth);
+            const bool last = b1 == maxDepth;
+            std::vector<unsigned char> key;
+            if (useGraph) {
+                appendKey(key, sc);
+                appendKey(key, wb);
+                appendKey(key, ls);
+                const long long params[] = {paths, N, frames, b0, b1, maxDepth, (long long)(uintptr_t)fb,
+                                            (long long)(uintptr_t)sum64, (long long)(uintptr_t)c.dAccum.p};
+                appendKey(key, params);
             }
+            CHECK_RC(runSegment(useGraph, key, [&]() {
+                if (b0 == 0) {
+                    launch_fill_i32(wb.counters, (long long)ctrInts, 0, c.stream);
+                    if (wb.hitCapture) launch_fill_i32(wb.hitCapture, paths * wb.maxRec, -2, c.stream);
+                    launch_generate(sc, wb, paths, N, c.stream);
+                }
+                for (int bb = b0; bb < b1; ++bb)
+                    launch_bounce(sc, wb, bb, maxDepth, ls, c.stream, timed ? &c.events[4 * (size_t)bb] : nullptr);
+                if (last) launch_present(sc, wb, N, frames, c.dAccum.as<float4>(), sum64, fb, c.stream);
+            }));
+            b = b1;
+            if (last) {
+                presented = true;
+                break;
+            }
+            int next = 0;
+            HIPCHK(hipMemcpyAsync(&next, wb.counters + (size_t)b1 * kCtrStride + kCtrRays, sizeof(int),
+                                  hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            if (next == 0) break;
         }
-        launch_present(sc, wb, N, frames, (int)c.frameIndex, c.dAccum.as<float4>(), sum64, fb, c.stream);
+        if (!presented) launch_present(sc, wb, N, frames, c.dAccum.as<float4>(), sum64, fb, c.stream);
         HIPCHK(hipMemcpyAsync(ctrHost.data(), wb.counters, ctrInts * sizeof(int), hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
         HIPCHK(hipGetLastError());
@@ -438,6 +547,7 @@ int renderWavefront(int nframes) {
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
     c.stats.renderMs = ms;
+    c.stats.graphLaunches = (int)(c.graphLaunches - graphLaunches0);
     if (wb.travCtr) {
         unsigned long long tc[4];
         HIPCHK(hipMemcpy(tc, c.dTrav.p, sizeof(tc), hipMemcpyDeviceToHost));
@@ -452,7 +562,7 @@ int renderWavefront(int nframes) {
 int renderMegakernel(int nframes) {
     Context& c = ctx();
     const DevScene sc = makeScene();
-    const LaunchShape ls = query_launch_shape(c.device, 8);
+    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4);
     CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
     double* sum64 = c.accum64 ? c.dSum64.as<double>() : nullptr;
@@ -724,6 +834,8 @@ int bte_get_render_mode(void) { return ctx().mode; }
 int bte_cleanup(void) {
     Context& c = ctx();
     if (c.deviceReady) (void)hipStreamSynchronize(c.stream);
+    dropGraphs();
+    c.dFrameBase.release();
     DBuf* bufs[] = {&c.dTris, &c.dMaterials, &c.dTriMat, &c.dTriBsdf, &c.dBsdfs, &c.dEmission, &c.dFlags,
                     &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes,
                     &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dEnvMargGuide,
@@ -755,6 +867,12 @@ int bte_set_config(const BteConfig* config) {
     if (config->traversalStack != 0 && config->traversalStack != 8 && config->traversalStack != 16 &&
         config->traversalStack != 32)
         return fail(BTE_ERR_INVALID, "traversalStack must be 0, 8, 16 or 32");
+    if (config->graphs != 0 && config->graphs != 1) return fail(BTE_ERR_INVALID, "graphs must be 0 or 1");
+    if (config->workChunk != 0 && config->workChunk != 1 && config->workChunk != 2 && config->workChunk != 4 &&
+        config->workChunk != 8)
+        return fail(BTE_ERR_INVALID, "workChunk must be 0, 1, 2, 4 or 8");
+    if (config->traversalBlocksPerCU < 0 || config->traversalBlocksPerCU > 64)
+        return fail(BTE_ERR_INVALID, "traversalBlocksPerCU must be in [0, 64]");
     ctx().cfg = *config;
     return BTE_OK;
 }
@@ -763,6 +881,67 @@ int bte_get_config(BteConfig* config) {
     *config = ctx().cfg;
     return BTE_OK;
 }
+int bte_autotune(int width, int height, int frames, double* msBest) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (width <= 0 || height <= 0 || frames <= 0) return fail(BTE_ERR_INVALID, "autotune needs positive sizes");
+    if (c.mode != BTE_MODE_WAVEFRONT) return fail(BTE_ERR_STATE, "autotune applies to the wavefront mode");
+    const BteConfig saved = c.cfg;
+    const int savedInstr = c.instr;
+    c.instr = 0;
+    // one warm-up pass (captures the candidate's graphs), then one timed pass
+    auto timeOne = [&](const BteConfig& cand, double& ms) -> int {
+        c.cfg = cand;
+        CHECK_RC(bte_reset_accumulation(width, height));
+        CHECK_RC(renderFrames(width, height, frames));
+        CHECK_RC(bte_reset_accumulation(width, height));
+        CHECK_RC(renderFrames(width, height, frames));
+        ms = c.stats.renderMs;
+        return BTE_OK;
+    };
+    BteConfig best = saved;
+    if (best.traversalStack == 0) best.traversalStack = 16;
+    if (best.workChunk == 0) best.workChunk = 4;
+    double bestMs = 0.0;
+    int rc = timeOne(best, bestMs);
+    auto tryCand = [&](BteConfig cand) {
+        double ms = 0.0;
+        if (rc == BTE_OK && (rc = timeOne(cand, ms)) == BTE_OK && ms < bestMs) {
+            bestMs = ms;
+            best = cand;
+        }
+    };
+    for (int st : {8, 16, 32}) {
+        BteConfig cand = best;
+        cand.traversalStack = st;
+        if (st != best.traversalStack) tryCand(cand);
+    }
+    const BteConfig afterStack = best;
+    for (int ch : {1, 2, 4, 8}) {
+        BteConfig cand = afterStack;
+        cand.workChunk = ch;
+        if (ch != afterStack.workChunk) tryCand(cand);
+    }
+    int cus = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus <= 0) cus = 1;
+    const int occ = std::max(1, query_launch_shape(c.device, best.traversalStack, 0, best.workChunk).extendBlocks / cus);
+    const BteConfig afterChunk = best;
+    for (int bpc : {0, (3 * occ) / 4, occ / 2}) {
+        BteConfig cand = afterChunk;
+        cand.traversalBlocksPerCU = bpc;
+        if (bpc != afterChunk.traversalBlocksPerCU && (bpc == 0 || bpc < occ)) tryCand(cand);
+    }
+    c.instr = savedInstr;
+    if (rc != BTE_OK) {
+        c.cfg = saved;
+        return rc;
+    }
+    c.cfg = best;
+    CHECK_RC(bte_reset_accumulation(width, height));
+    if (msBest) *msBest = bestMs;
+    return BTE_OK;
+}
+
 int bte_get_stats(BteStats* stats) {
     if (!stats) return fail(BTE_ERR_INVALID, "null stats");
     *stats = ctx().stats;
@@ -884,7 +1063,7 @@ int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, floa
     if (c.tris.empty()) return fail(BTE_ERR_STATE, "no triangles uploaded");
     CHECK_RC(ensureAccel());
     if (c.accW <= 0) { c.accW = std::max(1, c.sceneW); c.accH = std::max(1, c.sceneH); }
-    const LaunchShape ls = query_launch_shape(c.device, 8);
+    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4);
     CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     DBuf dr, dtm, dtri, dt, du, dv;
     CHECK_RC(dr.upload(rays, (size_t)n * 6 * sizeof(float)));

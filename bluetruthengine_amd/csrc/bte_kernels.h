@@ -74,23 +74,25 @@ struct WaveBuffers {
     int kinds;            // shadow kinds in use
     int kindArea, kindSpot0, kindEnv;   // kind index of each light class (-1 absent)
     int catMask;          // bit t set: the scene has BSDF type t (its shading kernel runs)
+    int grabRows;         // 64-slot rows a persistent wave takes per work-counter atomic (1, 2, 4, 8)
+    const int* frameBase; // device int: frame index of the batch's first frame (set before each batch)
 };
 
 struct LaunchShape {
     int extendBlocks, shadeBlocks, connectBlocks, auxBlocks;
     int ldsStack;         // traversal LDS stack entries per lane: 8, 16 or 32
+    int grabRows;         // work chunk of the persistent kernels, in 64-slot rows
 };
 
 // Launchers (all asynchronous on `stream`).
 void launch_fill_i32(int* p, long long n, int value, hipStream_t s);
-void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths, int pixels, int frame0,
-                     hipStream_t s);
+void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths, int pixels, hipStream_t s);
 // ev (optional): 4 events recorded before extend, after extend, after shade,
 // after the connect passes.
 void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
                    hipStream_t s, hipEvent_t* ev);
-void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, int frameBase,
-                    float4* accum, double* sum64, uint32_t* fb, hipStream_t s);
+void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, float4* accum,
+                    double* sum64, uint32_t* fb, hipStream_t s);
 void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex, float4* accum, double* sum64,
                        uint32_t* fb, int* ovfStack, hipStream_t s);
 void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxPerRay, long long n, float tmin,
@@ -98,7 +100,8 @@ void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxP
                        hipStream_t s);
 void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
                     int* margGuide, int* condGuide, int* monotone, hipStream_t s);
-LaunchShape query_launch_shape(int device, int ldsStack);
+// traversalBlocksPerCU 0 = as many as occupancy allows; grabRows 1, 2, 4 or 8 (else 4).
+LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows);
 int persistent_threads(const LaunchShape& ls);
 int overflow_stack_ints(const LaunchShape& ls);   // size of the overflow-stack buffer (ints)
 

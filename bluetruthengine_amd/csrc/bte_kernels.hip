@@ -23,7 +23,7 @@ constexpr int kLdsStack = 32;                 // LDS stack entries per lane (meg
 constexpr int kMinLdsStack = 8;               // smallest LDS stack of any traversal kernel variant
 constexpr int kOvfStack = kStackSize - kMinLdsStack;   // global overflow entries per thread
 constexpr uint32_t kSpecBit = 0x80000000u;
-constexpr int kTraversalDone = 0x7fffffff;   // sentinel: no node left to visit
+[[maybe_unused]] constexpr int kTraversalDone = 0x7fffffff;   // sentinel: no node left to visit
 #ifndef BTE_TRAVERSAL_WHILE_WHILE
 #define BTE_TRAVERSAL_WHILE_WHILE 0
 #endif
@@ -60,7 +60,6 @@ BTE_DEV int wave_grab(int* ctr, int n = 64) {
     if (lane_id() == 0) base = atomicAdd(ctr, n);
     return __shfl(base, 0, 64);
 }
-constexpr int kGrabRows = 4;   // persistent waves take 4 x 64 queue slots per atomic
 
 // ---------------------------------------------------------------------------
 // BVH2 traversal (replaces optixTrace: rt_optix_shading.cuh:57-95)
@@ -709,10 +708,10 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
     if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb, long long paths, int pixels,
-                                                      int frame0) {
+__global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb, long long paths, int pixels) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= paths) return;
+    const int frame0 = *wb.frameBase;
     const int pixel = (int)(i % pixels);
     const int frame = frame0 + (int)(i / pixels);
     const int px = pixel % sc.width, py = pixel / sc.width;
@@ -747,9 +746,9 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
     const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     uint32_t nodes = 0, tris = 0;
     for (;;) {
-        const int chunk = wave_grab(ctr + kCtrExtendWork, 64 * kGrabRows);
+        const int chunk = wave_grab(ctr + kCtrExtendWork, 64 * wb.grabRows);
         if (chunk >= count) break;
-        for (int row = 0; row < kGrabRows; ++row) {
+        for (int row = 0; row < wb.grabRows; ++row) {
         const int qi = chunk + row * 64 + lane_id();
         if (qi < count) {
             const float4 o = wb.rayO[cur][qi];
@@ -772,8 +771,24 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
 // iteration: ballot ranks within each wave, an LDS scan over the block, and one
 // global atomic per non-empty category (device-wide atomics on a hot counter
 // serialise, so they are amortised over the block).
+// A missed ray ends its path: env-map emission with MIS (A13) added to its
+// radiance, which becomes the path's final radiance.
+BTE_DEV void shadeMiss(const DevScene& sc, const WaveBuffers& wb, int qi, int bounce) {
+    const int cur = bounce & 1;
+    const float4 ro = wb.rayO[cur][qi], rd = wb.rayD[cur][qi], th = wb.thr[cur][qi], ra = wb.rad[cur][qi];
+    const uint2 st = wb.st[cur][qi];
+    const uint32_t pid = st.y & ~kSpecBit;
+    const bool spec = (st.y & kSpecBit) != 0;
+    if (wb.hitCapture && bounce < wb.maxRec)
+        wb.hitCapture[(size_t)pid * wb.maxRec + bounce] = __float_as_int(wb.hit[qi].x);
+    V3 rad = xyz(ra);
+    envHit(sc, xyz(rd), spec, ro.w, xyz(th), rad);
+    wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+}
+
 constexpr int kCatSub = 4;   // 64-slot rows per wave per iteration
-__global__ __launch_bounds__(kBlock) void k_categorize(WaveBuffers wb, int bounce) {
+// Misses are shaded here, in queue order, and are not queued.
+__global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers wb, int bounce) {
     __shared__ int sCnt[kCatSub][kBlock / 64][kNumCats];
     __shared__ int sBase[kNumCats];
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
@@ -785,13 +800,17 @@ __global__ __launch_bounds__(kBlock) void k_categorize(WaveBuffers wb, int bounc
         for (int j = 0; j < kCatSub; ++j) {
             const int i = base + j * kBlock + wave * 64 + lane;
             c[j] = (i < count) ? (int)wb.cat[i] : -1;
-            for (int k = 0; k < kNumCats; ++k) {
+            if (c[j] == kCatMiss) {
+                shadeMiss(sc, wb, i, bounce);
+                c[j] = -1;
+            }
+            for (int k = 1; k < kNumCats; ++k) {
                 const unsigned long long m = __ballot(c[j] == k);
                 if (lane == 0) sCnt[j][wave][k] = __popcll(m);
             }
         }
         __syncthreads();
-        if (threadIdx.x < kNumCats) {
+        if (threadIdx.x >= 1 && threadIdx.x < kNumCats) {
             const int k = threadIdx.x;
             int run = 0;
             for (int j = 0; j < kCatSub; ++j)
@@ -801,12 +820,12 @@ __global__ __launch_bounds__(kBlock) void k_categorize(WaveBuffers wb, int bounc
                     run += n;
                 }
             sBase[k] = run ? atomicAdd(ctr + kCtrCat + k, run) : 0;
-            if (k != kCatMiss && run) atomicAdd(ctr + kCtrHits, run);
+            if (run) atomicAdd(ctr + kCtrHits, run);
         }
         __syncthreads();
         for (int j = 0; j < kCatSub; ++j) {
             const int i = base + j * kBlock + wave * 64 + lane;
-            for (int k = 0; k < kNumCats; ++k) {
+            for (int k = 1; k < kNumCats; ++k) {
                 const unsigned long long m = __ballot(c[j] == k);
                 if (c[j] == k) {
                     const unsigned lo = (unsigned)(m & 0xffffffffull), hi = (unsigned)(m >> 32);
@@ -819,7 +838,7 @@ __global__ __launch_bounds__(kBlock) void k_categorize(WaveBuffers wb, int bounc
     }
 }
 
-// Shade one category of hits (T = BsdfType) or the misses (T = kCatMiss).
+// Shade one category of hits (T = BsdfType; misses are shaded by k_categorize).
 template <int T>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, int bounce, int maxDepth) {
     const int cur = bounce & 1, nxt = cur ^ 1;
@@ -827,12 +846,11 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
     int* ctrNext = wb.counters + (size_t)(bounce + 1) * kCtrStride;
     const int count = ctr[kCtrCat + T];
     const int* list = wb.catQ + (size_t)T * (size_t)wb.cap;
-    constexpr bool kMiss = (T == kCatMiss);
     constexpr bool kDiffuseType = (T == BTE_BSDF_DIFFUSE || T == BTE_BSDF_MICROFACET || T == BTE_BSDF_DISNEY);
     for (;;) {
-        const int chunk = wave_grab(ctr + kCtrShadeWork + T, 64 * kGrabRows);
+        const int chunk = wave_grab(ctr + kCtrShadeWork + T, 64 * wb.grabRows);
         if (chunk >= count) break;
-        for (int row = 0; row < kGrabRows; ++row) {
+        for (int row = 0; row < wb.grabRows; ++row) {
         const int li = chunk + row * 64 + lane_id();
         const bool active = li < count;
         V3 o = {0, 0, 0}, d = {0, 0, 0}, thr = {0, 0, 0}, rad = {0, 0, 0};
@@ -863,22 +881,18 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
         frame.t = frame.b = frame.n = V3{0, 0, 0};
         const V3 dIn = d, thrIn = thr;
         const uint32_t rngNee = rng;
-        if constexpr (kMiss) {
-            if (active) envHit(sc, d, spec, brdfPdf, thr, rad);       // path ends
-        } else {
-            if (active) {
-                its = makeIts(sc, o, d, h);
-                bsdf = loadBsdf(sc, its.tri);
-                if (sc.triFlags && sc.triFlags[its.tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
-                spec = !kDiffuseType;
-                if (kDiffuseType) {
-                    rngNext(rng); rngNext(rng);                       // area NEE draws (always)
-                    if (sc.hasEnv) { rngNext(rng); rngNext(rng); }    // env NEE draws
-                }
-                frame = makeFrame(its.n);
-                scatter<T>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
-                survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
+        if (active) {
+            its = makeIts(sc, o, d, h);
+            bsdf = loadBsdf(sc, its.tri);
+            if (sc.triFlags && sc.triFlags[its.tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
+            spec = !kDiffuseType;
+            if (kDiffuseType) {
+                rngNext(rng); rngNext(rng);                       // area NEE draws (always)
+                if (sc.hasEnv) { rngNext(rng); rngNext(rng); }    // env NEE draws
             }
+            frame = makeFrame(its.n);
+            scatter<T>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
+            survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
         }
         const int slot = wave_append(ctrNext + kCtrRays, survive);
         if (survive) {
@@ -933,9 +947,9 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
     const size_t off = (size_t)kind * (size_t)wb.cap;
     uint32_t nodes = 0, tris = 0;
     for (;;) {
-        const int chunk = wave_grab(ctr + kCtrConnectWork + kind, 64 * kGrabRows);
+        const int chunk = wave_grab(ctr + kCtrConnectWork + kind, 64 * wb.grabRows);
         if (chunk >= count) break;
-        for (int row = 0; row < kGrabRows; ++row) {
+        for (int row = 0; row < wb.grabRows; ++row) {
         const int si = chunk + row * 64 + lane_id();
         if (si < count) {
             const float4 so = wb.shO[off + si];
@@ -961,10 +975,11 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_present(WaveBuffers wb, int pixels, int frames, int frameBase,
-                                                     float4* accum, double* sum64, uint32_t* fb) {
+__global__ __launch_bounds__(kBlock) void k_present(WaveBuffers wb, int pixels, int frames, float4* accum,
+                                                     double* sum64, uint32_t* fb) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pixels) return;
+    const int frameBase = *wb.frameBase;
     float4 mean = accum[p];
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int f = 0; f < frames; ++f) {
@@ -1175,9 +1190,8 @@ void launch_fill_i32(int* p, long long n, int value, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_i32, dim3(blocksFor(n, 256)), dim3(256), 0, s, p, n, value);
 }
 
-void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths, int pixels, int frame0,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(k_generate, dim3(blocksFor(paths, kBlock)), dim3(kBlock), 0, s, sc, wb, paths, pixels, frame0);
+void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths, int pixels, hipStream_t s) {
+    hipLaunchKernelGGL(k_generate, dim3(blocksFor(paths, kBlock)), dim3(kBlock), 0, s, sc, wb, paths, pixels);
 }
 
 template <int L>
@@ -1187,10 +1201,9 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     if (ev) (void)hipEventRecord(ev[0], s);
     if (count) hipLaunchKernelGGL((k_extend<true, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
     else hipLaunchKernelGGL((k_extend<false, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
-    hipLaunchKernelGGL(k_categorize, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, wb, bounce);
     if (ev) (void)hipEventRecord(ev[1], s);
-    // misses are always shaded; a BSDF category only when the scene uses that type
-    hipLaunchKernelGGL(k_shade<kCatMiss>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce, maxDepth);
+    hipLaunchKernelGGL(k_categorize, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    // a BSDF category is shaded only when the scene uses that type
     if (wb.catMask & (1 << BTE_BSDF_DIFFUSE))
         hipLaunchKernelGGL(k_shade<BTE_BSDF_DIFFUSE>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
                            maxDepth);
@@ -1225,11 +1238,11 @@ void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int ma
     }
 }
 
-void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, int frameBase,
-                    float4* accum, double* sum64, uint32_t* fb, hipStream_t s) {
+void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, float4* accum,
+                    double* sum64, uint32_t* fb, hipStream_t s) {
     (void)sc;
-    hipLaunchKernelGGL(k_present, dim3(blocksFor(pixels, kBlock)), dim3(kBlock), 0, s, wb, pixels, frames, frameBase,
-                       accum, sum64, fb);
+    hipLaunchKernelGGL(k_present, dim3(blocksFor(pixels, kBlock)), dim3(kBlock), 0, s, wb, pixels, frames, accum,
+                       sum64, fb);
 }
 
 void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex, float4* accum, double* sum64,
@@ -1240,7 +1253,7 @@ void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex
     launch_fill_i32(work, 1, 0, s);
     int dev = 0;
     (void)hipGetDevice(&dev);
-    const LaunchShape ls = query_launch_shape(dev, 32);
+    const LaunchShape ls = query_launch_shape(dev, 32, 0, 4);
     hipLaunchKernelGGL(k_megakernel, dim3(ls.auxBlocks), dim3(kBlock), 0, s, sc, spp, maxDepth, frameIndex, accum,
                        sum64, fb, ovf, work);
 }
@@ -1252,7 +1265,7 @@ void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxP
     launch_fill_i32(work, 1, 0, s);
     int dev = 0;
     (void)hipGetDevice(&dev);
-    const LaunchShape ls = query_launch_shape(dev, 32);
+    const LaunchShape ls = query_launch_shape(dev, 32, 0, 4);
     hipLaunchKernelGGL(k_trace_rays, dim3(ls.auxBlocks), dim3(kBlock), 0, s, sc, rays, tmaxPerRay, n, tmin, tmax,
                        anyHit, tri, t, u, v, ovf, work);
 }
@@ -1273,7 +1286,7 @@ static int travOccupancy() {
     return std::max(1, std::min(a, b));
 }
 
-LaunchShape query_launch_shape(int device, int ldsStack) {
+LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows) {
     static int cachedDev = -1;
     static int cus = 1, occS = 1, occMT = 1, occL[3] = {1, 1, 1};
     if (device != cachedDev) {
@@ -1294,7 +1307,9 @@ LaunchShape query_launch_shape(int device, int ldsStack) {
     }
     LaunchShape ls{};
     ls.ldsStack = (ldsStack == 8 || ldsStack == 16) ? ldsStack : 32;
-    const int occ = occL[ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
+    int occ = occL[ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
+    if (traversalBlocksPerCU > 0) occ = std::min(occ, traversalBlocksPerCU);
+    ls.grabRows = (grabRows == 1 || grabRows == 2 || grabRows == 8) ? grabRows : 4;
     ls.extendBlocks = cus * occ;
     ls.connectBlocks = cus * occ;
     ls.shadeBlocks = cus * occS;

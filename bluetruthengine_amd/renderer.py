@@ -42,7 +42,7 @@ class BteFloat3(ctypes.Structure):
 
 class BteConfig(ctypes.Structure):
     _fields_ = [("maxDepth", _i), ("megakernelSpp", _i), ("framesInFlight", _i), ("memoryBudgetMB", _i),
-                ("traversalStack", _i)]
+                ("traversalStack", _i), ("graphs", _i), ("workChunk", _i), ("traversalBlocksPerCU", _i)]
 
 
 class BteStats(ctypes.Structure):
@@ -51,7 +51,8 @@ class BteStats(ctypes.Structure):
                 ("raysPerBounce", _ll * 64),
                 ("extendMs", ctypes.c_double), ("shadeMs", ctypes.c_double), ("connectMs", ctypes.c_double),
                 ("extendLaunches", _i), ("connectLaunches", _i),
-                ("extendNodes", _ll), ("extendTris", _ll), ("connectNodes", _ll), ("connectTris", _ll)]
+                ("extendNodes", _ll), ("extendTris", _ll), ("connectNodes", _ll), ("connectTris", _ll),
+                ("graphLaunches", _i)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "raysPerBounce"}
@@ -97,6 +98,7 @@ API = {
     "bte_read_hit_capture": [_p, _sz],
     "bte_read_env_cdf": [_p, _p, _p],
     "bte_trace_rays": [_p, _p, _ll, _f, _f, _i, _p, _p, _p, _p],
+    "bte_autotune": [_i, _i, _i, _p],
 }
 
 _lib = None
@@ -219,7 +221,7 @@ class Renderer:
     """Uploads a SceneData through the C ABI and renders into device buffers."""
 
     def __init__(self, scene, max_depth=None, frames_in_flight=0, memory_budget_mb=0, mode=BTE_MODE_WAVEFRONT,
-                 megakernel_spp=5, traversal_stack=0):
+                 megakernel_spp=5, traversal_stack=0, graphs=1, work_chunk=0, traversal_blocks_per_cu=0):
         self.scene = scene
         lib()
         cudaCleanup()
@@ -234,14 +236,28 @@ class Renderer:
         cudaInitSpotlights(scene.spotlights)
         cudaInitEnvMap(scene.env_rgba, scene.env_intensity, scene.env_rotation_deg)
         self.set_config(max_depth if max_depth is not None else scene.max_depth, frames_in_flight,
-                        memory_budget_mb, megakernel_spp, traversal_stack)
+                        memory_budget_mb, megakernel_spp, traversal_stack, graphs, work_chunk,
+                        traversal_blocks_per_cu)
         cudaSetRenderMode(mode)
         cudaResetAccumulation(scene.width, scene.height)
 
-    def set_config(self, max_depth, frames_in_flight=0, memory_budget_mb=0, megakernel_spp=5, traversal_stack=0):
+    def set_config(self, max_depth, frames_in_flight=0, memory_budget_mb=0, megakernel_spp=5, traversal_stack=0,
+                   graphs=1, work_chunk=0, traversal_blocks_per_cu=0):
         cfg = BteConfig(int(max_depth), int(megakernel_spp), int(frames_in_flight), int(memory_budget_mb),
-                        int(traversal_stack))
+                        int(traversal_stack), int(graphs), int(work_chunk), int(traversal_blocks_per_cu))
         _call("bte_set_config", ctypes.byref(cfg))
+
+    def config(self):
+        cfg = BteConfig()
+        _call("bte_get_config", ctypes.byref(cfg))
+        return {k: getattr(cfg, k) for k, _ in cfg._fields_}
+
+    def autotune(self, frames):
+        """Picks the fastest launch shape for `frames`-frame passes at the scene's resolution (resets the
+        accumulation); returns (best ms, config dict)."""
+        ms = ctypes.c_double(0.0)
+        _call("bte_autotune", self.scene.width, self.scene.height, int(frames), ctypes.byref(ms))
+        return ms.value, self.config()
 
     def build_accel(self):
         cudaInitOptix()

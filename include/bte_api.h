@@ -85,6 +85,9 @@ typedef struct {
     int framesInFlight;      /* wavefront: frames traced concurrently (0 = auto) */
     int memoryBudgetMB;      /* cap for wavefront path state (0 = default 32768) */
     int traversalStack;      /* BVH stack entries kept in LDS per lane: 8, 16 or 32 (0 = default 16) */
+    int graphs;              /* 1 = replay the bounce loop as captured HIP graphs (default), 0 = plain launches */
+    int workChunk;           /* 64-slot queue rows a persistent wave takes per atomic: 1, 2, 4, 8 (0 = 4) */
+    int traversalBlocksPerCU;/* resident traversal blocks per CU (0 = as many as occupancy allows) */
 } BteConfig;
 
 /* Counters of the last bte_render / bte_render_frames call. */
@@ -104,6 +107,7 @@ typedef struct {
     int extendLaunches, connectLaunches;
     long long extendNodes, extendTris;     /* BVH nodes visited / triangles tested */
     long long connectNodes, connectTris;
+    int graphLaunches;       /* captured bounce-loop segments replayed by this call */
 } BteStats;
 
 enum BteInstrument {
@@ -192,6 +196,15 @@ int bte_set_hit_capture(int maxRecord);
 int bte_read_hit_capture(int* hostTris, size_t count);
 /* Env-map CDF arrays as built on the GPU (marginal H+1, conditional H*(W+1)). */
 int bte_read_env_cdf(float* marginal, float* conditional, float* totalPower);
+/* Launch-shape autotuner (no counterpart in the reference, whose launch
+ * shape OptiX fixes).  Renders `frames` frames at width x height once per
+ * candidate (traversal LDS stack 8/16/32, then work chunk 1/2/4/8 rows, then
+ * traversal blocks per CU: all, 3/4, 1/2), by coordinate descent from the
+ * current configuration, keeps the fastest in the configuration and resets
+ * the accumulation.  Images do not depend on these settings.  msBest
+ * (optional) receives the best render time. */
+int bte_autotune(int width, int height, int frames, double* msBest);
+
 /* Trace caller rays with the BVH2 kernels: rays = n x (ox,oy,oz,dx,dy,dz).
  * anyHit = 0: closest hit in [tmin, tmax] -> tri (-1 miss), t, u, v.
  * anyHit = 1: occlusion in [tmin, tmaxPerRay[i]] -> tri = 1 occluded / 0. */

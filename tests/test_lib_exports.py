@@ -33,3 +33,25 @@ def test_library_loads_and_reports_errors_without_device():
     cfg = renderer.BteConfig(0, 5, 0, 0, 0)
     assert L.bte_set_config(ctypes.byref(cfg)) == -1        # invalid maxDepth
     assert b"maxDepth" in L.bte_last_error()
+
+
+def test_config_validation_without_device():
+    import ctypes
+    L = renderer.lib()
+    saved = renderer.BteConfig()
+    assert L.bte_get_config(ctypes.byref(saved)) == 0
+    assert (saved.graphs, saved.workChunk, saved.traversalBlocksPerCU) == (1, 4, 0)
+    for fields, word in [(dict(graphs=2), b"graphs"), (dict(workChunk=3), b"workChunk"),
+                         (dict(traversalBlocksPerCU=-1), b"traversalBlocksPerCU"),
+                         (dict(traversalStack=12), b"traversalStack")]:
+        cfg = renderer.BteConfig(12, 5, 0, 0, 16, 1, 4, 0)
+        for k, v in fields.items():
+            setattr(cfg, k, v)
+        assert L.bte_set_config(ctypes.byref(cfg)) == -1
+        assert word in L.bte_last_error()
+    good = renderer.BteConfig(12, 5, 0, 0, 8, 0, 2, 3)
+    assert L.bte_set_config(ctypes.byref(good)) == 0
+    back = renderer.BteConfig()
+    L.bte_get_config(ctypes.byref(back))
+    assert (back.traversalStack, back.graphs, back.workChunk, back.traversalBlocksPerCU) == (8, 0, 2, 3)
+    assert L.bte_set_config(ctypes.byref(saved)) == 0
