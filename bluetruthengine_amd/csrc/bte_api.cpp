@@ -21,10 +21,10 @@
 #include <vector>
 
 #include "bte_bvh.h"
+#include "bte_internal.h"
 #include "bte_kernels.h"
 
 namespace bte {
-namespace {
 
 char g_err[1024] = "";
 
@@ -36,44 +36,7 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIPCHK(expr)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return fail(BTE_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                        __FILE__, __LINE__);                                                  \
-    } while (0)
-
-// Owning device allocation.
-struct DBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    int ensure(size_t n) {
-        if (n <= bytes && p) return BTE_OK;
-        release();
-        if (n == 0) return BTE_OK;
-        if (hipMalloc(&p, n) != hipSuccess) {
-            p = nullptr;
-            (void)hipGetLastError();
-            return fail(BTE_ERR_MEMORY, "hipMalloc(%zu) failed", n);
-        }
-        bytes = n;
-        return BTE_OK;
-    }
-    int upload(const void* src, size_t n) {
-        int rc = ensure(n);
-        if (rc) return rc;
-        if (n) HIPCHK(hipMemcpy(p, src, n, hipMemcpyHostToDevice));
-        return BTE_OK;
-    }
-    template <typename T> T* as() const { return static_cast<T*>(p); }
-};
+namespace {
 
 struct Context {
     bool deviceReady = false;
@@ -115,11 +78,12 @@ struct Context {
     uint32_t frameIndex = 0;
     bool accum64 = false;
     int mode = BTE_MODE_WAVEFRONT;
-    bool denoiser = false;
 
     // wavefront buffers
     DBuf dRayO[2], dRayD[2], dThr[2], dRad[2], dSt[2], dHit, dCatQ, dCat, dFinal, dShO, dShD, dShC, dCounters, dHitCap, dOvf;
     DBuf dFrameBase;
+    bool aov = false;                 // albedo / normal AOVs of the first non-delta hit
+    DBuf dAovAlb, dAovNrm, dAovAccAlb, dAovAccNrm;
 
     // captured bounce-loop segments, keyed by every value their launches bake in
     struct GraphEntry {
@@ -167,11 +131,6 @@ int ensureDevice() {
 #define ENSURE_DEVICE()              \
     do {                             \
         int rc_ = ensureDevice();    \
-        if (rc_) return rc_;         \
-    } while (0)
-#define CHECK_RC(expr)               \
-    do {                             \
-        int rc_ = (expr);            \
         if (rc_) return rc_;         \
     } while (0)
 
@@ -277,6 +236,12 @@ int resetAccum(int w, int h) {
     CHECK_RC(c.dAccum.ensure(N * sizeof(float4)));
     CHECK_RC(c.dFb.ensure(N * sizeof(uint32_t)));
     HIPCHK(hipMemsetAsync(c.dAccum.p, 0, N * sizeof(float4), c.stream));
+    if (c.aov) {
+        CHECK_RC(c.dAovAccAlb.ensure(N * sizeof(float4)));
+        CHECK_RC(c.dAovAccNrm.ensure(N * sizeof(float4)));
+        HIPCHK(hipMemsetAsync(c.dAovAccAlb.p, 0, N * sizeof(float4), c.stream));
+        HIPCHK(hipMemsetAsync(c.dAovAccNrm.p, 0, N * sizeof(float4), c.stream));
+    }
     if (c.accum64) {
         CHECK_RC(c.dSum64.ensure(N * 3 * sizeof(double)));
         HIPCHK(hipMemsetAsync(c.dSum64.p, 0, N * 3 * sizeof(double), c.stream));
@@ -300,7 +265,7 @@ int kindsInUse(int& kindArea, int& kindSpot0, int& kindEnv) {
 
 size_t bytesPerPath(int kinds) {
     return 2 * (4 * sizeof(float4) + sizeof(uint2)) + 2 * sizeof(float4) + kNumCats * sizeof(int) + 1 +
-           (size_t)kinds * 3 * sizeof(float4);
+           (size_t)kinds * 3 * sizeof(float4) + (ctx().aov ? 2 * sizeof(float4) : 0);
 }
 
 int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
@@ -323,6 +288,10 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     CHECK_RC(c.dShC.ensure(K * P * sizeof(float4)));
     CHECK_RC(c.dCounters.ensure((size_t)kCtrStride * (size_t)(maxDepth + 1) * sizeof(int)));
     CHECK_RC(c.dFrameBase.ensure(sizeof(int)));
+    if (c.aov) {
+        CHECK_RC(c.dAovAlb.ensure(P * sizeof(float4)));
+        CHECK_RC(c.dAovNrm.ensure(P * sizeof(float4)));
+    }
     const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4);
     CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     if (c.hitCaptureRec > 0) CHECK_RC(c.dHitCap.ensure(P * (size_t)c.hitCaptureRec * sizeof(int)));
@@ -344,6 +313,10 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     wb.shC = c.dShC.as<float4>();
     wb.counters = c.dCounters.as<int>();
     wb.frameBase = c.dFrameBase.as<int>();
+    wb.aovAlb = c.aov ? c.dAovAlb.as<float4>() : nullptr;
+    wb.aovNrm = c.aov ? c.dAovNrm.as<float4>() : nullptr;
+    wb.aovAccAlb = c.aov ? c.dAovAccAlb.as<float4>() : nullptr;
+    wb.aovAccNrm = c.aov ? c.dAovAccNrm.as<float4>() : nullptr;
     wb.hitCapture = c.hitCaptureRec > 0 ? c.dHitCap.as<int>() : nullptr;
     wb.ovfStack = c.dOvf.as<int>() + 64;
     wb.cap = cap;
@@ -425,7 +398,9 @@ int renderWavefront(int nframes) {
         F = (int)std::max(1.0, std::floor(budget / ((double)N * (double)bytesPerPath(kinds))));
     }
     F = std::max(1, std::min(F, nframes));
-    if ((long long)N * F >= (1ll << 31) - 4096) F = std::max(1, (int)(((1ll << 31) - 4096) / N));
+    // path ids occupy 30 bits of the path word
+    if ((long long)N * F > (1ll << 30)) F = std::max(1, (int)((1ll << 30) / N));
+    if ((long long)N > (1ll << 30)) return fail(BTE_ERR_INVALID, "more than 2^30 pixels per frame");
     WaveBuffers wb;
     std::memset(&wb, 0, sizeof wb);   // padding too: compared bytewise as graph keys
     CHECK_RC(ensureWaveBuffers((long long)N * F, kinds, maxDepth, wb));
@@ -593,11 +568,33 @@ int renderFrames(int w, int h, int nframes) {
     CHECK_RC(validateForRender());
     if (!c.dAccum.p || c.accW != w || c.accH != h) CHECK_RC(resetAccum(w, h));
     CHECK_RC(ensureAccel());
-    if (c.mode == BTE_MODE_MEGAKERNEL) return renderMegakernel(nframes);
-    return renderWavefront(nframes);
+    CHECK_RC(c.mode == BTE_MODE_MEGAKERNEL ? renderMegakernel(nframes) : renderWavefront(nframes));
+    // the reference denoises after the frame when its denoiser is on (src/render_kernel.cu:139-145)
+    if (denoiser_active()) CHECK_RC(denoiser_present());
+    return BTE_OK;
 }
 
 }  // namespace
+
+int ensure_device() { return ensureDevice(); }
+hipStream_t main_stream() { return ctx().stream; }
+
+int render_view(RenderView& v) {
+    CHECK_RC(ensureDevice());
+    Context& c = ctx();
+    if (!c.dAccum.p || c.accW <= 0) return fail(BTE_ERR_STATE, "no accumulation buffer (render first)");
+    v.stream = c.stream;
+    v.device = c.device;
+    v.width = c.accW;
+    v.height = c.accH;
+    v.accum = c.dAccum.as<float4>();
+    v.aovAlb = (c.aov && c.dAovAccAlb.p) ? c.dAovAccAlb.as<float4>() : nullptr;
+    v.aovNrm = (c.aov && c.dAovAccNrm.p) ? c.dAovAccNrm.as<float4>() : nullptr;
+    v.fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
+    v.frameIndex = c.frameIndex;
+    return BTE_OK;
+}
+
 }  // namespace bte
 
 using namespace bte;
@@ -817,10 +814,6 @@ int bte_render_frames(int imageWidth, int imageHeight, int nframes) {
     return renderFrames(imageWidth, imageHeight, nframes);
 }
 
-int bte_toggle_denoiser(void) {
-    return fail(BTE_ERR_UNSUPPORTED, "the learned denoiser is not part of this build");
-}
-
 int bte_set_render_mode(int mode) {
     if (mode != BTE_MODE_MEGAKERNEL && mode != BTE_MODE_WAVEFRONT) return fail(BTE_ERR_INVALID, "unknown mode %d", mode);
     Context& c = ctx();
@@ -835,7 +828,10 @@ int bte_cleanup(void) {
     Context& c = ctx();
     if (c.deviceReady) (void)hipStreamSynchronize(c.stream);
     dropGraphs();
+    denoiser_release();
     c.dFrameBase.release();
+    c.dAovAlb.release(); c.dAovNrm.release(); c.dAovAccAlb.release(); c.dAovAccNrm.release();
+    c.aov = false;
     DBuf* bufs[] = {&c.dTris, &c.dMaterials, &c.dTriMat, &c.dTriBsdf, &c.dBsdfs, &c.dEmission, &c.dFlags,
                     &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes,
                     &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dEnvMargGuide,
@@ -977,6 +973,41 @@ int bte_read_accum(float* hostRGB, size_t count) {
         hostRGB[3 * i] = tmp[4 * i];
         hostRGB[3 * i + 1] = tmp[4 * i + 1];
         hostRGB[3 * i + 2] = tmp[4 * i + 2];
+    }
+    return BTE_OK;
+}
+
+int bte_enable_aov(int enable) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    c.aov = enable != 0;
+    if (c.aov && c.accW > 0) {
+        const size_t N = (size_t)c.accW * c.accH;
+        CHECK_RC(c.dAovAccAlb.ensure(N * sizeof(float4)));
+        CHECK_RC(c.dAovAccNrm.ensure(N * sizeof(float4)));
+        HIPCHK(hipMemset(c.dAovAccAlb.p, 0, N * sizeof(float4)));
+        HIPCHK(hipMemset(c.dAovAccNrm.p, 0, N * sizeof(float4)));
+    }
+    return BTE_OK;
+}
+
+int bte_read_aov(float* hostAlbedo, float* hostNormal, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    const size_t N = (size_t)c.accW * c.accH;
+    if (count < 3 * N) return fail(BTE_ERR_INVALID, "AOV copy needs %zu floats per buffer", 3 * N);
+    if (!c.aov || !c.dAovAccAlb.p) return fail(BTE_ERR_STATE, "AOVs are not enabled");
+    std::vector<float> tmp(4 * N);
+    float* dst[2] = {hostAlbedo, hostNormal};
+    const DBuf* src[2] = {&c.dAovAccAlb, &c.dAovAccNrm};
+    for (int k = 0; k < 2; ++k) {
+        if (!dst[k]) continue;
+        HIPCHK(hipMemcpy(tmp.data(), src[k]->p, N * sizeof(float4), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; ++i) {
+            dst[k][3 * i] = tmp[4 * i];
+            dst[k][3 * i + 1] = tmp[4 * i + 1];
+            dst[k][3 * i + 2] = tmp[4 * i + 2];
+        }
     }
     return BTE_OK;
 }

@@ -1,0 +1,442 @@
+// Denoiser kernels (bf16 MFMA implicit-GEMM convolutions and their
+// companions).  Shapes and conventions are documented in bte_denoise.h.
+#include "bte_denoise.h"
+
+#include "bte_device.h"
+
+namespace bte {
+namespace dn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u}); }
+__device__ __forceinline__ float f(bf16 x) { return (float)x; }
+
+// ---------------------------------------------------------------------------
+// 3x3 convolution as an implicit GEMM: M = pixels, N = output channels,
+// K = 9 taps x input channels.  A wave owns 32 consecutive pixels (two 16-row
+// MFMA tiles) and NT 16-column tiles; each K step takes 32 input channels of
+// one tap: lane (row r, group g) loads the 8 channels [8g, 8g + 8) of its
+// pixel's tapped neighbour (16 bytes, NHWC) and of its output channel's
+// weights, and the MFMA pairs them.  The accumulator lane layout of
+// v_mfma_f32_16x16x32_bf16 is D[4g + i][r] in element i.
+// ---------------------------------------------------------------------------
+template <int NT>
+__global__ __launch_bounds__(kBlock) void k_conv3x3(ConvArgs a) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, r16 = lane & 15;
+    const long long P = (long long)a.n * a.h * a.w_;
+    const long long base = ((long long)blockIdx.x * 4 + wave) * 32;
+    if (base >= P) return;
+    const int coBase = blockIdx.y * NT * 16;
+    const int cin = a.c0 + a.c1;
+    const int hh = a.h >> 1, wh = a.w_ >> 1;
+    int pn[2], py[2], px[2];
+    bool pv[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const long long p = base + mt * 16 + r16;
+        pv[mt] = p < P;
+        const long long pp = pv[mt] ? p : 0;
+        px[mt] = (int)(pp % a.w_);
+        const long long q = pp / a.w_;
+        py[mt] = (int)(q % a.h);
+        pn[mt] = (int)(q / a.h);
+    }
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int t = 0; t < 9; ++t) {
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+        const bf16* s0p[2];
+        const bf16* s1p[2];
+        bool ok[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            const int yy = py[mt] + dy, xx = px[mt] + dx;
+            ok[mt] = pv[mt] && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_;
+            const int y0 = ok[mt] ? yy : 0, x0 = ok[mt] ? xx : 0;
+            s0p[mt] = a.up0 ? a.src0 + (((size_t)pn[mt] * hh + (y0 >> 1)) * wh + (x0 >> 1)) * a.c0
+                            : a.src0 + (((size_t)pn[mt] * a.h + y0) * a.w_ + x0) * a.c0;
+            s1p[mt] = a.src1 ? a.src1 + (((size_t)pn[mt] * a.h + y0) * a.w_ + x0) * a.c1 : nullptr;
+        }
+        const bf16* wrow = a.w + ((size_t)(coBase + r16) * 9 + t) * cin + 8 * g;
+        for (int c = 0; c < cin; c += 32) {
+            bf16x8 A[2];
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const bf16* src = (c < a.c0) ? s0p[mt] + c : s1p[mt] + (c - a.c0);
+                A[mt] = ok[mt] ? *reinterpret_cast<const bf16x8*>(src + 8 * g) : zero8();
+            }
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const bf16x8 B = *reinterpret_cast<const bf16x8*>(wrow + (size_t)nt * 16 * 9 * cin + c);
+                acc[0][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[0], B, acc[0][nt], 0, 0, 0);
+                acc[1][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[1], B, acc[1][nt], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long long p = base + mt * 16 + 4 * g + i;
+            if (p >= P) continue;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = coBase + nt * 16 + r16;
+                if (co >= a.cout) continue;
+                float v = acc[mt][nt][i];
+                if (a.bias) v += a.bias[co];
+                if (a.resid) v += f(a.resid[(size_t)p * a.residC + co]);
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (a.mask && !(f(a.mask[(size_t)p * a.maskC + co]) > 0.0f)) v = 0.0f;
+                if (a.dstF) a.dstF[(size_t)p * a.dstC + co] = v;
+                else a.dstB[(size_t)p * a.dstC + co] = (bf16)v;
+            }
+        }
+    }
+}
+
+void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
+    const long long P = (long long)a.n * a.h * a.w_;
+    const int tiles = a.coutPad / 16;
+    const int nt = (tiles % 8 == 0) ? 8 : (tiles % 4 == 0) ? 4 : (tiles % 2 == 0) ? 2 : 1;
+    const dim3 grid((unsigned)((P + 127) / 128), (unsigned)(tiles / nt));
+    switch (nt) {
+        case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(kBlock), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_conv3x3<4>, grid, dim3(kBlock), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(kBlock), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(kBlock), 0, s, a); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: for one tap, dW_tap (cout x cin) = G^T (cout x P) X_tap
+// (P x cin), K = pixels.  A wave owns a 16 x 16 (co, ci) tile of one tap and
+// a range of pixels (split K); partial sums are added to dW with fp32
+// atomics.  Lane (row r, group g) reads 8 consecutive pixels [8g, 8g + 8) of
+// a 32-pixel step for its output channel (A) and for its input channel (B),
+// so both operands enumerate K identically.  Rows are a multiple of 32
+// pixels wide, so a step never crosses an image row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_wgrad3x3(WgradArgs a, int chunk) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int cin = a.c0 + a.c1;
+    const int ciTiles = cin / 16;
+    const int wz = blockIdx.z * 4 + wave;
+    if (wz >= 9 * ciTiles) return;
+    const int t = wz / ciTiles, cit = wz % ciTiles;
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const int co = blockIdx.y * 16 + r16;              // A row of this lane
+    const int ci = cit * 16 + r16;                     // B column of this lane
+    const bool fromS0 = ci < a.c0;
+    const int hh = a.h >> 1, wh = a.w_ >> 1;
+    const long long P = (long long)a.n * a.h * a.w_;
+    const long long p0 = (long long)blockIdx.x * chunk;
+    const long long p1 = p0 + chunk < P ? p0 + chunk : P;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (long long k0 = p0; k0 < p1; k0 += 32) {
+        const int x0 = (int)(k0 % a.w_);
+        const long long q = k0 / a.w_;
+        const int y = (int)(q % a.h), n = (int)(q / a.h);
+        bf16x8 A, B;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int x = x0 + 8 * g + e;
+            const long long p = k0 + 8 * g + e;
+            const bool in = p < p1;
+            A[e] = in ? a.g[(size_t)p * a.gC + co] : (bf16)0.0f;
+            const int yy = y + dy, xx = x + dx;
+            bf16 b = (bf16)0.0f;
+            if (in && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_) {
+                if (fromS0) {
+                    b = a.up0 ? a.src0[(((size_t)n * hh + (yy >> 1)) * wh + (xx >> 1)) * a.c0 + ci]
+                              : a.src0[(((size_t)n * a.h + yy) * a.w_ + xx) * a.c0 + ci];
+                } else {
+                    b = a.src1[(((size_t)n * a.h + yy) * a.w_ + xx) * a.c1 + (ci - a.c0)];
+                }
+            }
+            B[e] = b;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = blockIdx.y * 16 + 4 * g + i;
+        if (row < a.cout) atomicAdd(a.dW + ((size_t)row * 9 + t) * cin + ci, acc[i]);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_bias_grad(const bf16* g, int gC, long long P, int cout, int chunk,
+                                                      float* dB) {
+    const long long p0 = (long long)blockIdx.x * chunk;
+    const long long p1 = p0 + chunk < P ? p0 + chunk : P;
+    for (int co = threadIdx.x; co < cout; co += blockDim.x) {
+        float sum = 0.0f;
+        for (long long p = p0; p < p1; ++p) sum += f(g[(size_t)p * gC + co]);
+        atomicAdd(dB + co, sum);
+    }
+}
+
+void launch_wgrad3x3(const WgradArgs& a, hipStream_t s) {
+    const long long P = (long long)a.n * a.h * a.w_;
+    const int cin = a.c0 + a.c1;
+    const int waves = 9 * (cin / 16);
+    const int zBlocks = (waves + 3) / 4;
+    const int yBlocks = a.coutPad / 16;
+    // about 4096 blocks in all, chunks a multiple of 32 pixels
+    long long chunks = 4096 / ((long long)zBlocks * yBlocks);
+    if (chunks < 1) chunks = 1;
+    long long chunk = (P + chunks - 1) / chunks;
+    chunk = ((chunk + 31) / 32) * 32;
+    if (chunk < 32) chunk = 32;
+    const unsigned xBlocks = (unsigned)((P + chunk - 1) / chunk);
+    hipLaunchKernelGGL(k_wgrad3x3, dim3(xBlocks, yBlocks, zBlocks), dim3(kBlock), 0, s, a, (int)chunk);
+    if (a.dB) {
+        const long long bchunk = 2048;
+        hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((P + bchunk - 1) / bchunk)), dim3(128), 0, s, a.g, a.gC, P,
+                           a.cout, (int)bchunk, a.dB);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pooling, upsampling gradients
+// ---------------------------------------------------------------------------
+__global__ void k_avgpool(const bf16* src, bf16* dst, int n, int h, int w, int c) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h2 = h >> 1, w2 = w >> 1;
+    const long long total = (long long)n * h2 * w2 * c;
+    if (i >= total) return;
+    const int ch = (int)(i % c);
+    const long long q = i / c;
+    const int x = (int)(q % w2);
+    const long long r = q / w2;
+    const int y = (int)(r % h2), nn = (int)(r / h2);
+    const size_t b00 = (((size_t)nn * h + 2 * y) * w + 2 * x) * c + ch;
+    const float sum = f(src[b00]) + f(src[b00 + c]) + f(src[b00 + (size_t)w * c]) + f(src[b00 + (size_t)w * c + c]);
+    dst[i] = (bf16)(0.25f * sum);
+}
+
+void launch_avgpool(const bf16* src, bf16* dst, int n, int h, int w, int c, hipStream_t s) {
+    const long long total = (long long)n * (h / 2) * (w / 2) * c;
+    hipLaunchKernelGGL(k_avgpool, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dst, n, h, w, c);
+}
+
+__global__ void k_upsample_bwd(const bf16* g, int gC, const bf16* act, bf16* out, int n, int h, int w, int ca) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int h2 = h >> 1, w2 = w >> 1;
+    const long long total = (long long)n * h2 * w2 * ca;
+    if (i >= total) return;
+    const int ch = (int)(i % ca);
+    const long long q = i / ca;
+    const int x = (int)(q % w2);
+    const long long r = q / w2;
+    const int y = (int)(r % h2), nn = (int)(r / h2);
+    const size_t b00 = (((size_t)nn * h + 2 * y) * w + 2 * x) * gC + ch;
+    const float sum = f(g[b00]) + f(g[b00 + gC]) + f(g[b00 + (size_t)w * gC]) + f(g[b00 + (size_t)w * gC + gC]);
+    out[i] = (f(act[i]) > 0.0f) ? (bf16)sum : (bf16)0.0f;
+}
+
+void launch_upsample_bwd(const bf16* g, int gC, const bf16* act, bf16* out, int n, int h, int w, int ca,
+                         hipStream_t s) {
+    const long long total = (long long)n * (h / 2) * (w / 2) * ca;
+    hipLaunchKernelGGL(k_upsample_bwd, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, gC, act, out, n, h,
+                       w, ca);
+}
+
+__global__ void k_pool_bwd_add(const bf16* gp, const bf16* gcat, int gcatC, int off, const bf16* act, bf16* out, int n,
+                               int h, int w, int c) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)n * h * w * c;
+    if (i >= total) return;
+    const int ch = (int)(i % c);
+    const long long q = i / c;
+    const int x = (int)(q % w);
+    const long long r = q / w;
+    const int y = (int)(r % h), nn = (int)(r / h);
+    const size_t pp = (((size_t)nn * (h >> 1) + (y >> 1)) * (w >> 1) + (x >> 1)) * c + ch;
+    const float v = 0.25f * f(gp[pp]) + f(gcat[(size_t)q * gcatC + off + ch]);
+    out[i] = (f(act[i]) > 0.0f) ? (bf16)v : (bf16)0.0f;
+}
+
+void launch_pool_bwd_add(const bf16* gp, const bf16* gcat, int gcatC, int off, const bf16* act, bf16* out, int n,
+                         int h, int w, int c, hipStream_t s) {
+    const long long total = (long long)n * h * w * c;
+    hipLaunchKernelGGL(k_pool_bwd_add, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, gp, gcat, gcatC, off,
+                       act, out, n, h, w, c);
+}
+
+// ---------------------------------------------------------------------------
+// loss, optimiser, weight layouts
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_l1_loss(const float* o, const float* t, long long pixels, float invCount,
+                                                    bf16* go, int goC, float* lossAcc) {
+    __shared__ float part[kBlock / 64];
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.0f;
+    if (p < pixels) {
+        for (int c = 0; c < goC; ++c) {
+            float gv = 0.0f;
+            if (c < kOutChannels) {
+                const float d = o[(size_t)p * 3 + c] - t[(size_t)p * 3 + c];
+                l += fabsf(d);
+                gv = d > 0.0f ? invCount : (d < 0.0f ? -invCount : 0.0f);
+            }
+            go[(size_t)p * goC + c] = (bf16)gv;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) l += __shfl_down(l, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float s = 0.0f;
+        for (int i = 0; i < kBlock / 64; ++i) s += part[i];
+        atomicAdd(lossAcc, s * invCount);
+    }
+}
+
+void launch_l1_loss(const float* o, const float* t, long long pixels, float invCount, bf16* go, int goC,
+                    float* lossAcc, hipStream_t s) {
+    hipLaunchKernelGGL(k_l1_loss, dim3((unsigned)((pixels + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, o, t, pixels,
+                       invCount, go, goC, lossAcc);
+}
+
+// AdamW (decoupled weight decay), in the order w <- w (1 - lr wd);
+// w <- w - lr mhat / (sqrt(vhat) + eps).
+__global__ void k_adamw(AdamArgs a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.count) return;
+    const float gr = a.g[i];
+    const float m = a.beta1 * a.m[i] + (1.0f - a.beta1) * gr;
+    const float v = a.beta2 * a.v[i] + (1.0f - a.beta2) * gr * gr;
+    a.m[i] = m;
+    a.v[i] = v;
+    float w = a.w[i];
+    if (a.decay) w = w * (1.0f - a.lr * a.wd);
+    const float mh = m / a.bc1, vh = v / a.bc2;
+    w = w - a.lr * mh / (sqrtf(vh) + a.eps);
+    a.w[i] = w;
+}
+
+void launch_adamw(const AdamArgs& a, hipStream_t s) {
+    if (a.count <= 0) return;
+    hipLaunchKernelGGL(k_adamw, dim3((unsigned)((a.count + 255) / 256)), dim3(256), 0, s, a);
+}
+
+__global__ void k_weight_copies(const float* w, int coutPad, int cin, int coutT, bf16* w16, bf16* wt16) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)coutPad * 9 * cin;
+    if (i >= total) return;
+    const int ci = (int)(i % cin);
+    const int t = (int)((i / cin) % 9);
+    const int co = (int)(i / ((long long)9 * cin));
+    const bf16 v = (bf16)w[i];
+    w16[i] = v;
+    if (wt16 && co < coutT) wt16[((size_t)ci * 9 + (8 - t)) * coutT + co] = v;
+}
+
+void launch_weight_copies(const float* w, int coutPad, int cin, int coutT, bf16* w16, bf16* wt16, hipStream_t s) {
+    const long long total = (long long)coutPad * 9 * cin;
+    hipLaunchKernelGGL(k_weight_copies, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, w, coutPad, cin, coutT,
+                       w16, wt16);
+}
+
+// ---------------------------------------------------------------------------
+// render buffers <-> network tensors
+// ---------------------------------------------------------------------------
+__global__ void k_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
+                           bf16* x) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)wp * hp) return;
+    const int px = (int)(i % wp), py = (int)(i / wp);
+    float v[kInChannels] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (px < w && py < h) {
+        const size_t q = (size_t)py * w + px;
+        const float4 c = rgb[q];
+        v[0] = log1pf(fmaxf(c.x, 0.0f));
+        v[1] = log1pf(fmaxf(c.y, 0.0f));
+        v[2] = log1pf(fmaxf(c.z, 0.0f));
+        if (alb) {
+            const float4 a = alb[q], n = nrm[q];
+            v[3] = a.x; v[4] = a.y; v[5] = a.z;
+            v[6] = n.x; v[7] = n.y; v[8] = n.z;
+        }
+    }
+    bf16x8 out[kInPad / 8];
+#pragma unroll
+    for (int k = 0; k < kInPad / 8; ++k) out[k] = zero8();
+#pragma unroll
+    for (int c = 0; c < kInChannels; ++c) out[c / 8][c % 8] = (bf16)v[c];
+    bf16x8* dst = reinterpret_cast<bf16x8*>(x + (size_t)i * kInPad);
+#pragma unroll
+    for (int k = 0; k < kInPad / 8; ++k) dst[k] = out[k];
+}
+
+void launch_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
+                     bf16* x, hipStream_t s) {
+    const long long total = (long long)wp * hp;
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, rgb, alb, nrm, w, h, wp,
+                       hp, x);
+}
+
+__global__ void k_output(const float* o, int wp, int w, int h, float4* out, uint32_t* fb) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)w * h) return;
+    const int px = (int)(i % w), py = (int)(i / w);
+    const float* src = o + ((size_t)py * wp + px) * 3;
+    const float r = fmaxf(expm1f(src[0]), 0.0f), g = fmaxf(expm1f(src[1]), 0.0f), b = fmaxf(expm1f(src[2]), 0.0f);
+    out[i] = make_float4(r, g, b, 0.0f);
+    if (fb) fb[i] = packGamma(V3{r, g, b});
+}
+
+void launch_output(const float* o, int wp, int w, int h, float4* out, uint32_t* fb, hipStream_t s) {
+    const long long total = (long long)w * h;
+    hipLaunchKernelGGL(k_output, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, o, wp, w, h, out, fb);
+}
+
+__global__ void k_log_target(const float4* rgb, long long pixels, float* t) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pixels) return;
+    const float4 c = rgb[i];
+    t[3 * i] = log1pf(fmaxf(c.x, 0.0f));
+    t[3 * i + 1] = log1pf(fmaxf(c.y, 0.0f));
+    t[3 * i + 2] = log1pf(fmaxf(c.z, 0.0f));
+}
+
+void launch_log_target(const float4* rgb, long long pixels, float* t, hipStream_t s) {
+    hipLaunchKernelGGL(k_log_target, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, s, rgb, pixels, t);
+}
+
+__global__ void k_fill_f32(float* p, long long n, float v) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_f32(float* p, long long n, float v, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+}
+
+__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+void launch_copy_bytes(void* dst, const void* src, long long bytes, hipStream_t s) {
+    const long long n = bytes / 4;   // callers copy whole 4-byte words
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_copy_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, static_cast<uint32_t*>(dst),
+                       static_cast<const uint32_t*>(src), n);
+}
+
+}  // namespace dn
+}  // namespace bte

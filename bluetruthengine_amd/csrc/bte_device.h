@@ -707,4 +707,15 @@ BTE_DEV bool bsdfIsDiffuse(int type) {
     return type == BTE_BSDF_DIFFUSE || type == BTE_BSDF_MICROFACET || type == BTE_BSDF_DISNEY;
 }
 
+BTE_DEV uint32_t packGamma(V3 lin) {                               // rt_path_integrator.cuh:93-105
+    const float g = 1.0f / 2.2f;
+    const float dr = dpow(fmaxf(0.0f, fminf(lin.x, 1.0f)), g);
+    const float dg = dpow(fmaxf(0.0f, fminf(lin.y, 1.0f)), g);
+    const float db = dpow(fmaxf(0.0f, fminf(lin.z, 1.0f)), g);
+    const uint32_t r = (uint32_t)(uint8_t)(dr * 255.0f);
+    const uint32_t gg = (uint32_t)(uint8_t)(dg * 255.0f);
+    const uint32_t b = (uint32_t)(uint8_t)(db * 255.0f);
+    return (255u << 24) | (b << 16) | (gg << 8) | r;
+}
+
 }  // namespace bte

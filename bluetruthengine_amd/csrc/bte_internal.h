@@ -1,0 +1,85 @@
+// Host-side helpers shared by the renderer's C ABI (bte_api.cpp) and the
+// denoiser's (bte_denoise_api.cpp): error reporting, owning device buffers and
+// a view of the current render target.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/bte_api.h"
+
+namespace bte {
+
+// Records the message returned by bte_last_error() and returns code.
+int fail(int code, const char* fmt, ...);
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return ::bte::fail(BTE_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                               __FILE__, __LINE__);                                           \
+    } while (0)
+#define CHECK_RC(expr)               \
+    do {                             \
+        int rc_ = (expr);            \
+        if (rc_) return rc_;         \
+    } while (0)
+
+// Owning device allocation.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int ensure(size_t n) {
+        if (n <= bytes && p) return BTE_OK;
+        release();
+        if (n == 0) return BTE_OK;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(BTE_ERR_MEMORY, "hipMalloc(%zu) failed", n);
+        }
+        bytes = n;
+        return BTE_OK;
+    }
+    int upload(const void* src, size_t n) {
+        int rc = ensure(n);
+        if (rc) return rc;
+        if (n) HIPCHK(hipMemcpy(p, src, n, hipMemcpyHostToDevice));
+        return BTE_OK;
+    }
+    template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+// The renderer's current accumulation target.
+struct RenderView {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int width = 0, height = 0;
+    const float4* accum = nullptr;     // running mean, row 0 = bottom
+    const float4* aovAlb = nullptr;    // AOV means (null when AOVs are off)
+    const float4* aovNrm = nullptr;
+    uint32_t* fb = nullptr;            // display buffer
+    uint32_t frameIndex = 0;
+};
+int ensure_device();
+hipStream_t main_stream();             // the library's stream (after ensure_device)
+int render_view(RenderView& v);        // BTE_ERR_STATE when nothing was accumulated
+
+// Denoiser hooks called by the render entry points.
+bool denoiser_active();
+int denoiser_present();                // denoise the accumulation into the display buffer
+void denoiser_release();               // bte_cleanup
+
+}  // namespace bte

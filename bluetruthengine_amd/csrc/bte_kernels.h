@@ -75,6 +75,10 @@ struct WaveBuffers {
     int kindArea, kindSpot0, kindEnv;   // kind index of each light class (-1 absent)
     int catMask;          // bit t set: the scene has BSDF type t (its shading kernel runs)
     int grabRows;         // 64-slot rows a persistent wave takes per work-counter atomic (1, 2, 4, 8)
+    float4* aovAlb;       // optional, per path id: albedo at the first non-delta hit (0 if none)
+    float4* aovNrm;       // optional, per path id: shading normal there
+    float4* aovAccAlb;    // per pixel running means of the two AOVs (with aovAlb)
+    float4* aovAccNrm;
     const int* frameBase; // device int: frame index of the batch's first frame (set before each batch)
 };
 

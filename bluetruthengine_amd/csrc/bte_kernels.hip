@@ -22,7 +22,10 @@ constexpr int kBlock = 256;
 constexpr int kLdsStack = 32;                 // LDS stack entries per lane (megakernel, trace_rays)
 constexpr int kMinLdsStack = 8;               // smallest LDS stack of any traversal kernel variant
 constexpr int kOvfStack = kStackSize - kMinLdsStack;   // global overflow entries per thread
+// path word st.y: path id (bits 0-29) | AOV recorded (bit 30) | specular (bit 31)
 constexpr uint32_t kSpecBit = 0x80000000u;
+constexpr uint32_t kAovBit = 0x40000000u;
+constexpr uint32_t kPidMask = 0x3fffffffu;
 [[maybe_unused]] constexpr int kTraversalDone = 0x7fffffff;   // sentinel: no node left to visit
 #ifndef BTE_TRAVERSAL_WHILE_WHILE
 #define BTE_TRAVERSAL_WHILE_WHILE 0
@@ -689,17 +692,6 @@ BTE_DEV V3 primaryDir(const BteCamera& cam, int px, int py, int W, int H, float 
     return normalize3(add3(fw, add3(mul3(rt, sx), mul3(up, sy))));
 }
 
-BTE_DEV uint32_t packGamma(V3 lin) {                               // rt_path_integrator.cuh:93-105
-    const float g = 1.0f / 2.2f;
-    const float dr = dpow(fmaxf(0.0f, fminf(lin.x, 1.0f)), g);
-    const float dg = dpow(fmaxf(0.0f, fminf(lin.y, 1.0f)), g);
-    const float db = dpow(fmaxf(0.0f, fminf(lin.z, 1.0f)), g);
-    const uint32_t r = (uint32_t)(uint8_t)(dr * 255.0f);
-    const uint32_t gg = (uint32_t)(uint8_t)(dg * 255.0f);
-    const uint32_t b = (uint32_t)(uint8_t)(db * 255.0f);
-    return (255u << 24) | (b << 16) | (gg << 8) | r;
-}
-
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
@@ -724,6 +716,10 @@ __global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb
     wb.thr[0][i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     wb.rad[0][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     wb.st[0][i] = make_uint2(rng, (uint32_t)i | kSpecBit);
+    if (wb.aovAlb) {
+        wb.aovAlb[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        wb.aovNrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
     if (i == 0) wb.counters[kCtrRays] = (int)paths;
 }
 
@@ -777,7 +773,7 @@ BTE_DEV void shadeMiss(const DevScene& sc, const WaveBuffers& wb, int qi, int bo
     const int cur = bounce & 1;
     const float4 ro = wb.rayO[cur][qi], rd = wb.rayD[cur][qi], th = wb.thr[cur][qi], ra = wb.rad[cur][qi];
     const uint2 st = wb.st[cur][qi];
-    const uint32_t pid = st.y & ~kSpecBit;
+    const uint32_t pid = st.y & kPidMask;
     const bool spec = (st.y & kSpecBit) != 0;
     if (wb.hitCapture && bounce < wb.maxRec)
         wb.hitCapture[(size_t)pid * wb.maxRec + bounce] = __float_as_int(wb.hit[qi].x);
@@ -855,7 +851,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
         const bool active = li < count;
         V3 o = {0, 0, 0}, d = {0, 0, 0}, thr = {0, 0, 0}, rad = {0, 0, 0};
         float brdfPdf = 0.0f, eta = 0.0f;
-        uint32_t rng = 0, pid = 0;
+        uint32_t rng = 0, pid = 0, aovBit = 0;
         bool spec = false;
         float4 h = make_float4(0, 0, 0, 0);
         if (active) {
@@ -868,7 +864,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
             thr = xyz(th);
             rad = xyz(ra);
             rng = st.x;
-            pid = st.y & ~kSpecBit;
+            pid = st.y & kPidMask;
+            aovBit = st.y & kAovBit;
             spec = (st.y & kSpecBit) != 0;
             if (wb.hitCapture && bounce < wb.maxRec) wb.hitCapture[(size_t)pid * wb.maxRec + bounce] = __float_as_int(h.x);
         }
@@ -890,6 +887,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
                 rngNext(rng); rngNext(rng);                       // area NEE draws (always)
                 if (sc.hasEnv) { rngNext(rng); rngNext(rng); }    // env NEE draws
             }
+            if (kDiffuseType && wb.aovAlb && !aovBit) {
+                // first non-delta hit: base colour and shading normal
+                wb.aovAlb[pid] = make_float4(bsdf.p0.x, bsdf.p0.y, bsdf.p0.z, 0.0f);
+                wb.aovNrm[pid] = make_float4(its.n.x, its.n.y, its.n.z, 0.0f);
+                aovBit = kAovBit;
+            }
             frame = makeFrame(its.n);
             scatter<T>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
             survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
@@ -900,7 +903,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
             wb.rayD[nxt][slot] = make_float4(d.x, d.y, d.z, eta);
             wb.thr[nxt][slot] = make_float4(thr.x, thr.y, thr.z, 0.0f);
             wb.rad[nxt][slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
-            wb.st[nxt][slot] = make_uint2(rng, pid | (spec ? kSpecBit : 0u));
+            wb.st[nxt][slot] = make_uint2(rng, pid | aovBit | (spec ? kSpecBit : 0u));
         } else if (active) {
             wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
         }
@@ -980,6 +983,18 @@ __global__ __launch_bounds__(kBlock) void k_present(WaveBuffers wb, int pixels, 
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= pixels) return;
     const int frameBase = *wb.frameBase;
+    if (wb.aovAlb) {
+        // running means of the AOVs, same Welford update as the radiance
+        float4 ma = wb.aovAccAlb[p], mn = wb.aovAccNrm[p];
+        for (int f = 0; f < frames; ++f) {
+            const float4 a = wb.aovAlb[(size_t)f * pixels + p], nr = wb.aovNrm[(size_t)f * pixels + p];
+            const float n = (float)((uint32_t)(frameBase + f) + 1u);
+            ma.x = ma.x + (a.x - ma.x) / n; ma.y = ma.y + (a.y - ma.y) / n; ma.z = ma.z + (a.z - ma.z) / n;
+            mn.x = mn.x + (nr.x - mn.x) / n; mn.y = mn.y + (nr.y - mn.y) / n; mn.z = mn.z + (nr.z - mn.z) / n;
+        }
+        wb.aovAccAlb[p] = ma;
+        wb.aovAccNrm[p] = mn;
+    }
     float4 mean = accum[p];
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int f = 0; f < frames; ++f) {

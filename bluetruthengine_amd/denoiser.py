@@ -1,0 +1,105 @@
+"""Python mirror of the denoiser entry points of include/bte_api.h.
+
+The U-Net fills the reference's denoiser slot (src/host/denoiser.cu:72-116:
+cudaToggleDenoiser, denoiseAndPresent).  Every call goes through libbte.so;
+there is no host fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .renderer import _call, _ptr, lib
+
+WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_c4.npy")
+OUTPUT_TENSOR = 39
+
+
+def param_count():
+    return int(lib().bte_denoiser_param_count())
+
+
+def init(seed=0):
+    _call("bte_denoiser_init", int(seed))
+
+
+def set_weights(flat):
+    w = np.ascontiguousarray(flat, np.float32)
+    _call("bte_denoiser_set_weights", _ptr(w), w.size)
+
+
+def get_weights():
+    out = np.empty(param_count(), np.float32)
+    _call("bte_denoiser_get_weights", _ptr(out), out.size)
+    return out
+
+
+def get_grads():
+    out = np.empty(param_count(), np.float32)
+    _call("bte_denoiser_get_grads", _ptr(out), out.size)
+    return out
+
+
+def load(path=WEIGHTS_PATH):
+    """Loads trained weights (a float32 .npy in canonical order)."""
+    set_weights(np.load(path, allow_pickle=False))
+
+
+def save(path=WEIGHTS_PATH):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.save(path, get_weights(), allow_pickle=False)
+
+
+def forward(x9):
+    """x9 (n, h, w, 9) float32 -> network output (n, h, w, 3), log(1 + RGB)."""
+    x = np.ascontiguousarray(x9, np.float32)
+    n, h, w, _ = x.shape
+    out = np.empty((n, h, w, 3), np.float32)
+    _call("bte_denoiser_forward_host", _ptr(x), n, h, w, _ptr(out))
+    return out
+
+
+def train_step(x9, target3, lr=0.0, weight_decay=0.0):
+    x = np.ascontiguousarray(x9, np.float32)
+    t = np.ascontiguousarray(target3, np.float32)
+    n, h, w, _ = x.shape
+    loss = np.zeros(1, np.float32)
+    _call("bte_denoiser_train_step_host", _ptr(x), _ptr(t), n, h, w, float(lr), float(weight_decay), _ptr(loss))
+    return float(loss[0])
+
+
+def tensor(tid):
+    shape = np.zeros(4, np.int32)
+    _call("bte_denoiser_tensor_shape", int(tid), _ptr(shape))
+    out = np.empty(tuple(int(s) for s in shape), np.float32)
+    _call("bte_denoiser_read_tensor", int(tid), _ptr(out), out.size)
+    return out
+
+
+def dataset_reserve(slots, width, height):
+    _call("bte_denoiser_dataset_reserve", int(slots), int(width), int(height))
+
+
+def capture(slot, which):
+    _call("bte_denoiser_capture", int(slot), int(which))
+
+
+def train(steps, batch, lr=1e-3, weight_decay=1e-4, seed=0):
+    losses = np.zeros(int(steps), np.float32)
+    _call("bte_denoiser_train", int(steps), int(batch), float(lr), float(weight_decay), ctypes.c_uint(seed),
+          _ptr(losses))
+    return losses
+
+
+def denoise():
+    _call("bte_denoise")
+
+
+def toggle():
+    return _call("bte_toggle_denoiser") == 1
+
+
+def denoised(width, height):
+    out = np.empty((height, width, 3), np.float32)
+    _call("bte_read_denoised", _ptr(out), out.size)
+    return out

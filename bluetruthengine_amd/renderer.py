@@ -99,7 +99,24 @@ API = {
     "bte_read_env_cdf": [_p, _p, _p],
     "bte_trace_rays": [_p, _p, _ll, _f, _f, _i, _p, _p, _p, _p],
     "bte_autotune": [_i, _i, _i, _p],
+    "bte_enable_aov": [_i],
+    "bte_read_aov": [_p, _p, ctypes.c_size_t],
+    "bte_denoise": [],
+    "bte_read_denoised": [_p, _sz],
+    "bte_denoiser_param_count": [],
+    "bte_denoiser_init": [ctypes.c_uint],
+    "bte_denoiser_set_weights": [_p, _ll],
+    "bte_denoiser_get_weights": [_p, _ll],
+    "bte_denoiser_get_grads": [_p, _ll],
+    "bte_denoiser_dataset_reserve": [_i, _i, _i],
+    "bte_denoiser_capture": [_i, _i],
+    "bte_denoiser_train": [_i, _i, _f, _f, ctypes.c_uint, _p],
+    "bte_denoiser_forward_host": [_p, _i, _i, _i, _p],
+    "bte_denoiser_train_step_host": [_p, _p, _i, _i, _i, _f, _f, _p],
+    "bte_denoiser_tensor_shape": [_i, _p],
+    "bte_denoiser_read_tensor": [_i, _p, _ll],
 }
+RESTYPE = {"bte_last_error": ctypes.c_char_p, "bte_denoiser_param_count": _ll}
 
 _lib = None
 
@@ -114,7 +131,7 @@ def lib():
         for name, args in API.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = ctypes.c_char_p if name == "bte_last_error" else _i
+            fn.restype = RESTYPE.get(name, _i)
         _lib = L
     return _lib
 
@@ -299,6 +316,17 @@ class Renderer:
         out = np.empty((H, W, 3), np.float64)
         _call("bte_read_accum64", _ptr(out), out.size)
         return out
+
+    def enable_aov(self, on=True):
+        _call("bte_enable_aov", int(on))
+
+    def aov(self):
+        """(albedo, normal) running means, each (H, W, 3) float32, row 0 = bottom."""
+        W, H = self.scene.width, self.scene.height
+        alb = np.empty((H, W, 3), np.float32)
+        nrm = np.empty((H, W, 3), np.float32)
+        _call("bte_read_aov", _ptr(alb), _ptr(nrm), alb.size)
+        return alb, nrm
 
     def sample_radiance(self, frames):
         W, H = self.scene.width, self.scene.height

@@ -90,6 +90,19 @@ def make_camera(eye, target, up, fov_y_deg, width, height):
     return c
 
 
+def crop_camera(eye, target, up, fov_y_deg, full_w, full_h, cx, cy, size):
+    """A size x size view aimed at pixel (cx, cy) of the full_w x full_h view
+    (eye, target, up, fov_y_deg), with that window's field of view: training
+    crops keep the pixel footprint of the full-resolution frame."""
+    base = make_camera(eye, target, up, fov_y_deg, full_w, full_h)
+    th = np.tan(np.radians(fov_y_deg) / 2.0)
+    ndx = 2.0 * (cx + 0.5) / full_w - 1.0
+    ndy = 2.0 * (cy + 0.5) / full_h - 1.0
+    d = (base["forward"].astype(np.float64) + base["right"] * (ndx * full_w / full_h * th) + base["up"] * (ndy * th))
+    fov = np.degrees(2.0 * np.arctan(th * size / full_h))
+    return make_camera(eye, np.asarray(eye, np.float64) + d, up, fov, size, size)
+
+
 # ---------------------------------------------------------------------------
 # flattening (src/main.cpp:686-758)
 # ---------------------------------------------------------------------------
@@ -416,26 +429,30 @@ def random_disney_slots(rng, count):
     return out
 
 
-def terrain_c2(width=1280, height=720, spp=64, max_depth=10, freq=100):
+def terrain_c2(width=1280, height=720, spp=64, max_depth=10, freq=100, seed=None, camera=None):
+    """seed None: the fixed benchmark scene; otherwise another draw of the
+    same generator (materials, terrain noise, lights).  camera: optional
+    (eye, target, fov) override."""
+    s_mat, s_noise, s_light = (3, 2, 4) if seed is None else (1000 + 3 * seed, 1001 + 3 * seed, 1002 + 3 * seed)
     sb = SceneBuilder()
     white = sb.add_bsdf(pod.bsdf_diffuse((0.7, 0.7, 0.7)))
     sb.add_mesh(room_quads(), None, white)
-    slots = [sb.add_bsdf(b) for b in random_disney_slots(np.random.default_rng(3), 64)]
+    slots = [sb.add_bsdf(b) for b in random_disney_slots(np.random.default_rng(s_mat), 64)]
     v, f = geodesic_sphere(freq)
-    disp = value_noise3(v, seed=2)
+    disp = value_noise3(v, seed=s_noise)
     pos = v * (0.6 + 0.08 * disp)[:, None] + np.array([0.0, 0.6, 0.0])
     vn = vertex_normals(pos, f)
     p, n = mesh_arrays(pos, f, vn)
     tri_ids = np.arange(f.shape[0], dtype=np.uint32)
     slot_of = (_hash_u32(tri_ids * np.uint32(0x9E3779B9) + np.uint32(3)) % np.uint32(64)).astype(np.int32)
     sb.add_mesh(p, n, np.asarray(slots, np.int32)[slot_of])
-    rng = np.random.default_rng(4)
+    rng = np.random.default_rng(s_light)
     for _ in range(4):
         sx, sz = rng.uniform(0.2, 0.5, 2)
         cx, cz = rng.uniform(-0.7, 0.7, 2)
         le = rng.uniform(5, 20)
         sb.add_mesh(ceiling_light(cx, cz, sx, sz, 1.98), None, white, emission=(le, le, le))
-    cam = make_camera((0, 1, 3.6), (0, 1, 0), (0, 1, 0), 40.0, width, height)
+    cam = camera if camera is not None else make_camera((0, 1, 3.6), (0, 1, 0), (0, 1, 0), 40.0, width, height)
     return sb.build("c2_terrain", width, height, spp, max_depth, cam)
 
 
@@ -471,14 +488,20 @@ def sky_env(width=2048, height=1024, seed=5, sun_elev_deg=40.0, sun_azim_deg=60.
     return rgba
 
 
+C3_EYE, C3_TARGET, C3_FOV = (0.0, 2.5, 8.0), (0.0, 0.0, 0.0), 45.0
+
+
 def envlit_c3(width=1920, height=1080, spp=16, max_depth=12, n_spheres=500, slices=50, bands=21,
-              env_size=(2048, 1024)):
+              env_size=(2048, 1024), seed=None, camera=None):
+    """seed None: the fixed benchmark scene; otherwise another draw of the same
+    generators (sphere layout, materials, sky noise and sun position)."""
+    s_geo, s_mat, s_env = (6, 7, 5) if seed is None else (2000 + 3 * seed, 2001 + 3 * seed, 2002 + 3 * seed)
     sb = SceneBuilder()
     ground_b = sb.add_bsdf(pod.bsdf_diffuse((0.5, 0.5, 0.5)))
     G = 10.0
     sb.add_mesh(quad((-G, 0, -G), (-G, 0, G), (G, 0, G), (G, 0, -G)), None, ground_b)
-    rng = np.random.default_rng(6)
-    mat_rng = np.random.default_rng(7)
+    rng = np.random.default_rng(s_geo)
+    mat_rng = np.random.default_rng(s_mat)
     v, f = uv_sphere(slices, bands)
     for i in range(n_spheres):
         r = rng.uniform(0.05, 0.2)
@@ -494,8 +517,12 @@ def envlit_c3(width=1920, height=1080, spp=16, max_depth=12, n_spheres=500, slic
         bid = sb.add_bsdf(b)
         p, n = mesh_arrays(pos, f, vn)
         sb.add_mesh(p, n, bid)
-    cam = make_camera((0.0, 2.5, 8.0), (0.0, 0.0, 0.0), (0, 1, 0), 45.0, width, height)
-    env = sky_env(*env_size)
+    cam = camera if camera is not None else make_camera(C3_EYE, C3_TARGET, (0, 1, 0), C3_FOV, width, height)
+    if seed is None:
+        env = sky_env(*env_size)
+    else:
+        srng = np.random.default_rng(s_env)
+        env = sky_env(*env_size, seed=s_env, sun_elev_deg=srng.uniform(15, 75), sun_azim_deg=srng.uniform(0, 360))
     return sb.build("c3_envlit", width, height, spp, max_depth, cam, env=env, env_intensity=1.0)
 
 

@@ -161,8 +161,10 @@ int bte_render(int imageWidth, int imageHeight);
 /* nframes consecutive frames in as few wavefront passes as memory allows;
  * identical result to nframes calls of bte_render. */
 int bte_render_frames(int imageWidth, int imageHeight, int nframes);
-/* ≙ cudaToggleDenoiser (render_kernel.h:192): returns the new state (0/1) or
- * BTE_ERR_UNSUPPORTED while no denoiser is built. */
+/* ≙ cudaToggleDenoiser (render_kernel.h:192; src/host/denoiser.cu:111-116):
+ * flips the denoise-after-render switch and returns the new state (1 on,
+ * 0 off).  When on, every bte_render / bte_render_frames call ends with
+ * bte_denoise (src/render_kernel.cu:136-140).  Needs loaded weights. */
 int bte_toggle_denoiser(void);
 /* ≙ cudaSetRenderMode / cudaGetRenderMode (render_kernel.h:205,208). */
 int bte_set_render_mode(int mode);
@@ -182,6 +184,14 @@ int bte_frame_index(void);
  * (W*H*3 floats, row-major, row 0 = bottom image row as in the reference). */
 int bte_read_framebuffer(uint32_t* hostRGBA8, size_t count);
 int bte_read_accum(float* hostRGB, size_t count);
+/* Albedo and shading-normal AOVs (no counterpart in the reference; they feed
+ * the denoiser): per sample, the base colour (BsdfData p0.xyz) and the
+ * interpolated shading normal at the path's first non-delta hit (diffuse,
+ * microfacet or Disney), zero when the path ends before one; averaged per
+ * pixel with the radiance's Welford update.  Wavefront mode only.  Enabling
+ * clears the AOV means; bte_read_aov copies W*H*3 floats per non-null buffer. */
+int bte_enable_aov(int enable);
+int bte_read_aov(float* hostAlbedo, float* hostNormal, size_t count);
 /* Binary64 accumulation of plain sample sums (oracle images). */
 int bte_enable_accum64(int enable);
 int bte_read_accum64(double* hostRGB, size_t count);
@@ -196,6 +206,40 @@ int bte_set_hit_capture(int maxRecord);
 int bte_read_hit_capture(int* hostTris, size_t count);
 /* Env-map CDF arrays as built on the GPU (marginal H+1, conditional H*(W+1)). */
 int bte_read_env_cdf(float* marginal, float* conditional, float* totalPower);
+/* ---- learned denoiser (the slot of src/host/denoiser.cu:72-109) --------- */
+/* ≙ denoiseAndPresent (denoiser.cu:72-109): a U-Net (4 levels, 32/64/128/128
+ * channels, bf16 MFMA convolutions) over log(1+RGB) of the accumulation and
+ * the albedo / normal AOVs (bte_enable_aov) writes the denoised linear image
+ * (bte_read_denoised) and its gamma-packed pixels into the display buffer. */
+int bte_denoise(void);
+int bte_read_denoised(float* hostRGB, size_t count);
+/* Parameters in canonical order, layer by layer (c0a c0b c1a c1b c2a c2b c3a
+ * c3b d2a d2b d1a d1b d0a d0b out): W[cout][3][3][cin] then b[cout], fp32. */
+long long bte_denoiser_param_count(void);
+int bte_denoiser_init(unsigned seed);                 /* He-normal weights, zero biases */
+int bte_denoiser_set_weights(const float* params, long long count);
+int bte_denoiser_get_weights(float* params, long long count);
+int bte_denoiser_get_grads(float* grads, long long count);   /* of the last training step */
+/* Training on this renderer's own pairs: reserve slots of width x height
+ * (width a multiple of 256, height of 8), capture the current accumulation as
+ * a slot's input (which = 0: features incl. AOVs, e.g. after 4 spp) or target
+ * (which = 1: log(1+RGB), e.g. after 4096 spp), then run AdamW steps on random
+ * batches of complete pairs; losses (optional) receives each step's mean L1. */
+int bte_denoiser_dataset_reserve(int slots, int width, int height);
+int bte_denoiser_capture(int slot, int which);
+int bte_denoiser_train(int steps, int batch, float lr, float weightDecay, unsigned seed, float* losses);
+/* Host-array forms (tests): x9 is n*h*w*9 floats (log(1+RGB), albedo,
+ * normal; rounded to bf16 on upload), out3 / target3 are n*h*w*3 log-space
+ * values; h, w multiples of 8 (training: w a multiple of 256).  lr = 0 in a
+ * training step computes loss and gradients without updating. */
+int bte_denoiser_forward_host(const float* x9, int n, int h, int w, float* out3);
+int bte_denoiser_train_step_host(const float* x9, const float* target3, int n, int h, int w, float lr,
+                                 float weightDecay, float* loss);
+/* Intermediate tensors of the last pass as fp32 (NHWC); id 18 + k are the
+ * gradients, id 39 the fp32 output; shape4 = {n, h, w, c}. */
+int bte_denoiser_tensor_shape(int id, int* shape4);
+int bte_denoiser_read_tensor(int id, float* out, long long count);
+
 /* Launch-shape autotuner (no counterpart in the reference, whose launch
  * shape OptiX fixes).  Renders `frames` frames at width x height once per
  * candidate (traversal LDS stack 8/16/32, then work chunk 1/2/4/8 rows, then

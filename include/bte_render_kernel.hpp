@@ -88,7 +88,7 @@ inline void cudaResetAccumulation(int w, int h) { bte_compat::check(bte_reset_ac
 inline void cudaRender(int w, int h) { bte_compat::check(bte_render(w, h), "cudaRender"); }
 inline bool cudaToggleDenoiser() {
     const int rc = bte_toggle_denoiser();
-    return rc > 0;   // no denoiser in this build: stays off
+    return rc > 0;   // stays off until denoiser weights are loaded
 }
 inline void cudaSetRenderMode(RenderMode mode) {
     bte_compat::check(bte_set_render_mode(mode == RenderMode::Wavefront ? BTE_MODE_WAVEFRONT : BTE_MODE_MEGAKERNEL),

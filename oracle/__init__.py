@@ -72,8 +72,9 @@ def lib():
         L.ora_dir_to_equirect.argtypes = [_p, _f, _p]
         L.ora_sample_emitter.argtypes = [_p, _p, _f, _f, _p]
         L.ora_render.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p, _p, _i]
-        L.ora_trace_samples.argtypes = [_p, _p, _l, _i, _i, _i, _i, _p, _p, _i]
+        L.ora_trace_samples.argtypes = [_p, _p, _l, _i, _i, _i, _i, _p, _p, _i, _p]
         L.ora_render_megakernel.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p]
+        L.ora_render_aov.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
         L.ora_pack_gamma.restype = _u32
         L.ora_pack_gamma.argtypes = [_f, _f, _f]
         L.ora_num_threads.restype = _i
@@ -272,17 +273,25 @@ class OracleScene:
                          _ptr(out8), threads)
         return (accum, out8) if rgba8 else accum
 
+    def render_aov(self, W, H, frame0, nframes, max_depth):
+        """Albedo / normal running means (H, W, 3) of the first non-delta hit."""
+        alb = np.zeros((H, W, 3), np.float32)
+        nrm = np.zeros((H, W, 3), np.float32)
+        lib().ora_render_aov(self.handle, W, H, frame0, nframes, max_depth, _ptr(alb), _ptr(nrm))
+        return alb, nrm
+
     def render_megakernel(self, W, H, frame0, nframes, spp, max_depth, accum=None):
         if accum is None:
             accum = np.zeros((H, W, 3), np.float32)
         lib().ora_render_megakernel(self.handle, W, H, frame0, nframes, spp, max_depth, _ptr(accum))
         return accum
 
-    def trace_samples(self, pixels, W, H, frame, max_depth, max_rec=0):
+    def trace_samples(self, pixels, W, H, frame, max_depth, max_rec=0, aov=False):
         pixels = np.ascontiguousarray(pixels, np.int32)
         n = pixels.size
         rad = np.empty((n, 3), np.float32)
         tris = np.empty((n, max_rec), np.int32) if max_rec else None
+        aov6 = np.empty((n, 6), np.float32) if aov else None
         lib().ora_trace_samples(self.handle, _ptr(pixels), n, W, H, frame, max_depth, _ptr(rad),
-                                _ptr(tris), max_rec)
-        return rad, tris
+                                _ptr(tris), max_rec, _ptr(aov6))
+        return (rad, tris, aov6) if aov else (rad, tris)

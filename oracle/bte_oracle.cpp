@@ -1221,6 +1221,8 @@ static inline bool russianRoulette(PathState& ps, Rng& rng) {       // :54-65
 struct SampleTrace {   // optional per-bounce record for parity debugging
     int maxRec;
     int* tris;         // [maxRec] hit triangle per bounce (-1 miss, -2 not reached)
+    float* aov;        // optional [6]: albedo xyz and shading normal xyz at the first
+                       // non-delta hit (zeros when the path ends before one)
 };
 
 static inline Its makeIts(const Scene& s, const PathState& ps, const Hit& h) {
@@ -1263,6 +1265,8 @@ static F3 tracePath(const Scene& s, int px, int py, int W, int H, Rng& rng, int 
     ps.brdfPDF = 0.0f;
     ps.specular = true;
     if (dbg) for (int i = 0; i < dbg->maxRec; ++i) dbg->tris[i] = -2;
+    bool aovDone = !(dbg && dbg->aov);
+    if (!aovDone) for (int i = 0; i < 6; ++i) dbg->aov[i] = 0.0f;
     while (ps.bounce < maxDepth) {
         const Hit h = closestHit(s, ps.o, ps.d, RT_EPSILON, 1e30f);
         if (dbg && ps.bounce < dbg->maxRec) dbg->tris[ps.bounce] = h.tri;
@@ -1274,6 +1278,11 @@ static F3 tracePath(const Scene& s, int px, int py, int W, int H, Rng& rng, int 
         const OraBsdf& bsdf = s.d.bsdfs[s.d.triangleBsdfIds[its.tri]];
         if (s.d.triangleEmitterFlags && s.d.triangleEmitterFlags[its.tri] != 0) accumulateEmitterHit(s, ps, its);
         if (bsdfIsDiffuse(bsdf)) {
+            if (!aovDone) {
+                const float a[6] = {bsdf.p0.x, bsdf.p0.y, bsdf.p0.z, its.n.x, its.n.y, its.n.z};
+                for (int i = 0; i < 6; ++i) dbg->aov[i] = a[i];
+                aovDone = true;
+            }
             ps.specular = false;
             Shadow sh;
             if (prepareAreaNEE(s, ps, rng, its, bsdf, sh) && !occluded(s, sh.o, sh.d, sh.tMax))
@@ -1541,16 +1550,42 @@ void ora_render(void* p, int W, int H, int frame0, int nframes, int maxDepth, fl
     }
 }
 
-// One sample of selected pixels: radiance (3 floats each) and hit triangles of
-// the first maxRec bounces (-1 miss, -2 not reached).
+// AOV running means (albedo, shading normal of the first non-delta hit) of
+// frames [frame0, frame0+nframes), with the Welford update of ora_render.
+void ora_render_aov(void* p, int W, int H, int frame0, int nframes, int maxDepth, float* alb3, float* nrm3) {
+    const Scene& s = *static_cast<Scene*>(p);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int py = 0; py < H; ++py) {
+        for (int px = 0; px < W; ++px) {
+            const size_t pix = (size_t)py * W + px;
+            float m[6] = {alb3[3 * pix], alb3[3 * pix + 1], alb3[3 * pix + 2],
+                          nrm3[3 * pix], nrm3[3 * pix + 1], nrm3[3 * pix + 2]};
+            for (int f = frame0; f < frame0 + nframes; ++f) {
+                float a[6];
+                SampleTrace tr{0, nullptr, a};
+                (void)traceSample(s, px, py, W, H, (uint32_t)f, maxDepth, &tr);
+                const float nn = (float)((uint32_t)f + 1u);
+                for (int i = 0; i < 6; ++i) m[i] = m[i] + (a[i] - m[i]) / nn;
+            }
+            for (int i = 0; i < 3; ++i) {
+                alb3[3 * pix + i] = m[i];
+                nrm3[3 * pix + i] = m[3 + i];
+            }
+        }
+    }
+}
+
+// One sample of selected pixels: radiance (3 floats each), hit triangles of
+// the first maxRec bounces (-1 miss, -2 not reached) and the 6 AOV floats
+// (optional outputs).
 void ora_trace_samples(void* p, const int* pixels, long n, int W, int H, int frame, int maxDepth, float* rad3,
-                       int* tris, int maxRec) {
+                       int* tris, int maxRec, float* aov6) {
     const Scene& s = *static_cast<Scene*>(p);
 #pragma omp parallel for schedule(dynamic, 64)
     for (long i = 0; i < n; ++i) {
         const int px = pixels[i] % W, py = pixels[i] / W;
-        SampleTrace tr{maxRec, tris ? tris + i * maxRec : nullptr};
-        const F3 r = traceSample(s, px, py, W, H, (uint32_t)frame, maxDepth, tris ? &tr : nullptr);
+        SampleTrace tr{tris ? maxRec : 0, tris ? tris + i * maxRec : nullptr, aov6 ? aov6 + 6 * i : nullptr};
+        const F3 r = traceSample(s, px, py, W, H, (uint32_t)frame, maxDepth, (tris || aov6) ? &tr : nullptr);
         rad3[3 * i] = r.x; rad3[3 * i + 1] = r.y; rad3[3 * i + 2] = r.z;
     }
 }

@@ -1,0 +1,215 @@
+"""GPU parity of the learned denoiser (bf16 MFMA kernels) against the NumPy
+oracle (oracle/denoiser.py, itself pinned to torch on the CPU).
+
+Bars:
+* every kernel in isolation (GPU inputs fed to the oracle op): results equal
+  up to one bfloat16 rounding step, i.e. |gpu - ref| <= 2^-7 |ref| + 2^-7 *
+  (1/256 of the tensor's largest magnitude) elementwise, and at least 97 % of
+  the elements identical.  The GPU sums in float32 in MFMA order, the oracle in
+  float64, so a sum close to a rounding boundary may land one bf16 step apart;
+* weight gradients (float32 atomics) within 1e-3 of the largest entry;
+* AdamW: float32 results equal to the oracle's float32 update (same op order);
+* whole network end to end: max |gpu - ref| <= 2e-2 of the output range.
+"""
+import numpy as np
+import pytest
+
+from oracle import denoiser as od
+from bluetruthengine_amd import denoiser as dn
+from bluetruthengine_amd import renderer, scenes
+
+pytestmark = pytest.mark.gpu
+
+FWD = {name: i for i, name in enumerate(od.FORWARD)}
+GRD = {name: len(od.FORWARD) + i for i, name in enumerate(od.GRADS)}
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _params(seed):
+    rs = np.random.default_rng(seed + 100)
+    p = od.init_params(seed, scale_out=1.0)
+    for name in p:
+        p[name] = (p[name][0], rs.normal(0, 0.05, p[name][1].shape))
+    flat = od.join_params(p).astype(np.float32)
+    return od.split_params(flat.astype(np.float64)), flat
+
+
+def _close_bf16(gpu, ref, what, min_exact=0.97):
+    gpu = np.asarray(gpu, np.float64)
+    ref = np.asarray(ref, np.float64)
+    assert gpu.shape == ref.shape, (what, gpu.shape, ref.shape)
+    big = np.abs(ref).max() + 1e-30
+    tol = 2.0 ** -7 * np.abs(ref) + 2.0 ** -7 * big / 256
+    bad = np.abs(gpu - ref) > tol
+    exact = np.mean(gpu == ref)
+    _say(f"{what}: exact {exact:.4f}, max rel-to-max err {np.abs(gpu - ref).max() / big:.3g}")
+    assert not bad.any(), f"{what}: {bad.sum()} elements beyond one bf16 step"
+    assert exact >= min_exact, f"{what}: only {exact:.3f} identical"
+
+
+def _forward_case(seed, n, h, w):
+    params, flat = _params(seed)
+    dn.set_weights(flat)
+    rs = np.random.default_rng(seed)
+    x9 = rs.normal(0, 1, (n, h, w, 9)).astype(np.float32)
+    return params, x9
+
+
+def test_forward_kernels_match_oracle():
+    _say("case: U-Net forward, n=2 h=16 w=24, seed 0")
+    renderer.cudaCleanup()
+    params, x9 = _forward_case(0, 2, 16, 24)
+    out = dn.forward(x9)
+    net = od.Net(params, bf16=True)
+    g = {name: dn.tensor(FWD[name]).astype(np.float64) for name in od.FORWARD}
+    assert np.array_equal(g["x"][..., :9], od.bf16_round(x9).astype(np.float64))
+    assert np.all(g["x"][..., 9:] == 0)
+    x = g["x"][..., :9]
+    # each layer from the GPU's own inputs
+    _close_bf16(g["e0a"], net.conv("c0a", x, True), "c0a")
+    _close_bf16(g["s0"], net.conv("c0b", g["e0a"], True), "c0b")
+    assert np.array_equal(g["p0"], net.pool(g["s0"]))
+    _close_bf16(g["e1a"], net.conv("c1a", g["p0"], True), "c1a")
+    _close_bf16(g["s1"], net.conv("c1b", g["e1a"], True), "c1b")
+    assert np.array_equal(g["p1"], net.pool(g["s1"]))
+    _close_bf16(g["e2a"], net.conv("c2a", g["p1"], True), "c2a")
+    _close_bf16(g["s2"], net.conv("c2b", g["e2a"], True), "c2b")
+    assert np.array_equal(g["p2"], net.pool(g["s2"]))
+    _close_bf16(g["e3a"], net.conv("c3a", g["p2"], True), "c3a")
+    _close_bf16(g["b3"], net.conv("c3b", g["e3a"], True), "c3b")
+    _close_bf16(g["d2a"], net.conv("d2a", np.concatenate([net.up(g["b3"]), g["s2"]], -1), True), "d2a (up+concat)")
+    _close_bf16(g["d2b"], net.conv("d2b", g["d2a"], True), "d2b")
+    _close_bf16(g["d1a"], net.conv("d1a", np.concatenate([net.up(g["d2b"]), g["s1"]], -1), True), "d1a (up+concat)")
+    _close_bf16(g["d1b"], net.conv("d1b", g["d1a"], True), "d1b")
+    _close_bf16(g["d0a"], net.conv("d0a", np.concatenate([net.up(g["d1b"]), g["s0"]], -1), True), "d0a (up+concat)")
+    _close_bf16(g["d0b"], net.conv("d0b", g["d0a"], True), "d0b")
+    o_ref = net.conv("out", g["d0b"], False, resid=x[..., :3])
+    err = np.abs(out - o_ref).max() / (np.abs(o_ref).max() + 1e-30)
+    _say(f"out layer (fp32): max rel-to-max err {err:.3g}")
+    assert err < 1e-5
+    # end to end from the input alone
+    full = net.forward(x9)["o"]
+    e2e = np.abs(out - full).max() / (np.abs(full).max() + 1e-30)
+    _say(f"end to end: max rel-to-max err {e2e:.3g}")
+    assert e2e < 2e-2
+
+
+def test_backward_kernels_match_oracle():
+    _say("case: U-Net backward (lr 0), n=1 h=32 w=256, seed 1")
+    renderer.cudaCleanup()
+    params, x9 = _forward_case(1, 1, 32, 256)
+    target = np.random.default_rng(7).normal(0, 1, (1, 32, 256, 3)).astype(np.float32)
+    loss = dn.train_step(x9, target, lr=0.0)
+    net = od.Net(params, bf16=True)
+    t = {name: dn.tensor(FWD[name]).astype(np.float64) for name in od.FORWARD}
+    t["x"] = t["x"][..., :9]
+    t["o"] = dn.tensor(dn.OUTPUT_TENSOR).astype(np.float64)
+    ref_loss = np.abs(t["o"] - target).mean()
+    _say(f"loss gpu {loss:.6f} oracle {ref_loss:.6f}")
+    assert abs(loss - ref_loss) <= 1e-5 * max(1.0, ref_loss)
+    g = {name: dn.tensor(GRD[name]).astype(np.float64) for name in od.GRADS}
+    inv = np.float32(1) / np.float32(t["o"].size)
+    go_ref = od.bf16_round((np.sign(t["o"] - target) * inv).astype(np.float32))
+    assert np.array_equal(g["go"][..., :3], go_ref) and np.all(g["go"][..., 3:] == 0)
+    go = g["go"][..., :3]
+    _close_bf16(g["gd0b"], net._dgrad("out", go, t["d0b"]), "dgrad out")
+    _close_bf16(g["gd0a"], net._dgrad("d0b", g["gd0b"], t["d0a"]), "dgrad d0b")
+    _close_bf16(g["gcat0"], net._dgrad("d0a", g["gd0a"]), "dgrad d0a")
+    up_bwd = lambda gc, act: od.bf16_round(np.where(act > 0, net._up_bwd(gc.astype(np.float32)), 0).astype(np.float32))
+    assert np.array_equal(g["gd1b"], up_bwd(g["gcat0"][..., :64], t["d1b"]))
+    _close_bf16(g["gd1a"], net._dgrad("d1b", g["gd1b"], t["d1a"]), "dgrad d1b")
+    _close_bf16(g["gcat1"], net._dgrad("d1a", g["gd1a"]), "dgrad d1a")
+    assert np.array_equal(g["gd2b"], up_bwd(g["gcat1"][..., :128], t["d2b"]))
+    _close_bf16(g["gcat2"], net._dgrad("d2a", g["gd2a"]), "dgrad d2a")
+    _close_bf16(g["ge3a"], net._dgrad("c3b", g["gb3"], t["e3a"]), "dgrad c3b")
+    _close_bf16(g["gp2"], net._dgrad("c3a", g["ge3a"]), "dgrad c3a")
+    pool_bwd = lambda gp, gc, act: od.bf16_round(np.where(
+        act > 0, np.float32(0.25) * net.up(gp).astype(np.float32) + gc.astype(np.float32), 0).astype(np.float32))
+    assert np.array_equal(g["gs2"], pool_bwd(g["gp2"], g["gcat2"][..., 128:], t["s2"]))
+    assert np.array_equal(g["gs1"], pool_bwd(g["gp1"], g["gcat1"][..., 128:], t["s1"]))
+    assert np.array_equal(g["gs0"], pool_bwd(g["gp0"], g["gcat0"][..., 64:], t["s0"]))
+    _close_bf16(g["ge0a"], net._dgrad("c0b", g["gs0"], t["e0a"]), "dgrad c0b")
+    # weight gradients from the GPU's tensors
+    grads = od.split_params(dn.get_grads().astype(np.float64))
+    pairs = {
+        "out": (go, t["d0b"]), "d0b": (g["gd0b"], t["d0a"]),
+        "d0a": (g["gd0a"], np.concatenate([net.up(t["d1b"]), t["s0"]], -1)),
+        "d1b": (g["gd1b"], t["d1a"]), "d1a": (g["gd1a"], np.concatenate([net.up(t["d2b"]), t["s1"]], -1)),
+        "d2b": (g["gd2b"], t["d2a"]), "d2a": (g["gd2a"], np.concatenate([net.up(t["b3"]), t["s2"]], -1)),
+        "c3b": (g["gb3"], t["e3a"]), "c3a": (g["ge3a"], t["p2"]), "c2b": (g["gs2"], t["e2a"]),
+        "c2a": (g["ge2a"], t["p1"]), "c1b": (g["gs1"], t["e1a"]), "c1a": (g["ge1a"], t["p0"]),
+        "c0b": (g["gs0"], t["e0a"]), "c0a": (g["ge0a"], t["x"]),
+    }
+    for name, (gout, xin) in pairs.items():
+        dw, db = net._wgrad(gout, xin)
+        gw, gb = grads[name]
+        scale = np.abs(dw).max() + 1e-30
+        err = np.abs(gw - dw).max() / scale
+        errb = np.abs(gb - db).max() / (np.abs(db).max() + 1e-30)
+        _say(f"wgrad {name}: max rel-to-max err {err:.3g}, bias {errb:.3g}")
+        assert err < 1e-3 and errb < 1e-3, name
+
+
+def test_adamw_step_matches_oracle():
+    _say("case: AdamW step lr 1e-3 wd 1e-2 after one backward, n=1 h=16 w=256")
+    renderer.cudaCleanup()
+    params, x9 = _forward_case(2, 1, 16, 256)
+    w0 = dn.get_weights()
+    target = np.random.default_rng(8).normal(0, 1, (1, 16, 256, 3)).astype(np.float32)
+    dn.train_step(x9, target, lr=1e-3, weight_decay=1e-2)
+    g = od.split_params(dn.get_grads().astype(np.float64))
+    w1 = od.split_params(dn.get_weights().astype(np.float64))
+    p0 = od.split_params(w0.astype(np.float64))
+    for name in od.NAMES:
+        for k, decay in ((0, True), (1, False)):
+            z = np.zeros_like(p0[name][k], np.float32)
+            want, _, _ = od.adamw(p0[name][k].astype(np.float32), g[name][k].astype(np.float32), z, z, 1, 1e-3, 1e-2,
+                                  decay)
+            assert np.array_equal(w1[name][k].astype(np.float32), want), (name, k)
+
+
+def test_render_then_denoise_runs_and_matches_oracle():
+    _say("case: Cornell 64x64, 4 frames with AOVs, random weights, bte_denoise and the toggle")
+    sd = scenes.cornell_c1(width=64, height=64, spp=4, max_depth=8)
+    r = renderer.Renderer(sd)
+    r.enable_aov(True)
+    params, flat = _params(3)
+    dn.set_weights(flat)
+    r.render(4)
+    dn.denoise()
+    out = dn.denoised(64, 64)
+    assert np.all(np.isfinite(out)) and np.all(out >= 0)
+    alb, nrm = r.aov()
+    acc = r.accum()
+    x9 = np.concatenate([np.log1p(np.maximum(acc, 0)), alb, nrm], -1)[None]
+    ref = np.maximum(np.expm1(od.Net(params, bf16=True).forward(x9)["o"][0]), 0)
+    err = np.abs(np.log1p(out) - np.log1p(ref)).max()
+    _say(f"denoised vs oracle: max log-space err {err:.3g}")
+    assert err < 2e-2
+    assert dn.toggle() is True
+    r.render(1)                      # render + denoise-after-render
+    out2 = dn.denoised(64, 64)
+    assert np.all(np.isfinite(out2)) and not np.array_equal(out, out2)
+    assert dn.toggle() is False
+
+
+def test_dataset_capture_and_training_reduce_loss():
+    _say("case: 2 training pairs from Cornell 256x32 (2 and 64 frames), 30 AdamW steps, batch 2")
+    sd = scenes.cornell_c1(width=256, height=32, spp=2, max_depth=8)
+    r = renderer.Renderer(sd)
+    r.enable_aov(True)
+    dn.init(5)
+    dn.dataset_reserve(2, 256, 32)
+    for slot in range(2):
+        r.reset()
+        r.render(2)
+        dn.capture(slot, 0)
+        r.render(62)
+        dn.capture(slot, 1)
+    losses = dn.train(30, 2, lr=1e-3, weight_decay=1e-4, seed=1)
+    _say(f"losses: first {losses[:3]}, last {losses[-3:]}")
+    assert np.all(np.isfinite(losses))
+    assert losses[-5:].mean() < losses[:5].mean()

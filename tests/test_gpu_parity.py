@@ -214,3 +214,19 @@ def test_error_paths_are_reported():
         renderer.cudaRender(16, 16)
     with pytest.raises(renderer.BteError):
         renderer.cudaToggleDenoiser()
+
+
+def test_aov_means_match_oracle():
+    for sd, what in ((scenes.small_envlit(), "reduced env-lit 128x72"),
+                     (scenes.cornell_c1(width=64, height=64, spp=3, max_depth=8), "Cornell 64x64")):
+        _say(f"case: albedo/normal AOVs, {what}, 3 frames")
+        r = renderer.Renderer(sd)
+        r.enable_aov(True)
+        r.render(3)
+        alb, nrm = r.aov()
+        env_cdf = r.env_cdf() if sd.env_rgba is not None else None
+        ralb, rnrm = oracle.OracleScene(sd, env_cdf=env_cdf).render_aov(sd.width, sd.height, 0, 3, sd.max_depth)
+        assert np.array_equal(alb, ralb) and np.array_equal(nrm, rnrm)
+        plain = renderer.Renderer(sd)
+        plain.render(3)
+        assert np.array_equal(r.accum(), plain.accum())     # AOVs leave the radiance untouched
