@@ -28,8 +28,8 @@ sys.path.insert(0, ROOT)
 METRIC = "Msamples/s at 1080p; time to relMSE≤0.01 (4spp+denoiser vs plain spp)"
 HBM_PEAK_GBPS = 8000.0
 # Algorithmic bytes of one extend ray: path-state read (origin 16 B + direction
-# 16 B) + hit record write 16 B; per BVH2 node visited 64 B; per triangle tested
-# 48 B (three float4 vertices).
+# 16 B) + hit record write 16 B; per BVH2 node visited 64 B (BVH4: 128 B); per
+# triangle tested 48 B (three float4 vertices).
 EXTEND_RAY_BYTES = 48
 NODE_BYTES = 64
 TRI_BYTES = 48
@@ -59,6 +59,172 @@ def cpu_baseline(sd, seconds):
                       f"{dt2:.1f} s on {threads} OpenMP threads ({os.cpu_count()} logical CPUs visible)"}
 
 
+def relmse(x, ref):
+    """mean over pixels and RGB of (x - r)^2 / (r^2 + 1e-2)"""
+    x = np.asarray(x, np.float64)
+    return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
+
+
+def denoiser_bench(args):
+    """Inference time of the U-Net on the 1080p frame (4 spp + AOVs) and the
+    time of one AdamW training step (batch 8 of 256x256), with MFMA rooflines."""
+    from bluetruthengine_amd import denoiser, renderer, scenes
+    sd = scenes.envlit_c3(spp=4)
+    r = renderer.Renderer(sd)
+    r.enable_aov(True)
+    r.build_accel()
+    r.render(4)
+    if os.path.exists(denoiser.WEIGHTS_PATH):
+        denoiser.load()
+    else:
+        denoiser.init(0)
+    for _ in range(2):
+        denoiser.denoise()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        denoiser.denoise()
+        times.append(time.perf_counter() - t0)
+    inf_ms = float(np.median(times)) * 1e3
+    hp, wp = (sd.height + 7) // 8 * 8, (sd.width + 7) // 8 * 8
+    fwd_flops = denoiser.flops_per_pixel() * hp * wp
+    # training step: batch 8 of 256x256 captured pairs
+    tsd = scenes.small_envlit(width=256, height=256)
+    t = renderer.Renderer(tsd)
+    t.enable_aov(True)
+    t.render(4)
+    denoiser.dataset_reserve(8, 256, 256)
+    for i in range(8):
+        denoiser.capture(i, 0)
+        denoiser.capture(i, 1)
+    denoiser.train(3, 8)
+    t0 = time.perf_counter()
+    n_train = 20
+    denoiser.train(n_train, 8)
+    train_ms = (time.perf_counter() - t0) / n_train * 1e3
+    train_flops = 3.0 * denoiser.flops_per_pixel() * 8 * 256 * 256   # forward + input grads + weight grads
+    peak = 2500.0
+    out = {"metric": "denoiser", "inference_ms_1080p": round(inf_ms, 3), "frame": f"{sd.width}x{sd.height}",
+           "forward_tflop": round(fwd_flops / 1e12, 4),
+           "roofline": {"bound": "mfma", "achieved": round(fwd_flops / (inf_ms * 1e-3) / 1e12, 2), "peak": peak,
+                        "unit": "TFLOP/s", "frac": round(fwd_flops / (inf_ms * 1e-3) / 1e12 / peak, 4),
+                        "kernel": "U-Net forward (15 bf16 MFMA convs + pooling, features, output)"},
+           "train_step_ms_b8_256": round(train_ms, 3),
+           "train_tflops": round(train_flops / (train_ms * 1e-3) / 1e12, 2), "dtype": "bf16 (fp32 accumulate)"}
+    print(json.dumps(out), flush=True)
+
+
+def relmse_bench(args):
+    """Time to relMSE <= 0.01 on the 1080p env-lit frame: plain path tracing
+    (smallest spp that reaches it) against 4 spp + the learned denoiser.  The
+    reference is the same estimator at args.ref_spp samples per pixel summed in
+    binary64, from frames 2^24 on (independent of the test renders' frames)."""
+    from bluetruthengine_amd import denoiser, renderer, scenes
+    sd = scenes.envlit_c3(spp=16)
+    W, H = sd.width, sd.height
+    r = renderer.Renderer(sd)
+    r.build_accel()
+    r.autotune(16)
+    r.enable_accum64(True)
+    r.set_frame_index(1 << 24)          # reference samples independent of the test renders (frames 0..)
+    t0 = time.perf_counter()
+    chunk = 256
+    done = 0
+    while done < args.ref_spp:
+        n = min(chunk, args.ref_spp - done)
+        r.render(n)
+        done += n
+        if done % 4096 == 0 or done == args.ref_spp:
+            log(f"reference: {done}/{args.ref_spp} spp, {time.perf_counter() - t0:.0f} s")
+    ref = r.accum64() / float(args.ref_spp)
+    r.enable_accum64(False)
+
+    # One progressive pass (a render of n spp equals the first n frames of a
+    # longer one): relMSE at powers of two until it reaches 0.01, then a second
+    # pass re-accumulates to the last power of two below and checks every 1/64
+    # of the final interval.
+    curve = {}
+    r.reset()
+    spp, done = 1, 0
+    while True:
+        r.render(spp - done)
+        done = spp
+        curve[spp] = relmse(r.accum(), ref)
+        log(f"plain {spp} spp: relMSE {curve[spp]:.5f}")
+        if curve[spp] <= 0.01 or spp >= args.ref_spp // 4:
+            break
+        spp *= 2
+    plain_spp = spp
+    if spp > 1 and curve[spp] <= 0.01:
+        lo = spp // 2
+        step = max(1, (spp - lo) // 64)
+        r.reset()
+        r.render(lo)
+        done = lo
+        while done + step < spp:
+            r.render(step)
+            done += step
+            curve[done] = relmse(r.accum(), ref)
+            log(f"plain {done} spp: relMSE {curve[done]:.5f}")
+            if curve[done] <= 0.01:
+                plain_spp = done
+                break
+        log(f"plain: {plain_spp} spp reach relMSE {curve[plain_spp]:.5f}")
+
+    def timed_plain():
+        r.reset()
+        t = time.perf_counter()
+        r.render(plain_spp)
+        return time.perf_counter() - t
+    timed_plain()
+    plain_ms = min(timed_plain() for _ in range(2)) * 1e3
+
+    r.enable_aov(True)
+    if os.path.exists(denoiser.WEIGHTS_PATH):
+        denoiser.load()
+        trained = True
+    else:
+        denoiser.init(0)
+        trained = False
+
+    def timed_dn(spp):
+        r.reset()
+        t = time.perf_counter()
+        r.render(spp)
+        denoiser.denoise()
+        return time.perf_counter() - t
+
+    # 4 spp + denoiser (the metric), then the denoised curve at higher spp
+    dn_curve, noisy_curve = {}, {}
+    r.reset()
+    done = 0
+    for spp in (4, 8, 16, 32, 64):
+        r.render(spp - done)
+        done = spp
+        denoiser.denoise()
+        dn_curve[spp] = relmse(denoiser.denoised(W, H), ref)
+        noisy_curve[spp] = relmse(r.accum(), ref)
+        log(f"denoised {spp} spp: relMSE {dn_curve[spp]:.5f} (input {noisy_curve[spp]:.5f})")
+    timed_dn(4)
+    dn_ms = min(timed_dn(4) for _ in range(3)) * 1e3
+    reach = [k for k in sorted(dn_curve) if dn_curve[k] <= 0.01]
+    dn_reach_ms = min(timed_dn(reach[0]) for _ in range(2)) * 1e3 if reach else None
+    out = {"metric": "time to relMSE<=0.01, 1080p env-lit", "reference_spp": args.ref_spp,
+           "plain": {"spp": plain_spp, "relmse": round(curve[plain_spp], 5), "ms": round(plain_ms, 2)},
+           "denoised_4spp": {"relmse": round(dn_curve[4], 5), "ms": round(dn_ms, 2),
+                             "input_relmse": round(noisy_curve[4], 5),
+                             "weights": "trained" if trained else "untrained (random init)"},
+           "speedup_4spp": round(plain_ms / dn_ms, 2) if dn_curve[4] <= 0.01 else None,
+           "denoised_min_spp_reaching": reach[0] if reach else None,
+           "denoised_reach_ms": round(dn_reach_ms, 2) if dn_reach_ms else None,
+           "speedup_at_reach": round(plain_ms / dn_reach_ms, 2) if dn_reach_ms else None,
+           "target_speedup": 5.0,
+           "plain_curve": {str(k): round(v, 5) for k, v in sorted(curve.items())},
+           "denoised_curve": {str(k): round(v, 5) for k, v in sorted(dn_curve.items())},
+           "input_curve": {str(k): round(v, 5) for k, v in sorted(noisy_curve.items())}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,10 +237,15 @@ def main():
     ap.add_argument("--traversal-stack", type=int, default=0, help="LDS stack entries per lane (8/16/32)")
     ap.add_argument("--work-chunk", type=int, default=0, help="64-slot rows per work grab (1/2/4/8)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident traversal blocks per CU (0 = max)")
+    ap.add_argument("--bvh-width", type=int, default=0, help="2 or 4: BVH traversed by extend/connect")
     ap.add_argument("--no-graphs", action="store_true", help="plain launches instead of captured HIP graphs")
     ap.add_argument("--no-autotune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) to calibrate rocprof byte counters")
+    ap.add_argument("--denoiser", action="store_true", help="only time the U-Net denoiser (inference, training)")
+    ap.add_argument("--relmse", action="store_true",
+                    help="only measure time to relMSE<=0.01: plain spp vs 4 spp + denoiser")
+    ap.add_argument("--ref-spp", type=int, default=65536, help="samples per pixel of the --relmse reference")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch
@@ -86,6 +257,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "repeats": 3}), flush=True)
         return
+    if args.denoiser:
+        return denoiser_bench(args)
+    if args.relmse:
+        return relmse_bench(args)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "the path shards by sample index across GPUs with one final reduce, which is not run here",
@@ -98,7 +273,8 @@ def main():
     log(f"scene: {sd.triangles.shape[0]} triangles, {sd.bsdfs.shape[0]} bsdfs, generated in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     r = renderer.Renderer(sd, traversal_stack=args.traversal_stack, graphs=0 if args.no_graphs else 1,
-                          work_chunk=args.work_chunk, traversal_blocks_per_cu=args.blocks_per_cu)
+                          work_chunk=args.work_chunk, traversal_blocks_per_cu=args.blocks_per_cu,
+                          bvh_width=args.bvh_width)
     r.build_accel()
     log(f"upload + env CDF + BVH build: {time.perf_counter() - t0:.1f} s")
     W, H, spp = sd.width, sd.height, sd.spp
@@ -140,7 +316,8 @@ def main():
         r.set_instrumentation(0)
         launches = max(1, s1.extendLaunches)
         avg_ms = s1.extendMs / launches
-        bytes_total = EXTEND_RAY_BYTES * s2.extensionRays + NODE_BYTES * s2.extendNodes + TRI_BYTES * s2.extendTris
+        node_bytes = NODE_BYTES * (2 if cfg["bvhWidth"] == 4 else 1)
+        bytes_total = EXTEND_RAY_BYTES * s2.extensionRays + node_bytes * s2.extendNodes + TRI_BYTES * s2.extendTris
         bytes_per_launch = bytes_total / launches
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         traffic = None
@@ -152,7 +329,7 @@ def main():
                         traffic = int(line.split(":")[1])
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                    "kernel": "k_extend (BVH2 closest hit)", "launches": launches,
+                    "kernel": f"k_extend (BVH{cfg['bvhWidth']} closest hit)", "launches": launches,
                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
         kernel = {"extend_ms": round(s1.extendMs, 2), "shade_ms": round(s1.shadeMs, 2),
                   "connect_ms": round(s1.connectMs, 2), "nodes_per_ray": round(s2.extendNodes / max(1, s2.extensionRays), 2),
@@ -173,6 +350,7 @@ def main():
                    "width": W, "height": H, "spp": spp, "max_depth": sd.max_depth,
                    "frames_in_flight": st["framesInFlight"], "traversal_stack": cfg["traversalStack"],
                    "work_chunk_rows": cfg["workChunk"], "traversal_blocks_per_cu": cfg["traversalBlocksPerCU"],
+                   "bvh_width": cfg["bvhWidth"],
                    "hip_graphs": bool(cfg["graphs"]), "autotuned": not args.no_autotune},
         "mrays_per_s": round(rays * args.steps / elapsed / 1e6, 1) if rays else None,
         "rays_per_step": {"extension": st["extensionRays"], "shadow": st["shadowRays"], "hits": st["hits"]},

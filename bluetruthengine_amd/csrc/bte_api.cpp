@@ -61,7 +61,8 @@ struct Context {
     // device scene
     DBuf dTris, dMaterials, dTriMat, dTriBsdf, dBsdfs, dEmission, dFlags, dEmitterOfTri;
     DBuf dEmitters, dEmTriIdx, dEmTriCdf, dSceneCdf, dSpots;
-    DBuf dNodes, dLeafTris;
+    DBuf dNodes, dNodes4, dLeafTris;
+    int bvh4Depth = 0;
     DBuf dEnv, dEnvMarg, dEnvCond, dEnvRowSums, dEnvTotal, dEnvMargGuide, dEnvCondGuide, dEnvMono;
     bool envGuides = false;
     bool haveEmission = false, haveFlags = false;
@@ -100,7 +101,7 @@ struct Context {
     long long lastPassPaths = 0;
     int lastPassFrames = 0;
 
-    BteConfig cfg{200, 5, 0, 32768, 16, 1, 4, 0};
+    BteConfig cfg{200, 5, 0, 32768, 16, 1, 4, 0, 2};
     BteStats stats{};
     int instr = 0;
     DBuf dTrav;
@@ -159,6 +160,10 @@ int ensureAccel() {
     BvhBuild b = build_bvh2(c.tris.data(), (int)c.tris.size());
     (void)t0;
     CHECK_RC(c.dNodes.upload(b.nodes.data(), b.nodes.size() * sizeof(BvhHostNode)));
+    {
+        const std::vector<Bvh4HostNode> n4 = collapse_bvh4(b.nodes, c.bvh4Depth);
+        CHECK_RC(c.dNodes4.upload(n4.data(), n4.size() * sizeof(Bvh4HostNode)));
+    }
     std::vector<float> lt = std::move(b.leafTris);
     if (lt.empty()) lt.assign(12, 0.0f);
     CHECK_RC(c.dLeafTris.upload(lt.data(), lt.size() * sizeof(float)));
@@ -173,6 +178,7 @@ DevScene makeScene() {
     DevScene s;
     std::memset(&s, 0, sizeof s);   // padding too: scenes are compared bytewise as graph keys
     s.nodes = c.dNodes.as<float4>();
+    s.nodes4 = c.dNodes4.as<float4>();
     s.leafTris = c.dLeafTris.as<float4>();
     s.tris = c.dTris.as<BteTriangle>();
     s.triBsdf = c.dTriBsdf.as<int>();
@@ -292,7 +298,7 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
         CHECK_RC(c.dAovAlb.ensure(P * sizeof(float4)));
         CHECK_RC(c.dAovNrm.ensure(P * sizeof(float4)));
     }
-    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4);
+    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4, 2);
     CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     if (c.hitCaptureRec > 0) CHECK_RC(c.dHitCap.ensure(P * (size_t)c.hitCaptureRec * sizeof(int)));
     c.cap = cap;
@@ -426,7 +432,7 @@ int renderWavefront(int nframes) {
     wb.kindEnv = kindEnv;
     const DevScene sc = makeScene();
     const LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16,
-                                              c.cfg.traversalBlocksPerCU, c.cfg.workChunk);
+                                              c.cfg.traversalBlocksPerCU, c.cfg.workChunk, c.cfg.bvhWidth);
     wb.grabRows = ls.grabRows;
     // instrumented runs record events between launches: those stay uncaptured
     const bool useGraph = c.cfg.graphs != 0 && c.instr == 0;
@@ -537,7 +543,7 @@ int renderWavefront(int nframes) {
 int renderMegakernel(int nframes) {
     Context& c = ctx();
     const DevScene sc = makeScene();
-    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4);
+    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4, 2);
     CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
     double* sum64 = c.accum64 ? c.dSum64.as<double>() : nullptr;
@@ -833,7 +839,7 @@ int bte_cleanup(void) {
     c.dAovAlb.release(); c.dAovNrm.release(); c.dAovAccAlb.release(); c.dAovAccNrm.release();
     c.aov = false;
     DBuf* bufs[] = {&c.dTris, &c.dMaterials, &c.dTriMat, &c.dTriBsdf, &c.dBsdfs, &c.dEmission, &c.dFlags,
-                    &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes,
+                    &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes, &c.dNodes4,
                     &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dEnvMargGuide,
                     &c.dEnvCondGuide, &c.dEnvMono, &c.dAccum,
                     &c.dFb, &c.dSum64, &c.dHit, &c.dCatQ, &c.dCat, &c.dFinal, &c.dShO, &c.dShD, &c.dShC, &c.dCounters, &c.dHitCap,
@@ -869,6 +875,8 @@ int bte_set_config(const BteConfig* config) {
         return fail(BTE_ERR_INVALID, "workChunk must be 0, 1, 2, 4 or 8");
     if (config->traversalBlocksPerCU < 0 || config->traversalBlocksPerCU > 64)
         return fail(BTE_ERR_INVALID, "traversalBlocksPerCU must be in [0, 64]");
+    if (config->bvhWidth != 0 && config->bvhWidth != 2 && config->bvhWidth != 4)
+        return fail(BTE_ERR_INVALID, "bvhWidth must be 0, 2 or 4");
     ctx().cfg = *config;
     return BTE_OK;
 }
@@ -898,6 +906,7 @@ int bte_autotune(int width, int height, int frames, double* msBest) {
     BteConfig best = saved;
     if (best.traversalStack == 0) best.traversalStack = 16;
     if (best.workChunk == 0) best.workChunk = 4;
+    if (best.bvhWidth == 0) best.bvhWidth = 2;
     double bestMs = 0.0;
     int rc = timeOne(best, bestMs);
     auto tryCand = [&](BteConfig cand) {
@@ -907,6 +916,11 @@ int bte_autotune(int width, int height, int frames, double* msBest) {
             best = cand;
         }
     };
+    {
+        BteConfig cand = best;
+        cand.bvhWidth = best.bvhWidth == 4 ? 2 : 4;
+        tryCand(cand);
+    }
     for (int st : {8, 16, 32}) {
         BteConfig cand = best;
         cand.traversalStack = st;
@@ -920,7 +934,8 @@ int bte_autotune(int width, int height, int frames, double* msBest) {
     }
     int cus = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus <= 0) cus = 1;
-    const int occ = std::max(1, query_launch_shape(c.device, best.traversalStack, 0, best.workChunk).extendBlocks / cus);
+    const int occ = std::max(
+        1, query_launch_shape(c.device, best.traversalStack, 0, best.workChunk, best.bvhWidth).extendBlocks / cus);
     const BteConfig afterChunk = best;
     for (int bpc : {0, (3 * occ) / 4, occ / 2}) {
         BteConfig cand = afterChunk;
@@ -944,6 +959,11 @@ int bte_get_stats(BteStats* stats) {
     return BTE_OK;
 }
 int bte_frame_index(void) { return (int)ctx().frameIndex; }
+int bte_set_frame_index(int frameIndex) {
+    if (frameIndex < 0) return fail(BTE_ERR_INVALID, "frame index must be non-negative");
+    ctx().frameIndex = (uint32_t)frameIndex;
+    return BTE_OK;
+}
 int bte_set_instrumentation(int flags) {
     if (flags < 0 || flags > 3) return fail(BTE_ERR_INVALID, "unknown instrumentation flags %d", flags);
     ctx().instr = flags;
@@ -1094,7 +1114,7 @@ int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, floa
     if (c.tris.empty()) return fail(BTE_ERR_STATE, "no triangles uploaded");
     CHECK_RC(ensureAccel());
     if (c.accW <= 0) { c.accW = std::max(1, c.sceneW); c.accH = std::max(1, c.sceneH); }
-    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4);
+    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4, 2);
     CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     DBuf dr, dtm, dtri, dt, du, dv;
     CHECK_RC(dr.upload(rays, (size_t)n * 6 * sizeof(float)));

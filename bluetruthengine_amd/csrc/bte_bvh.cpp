@@ -207,4 +207,79 @@ BvhBuild build_bvh2(const BteTriangle* tris, int n) {
     return out;
 }
 
+namespace {
+
+struct Child2 {
+    int ref;
+    float lo[3], hi[3];
+    float area() const {
+        const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        return (dx < 0.0f || dy < 0.0f || dz < 0.0f) ? -1.0f : 2.0f * (dx * dy + dy * dz + dz * dx);
+    }
+};
+
+Child2 childOf(const BvhHostNode& n, int k) {
+    Child2 c;
+    c.ref = n.child[k];
+    if (k == 0) {
+        c.lo[0] = n.n[0]; c.hi[0] = n.n[1]; c.lo[1] = n.n[2]; c.hi[1] = n.n[3]; c.lo[2] = n.n[8]; c.hi[2] = n.n[9];
+    } else {
+        c.lo[0] = n.n[4]; c.hi[0] = n.n[5]; c.lo[1] = n.n[6]; c.hi[1] = n.n[7]; c.lo[2] = n.n[10]; c.hi[2] = n.n[11];
+    }
+    return c;
+}
+
+struct Collapser {
+    const std::vector<BvhHostNode>& n2;
+    std::vector<Bvh4HostNode> out;
+    int maxDepth = 0;
+
+    // BVH4 node for BVH2 inner node `i`; returns its index
+    int make(int i, int depth) {
+        maxDepth = std::max(maxDepth, depth);
+        Child2 ch[4];
+        int cnt = 0;
+        const BvhHostNode& root = n2[(size_t)i];
+        ch[cnt++] = childOf(root, 0);
+        ch[cnt++] = childOf(root, 1);
+        if (root.child[0] == root.child[1]) cnt = 1;   // single-leaf root (second box is the far point)
+        while (cnt < 4) {
+            int pick = -1;
+            float bestArea = -2.0f;
+            for (int k = 0; k < cnt; ++k)
+                if (ch[k].ref >= 0 && ch[k].area() > bestArea) { bestArea = ch[k].area(); pick = k; }
+            if (pick < 0) break;
+            const BvhHostNode& inner = n2[(size_t)ch[pick].ref];
+            ch[pick] = childOf(inner, 0);
+            ch[cnt++] = childOf(inner, 1);
+        }
+        const int idx = (int)out.size();
+        out.push_back(Bvh4HostNode{});
+        int refs[4];
+        for (int k = 0; k < 4; ++k) refs[k] = 0;
+        for (int k = 0; k < cnt; ++k) refs[k] = ch[k].ref >= 0 ? make(ch[k].ref, depth + 1) : ch[k].ref;
+        Bvh4HostNode& nd = out[(size_t)idx];
+        for (int k = 0; k < 4; ++k) {
+            const bool used = k < cnt;
+            for (int a = 0; a < 3; ++a) {
+                nd.box[2 * a][k] = used ? ch[k].lo[a] : -3.0e38f;
+                nd.box[2 * a + 1][k] = used ? ch[k].hi[a] : -3.0e38f;
+            }
+            nd.child[k] = refs[k];
+            nd.pad[k] = 0;
+        }
+        return idx;
+    }
+};
+
+}  // namespace
+
+std::vector<Bvh4HostNode> collapse_bvh4(const std::vector<BvhHostNode>& nodes2, int& depth4) {
+    Collapser c{nodes2, {}, 0};
+    c.out.reserve(nodes2.size() / 2 + 1);
+    c.make(0, 1);
+    depth4 = c.maxDepth;
+    return std::move(c.out);
+}
+
 }  // namespace bte

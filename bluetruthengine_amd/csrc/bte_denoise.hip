@@ -176,14 +176,24 @@ __global__ __launch_bounds__(kBlock) void k_wgrad3x3(WgradArgs a, int chunk) {
     }
 }
 
+// dB[co] += sum_p g[p][co]: thread t reads channel t % gC of pixels
+// t / gC, t / gC + 256 / gC, ... (consecutive threads read consecutive
+// channels), then the block folds its rows in LDS and adds one value per
+// channel.  gC divides 256.
 __global__ __launch_bounds__(kBlock) void k_bias_grad(const bf16* g, int gC, long long P, int cout, int chunk,
                                                       float* dB) {
+    __shared__ float part[kBlock];
+    const int c = threadIdx.x % gC, row = threadIdx.x / gC, rows = kBlock / gC;
     const long long p0 = (long long)blockIdx.x * chunk;
     const long long p1 = p0 + chunk < P ? p0 + chunk : P;
-    for (int co = threadIdx.x; co < cout; co += blockDim.x) {
-        float sum = 0.0f;
-        for (long long p = p0; p < p1; ++p) sum += f(g[(size_t)p * gC + co]);
-        atomicAdd(dB + co, sum);
+    float sum = 0.0f;
+    for (long long p = p0 + row; p < p1; p += rows) sum += f(g[(size_t)p * gC + c]);
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x < gC && threadIdx.x < cout) {
+        float s = 0.0f;
+        for (int r = 0; r < rows; ++r) s += part[r * gC + threadIdx.x];
+        atomicAdd(dB + threadIdx.x, s);
     }
 }
 
@@ -202,8 +212,8 @@ void launch_wgrad3x3(const WgradArgs& a, hipStream_t s) {
     const unsigned xBlocks = (unsigned)((P + chunk - 1) / chunk);
     hipLaunchKernelGGL(k_wgrad3x3, dim3(xBlocks, yBlocks, zBlocks), dim3(kBlock), 0, s, a, (int)chunk);
     if (a.dB) {
-        const long long bchunk = 2048;
-        hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((P + bchunk - 1) / bchunk)), dim3(128), 0, s, a.g, a.gC, P,
+        const long long bchunk = 256;
+        hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((P + bchunk - 1) / bchunk)), dim3(kBlock), 0, s, a.g, a.gC, P,
                            a.cout, (int)bchunk, a.dB);
     }
 }

@@ -20,7 +20,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "bte_denoise.h"
@@ -408,21 +410,18 @@ int denoiser_present() {
     return BTE_OK;
 }
 
+// bte_cleanup: frees the per-resolution state and switches the denoiser off
+// (src/host/denoiser.cu:121-129); the model and the training pairs stay, so
+// training can move from scene to scene (bte_denoiser_release frees them).
 void denoiser_release() {
     Net& N = net();
     for (DBuf& b : N.t) b.release();
     N.o.release(); N.target.release(); N.loss.release();
-    N.dsX.release(); N.dsT.release();
     N.denoised.release();
-    N.slots = 0;
-    N.dsHave.clear();
+    N.outW = N.outH = 0;
     N.enabled = false;
     N.n = N.h = N.wdt = 0;
     N.train = false;
-    // the model goes too (src/host/denoiser.cu:121-129 frees the denoiser)
-    N.w.release(); N.g.release(); N.m.release(); N.v.release(); N.w16.release(); N.wt16.release();
-    N.ready = false;
-    N.step = 0;
 }
 
 }  // namespace bte
@@ -436,6 +435,18 @@ int bte_toggle_denoiser(void) {
     if (!N.ready) return fail(BTE_ERR_STATE, "denoiser weights not loaded (bte_denoiser_init / set_weights)");
     N.enabled = !N.enabled;
     return N.enabled ? 1 : 0;
+}
+
+int bte_denoiser_release(void) {
+    Net& N = net();
+    denoiser_release();
+    N.dsX.release(); N.dsT.release();
+    N.slots = 0;
+    N.dsHave.clear();
+    N.w.release(); N.g.release(); N.m.release(); N.v.release(); N.w16.release(); N.wt16.release();
+    N.ready = false;
+    N.step = 0;
+    return BTE_OK;
 }
 
 int bte_denoise(void) {
@@ -499,6 +510,41 @@ int bte_denoiser_set_weights(const float* params, long long count) {
     std::vector<float> padded;
     canonicalToPadded(N, params, padded);
     return uploadParams(N, padded, s);
+}
+
+// Weights file: a NumPy .npy array of little-endian float32 in canonical order.
+int bte_denoiser_load_file(const char* path) {
+    if (!path) return fail(BTE_ERR_INVALID, "null path");
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return fail(BTE_ERR_INVALID, "cannot open %s", path);
+    unsigned char head[10];
+    std::vector<float> w;
+    int rc = BTE_OK;
+    if (std::fread(head, 1, 10, fp) != 10 || std::memcmp(head, "\x93NUMPY", 6) != 0) {
+        rc = fail(BTE_ERR_INVALID, "%s is not a .npy file", path);
+    } else {
+        size_t hlen = 0;
+        if (head[6] == 1) {
+            hlen = (size_t)head[8] | ((size_t)head[9] << 8);
+        } else {
+            unsigned char ext[2];
+            if (std::fread(ext, 1, 2, fp) != 2) hlen = 0;
+            else hlen = (size_t)head[8] | ((size_t)head[9] << 8) | ((size_t)ext[0] << 16) | ((size_t)ext[1] << 24);
+        }
+        std::string hdr(hlen, '\0');
+        if (hlen == 0 || std::fread(&hdr[0], 1, hlen, fp) != hlen || hdr.find("'<f4'") == std::string::npos ||
+            hdr.find("'fortran_order': False") == std::string::npos) {
+            rc = fail(BTE_ERR_INVALID, "%s: expected a C-ordered little-endian float32 array", path);
+        } else {
+            const long long n = canonicalCount();
+            w.resize((size_t)n);
+            if (std::fread(w.data(), sizeof(float), (size_t)n, fp) != (size_t)n || std::fgetc(fp) != EOF)
+                rc = fail(BTE_ERR_INVALID, "%s: expected exactly %lld floats", path, n);
+        }
+    }
+    std::fclose(fp);
+    if (rc != BTE_OK) return rc;
+    return bte_denoiser_set_weights(w.data(), (long long)w.size());
 }
 
 static int readCanonical(const DBuf& b, float* out, long long count) {

@@ -11,6 +11,7 @@ namespace bte {
 // Read-only scene view passed by value to every kernel.
 struct DevScene {
     const float4* nodes;          // BVH2, 4 float4 per node
+    const float4* nodes4;         // BVH4, 8 float4 per node (null when not built)
     const float4* leafTris;       // 3 float4 per leaf-ordered triangle (v0.w = triangle id bits)
     const BteTriangle* tris;      // original order, 96 B each
     const int* triBsdf;
@@ -86,6 +87,7 @@ struct LaunchShape {
     int extendBlocks, shadeBlocks, connectBlocks, auxBlocks;
     int ldsStack;         // traversal LDS stack entries per lane: 8, 16 or 32
     int grabRows;         // work chunk of the persistent kernels, in 64-slot rows
+    int bvhWidth;         // 2 or 4: BVH traversed by extend / connect
 };
 
 // Launchers (all asynchronous on `stream`).
@@ -105,7 +107,7 @@ void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxP
 void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
                     int* margGuide, int* condGuide, int* monotone, hipStream_t s);
 // traversalBlocksPerCU 0 = as many as occupancy allows; grabRows 1, 2, 4 or 8 (else 4).
-LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows);
+LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows, int bvhWidth);
 int persistent_threads(const LaunchShape& ls);
 int overflow_stack_ints(const LaunchShape& ls);   // size of the overflow-stack buffer (ints)
 

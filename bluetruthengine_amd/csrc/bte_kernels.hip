@@ -269,6 +269,83 @@ BTE_DEV int traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, con
 #endif
 }
 
+BTE_DEV void sortPair(float& ka, int& ca, float& kb, int& cb) {
+    if (kb < ka) {
+        const float tk = ka; ka = kb; kb = tk;
+        const int tc = ca; ca = cb; cb = tc;
+    }
+}
+
+// BVH4 traversal (same contract as traverse): the hit children of a node are
+// ordered by entry distance; the nearest is visited next, the others pushed
+// farthest first.
+template <bool ANY, bool COUNT = false, int L = kLdsStack>
+BTE_DEV int traverse4(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, const StackRef<L>& stk, float& tOut,
+                      float& uOut, float& vOut, uint32_t* nodeCount = nullptr, uint32_t* triCount = nullptr) {
+    const WtRay r = makeWtRay(o, d);
+    if (!r.valid || sc.triCount == 0) return ANY ? 0 : -1;
+    constexpr float kNone = __builtin_huge_valf();
+    float best = tmax;
+    int bestTri = -1;
+    int node = 0;
+    int sp = 0;
+    for (;;) {
+        if (node >= 0) {
+            if (COUNT) ++*nodeCount;
+            const float4* np = sc.nodes4 + 8 * (size_t)node;
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+            float f0, f1, f2, f3;
+            const float e0 = boxEnter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, tmin, best, f0);
+            const float e1 = boxEnter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, tmin, best, f1);
+            const float e2 = boxEnter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, tmin, best, f2);
+            const float e3 = boxEnter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, tmin, best, f3);
+            float k0 = e0 <= f0 ? e0 : kNone, k1 = e1 <= f1 ? e1 : kNone;
+            float k2 = e2 <= f2 ? e2 : kNone, k3 = e3 <= f3 ? e3 : kNone;
+            int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+            sortPair(k0, c0, k1, c1);
+            sortPair(k2, c2, k3, c3);
+            sortPair(k0, c0, k2, c2);
+            sortPair(k1, c1, k3, c3);
+            sortPair(k1, c1, k2, c2);
+            if (k0 != kNone) {
+                if (k3 != kNone) push(stk, sp, c3);
+                if (k2 != kNone) push(stk, sp, c2);
+                if (k1 != kNone) push(stk, sp, c1);
+                node = c0;
+                continue;
+            }
+        } else {
+            const int code = ~node;
+            const int start = code >> kLeafCountBits;
+            const int cnt = (code & (kMaxLeafTris - 1)) + 1;
+            for (int i = 0; i < cnt; ++i) {
+                const float4* tp = sc.leafTris + 3 * (size_t)(start + i);
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                if (COUNT) ++*triCount;
+                float t, u, v;
+                if (hitTriangle(r, a, b, c, tmin, best, t, u, v)) {
+                    if (ANY) return 1;
+                    const int id = __float_as_int(a.w);
+                    if (t < best || bestTri < 0 || id < bestTri) {
+                        best = t; bestTri = id; tOut = t; uOut = u; vOut = v;
+                    }
+                }
+            }
+        }
+        if (sp == 0) break;
+        node = pop(stk, sp);
+    }
+    return ANY ? 0 : bestTri;
+}
+
+template <int W, bool ANY, bool COUNT, int L>
+BTE_DEV int traverseW(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, const StackRef<L>& stk, float& t,
+                      float& u, float& v, uint32_t* nodeCount, uint32_t* triCount) {
+    if constexpr (W == 4) return traverse4<ANY, COUNT, L>(sc, o, d, tmin, tmax, stk, t, u, v, nodeCount, triCount);
+    else return traverse<ANY, COUNT, L>(sc, o, d, tmin, tmax, stk, t, u, v, nodeCount, triCount);
+}
+
 // ---------------------------------------------------------------------------
 // Scene helpers
 // ---------------------------------------------------------------------------
@@ -732,7 +809,7 @@ BTE_DEV int shadeCategory(const DevScene& sc, int tri) {
 // Closest hit of every queued ray (src/wavefront/ray_extend.cu:49-72).  Waves
 // take 64 consecutive queue slots per atomic: consecutive slots hold rays that
 // were adjacent before compaction, so a wave's lanes share BVH cache lines.
-template <bool COUNT, int L>
+template <bool COUNT, int L, int W>
 __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, int bounce) {
     __shared__ int lds[L * kBlock];
     const int cur = bounce & 1;
@@ -750,7 +827,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
             const float4 o = wb.rayO[cur][qi];
             const float4 d = wb.rayD[cur][qi];
             float t = 0.0f, u = 0.0f, v = 0.0f;
-            const int tri = traverse<false, COUNT, L>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, &nodes, &tris);
+            const int tri = traverseW<W, false, COUNT, L>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, &nodes, &tris);
             wb.hit[qi] = make_float4(__int_as_float(tri), t, u, v);
             wb.cat[qi] = (uint8_t)shadeCategory(sc, tri);
         }
@@ -939,7 +1016,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
 
 // Any-hit visibility of one light class's shadow rays; the unoccluded ones add
 // their contribution to the destination path (src/wavefront/ray_connect.cu:52-72).
-template <bool COUNT, int L>
+template <bool COUNT, int L, int W>
 __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb, int bounce, int kind) {
     __shared__ int lds[L * kBlock];
     const int nxt = (bounce & 1) ^ 1;
@@ -958,7 +1035,7 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
             const float4 so = wb.shO[off + si];
             const float4 sd = wb.shD[off + si];
             float t, u, v;
-            const int occ = traverse<true, COUNT, L>(sc, xyz(so), xyz(sd), RT_EPS, so.w, stk, t, u, v, &nodes, &tris);
+            const int occ = traverseW<W, true, COUNT, L>(sc, xyz(so), xyz(sd), RT_EPS, so.w, stk, t, u, v, &nodes, &tris);
             if (!occ) {
                 const float4 c = wb.shC[off + si];
                 const uint32_t dest = __float_as_uint(sd.w);
@@ -1209,13 +1286,13 @@ void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths,
     hipLaunchKernelGGL(k_generate, dim3(blocksFor(paths, kBlock)), dim3(kBlock), 0, s, sc, wb, paths, pixels);
 }
 
-template <int L>
+template <int L, int W>
 static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth,
                             const LaunchShape& ls, hipStream_t s, hipEvent_t* ev) {
     const bool count = wb.travCtr != nullptr;
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (count) hipLaunchKernelGGL((k_extend<true, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
-    else hipLaunchKernelGGL((k_extend<false, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    if (count) hipLaunchKernelGGL((k_extend<true, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    else hipLaunchKernelGGL((k_extend<false, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
     if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(k_categorize, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
     // a BSDF category is shaded only when the scene uses that type
@@ -1237,19 +1314,29 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int k = 0; k < wb.kinds; ++k) {
         if (count)
-            hipLaunchKernelGGL((k_connect<true, L>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+            hipLaunchKernelGGL((k_connect<true, L, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
         else
-            hipLaunchKernelGGL((k_connect<false, L>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+            hipLaunchKernelGGL((k_connect<false, L, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
     }
     if (ev) (void)hipEventRecord(ev[3], s);
 }
 
 void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
                    hipStream_t s, hipEvent_t* ev) {
+    const bool w4 = ls.bvhWidth == 4 && sc.nodes4 != nullptr;
     switch (ls.ldsStack) {
-        case 8: launch_bounce_l<8>(sc, wb, bounce, maxDepth, ls, s, ev); break;
-        case 16: launch_bounce_l<16>(sc, wb, bounce, maxDepth, ls, s, ev); break;
-        default: launch_bounce_l<32>(sc, wb, bounce, maxDepth, ls, s, ev); break;
+        case 8:
+            if (w4) launch_bounce_l<8, 4>(sc, wb, bounce, maxDepth, ls, s, ev);
+            else launch_bounce_l<8, 2>(sc, wb, bounce, maxDepth, ls, s, ev);
+            break;
+        case 16:
+            if (w4) launch_bounce_l<16, 4>(sc, wb, bounce, maxDepth, ls, s, ev);
+            else launch_bounce_l<16, 2>(sc, wb, bounce, maxDepth, ls, s, ev);
+            break;
+        default:
+            if (w4) launch_bounce_l<32, 4>(sc, wb, bounce, maxDepth, ls, s, ev);
+            else launch_bounce_l<32, 2>(sc, wb, bounce, maxDepth, ls, s, ev);
+            break;
     }
 }
 
@@ -1268,7 +1355,7 @@ void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex
     launch_fill_i32(work, 1, 0, s);
     int dev = 0;
     (void)hipGetDevice(&dev);
-    const LaunchShape ls = query_launch_shape(dev, 32, 0, 4);
+    const LaunchShape ls = query_launch_shape(dev, 32, 0, 4, 2);
     hipLaunchKernelGGL(k_megakernel, dim3(ls.auxBlocks), dim3(kBlock), 0, s, sc, spp, maxDepth, frameIndex, accum,
                        sum64, fb, ovf, work);
 }
@@ -1280,7 +1367,7 @@ void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxP
     launch_fill_i32(work, 1, 0, s);
     int dev = 0;
     (void)hipGetDevice(&dev);
-    const LaunchShape ls = query_launch_shape(dev, 32, 0, 4);
+    const LaunchShape ls = query_launch_shape(dev, 32, 0, 4, 2);
     hipLaunchKernelGGL(k_trace_rays, dim3(ls.auxBlocks), dim3(kBlock), 0, s, sc, rays, tmaxPerRay, n, tmin, tmax,
                        anyHit, tri, t, u, v, ovf, work);
 }
@@ -1295,13 +1382,15 @@ void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, d
 
 template <int L>
 static int travOccupancy() {
-    int a = 0, b = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false, L>, kBlock, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_connect<false, L>, kBlock, 0);
-    return std::max(1, std::min(a, b));
+    int a = 0, b = 0, c = 0, d = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false, L, 2>, kBlock, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_connect<false, L, 2>, kBlock, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_extend<false, L, 4>, kBlock, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_connect<false, L, 4>, kBlock, 0);
+    return std::max(1, std::min(std::min(a, b), std::min(c, d)));
 }
 
-LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows) {
+LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows, int bvhWidth) {
     static int cachedDev = -1;
     static int cus = 1, occS = 1, occMT = 1, occL[3] = {1, 1, 1};
     if (device != cachedDev) {
@@ -1325,6 +1414,7 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     int occ = occL[ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
     if (traversalBlocksPerCU > 0) occ = std::min(occ, traversalBlocksPerCU);
     ls.grabRows = (grabRows == 1 || grabRows == 2 || grabRows == 8) ? grabRows : 4;
+    ls.bvhWidth = bvhWidth == 4 ? 4 : 2;
     ls.extendBlocks = cus * occ;
     ls.connectBlocks = cus * occ;
     ls.shadeBlocks = cus * occS;

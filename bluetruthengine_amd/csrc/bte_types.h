@@ -16,8 +16,10 @@ namespace bte {
 // where [start, start+count) indexes the leaf-ordered triangle array.
 constexpr int kLeafCountBits = 3;
 constexpr int kMaxLeafTris = 1 << kLeafCountBits;
-constexpr int kMaxBvhDepth = 60;      // builder guarantee; traversal stack holds kStackSize entries
-constexpr int kStackSize = 64;
+constexpr int kMaxBvhDepth = 60;      // builder guarantee
+// Traversal stack entries: a BVH2 visit pushes at most 1 entry per level, a
+// BVH4 visit at most 3, and BVH4 depth <= BVH2 depth.
+constexpr int kStackSize = 3 * kMaxBvhDepth + 4;
 
 struct BvhHostNode {
     float n[12];
@@ -25,6 +27,17 @@ struct BvhHostNode {
     int pad[2];
 };
 static_assert(sizeof(BvhHostNode) == 64, "BVH node must be 64 B");
+
+// Device BVH4 node, 128 B = 8 x float4, child boxes in SoA:
+//   q0 = lo.x of children 0..3, q1 = hi.x, q2 = lo.y, q3 = hi.y, q4 = lo.z, q5 = hi.z,
+//   q6 = child refs (as in BVH2; empty slots have a far-away point box), q7 = 0.
+// Collapsed from the SAH BVH2 (same leaves and triangle order).
+struct Bvh4HostNode {
+    float box[6][4];
+    int child[4];
+    int pad[4];
+};
+static_assert(sizeof(Bvh4HostNode) == 128, "BVH4 node must be 128 B");
 
 // Shadow-slot kinds, traced in this order so contributions add in the
 // reference's order (area, spotlights, environment).

@@ -13,6 +13,16 @@ from .renderer import _call, _ptr, lib
 
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "unet_c4.npy")
 OUTPUT_TENSOR = 39
+# (name, input channels, output channels, level) of the 15 convolutions
+LAYERS = [("c0a", 9, 32, 0), ("c0b", 32, 32, 0), ("c1a", 32, 64, 1), ("c1b", 64, 64, 1), ("c2a", 64, 128, 2),
+          ("c2b", 128, 128, 2), ("c3a", 128, 128, 3), ("c3b", 128, 128, 3), ("d2a", 256, 128, 2),
+          ("d2b", 128, 128, 2), ("d1a", 192, 64, 1), ("d1b", 64, 64, 1), ("d0a", 96, 32, 0), ("d0b", 32, 32, 0),
+          ("out", 32, 3, 0)]
+
+
+def flops_per_pixel():
+    """Multiply-adds x 2 of one forward pass per full-resolution pixel."""
+    return sum(2.0 * 9 * ci * co / 4 ** lvl for _, ci, co, lvl in LAYERS)
 
 
 def param_count():
@@ -21,6 +31,10 @@ def param_count():
 
 def init(seed=0):
     _call("bte_denoiser_init", int(seed))
+
+
+def release():
+    _call("bte_denoiser_release")
 
 
 def set_weights(flat):
@@ -41,8 +55,8 @@ def get_grads():
 
 
 def load(path=WEIGHTS_PATH):
-    """Loads trained weights (a float32 .npy in canonical order)."""
-    set_weights(np.load(path, allow_pickle=False))
+    """Loads trained weights (a float32 .npy in canonical order) through the library's own reader."""
+    _call("bte_denoiser_load_file", os.fsencode(path))
 
 
 def save(path=WEIGHTS_PATH):

@@ -42,7 +42,8 @@ class BteFloat3(ctypes.Structure):
 
 class BteConfig(ctypes.Structure):
     _fields_ = [("maxDepth", _i), ("megakernelSpp", _i), ("framesInFlight", _i), ("memoryBudgetMB", _i),
-                ("traversalStack", _i), ("graphs", _i), ("workChunk", _i), ("traversalBlocksPerCU", _i)]
+                ("traversalStack", _i), ("graphs", _i), ("workChunk", _i), ("traversalBlocksPerCU", _i),
+                ("bvhWidth", _i)]
 
 
 class BteStats(ctypes.Structure):
@@ -88,6 +89,7 @@ API = {
     "bte_set_instrumentation": [_i],
     "bte_last_error": [],
     "bte_frame_index": [],
+    "bte_set_frame_index": [_i],
     "bte_read_framebuffer": [_p, _sz],
     "bte_read_accum": [_p, _sz],
     "bte_enable_accum64": [_i],
@@ -105,7 +107,9 @@ API = {
     "bte_read_denoised": [_p, _sz],
     "bte_denoiser_param_count": [],
     "bte_denoiser_init": [ctypes.c_uint],
+    "bte_denoiser_release": [],
     "bte_denoiser_set_weights": [_p, _ll],
+    "bte_denoiser_load_file": [ctypes.c_char_p],
     "bte_denoiser_get_weights": [_p, _ll],
     "bte_denoiser_get_grads": [_p, _ll],
     "bte_denoiser_dataset_reserve": [_i, _i, _i],
@@ -238,7 +242,7 @@ class Renderer:
     """Uploads a SceneData through the C ABI and renders into device buffers."""
 
     def __init__(self, scene, max_depth=None, frames_in_flight=0, memory_budget_mb=0, mode=BTE_MODE_WAVEFRONT,
-                 megakernel_spp=5, traversal_stack=0, graphs=1, work_chunk=0, traversal_blocks_per_cu=0):
+                 megakernel_spp=5, traversal_stack=0, graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=0):
         self.scene = scene
         lib()
         cudaCleanup()
@@ -254,14 +258,15 @@ class Renderer:
         cudaInitEnvMap(scene.env_rgba, scene.env_intensity, scene.env_rotation_deg)
         self.set_config(max_depth if max_depth is not None else scene.max_depth, frames_in_flight,
                         memory_budget_mb, megakernel_spp, traversal_stack, graphs, work_chunk,
-                        traversal_blocks_per_cu)
+                        traversal_blocks_per_cu, bvh_width)
         cudaSetRenderMode(mode)
         cudaResetAccumulation(scene.width, scene.height)
 
     def set_config(self, max_depth, frames_in_flight=0, memory_budget_mb=0, megakernel_spp=5, traversal_stack=0,
-                   graphs=1, work_chunk=0, traversal_blocks_per_cu=0):
+                   graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=0):
         cfg = BteConfig(int(max_depth), int(megakernel_spp), int(frames_in_flight), int(memory_budget_mb),
-                        int(traversal_stack), int(graphs), int(work_chunk), int(traversal_blocks_per_cu))
+                        int(traversal_stack), int(graphs), int(work_chunk), int(traversal_blocks_per_cu),
+                        int(bvh_width))
         _call("bte_set_config", ctypes.byref(cfg))
 
     def config(self):
@@ -295,6 +300,9 @@ class Renderer:
 
     def frame_index(self):
         return _call("bte_frame_index")
+
+    def set_frame_index(self, index):
+        _call("bte_set_frame_index", int(index))
 
     def accum(self):
         W, H = self.scene.width, self.scene.height

@@ -1,0 +1,120 @@
+"""Trains the U-Net denoiser on pairs rendered by this path tracer.
+
+Each training pair is a 256 x 256 crop (a camera aimed at a random window of
+a 1920 x 1080 view, so objects keep their full-resolution pixel footprint):
+the input is the 4-spp accumulation with its albedo / normal AOVs, the target
+the same crop at 4096 spp (log(1 + RGB)).  Scenes come from this package's
+generators with seeds other than the benchmark scenes': env-lit fields of
+displaced spheres (the 1080p workload's generator: sphere layout, metal and
+glass materials, sky noise and sun position redrawn) and terrain rooms with
+64 random Disney materials and 4 quad lights.  Everything stays on the GPU:
+captures go straight from the accumulation buffers into the device-side
+dataset, and AdamW steps run on device.
+
+    python -m bluetruthengine_amd.train_denoiser --pairs 256 --steps 4000
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import denoiser, renderer, scenes
+
+CROP = 256
+FULL_W, FULL_H = 1920, 1080
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def make_scene(kind, seed, rng):
+    cx = int(rng.integers(CROP // 2, FULL_W - CROP // 2))
+    cy = int(rng.integers(CROP // 2, FULL_H - CROP // 2))
+    if kind == "envlit":
+        eye = scenes.C3_EYE
+        cam = scenes.crop_camera(eye, scenes.C3_TARGET, (0, 1, 0), scenes.C3_FOV, FULL_W, FULL_H, cx, cy, CROP)
+        n = int(rng.integers(150, 500))
+        return scenes.envlit_c3(CROP, CROP, spp=1, max_depth=12, n_spheres=n, slices=24, bands=12,
+                                env_size=(1024, 512), seed=seed, camera=cam)
+    cam = scenes.crop_camera((0, 1, 3.6), (0, 1, 0), (0, 1, 0), 40.0, FULL_W, FULL_H, cx, cy, CROP)
+    return scenes.terrain_c2(CROP, CROP, spp=1, max_depth=10, freq=40, seed=seed, camera=cam)
+
+
+def render_pairs(pairs, input_spp, target_spp, envlit_frac, seed):
+    rng = np.random.default_rng(seed)
+    denoiser.dataset_reserve(pairs, CROP, CROP)
+    t0 = time.perf_counter()
+    for i in range(pairs):
+        kind = "envlit" if rng.uniform() < envlit_frac else "terrain"
+        sd = make_scene(kind, 10_000 + i, rng)
+        r = renderer.Renderer(sd)
+        r.enable_aov(True)
+        r.build_accel()
+        r.render(input_spp)
+        denoiser.capture(i, 0)
+        left = target_spp - input_spp
+        while left > 0:
+            n = min(left, 1024)
+            r.render(n)
+            left -= n
+        denoiser.capture(i, 1)
+        if i % 8 == 7 or i == pairs - 1:
+            log(f"pair {i + 1}/{pairs} ({kind}, {sd.triangles.shape[0]} tris) {time.perf_counter() - t0:.0f} s")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--pairs", type=int, default=256)
+    ap.add_argument("--input-spp", type=int, default=4)
+    ap.add_argument("--target-spp", type=int, default=4096)
+    ap.add_argument("--envlit-frac", type=float, default=0.75)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--weight-decay", type=float, default=1e-4)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--out", default=denoiser.WEIGHTS_PATH)
+    ap.add_argument("--log-json", default="")
+    args = ap.parse_args(argv)
+
+    renderer.lib()
+    denoiser.init(args.seed)
+    t0 = time.perf_counter()
+    render_pairs(args.pairs, args.input_spp, args.target_spp, args.envlit_frac, args.seed)
+    t_render = time.perf_counter() - t0
+    log(f"rendered {args.pairs} pairs in {t_render:.0f} s")
+    # AdamW at lr, then two tenfold decays over the last 30 % of the steps
+    phases = [(0.7, args.lr), (0.2, args.lr * 0.3), (0.1, args.lr * 0.1)]
+    history = []
+    t1 = time.perf_counter()
+    done = 0
+    for frac, lr in phases:
+        steps = max(1, int(round(args.steps * frac)))
+        chunk = 250
+        while steps > 0:
+            n = min(chunk, steps)
+            losses = denoiser.train(n, args.batch, lr=lr, weight_decay=args.weight_decay, seed=args.seed * 7919 + done)
+            history.extend(float(x) for x in losses)
+            done += n
+            steps -= n
+            log(f"step {done}: lr {lr:.1e}, mean L1 (log space) of the last {n}: {losses.mean():.5f}, "
+                f"{time.perf_counter() - t1:.0f} s")
+    t_train = time.perf_counter() - t1
+    denoiser.save(args.out)
+    log(f"saved {args.out} ({denoiser.param_count()} parameters); training {t_train:.0f} s")
+    if args.log_json:
+        with open(args.log_json, "w") as fh:
+            json.dump({"pairs": args.pairs, "input_spp": args.input_spp, "target_spp": args.target_spp,
+                       "steps": done, "batch": args.batch, "lr": args.lr, "weight_decay": args.weight_decay,
+                       "render_s": t_render, "train_s": t_train,
+                       "loss_first_100": float(np.mean(history[:100])),
+                       "loss_last_100": float(np.mean(history[-100:]))}, fh)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -88,6 +88,7 @@ typedef struct {
     int graphs;              /* 1 = replay the bounce loop as captured HIP graphs (default), 0 = plain launches */
     int workChunk;           /* 64-slot queue rows a persistent wave takes per atomic: 1, 2, 4, 8 (0 = 4) */
     int traversalBlocksPerCU;/* resident traversal blocks per CU (0 = as many as occupancy allows) */
+    int bvhWidth;            /* 2: BVH2 or 4: BVH4 (collapsed from the BVH2) for extend / connect (0 = 2) */
 } BteConfig;
 
 /* Counters of the last bte_render / bte_render_frames call. */
@@ -180,6 +181,11 @@ int bte_set_instrumentation(int flags);
 const char* bte_last_error(void);
 /* Current frame index (samples accumulated since the last reset). */
 int bte_frame_index(void);
+/* Sets the index (and so the seeds) of the next frame without touching the
+ * accumulation: frames [k, k + n) give a sample stream independent of frames
+ * [0, n).  The binary32 running mean takes frame f with Welford count f + 1,
+ * so an offset stream is read from the binary64 sums (bte_read_accum64). */
+int bte_set_frame_index(int frameIndex);
 /* Copies of the display buffer (W*H uint32) and the binary32 running mean
  * (W*H*3 floats, row-major, row 0 = bottom image row as in the reference). */
 int bte_read_framebuffer(uint32_t* hostRGBA8, size_t count);
@@ -217,7 +223,13 @@ int bte_read_denoised(float* hostRGB, size_t count);
  * c3b d2a d2b d1a d1b d0a d0b out): W[cout][3][3][cin] then b[cout], fp32. */
 long long bte_denoiser_param_count(void);
 int bte_denoiser_init(unsigned seed);                 /* He-normal weights, zero biases */
+/* bte_cleanup switches the denoiser off and frees its per-resolution buffers
+ * but keeps the weights and training pairs; this frees those too. */
+int bte_denoiser_release(void);
 int bte_denoiser_set_weights(const float* params, long long count);
+/* The same from a NumPy .npy file of little-endian float32 (e.g. the shipped
+ * bluetruthengine_amd/weights/unet_c4.npy). */
+int bte_denoiser_load_file(const char* path);
 int bte_denoiser_get_weights(float* params, long long count);
 int bte_denoiser_get_grads(float* grads, long long count);   /* of the last training step */
 /* Training on this renderer's own pairs: reserve slots of width x height
@@ -242,10 +254,10 @@ int bte_denoiser_read_tensor(int id, float* out, long long count);
 
 /* Launch-shape autotuner (no counterpart in the reference, whose launch
  * shape OptiX fixes).  Renders `frames` frames at width x height once per
- * candidate (traversal LDS stack 8/16/32, then work chunk 1/2/4/8 rows, then
- * traversal blocks per CU: all, 3/4, 1/2), by coordinate descent from the
- * current configuration, keeps the fastest in the configuration and resets
- * the accumulation.  Images do not depend on these settings.  msBest
+ * candidate (BVH width 2/4, then traversal LDS stack 8/16/32, then work chunk
+ * 1/2/4/8 rows, then traversal blocks per CU: all, 3/4, 1/2), by coordinate
+ * descent from the current configuration, keeps the fastest in the
+ * configuration and resets the accumulation.  Images do not depend on these settings.  msBest
  * (optional) receives the best render time. */
 int bte_autotune(int width, int height, int frames, double* msBest);
 

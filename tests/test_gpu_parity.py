@@ -73,8 +73,8 @@ def test_any_hit_matches_oracle(c1):
     assert np.array_equal(occ.astype(np.uint8), ref)
 
 
-def _sample_parity(sd, frames, max_rec=4, env_from_gpu=True):
-    r = renderer.Renderer(sd)
+def _sample_parity(sd, frames, max_rec=4, env_from_gpu=True, **cfg):
+    r = renderer.Renderer(sd, **cfg)
     r.set_hit_capture(max_rec)
     r.render(frames)
     gpu = r.sample_radiance(frames)
@@ -105,6 +105,13 @@ def test_envlit_samples_match_oracle():
     sd = scenes.small_envlit()
     _, _, exact = _sample_parity(sd, 2)
     assert min(exact) == 1.0
+
+
+def test_bvh4_traversal_samples_match_oracle():
+    _say("case: BVH4 extend/connect, terrain freq 20 + reduced env-lit, 2 frames each")
+    for sd in (scenes.terrain_c2(width=96, height=54, spp=2, max_depth=10, freq=20), scenes.small_envlit()):
+        _, _, exact = _sample_parity(sd, 2, bvh_width=4)
+        assert min(exact) == 1.0
 
 
 def test_terrain_disney_samples_match_oracle():
@@ -210,6 +217,7 @@ def test_megakernel_five_paths_per_frame_matches_oracle():
 def test_error_paths_are_reported():
     _say("case: C-ABI error reporting")
     renderer.cudaCleanup()
+    renderer._call("bte_denoiser_release")
     with pytest.raises(renderer.BteError):
         renderer.cudaRender(16, 16)
     with pytest.raises(renderer.BteError):

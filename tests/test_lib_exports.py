@@ -40,18 +40,36 @@ def test_config_validation_without_device():
     L = renderer.lib()
     saved = renderer.BteConfig()
     assert L.bte_get_config(ctypes.byref(saved)) == 0
-    assert (saved.graphs, saved.workChunk, saved.traversalBlocksPerCU) == (1, 4, 0)
+    assert (saved.graphs, saved.workChunk, saved.traversalBlocksPerCU, saved.bvhWidth) == (1, 4, 0, 2)
     for fields, word in [(dict(graphs=2), b"graphs"), (dict(workChunk=3), b"workChunk"),
                          (dict(traversalBlocksPerCU=-1), b"traversalBlocksPerCU"),
-                         (dict(traversalStack=12), b"traversalStack")]:
-        cfg = renderer.BteConfig(12, 5, 0, 0, 16, 1, 4, 0)
+                         (dict(traversalStack=12), b"traversalStack"), (dict(bvhWidth=3), b"bvhWidth")]:
+        cfg = renderer.BteConfig(12, 5, 0, 0, 16, 1, 4, 0, 2)
         for k, v in fields.items():
             setattr(cfg, k, v)
         assert L.bte_set_config(ctypes.byref(cfg)) == -1
         assert word in L.bte_last_error()
-    good = renderer.BteConfig(12, 5, 0, 0, 8, 0, 2, 3)
+    good = renderer.BteConfig(12, 5, 0, 0, 8, 0, 2, 3, 4)
     assert L.bte_set_config(ctypes.byref(good)) == 0
     back = renderer.BteConfig()
     L.bte_get_config(ctypes.byref(back))
-    assert (back.traversalStack, back.graphs, back.workChunk, back.traversalBlocksPerCU) == (8, 0, 2, 3)
+    assert (back.traversalStack, back.graphs, back.workChunk, back.traversalBlocksPerCU, back.bvhWidth) == (8, 0, 2, 3, 4)
     assert L.bte_set_config(ctypes.byref(saved)) == 0
+
+
+def test_weights_file_reader_rejects_bad_files(tmp_path):
+    L = renderer.lib()
+    bad = tmp_path / "bad.npy"
+    bad.write_bytes(b"not a numpy file")
+    assert L.bte_denoiser_load_file(str(bad).encode()) == -1
+    assert b"not a .npy" in L.bte_last_error()
+    import numpy as np
+    wrong = tmp_path / "wrong.npy"
+    np.save(wrong, np.zeros(10, np.float64))
+    assert L.bte_denoiser_load_file(str(wrong).encode()) == -1
+    assert b"float32" in L.bte_last_error()
+    short = tmp_path / "short.npy"
+    np.save(short, np.zeros(10, np.float32))
+    assert L.bte_denoiser_load_file(str(short).encode()) == -1
+    assert b"exactly" in L.bte_last_error()
+    assert L.bte_denoiser_param_count() == 1211907
