@@ -65,6 +65,19 @@ def relmse(x, ref):
     return float(np.mean((x - ref) ** 2 / (ref * ref + 1e-2)))
 
 
+def relmse_by_luminance(x, ref):
+    """share of the relMSE sum from pixels binned by reference luminance"""
+    e = ((np.asarray(x, np.float64) - ref) ** 2 / (ref * ref + 1e-2)).mean(axis=-1)
+    lum = ref.mean(axis=-1)
+    edges = [0.0, 0.01, 0.03, 0.1, 0.3, 1.0, 3.0, np.inf]
+    total = e.sum()
+    out = {}
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        m = (lum >= lo) & (lum < hi)
+        out[f"{lo:g}-{hi:g}"] = {"px_frac": round(float(m.mean()), 4), "err_share": round(float(e[m].sum() / total), 4)}
+    return out
+
+
 def denoiser_bench(args):
     """Inference time of the U-Net on the 1080p frame (4 spp + AOVs) and the
     time of one AdamW training step (batch 8 of 256x256), with MFMA rooflines."""
@@ -202,7 +215,10 @@ def relmse_bench(args):
         r.render(spp - done)
         done = spp
         denoiser.denoise()
-        dn_curve[spp] = relmse(denoiser.denoised(W, H), ref)
+        img = denoiser.denoised(W, H)
+        if spp == 4:
+            dn4_img = img
+        dn_curve[spp] = relmse(img, ref)
         noisy_curve[spp] = relmse(r.accum(), ref)
         log(f"denoised {spp} spp: relMSE {dn_curve[spp]:.5f} (input {noisy_curve[spp]:.5f})")
     timed_dn(4)
@@ -222,6 +238,24 @@ def relmse_bench(args):
            "plain_curve": {str(k): round(v, 5) for k, v in sorted(curve.items())},
            "denoised_curve": {str(k): round(v, 5) for k, v in sorted(dn_curve.items())},
            "input_curve": {str(k): round(v, 5) for k, v in sorted(noisy_curve.items())}}
+    out["denoised_4spp"]["error_by_reference_luminance"] = relmse_by_luminance(dn4_img, ref)
+    # relMSE(N) = a / N + b for independent samples: b is the reference's own
+    # noise, a / 0.01 the spp a noise-free reference would need
+    ks = sorted(k for k in curve if k >= 1024)
+    if len(ks) >= 2:
+        n1, n2 = ks[-2], ks[-1]
+        a = (curve[n1] - curve[n2]) / (1.0 / n1 - 1.0 / n2)
+        b = curve[n2] - a / n2
+        ms_per_spp = plain_ms / plain_spp
+        fit = {"a": round(a, 3), "reference_noise_b": round(b, 5), "from_spp": [n1, n2]}
+        if b < 0.01:
+            need = a / (0.01 - b)
+            fit["plain_spp_needed_vs_this_reference"] = int(np.ceil(need))
+            fit["plain_ms_needed_vs_this_reference"] = round(need * ms_per_spp, 1)
+        fit["plain_spp_needed_noise_free_reference"] = int(np.ceil(a / 0.01))
+        fit["plain_ms_needed_noise_free_reference"] = round(a / 0.01 * ms_per_spp, 1)
+        fit["denoised_4spp_relmse_minus_b"] = round(dn_curve[4] - b, 5)
+        out["extrapolation"] = fit
     print(json.dumps(out), flush=True)
 
 

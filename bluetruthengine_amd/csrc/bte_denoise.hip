@@ -105,10 +105,138 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3(ConvArgs a) {
     }
 }
 
+// The same convolution with the weight tile of each K step staged in LDS and
+// shared by the block's four waves, and four 16-pixel M tiles per wave (64
+// pixels), so every weight fragment read from LDS feeds four MFMAs.  The next
+// step's weights are fetched into registers while the current step computes.
+// LDS rows are padded to 80 bytes to spread the 16-byte reads over the banks.
+constexpr int kLdsRow = 40;   // bf16 per staged weight row (32 used)
+
+template <int NT, int MT>
+__global__ __launch_bounds__(kBlock) void k_conv3x3_lds(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) bf16 sB[NT * 16 * kLdsRow];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, r16 = lane & 15;
+    const long long P = (long long)a.n * a.h * a.w_;
+    const long long base = ((long long)blockIdx.x * 4 + wave) * (16 * MT);
+    const bool waveActive = base < P;
+    const int coBase = blockIdx.y * NT * 16;
+    const int cin = a.c0 + a.c1;
+    const int hh = a.h >> 1, wh = a.w_ >> 1;
+    int pn[MT], py[MT], px[MT];
+    bool pv[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const long long p = base + mt * 16 + r16;
+        pv[mt] = p < P;
+        const long long pp = pv[mt] ? p : 0;
+        px[mt] = (int)(pp % a.w_);
+        const long long q = pp / a.w_;
+        py[mt] = (int)(q % a.h);
+        pn[mt] = (int)(q / a.h);
+    }
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    // cooperative staging: 16-byte pieces (row, quarter) of the NT*16 x 32 tile
+    constexpr int kPieces = NT * 16 * 4;
+    constexpr int kPer = (kPieces + kBlock - 1) / kBlock;
+    const int cSteps = cin / 32;
+    const int kSteps = 9 * cSteps;
+    bf16x8 stage[kPer];
+    auto fetch = [&](int ks) {
+        const int t = ks / cSteps, c = (ks % cSteps) * 32;
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            if (piece < kPieces) {
+                const int row = piece >> 2, qt = piece & 3;
+                stage[j] = *reinterpret_cast<const bf16x8*>(a.w + ((size_t)(coBase + row) * 9 + t) * cin + c + 8 * qt);
+            }
+        }
+    };
+    fetch(0);
+    for (int ks = 0; ks < kSteps; ++ks) {
+        __syncthreads();                       // every wave is done with the previous tile
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            if (piece < kPieces) {
+                const int row = piece >> 2, qt = piece & 3;
+                *reinterpret_cast<bf16x8*>(sB + row * kLdsRow + 8 * qt) = stage[j];
+            }
+        }
+        __syncthreads();
+        if (ks + 1 < kSteps) fetch(ks + 1);
+        if (!waveActive) continue;
+        const int t = ks / cSteps, c = (ks % cSteps) * 32;
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+        bf16x8 A[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const int yy = py[mt] + dy, xx = px[mt] + dx;
+            const bool ok = pv[mt] && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_;
+            const bf16* src;
+            if (c < a.c0) {
+                src = a.up0 ? a.src0 + (((size_t)pn[mt] * hh + (yy >> 1)) * wh + (xx >> 1)) * a.c0 + c
+                            : a.src0 + (((size_t)pn[mt] * a.h + yy) * a.w_ + xx) * a.c0 + c;
+            } else {
+                src = a.src1 + (((size_t)pn[mt] * a.h + yy) * a.w_ + xx) * a.c1 + (c - a.c0);
+            }
+            A[mt] = ok ? *reinterpret_cast<const bf16x8*>(src + 8 * g) : zero8();
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const bf16x8 B = *reinterpret_cast<const bf16x8*>(sB + (nt * 16 + r16) * kLdsRow + 8 * g);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[mt], B, acc[mt][nt], 0, 0, 0);
+        }
+    }
+    if (!waveActive) return;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long long p = base + mt * 16 + 4 * g + i;
+            if (p >= P) continue;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = coBase + nt * 16 + r16;
+                if (co >= a.cout) continue;
+                float v = acc[mt][nt][i];
+                if (a.bias) v += a.bias[co];
+                if (a.resid) v += f(a.resid[(size_t)p * a.residC + co]);
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (a.mask && !(f(a.mask[(size_t)p * a.maskC + co]) > 0.0f)) v = 0.0f;
+                if (a.dstF) a.dstF[(size_t)p * a.dstC + co] = v;
+                else a.dstB[(size_t)p * a.dstC + co] = (bf16)v;
+            }
+        }
+    }
+}
+
+static int g_convVariant = 1;   // 0: direct loads, 1: LDS-staged weights
+void set_conv_variant(int v) { g_convVariant = v; }
+
 void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
     const long long P = (long long)a.n * a.h * a.w_;
     const int tiles = a.coutPad / 16;
     const int nt = (tiles % 8 == 0) ? 8 : (tiles % 4 == 0) ? 4 : (tiles % 2 == 0) ? 2 : 1;
+    if (g_convVariant == 1) {
+        // pixels per block: 4 waves x MT x 16 (MT = 2 for 128-wide N tiles)
+        const unsigned gy = (unsigned)(tiles / nt);
+        switch (nt) {
+            case 8: hipLaunchKernelGGL((k_conv3x3_lds<8, 2>), dim3((unsigned)((P + 127) / 128), gy), dim3(kBlock), 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_conv3x3_lds<4, 4>), dim3((unsigned)((P + 255) / 256), gy), dim3(kBlock), 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_conv3x3_lds<2, 4>), dim3((unsigned)((P + 255) / 256), gy), dim3(kBlock), 0, s, a); break;
+            default: hipLaunchKernelGGL((k_conv3x3_lds<1, 4>), dim3((unsigned)((P + 255) / 256), gy), dim3(kBlock), 0, s, a); break;
+        }
+        return;
+    }
     const dim3 grid((unsigned)((P + 127) / 128), (unsigned)(tiles / nt));
     switch (nt) {
         case 8: hipLaunchKernelGGL(k_conv3x3<8>, grid, dim3(kBlock), 0, s, a); break;
