@@ -46,6 +46,20 @@ class BteConfig(ctypes.Structure):
                 ("bvhWidth", _i)]
 
 
+class BteSceneInfo(ctypes.Structure):
+    _fields_ = [("width", _i), ("height", _i), ("triangleCount", _i), ("materialCount", _i), ("bsdfCount", _i),
+                ("emitterCount", _i), ("emitterTriCount", _i), ("emitterCdfCount", _i), ("spotlightCount", _i),
+                ("envWidth", _i), ("envHeight", _i), ("envIntensity", _f), ("envRotationDeg", _f),
+                ("camera", ctypes.c_byte * 56)]
+
+
+class BteSceneArrays(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "triangles", "materials", "triangleMaterialIds", "triangleBsdfIds", "bsdfs", "triangleEmission",
+        "triangleEmitterFlags", "emitters", "emitterTriIndices", "emitterTriCdf", "sceneEmitterCdf", "spotlights",
+        "envRGBA")]
+
+
 class BteStats(ctypes.Structure):
     _fields_ = [("renderMs", ctypes.c_double), ("extensionRays", _ll), ("hits", _ll), ("shadowRays", _ll),
                 ("pathsTraced", _ll), ("bounces", _i), ("framesInFlight", _i), ("bvhNodes", _i), ("bvhDepth", _i),
@@ -102,6 +116,12 @@ API = {
     "bte_trace_rays": [_p, _p, _ll, _f, _f, _i, _p, _p, _p, _p],
     "bte_autotune": [_i, _i, _i, _p],
     "bte_enable_aov": [_i],
+    "bte_scene_load_xml": [ctypes.c_char_p, ctypes.c_char_p, _p],
+    "bte_scene_last_error": [],
+    "bte_scene_info": [_p, _p],
+    "bte_scene_arrays": [_p, _p],
+    "bte_scene_upload": [_p],
+    "bte_scene_free": [_p],
     "bte_read_aov": [_p, _p, ctypes.c_size_t],
     "bte_denoise": [],
     "bte_read_denoised": [_p, _sz],
@@ -120,7 +140,8 @@ API = {
     "bte_denoiser_tensor_shape": [_i, _p],
     "bte_denoiser_read_tensor": [_i, _p, _ll],
 }
-RESTYPE = {"bte_last_error": ctypes.c_char_p, "bte_denoiser_param_count": _ll}
+RESTYPE = {"bte_last_error": ctypes.c_char_p, "bte_scene_last_error": ctypes.c_char_p,
+           "bte_denoiser_param_count": _ll, "bte_scene_free": None}
 
 _lib = None
 

@@ -18,6 +18,7 @@ so every buffer is reproducible (tests pin their SHA-256).
 """
 from dataclasses import dataclass, field
 import hashlib
+import os
 
 import numpy as np
 
@@ -534,3 +535,63 @@ def small_envlit(width=128, height=72, n_spheres=12, env_size=(256, 128)):
 
 
 CONFIGS = {"c1": cornell_c1, "c2": terrain_c2, "c3": envlit_c3}
+
+
+# ---------------------------------------------------------------------------
+# scene.xml / OBJ / .hdr through the library's host layer (src/scene.cpp,
+# src/main.cpp:524-930)
+# ---------------------------------------------------------------------------
+def load_xml(path, base_dir=None, spp=1, max_depth=8, name=None):
+    """Parses a scene.xml with libbte's loader and returns its flattened scene
+    as SceneData (the arrays the reference's main.cpp hands to the renderer).
+    Relative mesh / map paths resolve against base_dir, default the working
+    directory (as the reference's)."""
+    import ctypes
+    from . import renderer
+    L = renderer.lib()
+    handle = ctypes.c_void_p()
+    base = "" if base_dir is None else base_dir     # the reference resolves against the working directory
+    if L.bte_scene_load_xml(os.fsencode(path), os.fsencode(base), ctypes.byref(handle)) != 0:
+        raise ValueError(L.bte_scene_last_error().decode(errors="replace"))
+    try:
+        info = renderer.BteSceneInfo()
+        arr = renderer.BteSceneArrays()
+        L.bte_scene_info(handle, ctypes.byref(info))
+        L.bte_scene_arrays(handle, ctypes.byref(arr))
+
+        def take(ptr, dtype, count, shape=None):
+            if count == 0 or not ptr:
+                return np.zeros(0 if shape is None else (0,) + tuple(shape[1:]), dtype)
+            n = count * np.dtype(dtype).itemsize
+            buf = (ctypes.c_char * n).from_address(ptr)
+            a = np.frombuffer(bytes(buf), dtype=dtype).copy()
+            return a.reshape(shape) if shape is not None else a
+
+        T = info.triangleCount
+        cam = np.frombuffer(bytes(info.camera), dtype=pod.CAMERA).reshape(()).copy()
+        sd = SceneData(
+            name=name or os.path.splitext(os.path.basename(path))[0], width=info.width, height=info.height,
+            spp=spp, max_depth=max_depth,
+            triangles=take(arr.triangles, pod.TRIANGLE, T),
+            bsdfs=take(arr.bsdfs, pod.BSDF, info.bsdfCount),
+            tri_bsdf_ids=take(arr.triangleBsdfIds, np.int32, T),
+            tri_material_ids=take(arr.triangleMaterialIds, np.int32, T),
+            tri_emission=take(arr.triangleEmission, np.float32, 3 * T, (T, 3)),
+            tri_emitter_flags=take(arr.triangleEmitterFlags, np.uint8, T),
+            camera=cam)
+        if info.emitterCount:
+            sd.emitters = take(arr.emitters, pod.EMITTER, info.emitterCount)
+            sd.emitter_tri_indices = take(arr.emitterTriIndices, np.int32, info.emitterTriCount)
+            sd.emitter_tri_cdf = take(arr.emitterTriCdf, np.float32, info.emitterCdfCount)
+            sd.scene_emitter_cdf = take(arr.sceneEmitterCdf, np.float32, info.emitterCount + 1)
+        if info.spotlightCount:
+            sd.spotlights = take(arr.spotlights, pod.SPOTLIGHT, info.spotlightCount)
+        if info.envWidth:
+            sd.env_rgba = take(arr.envRGBA, np.float32, info.envWidth * info.envHeight * 4,
+                               (info.envHeight, info.envWidth, 4))
+            sd.env_intensity = float(info.envIntensity)
+            sd.env_rotation_deg = float(info.envRotationDeg)
+        sd.extra["materials"] = take(arr.materials, np.float32, 3 * info.materialCount, (info.materialCount, 3))
+        return sd
+    finally:
+        L.bte_scene_free(handle)

@@ -252,6 +252,46 @@ int bte_denoiser_train_step_host(const float* x9, const float* target3, int n, i
 int bte_denoiser_tensor_shape(int id, int* shape4);
 int bte_denoiser_read_tensor(int id, float* out, long long count);
 
+/* ---- scene host layer: scene.xml / OBJ / .hdr (src/scene.cpp, src/main.cpp) */
+/* ≙ loadSceneDescription (src/scene.cpp:83-523) plus the set-up of
+ * src/main.cpp:524-930: parses a scene.xml, imports its OBJ meshes (normal
+ * and planar-UV fallbacks, Scale->Rx->Ry->Rz->Translate transforms), packs
+ * the BSDF table, builds the emitter tables, the camera basis, spotlights and
+ * the Radiance .hdr environment.  Relative paths resolve against baseDir (the
+ * reference resolves them against its working directory).  Host only: no
+ * device needed until bte_scene_upload.  Errors: bte_scene_last_error(). */
+typedef struct BteScene BteScene;      /* opaque */
+typedef struct {
+    int width, height;                 /* <window> */
+    int triangleCount, materialCount, bsdfCount, emitterCount, emitterTriCount, emitterCdfCount;
+    int spotlightCount;
+    int envWidth, envHeight;           /* 0 when the scene has no <environment> */
+    float envIntensity, envRotationDeg;
+    BteCamera camera;
+} BteSceneInfo;
+typedef struct {                       /* views into the scene's arrays (valid until bte_scene_free) */
+    const BteTriangle* triangles;
+    const BteFloat3* materials;
+    const int* triangleMaterialIds;
+    const int* triangleBsdfIds;
+    const BteBsdf* bsdfs;
+    const BteFloat3* triangleEmission;
+    const uint8_t* triangleEmitterFlags;
+    const BteEmitter* emitters;
+    const int* emitterTriIndices;
+    const float* emitterTriCdf;
+    const float* sceneEmitterCdf;      /* emitterCount + 1 entries */
+    const BteSpotlight* spotlights;
+    const float* envRGBA;              /* envWidth * envHeight * 4, row 0 = top; null without env */
+} BteSceneArrays;
+int bte_scene_load_xml(const char* path, const char* baseDir, BteScene** scene);
+const char* bte_scene_last_error(void);
+int bte_scene_info(const BteScene* scene, BteSceneInfo* info);
+int bte_scene_arrays(const BteScene* scene, BteSceneArrays* arrays);
+/* bte_init_scene … bte_init_env_map and bte_init_camera in the reference's order */
+int bte_scene_upload(const BteScene* scene);
+void bte_scene_free(BteScene* scene);
+
 /* Launch-shape autotuner (no counterpart in the reference, whose launch
  * shape OptiX fixes).  Renders `frames` frames at width x height once per
  * candidate (BVH width 2/4, then traversal LDS stack 8/16/32, then work chunk

@@ -12,7 +12,7 @@ HDR = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 
 
 def declared():
     src = open(HDR).read()
-    return sorted(set(re.findall(r"^\s*(?:int|long long|const char\*)\s+(bte_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|void|long long|const char\*)\s+(bte_\w+)\s*\(", src, re.M)))
 
 
 def test_every_declared_symbol_is_exported():
