@@ -193,12 +193,18 @@ def relmse_bench(args):
     plain_ms = min(timed_plain() for _ in range(2)) * 1e3
 
     r.enable_aov(True)
-    if os.path.exists(denoiser.WEIGHTS_PATH):
-        denoiser.load()
-        trained = True
-    else:
-        denoiser.init(0)
-        trained = False
+    weight_files = [w for w in args.weights.split(",") if w] or [denoiser.WEIGHTS_PATH]
+    compared = {}
+    for wf in weight_files:
+        denoiser.load(wf)
+        r.reset()
+        r.render(4)
+        denoiser.denoise()
+        compared[wf] = relmse(denoiser.denoised(W, H), ref)
+        log(f"weights {wf}: 4 spp denoised relMSE {compared[wf]:.5f}")
+    best = min(compared, key=compared.get)
+    denoiser.load(best)
+    trained = True
 
     def timed_dn(spp):
         r.reset()
@@ -229,7 +235,8 @@ def relmse_bench(args):
            "plain": {"spp": plain_spp, "relmse": round(curve[plain_spp], 5), "ms": round(plain_ms, 2)},
            "denoised_4spp": {"relmse": round(dn_curve[4], 5), "ms": round(dn_ms, 2),
                              "input_relmse": round(noisy_curve[4], 5),
-                             "weights": "trained" if trained else "untrained (random init)"},
+                             "weights": os.path.basename(best) if trained else "untrained (random init)",
+                             "weights_compared": {os.path.basename(k): round(v, 5) for k, v in compared.items()}},
            "speedup_4spp": round(plain_ms / dn_ms, 2) if dn_curve[4] <= 0.01 else None,
            "denoised_min_spp_reaching": reach[0] if reach else None,
            "denoised_reach_ms": round(dn_reach_ms, 2) if dn_reach_ms else None,
@@ -280,6 +287,8 @@ def main():
     ap.add_argument("--relmse", action="store_true",
                     help="only measure time to relMSE<=0.01: plain spp vs 4 spp + denoiser")
     ap.add_argument("--ref-spp", type=int, default=65536, help="samples per pixel of the --relmse reference")
+    ap.add_argument("--weights", default="", help="--relmse: comma-separated denoiser weight files to compare "
+                                                  "(default: the shipped weights)")
     args = ap.parse_args()
     if args.copy_calibration:
         import torch

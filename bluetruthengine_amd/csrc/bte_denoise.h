@@ -68,10 +68,12 @@ void launch_upsample_bwd(const bf16* g, int gC, const bf16* act, bf16* out, int 
 void launch_pool_bwd_add(const bf16* gp, const bf16* gcat, int gcatC, int off, const bf16* act, bf16* out, int n,
                          int h, int w, int c, hipStream_t s);
 
-// L1 loss in log space: loss += sum |o - t| / count over the 3 channels of
-// every pixel; go[p][c] = sign(o - t) / count (c < 3), 0 for c in [3, goC).
-void launch_l1_loss(const float* o, const float* t, long long pixels, float invCount, bf16* go, int goC,
-                    float* lossAcc, hipStream_t s);
+// Training loss over the 3 channels of every pixel (o, t in log(1 + RGB)):
+// mode 0 = L1 in log space, mode 1 = relative MSE of the linear radiance with
+// the prediction's own value (held constant) in the denominator.  Adds the
+// mean loss to *lossAcc and writes go[p][c] = dLoss/do (c < 3), 0 up to goC.
+void launch_loss(const float* o, const float* t, long long pixels, float invCount, int mode, bf16* go, int goC,
+                 float* lossAcc, hipStream_t s);
 
 // AdamW on one parameter block of `count` fp32 values.
 struct AdamArgs {

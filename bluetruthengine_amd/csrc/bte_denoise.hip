@@ -416,8 +416,14 @@ void launch_pool_bwd_add(const bf16* gp, const bf16* gcat, int gcatC, int off, c
 // ---------------------------------------------------------------------------
 // loss, optimiser, weight layouts
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_l1_loss(const float* o, const float* t, long long pixels, float invCount,
-                                                    bf16* go, int goC, float* lossAcc) {
+// Loss over the 3 output channels of every pixel, mean over pixels and
+// channels (invCount = 1 / (3 P)).  mode 0: L1 in log space, |o - t|,
+// gradient sign(o - t).  mode 1: relative MSE on linear radiance,
+// (y - T)^2 / (yd^2 + 0.01) with y = e^o - 1, T = e^t - 1 and yd = max(y, 0)
+// held constant (the evaluation's error measure), gradient
+// 2 (y - T) e^o / (yd^2 + 0.01) clipped to [-4, 4].
+__global__ __launch_bounds__(kBlock) void k_loss(const float* o, const float* t, long long pixels, float invCount,
+                                                 int mode, bf16* go, int goC, float* lossAcc) {
     __shared__ float part[kBlock / 64];
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.0f;
@@ -425,9 +431,21 @@ __global__ __launch_bounds__(kBlock) void k_l1_loss(const float* o, const float*
         for (int c = 0; c < goC; ++c) {
             float gv = 0.0f;
             if (c < kOutChannels) {
-                const float d = o[(size_t)p * 3 + c] - t[(size_t)p * 3 + c];
-                l += fabsf(d);
-                gv = d > 0.0f ? invCount : (d < 0.0f ? -invCount : 0.0f);
+                const float oc = o[(size_t)p * 3 + c], tc = t[(size_t)p * 3 + c];
+                if (mode == 0) {
+                    const float d = oc - tc;
+                    l += fabsf(d);
+                    gv = d > 0.0f ? 1.0f : (d < 0.0f ? -1.0f : 0.0f);
+                } else {
+                    const float eo = expf(oc);
+                    const float y = eo - 1.0f, T = expm1f(tc);
+                    const float yd = fmaxf(y, 0.0f);
+                    const float den = yd * yd + 0.01f;
+                    const float d = y - T;
+                    l += d * d / den;
+                    gv = fminf(fmaxf(2.0f * d * eo / den, -4.0f), 4.0f);
+                }
+                gv *= invCount;
             }
             go[(size_t)p * goC + c] = (bf16)gv;
         }
@@ -442,10 +460,10 @@ __global__ __launch_bounds__(kBlock) void k_l1_loss(const float* o, const float*
     }
 }
 
-void launch_l1_loss(const float* o, const float* t, long long pixels, float invCount, bf16* go, int goC,
-                    float* lossAcc, hipStream_t s) {
-    hipLaunchKernelGGL(k_l1_loss, dim3((unsigned)((pixels + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, o, t, pixels,
-                       invCount, go, goC, lossAcc);
+void launch_loss(const float* o, const float* t, long long pixels, float invCount, int mode, bf16* go, int goC,
+                 float* lossAcc, hipStream_t s) {
+    hipLaunchKernelGGL(k_loss, dim3((unsigned)((pixels + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, o, t, pixels,
+                       invCount, mode, go, goC, lossAcc);
 }
 
 // AdamW (decoupled weight decay), in the order w <- w (1 - lr wd);

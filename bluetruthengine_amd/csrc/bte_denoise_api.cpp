@@ -14,7 +14,8 @@
 // nearest 2x upsampling up, and a residual: the output (log(1 + RGB)) is the
 // input's log channels plus the network's last convolution.  Input channels:
 // log(1 + max(RGB, 0)) of the accumulation, albedo and normal AOVs.  Training
-// minimises the mean L1 distance in that log space with AdamW.
+// minimises the mean L1 distance in that log space (or, selectable, the
+// relative MSE of the linear radiance) with AdamW.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -87,6 +88,7 @@ struct Net {
     DBuf denoised;
     int outW = 0, outH = 0;
     bool enabled = false;
+    int lossMode = 0;   // 0: L1 in log space, 1: relative MSE (linear)
 };
 
 Net& net() {
@@ -288,8 +290,8 @@ void backward(Net& N, hipStream_t s) {
     const long long P = (long long)n * h * w;
     dn::launch_fill_f32(N.g.as<float>(), (long long)N.nW, 0.0f, s);
     dn::launch_fill_f32(N.loss.as<float>(), 1, 0.0f, s);
-    dn::launch_l1_loss(N.o.as<float>(), N.target.as<float>(), P, 1.0f / (float)(P * 3), T(N, T_GO), 32,
-                       N.loss.as<float>(), s);
+    dn::launch_loss(N.o.as<float>(), N.target.as<float>(), P, 1.0f / (float)(P * 3), N.lossMode, T(N, T_GO), 32,
+                    N.loss.as<float>(), s);
     // decoder, level 0
     wgrad(N, OUT, T(N, T_GO), one(T(N, T_D0B), 32), 0, s);
     dgrad(N, OUT, T(N, T_GO), 0, T(N, T_D0B), T(N, T_GD0B), s);
@@ -626,6 +628,12 @@ int bte_denoiser_read_tensor(int id, float* out, long long count) {
     std::vector<uint16_t> tmp((size_t)total);
     HIPCHK(hipMemcpy(tmp.data(), N.t[id].p, (size_t)total * 2, hipMemcpyDeviceToHost));
     for (long long i = 0; i < total; ++i) out[i] = fromBf16(tmp[(size_t)i]);
+    return BTE_OK;
+}
+
+int bte_denoiser_set_loss(int mode) {
+    if (mode != 0 && mode != 1) return fail(BTE_ERR_INVALID, "loss mode must be 0 (L1, log space) or 1 (relMSE)");
+    net().lossMode = mode;
     return BTE_OK;
 }
 

@@ -90,6 +90,13 @@ def tensor(tid):
     return out
 
 
+LOSS_L1_LOG, LOSS_RELMSE = 0, 1
+
+
+def set_loss(mode):
+    _call("bte_denoiser_set_loss", int(mode))
+
+
 def dataset_reserve(slots, width, height):
     _call("bte_denoiser_dataset_reserve", int(slots), int(width), int(height))
 

@@ -77,6 +77,12 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--weight-decay", type=float, default=1e-4)
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--loss", choices=["l1", "relmse"], default="l1",
+                    help="l1: L1 of log(1 + RGB) (default); relmse: relative MSE of the linear radiance")
+    ap.add_argument("--init", default="", help="start from these weights (.npy) instead of a random init")
+    ap.add_argument("--finetune-steps", type=int, default=0,
+                    help="after the main run, this many more steps with the relMSE loss at lr/3 (saved to --finetune-out)")
+    ap.add_argument("--finetune-out", default="")
     ap.add_argument("--out", default=denoiser.WEIGHTS_PATH)
     ap.add_argument("--log-json", default="")
     args = ap.parse_args(argv)
@@ -87,6 +93,10 @@ def main(argv=None):
     render_pairs(args.pairs, args.input_spp, args.target_spp, args.envlit_frac, args.seed)
     t_render = time.perf_counter() - t0
     log(f"rendered {args.pairs} pairs in {t_render:.0f} s")
+    if args.init:
+        denoiser.load(args.init)          # after the renders: Renderer() keeps the weights
+        log(f"starting from {args.init}")
+    denoiser.set_loss(denoiser.LOSS_RELMSE if args.loss == "relmse" else denoiser.LOSS_L1_LOG)
     # AdamW at lr, then two tenfold decays over the last 30 % of the steps
     phases = [(0.7, args.lr), (0.2, args.lr * 0.3), (0.1, args.lr * 0.1)]
     history = []
@@ -101,18 +111,35 @@ def main(argv=None):
             history.extend(float(x) for x in losses)
             done += n
             steps -= n
-            log(f"step {done}: lr {lr:.1e}, mean L1 (log space) of the last {n}: {losses.mean():.5f}, "
+            log(f"step {done}: lr {lr:.1e}, mean {args.loss} loss of the last {n}: {losses.mean():.5f}, "
                 f"{time.perf_counter() - t1:.0f} s")
     t_train = time.perf_counter() - t1
     denoiser.save(args.out)
     log(f"saved {args.out} ({denoiser.param_count()} parameters); training {t_train:.0f} s")
+    ft_history = []
+    if args.finetune_steps > 0:
+        denoiser.set_loss(denoiser.LOSS_RELMSE)
+        left = args.finetune_steps
+        while left > 0:
+            n = min(250, left)
+            losses = denoiser.train(n, args.batch, lr=args.lr / 3, weight_decay=args.weight_decay,
+                                    seed=args.seed * 104729 + left)
+            ft_history.extend(float(x) for x in losses)
+            left -= n
+            log(f"fine-tune (relMSE) {args.finetune_steps - left}/{args.finetune_steps}: mean loss of the last {n}: "
+                f"{losses.mean():.5f}")
+        denoiser.save(args.finetune_out or args.out.replace(".npy", "_relmse.npy"))
+        denoiser.set_loss(denoiser.LOSS_L1_LOG)
     if args.log_json:
         with open(args.log_json, "w") as fh:
             json.dump({"pairs": args.pairs, "input_spp": args.input_spp, "target_spp": args.target_spp,
                        "steps": done, "batch": args.batch, "lr": args.lr, "weight_decay": args.weight_decay,
                        "render_s": t_render, "train_s": t_train,
+                       "loss": args.loss, "init": args.init,
                        "loss_first_100": float(np.mean(history[:100])),
-                       "loss_last_100": float(np.mean(history[-100:]))}, fh)
+                       "loss_last_100": float(np.mean(history[-100:])),
+                       "finetune_relmse_steps": args.finetune_steps,
+                       "finetune_loss_last_100": float(np.mean(ft_history[-100:])) if ft_history else None}, fh)
     return 0
 
 

@@ -237,6 +237,10 @@ int bte_denoiser_get_grads(float* grads, long long count);   /* of the last trai
  * a slot's input (which = 0: features incl. AOVs, e.g. after 4 spp) or target
  * (which = 1: log(1+RGB), e.g. after 4096 spp), then run AdamW steps on random
  * batches of complete pairs; losses (optional) receives each step's mean L1. */
+/* Training loss: 0 = L1 of log(1 + RGB) (default), 1 = relative MSE of the
+ * linear radiance, (y - t)^2 / (y^2 + 0.01) with the prediction y held
+ * constant in the denominator (the relMSE used to judge images). */
+int bte_denoiser_set_loss(int mode);
 int bte_denoiser_dataset_reserve(int slots, int width, int height);
 int bte_denoiser_capture(int slot, int which);
 int bte_denoiser_train(int steps, int batch, float lr, float weightDecay, unsigned seed, float* losses);

@@ -98,3 +98,25 @@ inline RenderMode cudaGetRenderMode() {
     return bte_get_render_mode() == BTE_MODE_WAVEFRONT ? RenderMode::Wavefront : RenderMode::Megakernel;
 }
 inline void cudaCleanup() { bte_compat::check(bte_cleanup(), "cudaCleanup"); }
+
+// The scene set-up of the reference's main() (src/main.cpp:524-935) in one
+// call: loadSceneDescription + OBJ import + BSDF/emitter tables + spotlights +
+// .hdr environment, uploaded in the reference's order.  Relative paths resolve
+// against the working directory, as in the reference.  Returns the <window>
+// size; exits with a message on failure (the reference's behaviour).
+struct BteWindowSize {
+    int width, height;
+};
+inline BteWindowSize loadSceneXml(const char* xmlPath) {
+    BteScene* scene = nullptr;
+    if (bte_scene_load_xml(xmlPath, "", &scene) != BTE_OK) {
+        std::fprintf(stderr, "[scene] %s\n", bte_scene_last_error());
+        std::exit(EXIT_FAILURE);
+    }
+    BteSceneInfo info{};
+    bte_scene_info(scene, &info);
+    const int rc = bte_scene_upload(scene);
+    bte_scene_free(scene);
+    bte_compat::check(rc, "loadSceneXml");
+    return BteWindowSize{info.width, info.height};
+}

@@ -179,14 +179,29 @@ class Net:
     def _up_bwd(g):
         return ((g[:, 0::2, 0::2] + g[:, 0::2, 1::2]) + g[:, 1::2, 0::2]) + g[:, 1::2, 1::2]
 
-    def backward(self, t, target):
-        """Mean-L1 (log space) loss and parameter gradients {name: (dW, db)}."""
-        o = t["o"]
-        d = o - target
+    def loss_grad(self, o, target, mode=0):
+        """Training loss and its gradient w.r.t. the log-space output o.
+        mode 0: mean |o - t|.  mode 1: mean (y - T)^2 / (max(y, 0)^2 + 0.01),
+        y = e^o - 1, T = e^t - 1, the denominator held constant, gradient
+        clipped to [-4, 4] before the 1 / count scale."""
         count = o.size
-        loss = np.abs(d).sum() / count
         inv = np.float32(1) / np.float32(count) if self.bf16 else 1.0 / count
-        go = np.sign(d) * inv
+        if mode == 0:
+            d = o - target
+            return np.abs(d).sum() / count, np.sign(d) * inv
+        eo = np.exp(o)
+        y = eo - 1.0
+        T = np.expm1(target)
+        den = np.maximum(y, 0.0) ** 2 + 0.01
+        d = y - T
+        g = np.clip(2.0 * d * eo / den, -4.0, 4.0)
+        return (d * d / den).sum() / count, (g.astype(np.float32) * inv) if self.bf16 else g * inv
+
+    def backward(self, t, target, loss_mode=0):
+        """Loss (mean L1 in log space, or relMSE with loss_mode=1) and
+        parameter gradients {name: (dW, db)}."""
+        o = t["o"]
+        loss, go = self.loss_grad(o, target, loss_mode)
         g = {"go": self._store(go)}
         grads = {}
         st = self._store

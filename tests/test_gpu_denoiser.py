@@ -213,3 +213,24 @@ def test_dataset_capture_and_training_reduce_loss():
     _say(f"losses: first {losses[:3]}, last {losses[-3:]}")
     assert np.all(np.isfinite(losses))
     assert losses[-5:].mean() < losses[:5].mean()
+
+
+def test_relmse_training_loss_matches_oracle():
+    _say("case: relMSE training loss (mode 1), n=1 h=16 w=256, seed 4")
+    renderer.cudaCleanup()
+    params, x9 = _forward_case(4, 1, 16, 256)
+    rs = np.random.default_rng(11)
+    target = rs.uniform(0.0, 1.5, (1, 16, 256, 3)).astype(np.float32)
+    dn.set_loss(dn.LOSS_RELMSE)
+    try:
+        loss = dn.train_step(x9, target, lr=0.0)
+    finally:
+        dn.set_loss(dn.LOSS_L1_LOG)
+    o = dn.tensor(dn.OUTPUT_TENSOR).astype(np.float64)
+    net = od.Net(params, bf16=True)
+    ref_loss, ref_go = net.loss_grad(o, target.astype(np.float64), mode=1)
+    _say(f"relMSE loss gpu {loss:.6f} oracle {ref_loss:.6f}")
+    assert abs(loss - ref_loss) <= 1e-4 * max(1.0, ref_loss)
+    go = dn.tensor(GRD["go"])
+    assert np.all(go[..., 3:] == 0)
+    _close_bf16(go[..., :3], od.bf16_round(ref_go.astype(np.float32)), "relMSE loss gradient", min_exact=0.99)

@@ -92,6 +92,24 @@ def test_forward_and_backward_match_torch_autograd():
         assert np.allclose(db, rb, rtol=1e-9, atol=1e-12 * scale), name
 
 
+def test_relmse_loss_gradient_matches_torch():
+    # the relMSE loss with the prediction held constant in the denominator;
+    # values kept small enough that the [-4, 4] clip is inactive
+    rs = np.random.default_rng(9)
+    o = rs.uniform(0.5, 1.5, (1, 4, 8, 3))
+    tgt = o + rs.normal(0, 0.02, o.shape)
+    net = od.Net({}, bf16=False)
+    loss, g = net.loss_grad(o, tgt, mode=1)
+    ot = torch.tensor(o, requires_grad=True)
+    y = torch.exp(ot) - 1
+    den = torch.clamp(y.detach(), min=0) ** 2 + 0.01
+    lt = ((y - torch.expm1(torch.tensor(tgt))) ** 2 / den).mean()
+    lt.backward()
+    assert abs(loss - lt.item()) < 1e-12
+    assert np.all(np.abs(g * o.size) < 4)
+    assert np.allclose(g, ot.grad.numpy(), rtol=1e-10, atol=1e-14)
+
+
 def test_adamw_matches_torch():
     rs = np.random.default_rng(5)
     w0 = rs.normal(size=50)
