@@ -101,7 +101,7 @@ struct Context {
     long long lastPassPaths = 0;
     int lastPassFrames = 0;
 
-    BteConfig cfg{200, 5, 0, 32768, 16, 1, 4, 0, 2};
+    BteConfig cfg{200, 5, 0, 32768, 16, 1, 4, 0, 4};
     BteStats stats{};
     int instr = 0;
     DBuf dTrav;

@@ -276,9 +276,9 @@ BTE_DEV void sortPair(float& ka, int& ca, float& kb, int& cb) {
     }
 }
 
-// BVH4 traversal (same contract as traverse): the hit children of a node are
-// ordered by entry distance; the nearest is visited next, the others pushed
-// farthest first.
+// BVH4 traversal (same contract as traverse): for closest hits the hit
+// children of a node are ordered by entry distance, the nearest visited next
+// and the others pushed farthest first; any-hit queries skip the ordering.
 template <bool ANY, bool COUNT = false, int L = kLdsStack>
 BTE_DEV int traverse4(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, const StackRef<L>& stk, float& tOut,
                       float& uOut, float& vOut, uint32_t* nodeCount = nullptr, uint32_t* triCount = nullptr) {
@@ -303,11 +303,20 @@ BTE_DEV int traverse4(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, co
             float k0 = e0 <= f0 ? e0 : kNone, k1 = e1 <= f1 ? e1 : kNone;
             float k2 = e2 <= f2 ? e2 : kNone, k3 = e3 <= f3 ? e3 : kNone;
             int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
-            sortPair(k0, c0, k1, c1);
-            sortPair(k2, c2, k3, c3);
-            sortPair(k0, c0, k2, c2);
-            sortPair(k1, c1, k3, c3);
-            sortPair(k1, c1, k2, c2);
+            if constexpr (!ANY) {
+                sortPair(k0, c0, k1, c1);
+                sortPair(k2, c2, k3, c3);
+                sortPair(k0, c0, k2, c2);
+                sortPair(k1, c1, k3, c3);
+                sortPair(k1, c1, k2, c2);
+            } else {
+                // any hit ends the search: order is irrelevant, only move a hit child to slot 0
+                if (k0 == kNone) {
+                    if (k1 != kNone) { k0 = k1; c0 = c1; k1 = kNone; }
+                    else if (k2 != kNone) { k0 = k2; c0 = c2; k2 = kNone; }
+                    else if (k3 != kNone) { k0 = k3; c0 = c3; k3 = kNone; }
+                }
+            }
             if (k0 != kNone) {
                 if (k3 != kNone) push(stk, sp, c3);
                 if (k2 != kNone) push(stk, sp, c2);

@@ -264,7 +264,7 @@ class Renderer:
     """Uploads a SceneData through the C ABI and renders into device buffers."""
 
     def __init__(self, scene, max_depth=None, frames_in_flight=0, memory_budget_mb=0, mode=BTE_MODE_WAVEFRONT,
-                 megakernel_spp=5, traversal_stack=0, graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=0):
+                 megakernel_spp=5, traversal_stack=0, graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=4):
         self.scene = scene
         lib()
         cudaCleanup()
@@ -285,7 +285,7 @@ class Renderer:
         cudaResetAccumulation(scene.width, scene.height)
 
     def set_config(self, max_depth, frames_in_flight=0, memory_budget_mb=0, megakernel_spp=5, traversal_stack=0,
-                   graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=0):
+                   graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=4):
         cfg = BteConfig(int(max_depth), int(megakernel_spp), int(frames_in_flight), int(memory_budget_mb),
                         int(traversal_stack), int(graphs), int(work_chunk), int(traversal_blocks_per_cu),
                         int(bvh_width))

@@ -88,7 +88,7 @@ typedef struct {
     int graphs;              /* 1 = replay the bounce loop as captured HIP graphs (default), 0 = plain launches */
     int workChunk;           /* 64-slot queue rows a persistent wave takes per atomic: 1, 2, 4, 8 (0 = 4) */
     int traversalBlocksPerCU;/* resident traversal blocks per CU (0 = as many as occupancy allows) */
-    int bvhWidth;            /* 2: BVH2 or 4: BVH4 (collapsed from the BVH2) for extend / connect (0 = 2) */
+    int bvhWidth;            /* 2: BVH2 or 4: BVH4 (collapsed from the BVH2) for extend / connect (default 4; 0 = 2) */
 } BteConfig;
 
 /* Counters of the last bte_render / bte_render_frames call. */

@@ -64,8 +64,8 @@ def test_deep_paths_run_in_segments_with_early_exit():
 
 
 @pytest.mark.parametrize("cfg", [dict(traversal_stack=8), dict(traversal_stack=32), dict(work_chunk=1),
-                                 dict(work_chunk=8), dict(traversal_blocks_per_cu=2), dict(bvh_width=4),
-                                 dict(bvh_width=4, traversal_stack=8, work_chunk=1)])
+                                 dict(work_chunk=8), dict(traversal_blocks_per_cu=2), dict(bvh_width=2),
+                                 dict(bvh_width=2, traversal_stack=8, work_chunk=1)])
 def test_launch_shape_does_not_change_result(cfg):
     _say(f"case: reduced env-lit 128x72 depth 12, 2 frames, launch shape {cfg} vs default")
     sd = scenes.small_envlit()

@@ -40,7 +40,7 @@ def test_config_validation_without_device():
     L = renderer.lib()
     saved = renderer.BteConfig()
     assert L.bte_get_config(ctypes.byref(saved)) == 0
-    assert (saved.graphs, saved.workChunk, saved.traversalBlocksPerCU, saved.bvhWidth) == (1, 4, 0, 2)
+    assert (saved.graphs, saved.workChunk, saved.traversalBlocksPerCU, saved.bvhWidth) == (1, 4, 0, 4)
     for fields, word in [(dict(graphs=2), b"graphs"), (dict(workChunk=3), b"workChunk"),
                          (dict(traversalBlocksPerCU=-1), b"traversalBlocksPerCU"),
                          (dict(traversalStack=12), b"traversalStack"), (dict(bvhWidth=3), b"bvhWidth")]:
