@@ -140,16 +140,26 @@ def relmse_bench(args):
     r.autotune(16)
     r.enable_accum64(True)
     r.set_frame_index(1 << 24)          # reference samples independent of the test renders (frames 0..)
-    t0 = time.perf_counter()
-    chunk = 256
-    done = 0
-    while done < args.ref_spp:
-        n = min(chunk, args.ref_spp - done)
-        r.render(n)
-        done += n
-        if done % 4096 == 0 or done == args.ref_spp:
-            log(f"reference: {done}/{args.ref_spp} spp, {time.perf_counter() - t0:.0f} s")
-    ref = r.accum64() / float(args.ref_spp)
+    ref = None
+    if args.ref_cache and os.path.exists(args.ref_cache):
+        with np.load(args.ref_cache) as z:
+            if int(z["spp"]) == args.ref_spp and z["ref"].shape == (H, W, 3):
+                ref = z["ref"].astype(np.float64)
+                log(f"reference: {args.ref_spp} spp read from {args.ref_cache}")
+    if ref is None:
+        t0 = time.perf_counter()
+        chunk = 256
+        done = 0
+        while done < args.ref_spp:
+            n = min(chunk, args.ref_spp - done)
+            r.render(n)
+            done += n
+            if done % 4096 == 0 or done == args.ref_spp:
+                log(f"reference: {done}/{args.ref_spp} spp, {time.perf_counter() - t0:.0f} s")
+        ref = r.accum64() / float(args.ref_spp)
+        if args.ref_cache:
+            np.savez(args.ref_cache, ref=ref.astype(np.float32), spp=np.int64(args.ref_spp))
+            ref = ref.astype(np.float32).astype(np.float64)   # same values as a later read of the file
     r.enable_accum64(False)
 
     # One progressive pass (a render of n spp equals the first n frames of a
@@ -353,6 +363,9 @@ def main():
                     help="only measure time to relMSE<=0.01: plain spp vs 4 spp + denoiser")
     ap.add_argument("--ref-spp", type=int, default=65536, help="samples per pixel of the --relmse reference")
     ap.add_argument("--skip-plain", action="store_true", help="--relmse: only the denoised side")
+    ap.add_argument("--ref-cache", default="", metavar="NPZ",
+                    help="--relmse: read the reference image from this file if it holds --ref-spp samples, "
+                         "else render it and write it there (float32)")
     ap.add_argument("--serve", action="store_true", help="only run the 64-scene serving benchmark")
     ap.add_argument("--serve-scenes", type=int, default=64)
     ap.add_argument("--weights", default="", help="--relmse: comma-separated denoiser weight files to compare "
