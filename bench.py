@@ -66,7 +66,8 @@ def relmse(x, ref):
 
 
 def relmse_by_luminance(x, ref):
-    """share of the relMSE sum from pixels binned by reference luminance"""
+    """share of the relMSE sum from pixels binned by reference luminance, and
+    each bin's energy bias sum(x) / sum(ref) - 1"""
     e = ((np.asarray(x, np.float64) - ref) ** 2 / (ref * ref + 1e-2)).mean(axis=-1)
     lum = ref.mean(axis=-1)
     edges = [0.0, 0.01, 0.03, 0.1, 0.3, 1.0, 3.0, np.inf]
@@ -74,7 +75,9 @@ def relmse_by_luminance(x, ref):
     out = {}
     for lo, hi in zip(edges[:-1], edges[1:]):
         m = (lum >= lo) & (lum < hi)
-        out[f"{lo:g}-{hi:g}"] = {"px_frac": round(float(m.mean()), 4), "err_share": round(float(e[m].sum() / total), 4)}
+        xs, rs = np.asarray(x, np.float64)[m].sum(), ref[m].sum()
+        out[f"{lo:g}-{hi:g}"] = {"px_frac": round(float(m.mean()), 4), "err_share": round(float(e[m].sum() / total), 4),
+                                 "bias": round(float(xs / rs - 1.0), 4) if rs > 0 else None}
     return out
 
 
@@ -212,17 +215,6 @@ def relmse_bench(args):
 
     r.enable_aov(True)
     weight_files = [w for w in args.weights.split(",") if w] or [denoiser.WEIGHTS_PATH]
-    compared = {}
-    for wf in weight_files:
-        denoiser.load(wf)
-        r.reset()
-        r.render(4)
-        denoiser.denoise()
-        compared[wf] = relmse(denoiser.denoised(W, H), ref)
-        log(f"weights {wf}: 4 spp denoised relMSE {compared[wf]:.5f}")
-    best = min(compared, key=compared.get)
-    denoiser.load(best)
-    trained = True
 
     def timed_dn(spp):
         r.reset()
@@ -231,24 +223,44 @@ def relmse_bench(args):
         denoiser.denoise()
         return time.perf_counter() - t
 
-    # 4 spp + denoiser (the metric), then the denoised curve at higher spp
-    dn_curve, noisy_curve = {}, {}
-    r.reset()
-    done = 0
-    for spp in (4, 8, 16, 32, 64):
-        r.render(spp - done)
-        done = spp
-        denoiser.denoise()
-        img = denoiser.denoised(W, H)
-        if spp == 4:
-            dn4_img = img
-        dn_curve[spp] = relmse(img, ref)
-        noisy_curve[spp] = relmse(r.accum(), ref)
-        log(f"denoised {spp} spp: relMSE {dn_curve[spp]:.5f} (input {noisy_curve[spp]:.5f})")
+    # the denoised curve (4 spp is the metric; weights trained on several
+    # input spp improve with it) for every weight file
+    curves, noisy_curve, dn4 = {}, {}, {}
+    for wf in weight_files:
+        denoiser.load(wf)
+        r.reset()
+        done = 0
+        cur = curves[wf] = {}
+        for spp in (4, 8, 16, 32, 64):
+            r.render(spp - done)
+            done = spp
+            denoiser.denoise()
+            img = denoiser.denoised(W, H)
+            if spp == 4:
+                dn4[wf] = img
+            cur[spp] = relmse(img, ref)
+            noisy_curve[spp] = relmse(r.accum(), ref)
+            log(f"{os.path.basename(wf)}: denoised {spp} spp: relMSE {cur[spp]:.5f} (input {noisy_curve[spp]:.5f})")
+    compared = {wf: curves[wf][4] for wf in weight_files}
+    best = min(compared, key=compared.get)
+    dn4_img = dn4[best]
+    trained = True
+    dn_curve = curves[best]
+    denoiser.load(best)
     timed_dn(4)
     dn_ms = min(timed_dn(4) for _ in range(3)) * 1e3
-    reach = [k for k in sorted(dn_curve) if dn_curve[k] <= 0.01]
-    dn_reach_ms = min(timed_dn(reach[0]) for _ in range(2)) * 1e3 if reach else None
+    # smallest spp at which some weight file reaches 0.01
+    reach, reach_wf = [], None
+    for spp in (4, 8, 16, 32, 64):
+        ok = [wf for wf in weight_files if curves[wf][spp] <= 0.01]
+        if ok:
+            reach, reach_wf = [spp], min(ok, key=lambda w: curves[w][spp])
+            break
+    dn_reach_ms = None
+    if reach:
+        denoiser.load(reach_wf)
+        dn_reach_ms = min(timed_dn(reach[0]) for _ in range(2)) * 1e3
+        denoiser.load(best)
     out = {"metric": "time to relMSE<=0.01, 1080p env-lit", "reference_spp": args.ref_spp,
            "plain": None if args.skip_plain else {
                "spp": plain_spp, "relmse": round(curve[plain_spp], 5), "ms": round(plain_ms, 2),
@@ -259,6 +271,9 @@ def relmse_bench(args):
                              "weights_compared": {os.path.basename(k): round(v, 5) for k, v in compared.items()}},
            "speedup_4spp": round(plain_ms / dn_ms, 2) if (plain_ms and dn_curve[4] <= 0.01) else None,
            "denoised_min_spp_reaching": reach[0] if reach else None,
+           "denoised_reach_weights": os.path.basename(reach_wf) if reach_wf else None,
+           "denoised_curves": {os.path.basename(w): {str(k): round(v, 5) for k, v in sorted(c.items())}
+                               for w, c in curves.items()},
            "denoised_reach_ms": round(dn_reach_ms, 2) if dn_reach_ms else None,
            "speedup_at_reach": round(plain_ms / dn_reach_ms, 2) if (plain_ms and dn_reach_ms) else None,
            "target_speedup": 5.0,
@@ -286,20 +301,14 @@ def relmse_bench(args):
     print(json.dumps(out), flush=True)
 
 
-def serve_bench(args):
-    """Serving of 64 uneven scenes (seeds 2000..2063): square frames of
-    256..1024 px (multiples of 64), 10k..500k triangles (log-uniform), the
-    terrain-room generator's Disney materials and quad lights plus a
-    procedural sky, 8 spp, depth 10, each frame followed by the denoiser.
-    Scene generation, upload and BVH build are outside the timed region; the
-    first frame of a scene (graph capture) is reported separately from the
-    steady-state frame latency."""
-    from bluetruthengine_amd import denoiser, renderer, scenes
+def serve_scenes(n):
+    """64 uneven scenes (seeds 2000..2063): square frames of 256..1024 px
+    (multiples of 64), 10k..500k triangles (log-uniform), the terrain-room
+    generator's Disney materials and quad lights plus a procedural sky."""
+    from bluetruthengine_amd import scenes
     rs = np.random.default_rng(2000)
-    cold, warm, dn_ms, samples = [], [], [], 0
-    t_all = 0.0
-    have_weights = os.path.exists(denoiser.WEIGHTS_PATH)
-    for k in range(args.serve_scenes):
+    out = []
+    for k in range(n):
         side = int(rs.integers(4, 17)) * 64
         tris = float(np.exp(rs.uniform(np.log(1e4), np.log(5e5))))
         freq = max(4, int(round(np.sqrt(tris / 20.0))))
@@ -308,13 +317,37 @@ def serve_bench(args):
         srng = np.random.default_rng(seed)
         sd.env_rgba = scenes.sky_env(512, 256, seed=seed, sun_elev_deg=srng.uniform(15, 75),
                                      sun_azim_deg=srng.uniform(0, 360))
-        r = renderer.Renderer(sd)
-        r.enable_aov(True)
-        r.build_accel()
+        out.append(sd)
+        if k % 8 == 7:
+            log(f"generated {k + 1}/{n} scenes")
+    return out
+
+
+def serve_bench(args):
+    """Serving of the 64 uneven scenes at 8 spp, depth 10, each frame followed
+    by the denoiser, two ways: all scenes as one batch (every bounce of every
+    scene in one persistent launch, then the U-Net over every slot), and one
+    scene after another.  Scene generation, upload and BVH build are outside
+    the timed regions; a first untimed round captures the graphs."""
+    from bluetruthengine_amd import denoiser, renderer
+    sds = serve_scenes(args.serve_scenes)
+    have_weights = os.path.exists(denoiser.WEIGHTS_PATH)
+    samples = sum(sd.width * sd.height * 8 for sd in sds)
+
+    def load_weights():
         if have_weights:
             denoiser.load()
         else:
             denoiser.init(0)
+
+    # one scene after another
+    cold, warm, dn_ms = [], [], []
+    t_all = 0.0
+    for k, sd in enumerate([] if args.skip_sequential else sds):
+        r = renderer.Renderer(sd)
+        r.enable_aov(True)
+        r.build_accel()
+        load_weights()
         t0 = time.perf_counter()
         r.render(8)
         denoiser.denoise()
@@ -328,15 +361,56 @@ def serve_bench(args):
         warm.append((t2 - t0) * 1e3)
         dn_ms.append((t2 - t1) * 1e3)
         t_all += t2 - t0
-        samples += side * side * 8
-        log(f"scene {k}: {side}x{side}, {sd.triangles.shape[0]} tris: first frame {cold[-1]:.1f} ms, "
-            f"frame {warm[-1]:.1f} ms (denoise {dn_ms[-1]:.2f} ms)")
-    out = {"metric": "batched serving: per-frame latency, 8 spp + denoiser", "scenes": args.serve_scenes,
-           "p50_ms": round(float(np.percentile(warm, 50)), 2), "p99_ms": round(float(np.percentile(warm, 99)), 2),
-           "first_frame_p50_ms": round(float(np.percentile(cold, 50)), 2),
-           "first_frame_p99_ms": round(float(np.percentile(cold, 99)), 2),
-           "denoise_p50_ms": round(float(np.percentile(dn_ms, 50)), 2),
-           "msamples_per_s": round(samples / t_all / 1e6, 1), "mode": "scenes one after another (one scene per launch)",
+        if k % 8 == 7:
+            log(f"sequential: scene {k}: {sd.width}x{sd.height}, {sd.triangles.shape[0]} tris: first frame "
+                f"{cold[-1]:.1f} ms, frame {warm[-1]:.1f} ms (denoise {dn_ms[-1]:.2f} ms)")
+    sequential = None if args.skip_sequential else {"frame_p50_ms": round(float(np.percentile(warm, 50)), 2),
+                  "frame_p99_ms": round(float(np.percentile(warm, 99)), 2),
+                  "first_frame_p50_ms": round(float(np.percentile(cold, 50)), 2),
+                  "denoise_p50_ms": round(float(np.percentile(dn_ms, 50)), 2),
+                  "all_scenes_ms": round(t_all * 1e3, 1),
+                  "msamples_per_s": round(samples / t_all / 1e6, 1)}
+
+    # one batch
+    t0 = time.perf_counter()
+    b = renderer.Batch(max_depth=10, aov=True, graphs=0 if args.no_graphs else 1,
+                       frames_in_flight=args.frames_in_flight, memory_budget_mb=args.memory_budget_mb)
+    for k, sd in enumerate(sds):
+        b.add(sd)
+        if k % 8 == 7:
+            log(f"batch: {k + 1}/{len(sds)} scenes uploaded, BVHs built ({time.perf_counter() - t0:.0f} s)")
+    load_weights()
+    tuned = None
+    if not args.no_autotune:
+        ms, cfg = b.autotune(8)
+        tuned = {k: cfg[k] for k in ("traversalStack", "workChunk", "traversalBlocksPerCU", "bvhWidth")}
+        log(f"batch autotune: best {ms:.1f} ms with {tuned}")
+    rounds = []
+    for it in range(1 + max(1, args.steps)):
+        b.reset()
+        t0 = time.perf_counter()
+        b.render(8)
+        t1 = time.perf_counter()
+        b.denoise()
+        t2 = time.perf_counter()
+        if it > 0:
+            rounds.append(((t2 - t0) * 1e3, (t1 - t0) * 1e3, (t2 - t1) * 1e3))
+        log(f"batch round {it}: {len(sds)} frames in {(t2 - t0) * 1e3:.1f} ms "
+            f"(trace {(t1 - t0) * 1e3:.1f}, denoise {(t2 - t1) * 1e3:.1f})" + (" [warm-up]" if it == 0 else ""))
+    st = b.stats()
+    lat = [x[0] for x in rounds]
+    batch = {"frame_p50_ms": round(float(np.percentile(lat, 50)), 2),
+             "frame_p99_ms": round(float(np.percentile(lat, 99)), 2),
+             "trace_ms": round(float(np.median([x[1] for x in rounds])), 2),
+             "denoise_ms": round(float(np.median([x[2] for x in rounds])), 2),
+             "msamples_per_s": round(samples / (np.median(lat) * 1e-3) / 1e6, 1),
+             "frames_in_flight": st.framesInFlight, "rounds": len(rounds), "autotuned": tuned,
+             "latency_note": "every frame of the batch completes when the batch does"}
+    out = {"metric": "batched serving: per-frame latency, 8 spp + denoiser", "scenes": len(sds),
+           "pixels": int(sum(sd.width * sd.height for sd in sds)),
+           "triangles": int(sum(sd.triangles.shape[0] for sd in sds)),
+           "p50_ms": batch["frame_p50_ms"], "p99_ms": batch["frame_p99_ms"],
+           "batch": batch, "sequential": sequential,
            "weights": "trained" if have_weights else "untrained (random init)"}
     print(json.dumps(out), flush=True)
 
@@ -355,6 +429,10 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident traversal blocks per CU (0 = max)")
     ap.add_argument("--bvh-width", type=int, default=4, help="2 or 4: BVH traversed by extend/connect")
     ap.add_argument("--no-graphs", action="store_true", help="plain launches instead of captured HIP graphs")
+    ap.add_argument("--frames-in-flight", type=int, default=0,
+                    help="frames traced per wavefront pass (0 = as many as the memory budget allows)")
+    ap.add_argument("--memory-budget-mb", type=int, default=0, help="path-state budget (0 = 32768 MB)")
+    ap.add_argument("--skip-sequential", action="store_true", help="--serve: only the batched mode")
     ap.add_argument("--no-autotune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 times) to calibrate rocprof byte counters")
@@ -398,7 +476,8 @@ def main():
     sd = scenes.envlit_c3(spp=args.spp)
     log(f"scene: {sd.triangles.shape[0]} triangles, {sd.bsdfs.shape[0]} bsdfs, generated in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    r = renderer.Renderer(sd, traversal_stack=args.traversal_stack, graphs=0 if args.no_graphs else 1,
+    r = renderer.Renderer(sd, frames_in_flight=args.frames_in_flight, memory_budget_mb=args.memory_budget_mb,
+                          traversal_stack=args.traversal_stack, graphs=0 if args.no_graphs else 1,
                           work_chunk=args.work_chunk, traversal_blocks_per_cu=args.blocks_per_cu,
                           bvh_width=args.bvh_width)
     r.build_accel()

@@ -17,6 +17,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -38,12 +39,9 @@ int fail(int code, const char* fmt, ...) {
 
 namespace {
 
-struct Context {
-    bool deviceReady = false;
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-
+// Everything bte_init_* sets up for one scene.  The context holds the current
+// scene; bte_batch_add_scene moves it into a batch slot.
+struct SceneData {
     // host copies (the BVH is built from triangles on the host)
     std::vector<BteTriangle> tris;
     std::vector<BteFloat3> materials;
@@ -71,6 +69,30 @@ struct Context {
     int envW = 0, envH = 0;
     float envIntensity = 0.0f, envRot = 0.0f, envTotal = 0.0f;
     bool hasEnv = false;
+};
+
+// Multi-scene batch: its scenes, their output buffers and its frame counter.
+struct BatchScene {
+    std::unique_ptr<SceneData> data;
+    int w = 0, h = 0;
+    long long pixelBase = 0;
+};
+
+struct Context : SceneData {
+    bool deviceReady = false;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    std::vector<BatchScene> batch;
+    unsigned long long batchGen = 0;          // changes whenever the slot list changes
+    long long batchPixels = 0;
+    uint32_t batchFrame = 0;
+    DBuf dBatchScenes, dBatchSlots, dBatchChunk, dBatchAccum, dBatchFb, dBatchSum64, dBatchAovAlb, dBatchAovNrm,
+        dBatchDenoised;
+    long long batchLayoutKey = -1;            // frames per pass the slot table was built for
+    unsigned long long batchLayoutGen = ~0ull;
+    bool batchDenoised = false;
 
     // output
     DBuf dAccum, dFb, dSum64;
@@ -136,7 +158,7 @@ int ensureDevice() {
     } while (0)
 
 int rebuildEmitterOfTri() {
-    Context& c = ctx();
+    SceneData& c = ctx();
     const int T = (int)c.tris.size();
     if (c.emitterCount <= 0 || T == 0) {
         c.dEmitterOfTri.release();
@@ -153,8 +175,7 @@ int rebuildEmitterOfTri() {
     return c.dEmitterOfTri.upload(eot.data(), eot.size() * sizeof(int));
 }
 
-int ensureAccel() {
-    Context& c = ctx();
+int ensureAccel(SceneData& c) {
     if (!c.bvhDirty) return BTE_OK;
     const auto t0 = std::chrono::steady_clock::now();
     BvhBuild b = build_bvh2(c.tris.data(), (int)c.tris.size());
@@ -173,8 +194,7 @@ int ensureAccel() {
     return BTE_OK;
 }
 
-DevScene makeScene() {
-    Context& c = ctx();
+DevScene makeScene(const SceneData& c, int width, int height) {
     DevScene s;
     std::memset(&s, 0, sizeof s);   // padding too: scenes are compared bytewise as graph keys
     s.nodes = c.dNodes.as<float4>();
@@ -205,14 +225,13 @@ DevScene makeScene() {
     s.envRot = c.envRot;
     s.envTotal = c.envTotal;
     s.hasEnv = c.hasEnv ? 1 : 0;
-    s.width = c.accW;
-    s.height = c.accH;
+    s.width = width;
+    s.height = height;
     s.triCount = (int)c.tris.size();
     return s;
 }
 
-int validateForRender() {
-    Context& c = ctx();
+int validateForRender(const SceneData& c) {
     if (c.tris.empty()) return fail(BTE_ERR_STATE, "no triangles uploaded (bte_init_scene)");
     if (!c.haveCamera) return fail(BTE_ERR_STATE, "no camera uploaded (bte_init_camera)");
     if (c.bsdfsHost.empty() || c.triBsdfHost.size() != c.tris.size())
@@ -430,7 +449,7 @@ int renderWavefront(int nframes) {
     wb.kindArea = kindArea;
     wb.kindSpot0 = kindSpot0;
     wb.kindEnv = kindEnv;
-    const DevScene sc = makeScene();
+    const DevScene sc = makeScene(c, c.accW, c.accH);
     const LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16,
                                               c.cfg.traversalBlocksPerCU, c.cfg.workChunk, c.cfg.bvhWidth);
     wb.grabRows = ls.grabRows;
@@ -542,7 +561,7 @@ int renderWavefront(int nframes) {
 
 int renderMegakernel(int nframes) {
     Context& c = ctx();
-    const DevScene sc = makeScene();
+    const DevScene sc = makeScene(c, c.accW, c.accH);
     const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4, 2);
     CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
     uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
@@ -571,12 +590,314 @@ int renderFrames(int w, int h, int nframes) {
     ENSURE_DEVICE();
     Context& c = ctx();
     if (nframes <= 0) return fail(BTE_ERR_INVALID, "nframes must be positive");
-    CHECK_RC(validateForRender());
+    CHECK_RC(validateForRender(c));
     if (!c.dAccum.p || c.accW != w || c.accH != h) CHECK_RC(resetAccum(w, h));
-    CHECK_RC(ensureAccel());
+    CHECK_RC(ensureAccel(c));
     CHECK_RC(c.mode == BTE_MODE_MEGAKERNEL ? renderMegakernel(nframes) : renderWavefront(nframes));
     // the reference denoises after the frame when its denoiser is on (src/render_kernel.cu:139-145)
     if (denoiser_active()) CHECK_RC(denoiser_present());
+    return BTE_OK;
+}
+
+// ---------------------------------------------------------------------------
+// multi-scene batches
+// ---------------------------------------------------------------------------
+void batchClear(Context& c) {
+    c.batch.clear();
+    c.batchPixels = 0;
+    c.batchFrame = 0;
+    ++c.batchGen;
+    DBuf* bufs[] = {&c.dBatchScenes, &c.dBatchSlots, &c.dBatchChunk, &c.dBatchAccum, &c.dBatchFb,
+                    &c.dBatchSum64, &c.dBatchAovAlb, &c.dBatchAovNrm, &c.dBatchDenoised};
+    for (DBuf* b : bufs) b->release();
+    c.batchLayoutKey = -1;
+    c.batchDenoised = false;
+}
+
+// (Re)allocates and zeroes the batch's per-pixel outputs; frame index 0.
+int batchResetAccum(Context& c) {
+    const size_t N = (size_t)c.batchPixels;
+    if (N == 0) return BTE_OK;
+    CHECK_RC(c.dBatchAccum.ensure(N * sizeof(float4)));
+    CHECK_RC(c.dBatchFb.ensure(N * sizeof(uint32_t)));
+    HIPCHK(hipMemsetAsync(c.dBatchAccum.p, 0, N * sizeof(float4), c.stream));
+    HIPCHK(hipMemsetAsync(c.dBatchFb.p, 0, N * sizeof(uint32_t), c.stream));
+    if (c.aov) {
+        CHECK_RC(c.dBatchAovAlb.ensure(N * sizeof(float4)));
+        CHECK_RC(c.dBatchAovNrm.ensure(N * sizeof(float4)));
+        HIPCHK(hipMemsetAsync(c.dBatchAovAlb.p, 0, N * sizeof(float4), c.stream));
+        HIPCHK(hipMemsetAsync(c.dBatchAovNrm.p, 0, N * sizeof(float4), c.stream));
+    }
+    if (c.accum64) {
+        CHECK_RC(c.dBatchSum64.ensure(N * 3 * sizeof(double)));
+        HIPCHK(hipMemsetAsync(c.dBatchSum64.p, 0, N * 3 * sizeof(double), c.stream));
+    }
+    HIPCHK(hipStreamSynchronize(c.stream));
+    c.batchFrame = 0;
+    c.batchDenoised = false;
+    return BTE_OK;
+}
+
+// Light classes of a batch: the union over its scenes (a scene without a
+// class simply makes no records of it).
+int batchKinds(const Context& c, int& kindArea, int& kindSpot0, int& kindEnv, int& catMask) {
+    bool area = false, env = false;
+    int spots = 0;
+    catMask = 0;
+    for (const BatchScene& b : c.batch) {
+        area = area || b.data->emitterCount > 0;
+        env = env || b.data->hasEnv;
+        spots = std::max(spots, b.data->spotCount);
+        for (const BteBsdf& x : b.data->bsdfsHost) catMask |= 1 << x.type;
+    }
+    int k = 0;
+    kindArea = area ? k++ : -1;
+    kindSpot0 = k;
+    k += spots;
+    kindEnv = env ? k++ : -1;
+    return k;
+}
+
+// Path-id layout of a pass of F frames: scene after scene, each range rounded
+// up to kBatchChunk so that one byte per chunk names the scene.
+long long batchPaddedPaths(const Context& c, int F) {
+    long long pb = 0;
+    for (const BatchScene& b : c.batch) pb += ((long long)b.w * b.h * F + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
+    return pb;
+}
+
+int batchLayout(Context& c, int F, long long& paddedPaths, long long& slots, int& maxPaths, int& maxPixels) {
+    const int S = (int)c.batch.size();
+    std::vector<BatchSlot> tab((size_t)S);
+    long long pb = 0, sb = 0;
+    maxPaths = maxPixels = 0;
+    for (int i = 0; i < S; ++i) {
+        const long long px = (long long)c.batch[i].w * c.batch[i].h;
+        tab[i] = BatchSlot{pb, sb, (int)(px * F), (int)px, (int)c.batch[i].pixelBase, 0};
+        pb += (px * F + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
+        sb += px * F;
+        maxPaths = std::max(maxPaths, (int)(px * F));
+        maxPixels = std::max(maxPixels, (int)px);
+    }
+    paddedPaths = pb;
+    slots = sb;
+    if (c.batchLayoutKey != F || c.batchLayoutGen != c.batchGen) {
+        std::vector<uint8_t> chunk((size_t)(pb / kBatchChunk), 0);
+        for (int i = 0; i < S; ++i)
+            for (long long k = tab[i].pathBase / kBatchChunk; k * kBatchChunk < tab[i].pathBase + tab[i].paths; ++k)
+                chunk[(size_t)k] = (uint8_t)i;
+        CHECK_RC(c.dBatchSlots.upload(tab.data(), tab.size() * sizeof(BatchSlot)));
+        CHECK_RC(c.dBatchChunk.upload(chunk.data(), chunk.size()));
+        c.batchLayoutKey = F;
+        c.batchLayoutGen = c.batchGen;
+    }
+    return BTE_OK;
+}
+
+int renderBatch(int nframes) {
+    Context& c = ctx();
+    if (c.batch.empty()) return fail(BTE_ERR_STATE, "the batch has no scenes (bte_batch_add_scene)");
+    if (nframes <= 0) return fail(BTE_ERR_INVALID, "nframes must be positive");
+    if (c.mode == BTE_MODE_MEGAKERNEL) return fail(BTE_ERR_UNSUPPORTED, "batches run the wavefront path only");
+    if (!c.dBatchAccum.p || (c.aov && !c.dBatchAovAlb.p) || (c.accum64 && !c.dBatchSum64.p))
+        CHECK_RC(batchResetAccum(c));
+    const int S = (int)c.batch.size();
+    if (!c.dBatchScenes.p || c.dBatchScenes.bytes < (size_t)S * sizeof(DevScene) || c.batchLayoutGen != c.batchGen) {
+        std::vector<DevScene> scenes((size_t)S);
+        for (int i = 0; i < S; ++i) scenes[i] = makeScene(*c.batch[i].data, c.batch[i].w, c.batch[i].h);
+        CHECK_RC(c.dBatchScenes.upload(scenes.data(), scenes.size() * sizeof(DevScene)));
+    }
+    int kindArea, kindSpot0, kindEnv, catMask;
+    const int kinds = batchKinds(c, kindArea, kindSpot0, kindEnv, catMask);
+    const int maxDepth = std::max(1, c.cfg.maxDepth);
+    int F = c.cfg.framesInFlight;
+    if (F <= 0) {
+        const double budget = (double)(c.cfg.memoryBudgetMB > 0 ? c.cfg.memoryBudgetMB : 32768) * 1048576.0;
+        F = (int)std::max(1.0, std::floor(budget / ((double)c.batchPixels * (double)bytesPerPath(kinds))));
+    }
+    F = std::max(1, std::min(F, nframes));
+    // path ids (padded layout) occupy 30 bits of the path word
+    while (F > 1 && batchPaddedPaths(c, F) > (1ll << 30)) --F;
+    if (batchPaddedPaths(c, F) > (1ll << 30)) return fail(BTE_ERR_INVALID, "the batch exceeds 2^30 paths per frame");
+
+    LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16,
+                                        c.cfg.traversalBlocksPerCU, c.cfg.workChunk, c.cfg.bvhWidth);
+    for (const BatchScene& b : c.batch)
+        if (!b.data->dNodes4.p) ls.bvhWidth = 2;   // a scene without the compressed BVH4
+    const bool useGraph = c.cfg.graphs != 0;
+    const size_t ctrInts = (size_t)kCtrStride * (size_t)(maxDepth + 1);
+    std::vector<int> ctrHost(ctrInts);
+    const DevScene* scenesDev = c.dBatchScenes.as<DevScene>();
+    DevScene none;
+    std::memset(&none, 0, sizeof none);
+    double* sum64 = c.accum64 ? c.dBatchSum64.as<double>() : nullptr;
+
+    c.stats = BteStats{};
+    c.stats.framesInFlight = F;
+    HIPCHK(hipEventRecord(c.ev0, c.stream));
+    const long long graphLaunches0 = c.graphLaunches;
+    int done = 0;
+    while (done < nframes) {
+        const int frames = std::min(F, nframes - done);
+        long long padded = 0, slots = 0;
+        int maxPaths = 0, maxPixels = 0;
+        CHECK_RC(batchLayout(c, frames, padded, slots, maxPaths, maxPixels));
+        WaveBuffers wb;
+        std::memset(&wb, 0, sizeof wb);   // padding too: compared bytewise as graph keys
+        const int savedCapture = c.hitCaptureRec;
+        c.hitCaptureRec = 0;
+        const int rc = ensureWaveBuffers(padded, kinds, maxDepth, wb);
+        c.hitCaptureRec = savedCapture;
+        CHECK_RC(rc);
+        wb.hitCapture = nullptr;
+        wb.maxRec = 0;
+        wb.kinds = kinds;
+        wb.travCtr = nullptr;
+        wb.catMask = catMask;
+        wb.kindArea = kindArea;
+        wb.kindSpot0 = kindSpot0;
+        wb.kindEnv = kindEnv;
+        wb.grabRows = ls.grabRows;
+        wb.aovAccAlb = c.aov ? c.dBatchAovAlb.as<float4>() : nullptr;
+        wb.aovAccNrm = c.aov ? c.dBatchAovNrm.as<float4>() : nullptr;
+        wb.sceneOfChunk = c.dBatchChunk.as<uint8_t>();
+        wb.slots = c.dBatchSlots.as<BatchSlot>();
+        wb.sceneCount = S;
+        launch_fill_i32(c.dFrameBase.as<int>(), 1, (int)c.batchFrame, c.stream);
+        constexpr int kSegment = 16;
+        int b = 0;
+        bool presented = false;
+        for (int b0 = 0; b0 < maxDepth; b0 += kSegment) {
+            const int b1 = std::min(b0 + kSegment, maxDepth);
+            const bool last = b1 == maxDepth;
+            std::vector<unsigned char> key;
+            if (useGraph) {
+                const char tag[8] = {'b', 'a', 't', 'c', 'h', 0, 0, 0};
+                appendKey(key, tag);
+                appendKey(key, c.batchGen);
+                appendKey(key, wb);
+                appendKey(key, ls);
+                const long long params[] = {padded, slots, maxPaths, maxPixels, frames, b0, b1, maxDepth,
+                                            (long long)(uintptr_t)scenesDev, (long long)(uintptr_t)sum64,
+                                            (long long)(uintptr_t)c.dBatchAccum.p, (long long)(uintptr_t)c.dBatchFb.p};
+                appendKey(key, params);
+            }
+            CHECK_RC(runSegment(useGraph, key, [&]() {
+                if (b0 == 0) {
+                    launch_fill_i32(wb.counters, (long long)ctrInts, 0, c.stream);
+                    launch_generate_batch(scenesDev, wb, maxPaths, slots, c.stream);
+                }
+                for (int bb = b0; bb < b1; ++bb) launch_bounce(none, wb, bb, maxDepth, ls, c.stream, nullptr, scenesDev);
+                if (last)
+                    launch_present_batch(wb, maxPixels, frames, c.dBatchAccum.as<float4>(), sum64,
+                                         c.dBatchFb.as<uint32_t>(), c.stream);
+            }));
+            b = b1;
+            if (last) {
+                presented = true;
+                break;
+            }
+            int next = 0;
+            HIPCHK(hipMemcpyAsync(&next, wb.counters + (size_t)b1 * kCtrStride + kCtrRays, sizeof(int),
+                                  hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            if (next == 0) break;
+        }
+        if (!presented)
+            launch_present_batch(wb, maxPixels, frames, c.dBatchAccum.as<float4>(), sum64, c.dBatchFb.as<uint32_t>(),
+                                 c.stream);
+        HIPCHK(hipMemcpyAsync(ctrHost.data(), wb.counters, ctrInts * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        HIPCHK(hipGetLastError());
+        for (int i = 0; i < b; ++i) {
+            const int* cb = ctrHost.data() + (size_t)i * kCtrStride;
+            c.stats.extensionRays += cb[kCtrRays];
+            c.stats.hits += cb[kCtrHits];
+            for (int k = 0; k < kinds; ++k) c.stats.shadowRays += cb[kCtrShadow + k];
+            if (i < 64) c.stats.raysPerBounce[i] += cb[kCtrRays];
+        }
+        c.stats.bounces = std::max(c.stats.bounces, b);
+        c.stats.pathsTraced += slots;
+        c.batchFrame += (uint32_t)frames;
+        done += frames;
+    }
+    HIPCHK(hipEventRecord(c.ev1, c.stream));
+    HIPCHK(hipEventSynchronize(c.ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c.ev0, c.ev1));
+    c.stats.renderMs = ms;
+    c.stats.graphLaunches = (int)(c.graphLaunches - graphLaunches0);
+    c.batchDenoised = false;
+    return BTE_OK;
+}
+
+// Coordinate descent over the launch shape (BVH width, LDS stack, work chunk,
+// traversal blocks per CU).  timed(ms) renders one candidate's workload from a
+// reset accumulation; each candidate runs twice (the first run captures its
+// graphs) and keeps the second time.  Leaves the best configuration set.
+template <class F>
+int autotune(F&& timed, double* msBest) {
+    Context& c = ctx();
+    const BteConfig saved = c.cfg;
+    const int savedInstr = c.instr;
+    c.instr = 0;
+    auto timeOne = [&](const BteConfig& cand, double& ms) -> int {
+        c.cfg = cand;
+        CHECK_RC(timed(ms));
+        return timed(ms);
+    };
+    BteConfig best = saved;
+    if (best.traversalStack == 0) best.traversalStack = 16;
+    if (best.workChunk == 0) best.workChunk = 4;
+    if (best.bvhWidth == 0) best.bvhWidth = 2;
+    double bestMs = 0.0;
+    int rc = timeOne(best, bestMs);
+    auto tryCand = [&](BteConfig cand) {
+        double ms = 0.0;
+        if (rc == BTE_OK && (rc = timeOne(cand, ms)) == BTE_OK && ms < bestMs) {
+            bestMs = ms;
+            best = cand;
+        }
+    };
+    {
+        BteConfig cand = best;
+        cand.bvhWidth = best.bvhWidth == 4 ? 2 : 4;
+        tryCand(cand);
+    }
+    for (int st : {8, 16, 32}) {
+        BteConfig cand = best;
+        cand.traversalStack = st;
+        if (st != best.traversalStack) tryCand(cand);
+    }
+    const BteConfig afterStack = best;
+    for (int ch : {1, 2, 4, 8}) {
+        BteConfig cand = afterStack;
+        cand.workChunk = ch;
+        if (ch != afterStack.workChunk) tryCand(cand);
+    }
+    int cus = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus <= 0) cus = 1;
+    const int occ = std::max(
+        1, query_launch_shape(c.device, best.traversalStack, 0, best.workChunk, best.bvhWidth).extendBlocks / cus);
+    const BteConfig afterChunk = best;
+    for (int bpc : {0, (3 * occ) / 4, occ / 2}) {
+        BteConfig cand = afterChunk;
+        cand.traversalBlocksPerCU = bpc;
+        if (bpc != afterChunk.traversalBlocksPerCU && (bpc == 0 || bpc < occ)) tryCand(cand);
+    }
+    c.instr = savedInstr;
+    if (rc != BTE_OK) {
+        c.cfg = saved;
+        return rc;
+    }
+    c.cfg = best;
+    if (msBest) *msBest = bestMs;
+    return BTE_OK;
+}
+
+int batchSlot(const Context& c, int slot) {
+    if (slot < 0 || slot >= (int)c.batch.size())
+        return fail(BTE_ERR_INVALID, "batch slot %d out of range (%d scenes)", slot, (int)c.batch.size());
     return BTE_OK;
 }
 
@@ -801,7 +1122,7 @@ int bte_init_env_map(const float* hdriPixelsRGBA, int width, int height, float i
 int bte_build_accel(void) {
     ENSURE_DEVICE();
     ctx().bvhDirty = true;
-    return ensureAccel();
+    return ensureAccel(ctx());
 }
 
 int bte_register_framebuffer(uint32_t* deviceRGBA8) {
@@ -835,6 +1156,7 @@ int bte_cleanup(void) {
     if (c.deviceReady) (void)hipStreamSynchronize(c.stream);
     dropGraphs();
     denoiser_release();
+    batchClear(c);
     c.dFrameBase.release();
     c.dAovAlb.release(); c.dAovNrm.release(); c.dAovAccAlb.release(); c.dAovAccNrm.release();
     c.aov = false;
@@ -890,67 +1212,28 @@ int bte_autotune(int width, int height, int frames, double* msBest) {
     Context& c = ctx();
     if (width <= 0 || height <= 0 || frames <= 0) return fail(BTE_ERR_INVALID, "autotune needs positive sizes");
     if (c.mode != BTE_MODE_WAVEFRONT) return fail(BTE_ERR_STATE, "autotune applies to the wavefront mode");
-    const BteConfig saved = c.cfg;
-    const int savedInstr = c.instr;
-    c.instr = 0;
-    // one warm-up pass (captures the candidate's graphs), then one timed pass
-    auto timeOne = [&](const BteConfig& cand, double& ms) -> int {
-        c.cfg = cand;
-        CHECK_RC(bte_reset_accumulation(width, height));
-        CHECK_RC(renderFrames(width, height, frames));
+    CHECK_RC(autotune([&](double& ms) -> int {
         CHECK_RC(bte_reset_accumulation(width, height));
         CHECK_RC(renderFrames(width, height, frames));
         ms = c.stats.renderMs;
         return BTE_OK;
-    };
-    BteConfig best = saved;
-    if (best.traversalStack == 0) best.traversalStack = 16;
-    if (best.workChunk == 0) best.workChunk = 4;
-    if (best.bvhWidth == 0) best.bvhWidth = 2;
-    double bestMs = 0.0;
-    int rc = timeOne(best, bestMs);
-    auto tryCand = [&](BteConfig cand) {
-        double ms = 0.0;
-        if (rc == BTE_OK && (rc = timeOne(cand, ms)) == BTE_OK && ms < bestMs) {
-            bestMs = ms;
-            best = cand;
-        }
-    };
-    {
-        BteConfig cand = best;
-        cand.bvhWidth = best.bvhWidth == 4 ? 2 : 4;
-        tryCand(cand);
-    }
-    for (int st : {8, 16, 32}) {
-        BteConfig cand = best;
-        cand.traversalStack = st;
-        if (st != best.traversalStack) tryCand(cand);
-    }
-    const BteConfig afterStack = best;
-    for (int ch : {1, 2, 4, 8}) {
-        BteConfig cand = afterStack;
-        cand.workChunk = ch;
-        if (ch != afterStack.workChunk) tryCand(cand);
-    }
-    int cus = 1;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device) != hipSuccess || cus <= 0) cus = 1;
-    const int occ = std::max(
-        1, query_launch_shape(c.device, best.traversalStack, 0, best.workChunk, best.bvhWidth).extendBlocks / cus);
-    const BteConfig afterChunk = best;
-    for (int bpc : {0, (3 * occ) / 4, occ / 2}) {
-        BteConfig cand = afterChunk;
-        cand.traversalBlocksPerCU = bpc;
-        if (bpc != afterChunk.traversalBlocksPerCU && (bpc == 0 || bpc < occ)) tryCand(cand);
-    }
-    c.instr = savedInstr;
-    if (rc != BTE_OK) {
-        c.cfg = saved;
-        return rc;
-    }
-    c.cfg = best;
-    CHECK_RC(bte_reset_accumulation(width, height));
-    if (msBest) *msBest = bestMs;
-    return BTE_OK;
+    }, msBest));
+    return bte_reset_accumulation(width, height);
+}
+
+int bte_batch_autotune(int frames, double* msBest) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (frames <= 0) return fail(BTE_ERR_INVALID, "autotune needs a positive frame count");
+    if (c.batch.empty()) return fail(BTE_ERR_STATE, "the batch has no scenes");
+    if (c.mode != BTE_MODE_WAVEFRONT) return fail(BTE_ERR_STATE, "autotune applies to the wavefront mode");
+    CHECK_RC(autotune([&](double& ms) -> int {
+        CHECK_RC(batchResetAccum(c));
+        CHECK_RC(renderBatch(frames));
+        ms = c.stats.renderMs;
+        return BTE_OK;
+    }, msBest));
+    return batchResetAccum(c);
 }
 
 int bte_get_stats(BteStats* stats) {
@@ -1112,7 +1395,7 @@ int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, floa
     if (!rays || !tri || (!anyHit && (!t || !u || !v))) return fail(BTE_ERR_INVALID, "bte_trace_rays: null buffer");
     if (n >= (1ll << 31) - 128) return fail(BTE_ERR_INVALID, "bte_trace_rays: too many rays");
     if (c.tris.empty()) return fail(BTE_ERR_STATE, "no triangles uploaded");
-    CHECK_RC(ensureAccel());
+    CHECK_RC(ensureAccel(c));
     if (c.accW <= 0) { c.accW = std::max(1, c.sceneW); c.accH = std::max(1, c.sceneH); }
     const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4, 2);
     CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
@@ -1123,7 +1406,7 @@ int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, floa
     CHECK_RC(dt.ensure((size_t)n * sizeof(float)));
     CHECK_RC(du.ensure((size_t)n * sizeof(float)));
     CHECK_RC(dv.ensure((size_t)n * sizeof(float)));
-    const DevScene sc = makeScene();
+    const DevScene sc = makeScene(c, c.accW, c.accH);
     launch_trace_rays(sc, dr.as<float>(), (anyHit && tmaxPerRay) ? dtm.as<float>() : nullptr, n, tmin, tmax, anyHit,
                       dtri.as<int>(), dt.as<float>(), du.as<float>(), dv.as<float>(), c.dOvf.as<int>(), c.stream);
     HIPCHK(hipStreamSynchronize(c.stream));
@@ -1133,6 +1416,144 @@ int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, floa
         HIPCHK(hipMemcpy(t, dt.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(u, du.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(v, dv.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return BTE_OK;
+}
+
+// ---- multi-scene batches -------------------------------------------------
+int bte_batch_add_scene(int width, int height) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (width <= 0 || height <= 0) return fail(BTE_ERR_INVALID, "invalid resolution %dx%d", width, height);
+    if ((long long)width * height > (1ll << 30)) return fail(BTE_ERR_INVALID, "more than 2^30 pixels");
+    if (c.batch.size() >= 255) return fail(BTE_ERR_UNSUPPORTED, "a batch holds at most 255 scenes");
+    CHECK_RC(validateForRender(c));
+    CHECK_RC(ensureAccel(c));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    dropGraphs();
+    BatchScene b;
+    b.data = std::make_unique<SceneData>(std::move(static_cast<SceneData&>(c)));
+    static_cast<SceneData&>(c) = SceneData();
+    b.w = width;
+    b.h = height;
+    b.pixelBase = c.batchPixels;
+    c.batchPixels += (long long)width * height;
+    c.batch.push_back(std::move(b));
+    ++c.batchGen;
+    CHECK_RC(batchResetAccum(c));
+    return (int)c.batch.size() - 1;
+}
+
+int bte_batch_scene_count(void) { return (int)ctx().batch.size(); }
+
+int bte_batch_render(int nframes) {
+    ENSURE_DEVICE();
+    CHECK_RC(renderBatch(nframes));
+    return BTE_OK;
+}
+
+int bte_batch_reset(void) {
+    ENSURE_DEVICE();
+    return batchResetAccum(ctx());
+}
+
+int bte_batch_clear(void) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    HIPCHK(hipStreamSynchronize(c.stream));
+    dropGraphs();
+    batchClear(c);
+    return BTE_OK;
+}
+
+int bte_batch_frame_index(void) { return (int)ctx().batchFrame; }
+
+int bte_batch_read_accum(int slot, float* hostRGB, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    CHECK_RC(batchSlot(c, slot));
+    const BatchScene& b = c.batch[slot];
+    const size_t N = (size_t)b.w * b.h;
+    if (!hostRGB || count < 3 * N) return fail(BTE_ERR_INVALID, "accum copy needs %zu floats", 3 * N);
+    if (!c.dBatchAccum.p) return fail(BTE_ERR_STATE, "no batch accumulation (render first)");
+    std::vector<float> tmp(4 * N);
+    HIPCHK(hipMemcpy(tmp.data(), c.dBatchAccum.as<float4>() + b.pixelBase, N * sizeof(float4),
+                     hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < N; ++i) {
+        hostRGB[3 * i] = tmp[4 * i];
+        hostRGB[3 * i + 1] = tmp[4 * i + 1];
+        hostRGB[3 * i + 2] = tmp[4 * i + 2];
+    }
+    return BTE_OK;
+}
+
+int bte_batch_read_framebuffer(int slot, uint32_t* hostRGBA8, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    CHECK_RC(batchSlot(c, slot));
+    const BatchScene& b = c.batch[slot];
+    const size_t N = (size_t)b.w * b.h;
+    if (!hostRGBA8 || count < N) return fail(BTE_ERR_INVALID, "framebuffer copy needs %zu pixels", N);
+    if (!c.dBatchFb.p) return fail(BTE_ERR_STATE, "no batch framebuffer (render first)");
+    HIPCHK(hipMemcpy(hostRGBA8, c.dBatchFb.as<uint32_t>() + b.pixelBase, N * sizeof(uint32_t),
+                     hipMemcpyDeviceToHost));
+    return BTE_OK;
+}
+
+int bte_batch_read_accum64(int slot, double* hostRGB, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    CHECK_RC(batchSlot(c, slot));
+    const BatchScene& b = c.batch[slot];
+    const size_t N = (size_t)b.w * b.h;
+    if (!hostRGB || count < 3 * N) return fail(BTE_ERR_INVALID, "accum64 copy needs %zu doubles", 3 * N);
+    if (!c.accum64 || !c.dBatchSum64.p) return fail(BTE_ERR_STATE, "binary64 accumulation is off");
+    HIPCHK(hipMemcpy(hostRGB, c.dBatchSum64.as<double>() + 3 * (size_t)b.pixelBase, 3 * N * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    return BTE_OK;
+}
+
+int bte_batch_denoise(void) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    if (c.batch.empty()) return fail(BTE_ERR_STATE, "the batch has no scenes");
+    if (!c.aov || !c.dBatchAovAlb.p || c.batchFrame == 0)
+        return fail(BTE_ERR_STATE, "batch denoising needs AOVs (bte_enable_aov before bte_batch_render) and a render");
+    CHECK_RC(c.dBatchDenoised.ensure((size_t)c.batchPixels * sizeof(float4)));
+    for (const BatchScene& b : c.batch) {
+        RenderView v;
+        v.stream = c.stream;
+        v.device = c.device;
+        v.width = b.w;
+        v.height = b.h;
+        v.accum = c.dBatchAccum.as<float4>() + b.pixelBase;
+        v.aovAlb = c.dBatchAovAlb.as<float4>() + b.pixelBase;
+        v.aovNrm = c.dBatchAovNrm.as<float4>() + b.pixelBase;
+        v.fb = c.dBatchFb.as<uint32_t>() + b.pixelBase;
+        v.frameIndex = c.batchFrame;
+        CHECK_RC(denoise_view(v, c.dBatchDenoised.as<float4>() + b.pixelBase));
+    }
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipGetLastError());
+    c.batchDenoised = true;
+    return BTE_OK;
+}
+
+int bte_batch_read_denoised(int slot, float* hostRGB, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    CHECK_RC(batchSlot(c, slot));
+    const BatchScene& b = c.batch[slot];
+    const size_t N = (size_t)b.w * b.h;
+    if (!hostRGB || count < 3 * N) return fail(BTE_ERR_INVALID, "denoised copy needs %zu floats", 3 * N);
+    if (!c.batchDenoised) return fail(BTE_ERR_STATE, "the batch has not been denoised since its last render");
+    std::vector<float> tmp(4 * N);
+    HIPCHK(hipMemcpy(tmp.data(), c.dBatchDenoised.as<float4>() + b.pixelBase, N * sizeof(float4),
+                     hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < N; ++i) {
+        hostRGB[3 * i] = tmp[4 * i];
+        hostRGB[3 * i + 1] = tmp[4 * i + 1];
+        hostRGB[3 * i + 2] = tmp[4 * i + 2];
     }
     return BTE_OK;
 }

@@ -394,17 +394,24 @@ uint64_t splitmix(uint64_t& s) {
 
 bool denoiser_active() { return net().enabled && net().ready; }
 
+int denoise_view(const RenderView& v, float4* out) {
+    Net& N = net();
+    if (!N.ready) return fail(BTE_ERR_STATE, "denoiser weights not loaded");
+    if (!v.aovAlb) return fail(BTE_ERR_STATE, "the denoiser needs the albedo/normal AOVs (bte_enable_aov)");
+    const int hp = (v.height + 7) / 8 * 8, wp = (v.width + 7) / 8 * 8;
+    CHECK_RC(ensureWorkspace(N, 1, hp, wp, false));
+    dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, wp, hp, T(N, T_X), v.stream);
+    forward(N, v.stream);
+    dn::launch_output(N.o.as<float>(), wp, v.width, v.height, out, v.fb, v.stream);
+    return BTE_OK;
+}
+
 int denoiser_present() {
     Net& N = net();
     RenderView v;
     CHECK_RC(render_view(v));
-    if (!v.aovAlb) return fail(BTE_ERR_STATE, "the denoiser needs the albedo/normal AOVs (bte_enable_aov)");
-    const int hp = (v.height + 7) / 8 * 8, wp = (v.width + 7) / 8 * 8;
-    CHECK_RC(ensureWorkspace(N, 1, hp, wp, false));
     CHECK_RC(N.denoised.ensure((size_t)v.width * v.height * sizeof(float4)));
-    dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, wp, hp, T(N, T_X), v.stream);
-    forward(N, v.stream);
-    dn::launch_output(N.o.as<float>(), wp, v.width, v.height, N.denoised.as<float4>(), v.fb, v.stream);
+    CHECK_RC(denoise_view(v, N.denoised.as<float4>()));
     HIPCHK(hipStreamSynchronize(v.stream));
     HIPCHK(hipGetLastError());
     N.outW = v.width;

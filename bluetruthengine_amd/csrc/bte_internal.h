@@ -35,6 +35,20 @@ struct DBuf {
     DBuf() = default;
     DBuf(const DBuf&) = delete;
     DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), bytes(o.bytes) {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     ~DBuf() { release(); }
     void release() {
         if (p) (void)hipFree(p);
@@ -63,6 +77,7 @@ struct DBuf {
 };
 
 // The renderer's current accumulation target.
+// A render target of the batch API or the single-scene one.
 struct RenderView {
     hipStream_t stream = nullptr;
     int device = 0;
@@ -80,6 +95,9 @@ int render_view(RenderView& v);        // BTE_ERR_STATE when nothing was accumul
 // Denoiser hooks called by the render entry points.
 bool denoiser_active();
 int denoiser_present();                // denoise the accumulation into the display buffer
+// Denoise one view into `out` (float4 per pixel, linear) and the view's display
+// buffer; asynchronous on v.stream.
+int denoise_view(const RenderView& v, float4* out);
 void denoiser_release();               // bte_cleanup
 
 }  // namespace bte

@@ -51,6 +51,20 @@ constexpr int kCtrCat = 4 + 2 * kMaxShadowKinds;     // + category: rays queued 
 constexpr int kCtrShadeWork = kCtrCat + kNumCats;    // + category
 constexpr int kCtrStride = kCtrShadeWork + kNumCats;
 
+// Multi-scene batch: where each scene's paths and pixels live.  Path ids are
+// laid out scene by scene, each scene's range starting at a multiple of
+// kBatchChunk, so the scene of a path id is one byte-table lookup.
+constexpr int kBatchChunkShift = 12;
+constexpr long long kBatchChunk = 1ll << kBatchChunkShift;
+struct BatchSlot {
+    long long pathBase;   // first path id of the scene (multiple of kBatchChunk)
+    long long slotBase;   // first queue slot of the scene in the generated queue
+    int paths;            // pixels * frames of this pass
+    int pixels;
+    int pixelBase;        // first pixel of the scene in the batch's output buffers
+    int pad;
+};
+
 // Wavefront buffers.  Path state lives in queue order and is compacted into
 // the other half of each double buffer by the shade kernel.
 struct WaveBuffers {
@@ -81,6 +95,10 @@ struct WaveBuffers {
     float4* aovAccAlb;    // per pixel running means of the two AOVs (with aovAlb)
     float4* aovAccNrm;
     const int* frameBase; // device int: frame index of the batch's first frame (set before each batch)
+    // multi-scene batches only (null / 0 for a single scene)
+    const uint8_t* sceneOfChunk;   // scene of each kBatchChunk-aligned block of path ids
+    const BatchSlot* slots;        // [sceneCount]
+    int sceneCount;
 };
 
 struct LaunchShape {
@@ -94,11 +112,18 @@ struct LaunchShape {
 void launch_fill_i32(int* p, long long n, int value, hipStream_t s);
 void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths, int pixels, hipStream_t s);
 // ev (optional): 4 events recorded before extend, after extend, after shade,
-// after the connect passes.
+// after the connect passes.  scenes (device array, wb.sceneCount entries) selects
+// the multi-scene kernels; sc is then unused.
 void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
-                   hipStream_t s, hipEvent_t* ev);
+                   hipStream_t s, hipEvent_t* ev, const DevScene* scenes = nullptr);
 void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, float4* accum,
                     double* sum64, uint32_t* fb, hipStream_t s);
+// Multi-scene pass: paths of every scene (wb.slots) into queue slots
+// [0, totalSlots); present writes each scene's pixels at its pixelBase.
+void launch_generate_batch(const DevScene* scenes, const WaveBuffers& wb, int maxScenePaths, long long totalSlots,
+                           hipStream_t s);
+void launch_present_batch(const WaveBuffers& wb, int maxScenePixels, int frames, float4* accum, double* sum64,
+                          uint32_t* fb, hipStream_t s);
 void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex, float4* accum, double* sum64,
                        uint32_t* fb, int* ovfStack, hipStream_t s);
 void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxPerRay, long long n, float tmin,

@@ -65,6 +65,64 @@ BTE_DEV int wave_grab(int* ctr, int n = 64) {
 }
 
 // ---------------------------------------------------------------------------
+// multi-scene batches
+// A batch keeps one DevScene per scene in device memory.  A wave whose lanes
+// hold paths of several scenes handles them one scene per round (a
+// "waterfall"): within a round the scene index is wave-uniform, so its
+// descriptor arrives through scalar loads, as a kernel argument would.  The
+// rounds are uniform loops (every lane iterates) so that the round body may
+// hold wave-wide operations.
+// ---------------------------------------------------------------------------
+BTE_DEV int sceneOfPath(const WaveBuffers& wb, uint32_t pid) {
+    return (int)wb.sceneOfChunk[pid >> kBatchChunkShift];
+}
+
+// A pointer read from memory is generic to the compiler; rebuilt from its
+// integer value as a global-space pointer, the loads through it become global
+// (not flat) loads.
+template <class T>
+BTE_DEV const T* globalPtr(const T* p) {
+    return (const T*)(const __attribute__((address_space(1))) T*)(uintptr_t)p;
+}
+
+// scenes must be a __restrict__ kernel argument and s wave-uniform.
+BTE_DEV DevScene fetchScene(const DevScene* __restrict__ scenes, int s) {
+    DevScene sc = scenes[s];
+    sc.nodes = globalPtr(sc.nodes);
+    sc.nodes4 = globalPtr(sc.nodes4);
+    sc.leafTris = globalPtr(sc.leafTris);
+    sc.tris = globalPtr(sc.tris);
+    sc.triBsdf = globalPtr(sc.triBsdf);
+    sc.bsdfs = globalPtr(sc.bsdfs);
+    sc.triEmission = globalPtr(sc.triEmission);
+    sc.triFlags = globalPtr(sc.triFlags);
+    sc.emitterOfTri = globalPtr(sc.emitterOfTri);
+    sc.emitters = globalPtr(sc.emitters);
+    sc.emTriIdx = globalPtr(sc.emTriIdx);
+    sc.emTriCdf = globalPtr(sc.emTriCdf);
+    sc.sceneCdf = globalPtr(sc.sceneCdf);
+    sc.spots = globalPtr(sc.spots);
+    sc.env = globalPtr(sc.env);
+    sc.envMarg = globalPtr(sc.envMarg);
+    sc.envCond = globalPtr(sc.envCond);
+    sc.envMargGuide = globalPtr(sc.envMargGuide);
+    sc.envCondGuide = globalPtr(sc.envCondGuide);
+    return sc;
+}
+
+// One waterfall round: the scene of the first pending lane (wave-uniform) and
+// whether this lane belongs to it.  The membership test goes through a ballot
+// so that the compiler cannot substitute the lane's own (divergent) scene id
+// for the uniform one inside the round.
+BTE_DEV int waterfallScene(bool pending, int sid, bool& mine) {
+    const unsigned long long pm = __ballot(pending);
+    const int s = __builtin_amdgcn_readfirstlane(__shfl(sid, __ffsll((long long)pm) - 1, 64));
+    const unsigned long long match = __ballot(pending && sid == s);
+    mine = ((match >> lane_id()) & 1ull) != 0;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
 // BVH2 traversal (replaces optixTrace: rt_optix_shading.cuh:57-95)
 // Watertight ray/triangle test of Woop, Benthin & Wald (JCGT 2013); ties on t
 // resolve to the lower triangle index so the result is independent of the
@@ -786,27 +844,47 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
     if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb, long long paths, int pixels) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= paths) return;
-    const int frame0 = *wb.frameBase;
-    const int pixel = (int)(i % pixels);
-    const int frame = frame0 + (int)(i / pixels);
+// One camera path: queue slot `slot`, path id `pid`, pixel / frame seed it.
+BTE_DEV void generateOne(const DevScene& sc, const WaveBuffers& wb, long long slot, uint32_t pid, int pixel,
+                         int frame) {
     const int px = pixel % sc.width, py = pixel / sc.width;
     uint32_t rng = seedRng(hashUint((uint32_t)pixel ^ ((uint32_t)frame * 0x9e3779b9u)));
     const float jx = rngNext(rng) - 0.5f;
     const float jy = rngNext(rng) - 0.5f;
     const V3 d = primaryDir(sc.cam, px, py, sc.width, sc.height, jx, jy);
-    wb.rayO[0][i] = make_float4(sc.cam.origin.x, sc.cam.origin.y, sc.cam.origin.z, 0.0f);
-    wb.rayD[0][i] = make_float4(d.x, d.y, d.z, 1.0f);
-    wb.thr[0][i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    wb.rad[0][i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    wb.st[0][i] = make_uint2(rng, (uint32_t)i | kSpecBit);
+    wb.rayO[0][slot] = make_float4(sc.cam.origin.x, sc.cam.origin.y, sc.cam.origin.z, 0.0f);
+    wb.rayD[0][slot] = make_float4(d.x, d.y, d.z, 1.0f);
+    wb.thr[0][slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    wb.rad[0][slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    wb.st[0][slot] = make_uint2(rng, pid | kSpecBit);
     if (wb.aovAlb) {
-        wb.aovAlb[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        wb.aovNrm[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        wb.aovAlb[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        wb.aovNrm[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb, long long paths, int pixels) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= paths) return;
+    const int frame0 = *wb.frameBase;
+    generateOne(sc, wb, i, (uint32_t)i, (int)(i % pixels), frame0 + (int)(i / pixels));
     if (i == 0) wb.counters[kCtrRays] = (int)paths;
+}
+
+// Batch: blockIdx.y = scene.  Scene s's paths take path ids from its pathBase
+// and queue slots from its slotBase; pixel and frame seeds are the scene's own,
+// so every scene renders exactly as it would alone.
+__global__ __launch_bounds__(kBlock) void k_generate_batch(const DevScene* __restrict__ scenes, WaveBuffers wb,
+                                                            long long totalSlots) {
+    const int s = blockIdx.y;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const BatchSlot b = wb.slots[s];
+    if (i >= b.paths) return;
+    const DevScene sc = fetchScene(scenes, s);
+    const int frame0 = *wb.frameBase;
+    generateOne(sc, wb, b.slotBase + i, (uint32_t)(b.pathBase + i), (int)(i % b.pixels),
+                frame0 + (int)(i / b.pixels));
+    if (s == 0 && i == 0) wb.counters[kCtrRays] = (int)totalSlots;
 }
 
 // Shading category of a finished extension ray: 0 = miss, else the BsdfType.
@@ -819,7 +897,19 @@ BTE_DEV int shadeCategory(const DevScene& sc, int tri) {
 // take 64 consecutive queue slots per atomic: consecutive slots hold rays that
 // were adjacent before compaction, so a wave's lanes share BVH cache lines.
 template <bool COUNT, int L, int W>
-__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, int bounce) {
+BTE_DEV void extendOne(const DevScene& sc, const WaveBuffers& wb, int cur, int qi, const StackRef<L>& stk,
+                       uint32_t* nodes, uint32_t* tris) {
+    const float4 o = wb.rayO[cur][qi];
+    const float4 d = wb.rayD[cur][qi];
+    float t = 0.0f, u = 0.0f, v = 0.0f;
+    const int tri = traverseW<W, false, COUNT, L>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, nodes, tris);
+    wb.hit[qi] = make_float4(__int_as_float(tri), t, u, v);
+    wb.cat[qi] = (uint8_t)shadeCategory(sc, tri);
+}
+
+template <bool COUNT, int L, int W, bool BATCH = false>
+__global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, int bounce,
+                                                    const DevScene* __restrict__ scenes) {
     __shared__ int lds[L * kBlock];
     const int cur = bounce & 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
@@ -831,15 +921,21 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
         const int chunk = wave_grab(ctr + kCtrExtendWork, 64 * wb.grabRows);
         if (chunk >= count) break;
         for (int row = 0; row < wb.grabRows; ++row) {
-        const int qi = chunk + row * 64 + lane_id();
-        if (qi < count) {
-            const float4 o = wb.rayO[cur][qi];
-            const float4 d = wb.rayD[cur][qi];
-            float t = 0.0f, u = 0.0f, v = 0.0f;
-            const int tri = traverseW<W, false, COUNT, L>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, &nodes, &tris);
-            wb.hit[qi] = make_float4(__int_as_float(tri), t, u, v);
-            wb.cat[qi] = (uint8_t)shadeCategory(sc, tri);
-        }
+            const int qi = chunk + row * 64 + lane_id();
+            if constexpr (!BATCH) {
+                if (qi < count) extendOne<COUNT, L, W>(sc, wb, cur, qi, stk, &nodes, &tris);
+            } else {
+                bool pending = qi < count;
+                const int sid = pending ? sceneOfPath(wb, wb.st[cur][qi].y & kPidMask) : 0;
+                while (__ballot(pending) != 0ull) {
+                    bool mine;
+                    const int s = waterfallScene(pending, sid, mine);
+                    if (mine) {
+                        extendOne<COUNT, L, W>(fetchScene(scenes, s), wb, cur, qi, stk, &nodes, &tris);
+                        pending = false;
+                    }
+                }
+            }
         }
     }
     if (COUNT) {
@@ -870,7 +966,9 @@ BTE_DEV void shadeMiss(const DevScene& sc, const WaveBuffers& wb, int qi, int bo
 
 constexpr int kCatSub = 4;   // 64-slot rows per wave per iteration
 // Misses are shaded here, in queue order, and are not queued.
-__global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers wb, int bounce) {
+template <bool BATCH = false>
+__global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers wb, int bounce,
+                                                        const DevScene* __restrict__ scenes) {
     __shared__ int sCnt[kCatSub][kBlock / 64][kNumCats];
     __shared__ int sBase[kNumCats];
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
@@ -882,9 +980,23 @@ __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers 
         for (int j = 0; j < kCatSub; ++j) {
             const int i = base + j * kBlock + wave * 64 + lane;
             c[j] = (i < count) ? (int)wb.cat[i] : -1;
-            if (c[j] == kCatMiss) {
-                shadeMiss(sc, wb, i, bounce);
-                c[j] = -1;
+            if constexpr (!BATCH) {
+                if (c[j] == kCatMiss) {
+                    shadeMiss(sc, wb, i, bounce);
+                    c[j] = -1;
+                }
+            } else {
+                bool pending = c[j] == kCatMiss;
+                const int sid = pending ? sceneOfPath(wb, wb.st[bounce & 1][i].y & kPidMask) : 0;
+                if (pending) c[j] = -1;
+                while (__ballot(pending) != 0ull) {
+                    bool mine;
+                    const int s = waterfallScene(pending, sid, mine);
+                    if (mine) {
+                        shadeMiss(fetchScene(scenes, s), wb, i, bounce);
+                        pending = false;
+                    }
+                }
             }
             for (int k = 1; k < kNumCats; ++k) {
                 const unsigned long long m = __ballot(c[j] == k);
@@ -920,105 +1032,128 @@ __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers 
     }
 }
 
-// Shade one category of hits (T = BsdfType; misses are shaded by k_categorize).
+// Shades one 64-slot row of a category queue: lanes with `active` set hold
+// the hits at list[li].  Every lane of the wave must call it (wave-wide
+// appends).  sid is stored with the shadow records for the connect pass.
 template <int T>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, int bounce, int maxDepth) {
+BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, int li, bool active,
+                      int sid) {
     const int cur = bounce & 1, nxt = cur ^ 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
     int* ctrNext = wb.counters + (size_t)(bounce + 1) * kCtrStride;
-    const int count = ctr[kCtrCat + T];
     const int* list = wb.catQ + (size_t)T * (size_t)wb.cap;
     constexpr bool kDiffuseType = (T == BTE_BSDF_DIFFUSE || T == BTE_BSDF_MICROFACET || T == BTE_BSDF_DISNEY);
+    V3 o = {0, 0, 0}, d = {0, 0, 0}, thr = {0, 0, 0}, rad = {0, 0, 0};
+    float brdfPdf = 0.0f, eta = 0.0f;
+    uint32_t rng = 0, pid = 0, aovBit = 0;
+    bool spec = false;
+    float4 h = make_float4(0, 0, 0, 0);
+    if (active) {
+        const int qi = list[li];
+        const float4 ro = wb.rayO[cur][qi], rd = wb.rayD[cur][qi], th = wb.thr[cur][qi], ra = wb.rad[cur][qi];
+        const uint2 st = wb.st[cur][qi];
+        h = wb.hit[qi];
+        o = xyz(ro); brdfPdf = ro.w;
+        d = xyz(rd); eta = rd.w;
+        thr = xyz(th);
+        rad = xyz(ra);
+        rng = st.x;
+        pid = st.y & kPidMask;
+        aovBit = st.y & kAovBit;
+        spec = (st.y & kSpecBit) != 0;
+        if (wb.hitCapture && bounce < wb.maxRec) wb.hitCapture[(size_t)pid * wb.maxRec + bounce] = __float_as_int(h.x);
+    }
+    bool survive = false;
+    Its its;
+    its.tri = -1; its.p = {0, 0, 0}; its.n = {0, 0, 0};
+    Bsdf bsdf;
+    bsdf.type = T;
+    Frame frame;
+    frame.t = frame.b = frame.n = V3{0, 0, 0};
+    const V3 dIn = d, thrIn = thr;
+    const uint32_t rngNee = rng;
+    if (active) {
+        its = makeIts(sc, o, d, h);
+        bsdf = loadBsdf(sc, its.tri);
+        if (sc.triFlags && sc.triFlags[its.tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
+        spec = !kDiffuseType;
+        if (kDiffuseType) {
+            rngNext(rng); rngNext(rng);                       // area NEE draws (always)
+            if (sc.hasEnv) { rngNext(rng); rngNext(rng); }    // env NEE draws
+        }
+        if (kDiffuseType && wb.aovAlb && !aovBit) {
+            // first non-delta hit: base colour and shading normal
+            wb.aovAlb[pid] = make_float4(bsdf.p0.x, bsdf.p0.y, bsdf.p0.z, 0.0f);
+            wb.aovNrm[pid] = make_float4(its.n.x, its.n.y, its.n.z, 0.0f);
+            aovBit = kAovBit;
+        }
+        frame = makeFrame(its.n);
+        scatter<T>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
+        survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
+    }
+    const int slot = wave_append(ctrNext + kCtrRays, survive);
+    if (survive) {
+        wb.rayO[nxt][slot] = make_float4(o.x, o.y, o.z, brdfPdf);
+        wb.rayD[nxt][slot] = make_float4(d.x, d.y, d.z, eta);
+        wb.thr[nxt][slot] = make_float4(thr.x, thr.y, thr.z, 0.0f);
+        wb.rad[nxt][slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+        wb.st[nxt][slot] = make_uint2(rng, pid | aovBit | (spec ? kSpecBit : 0u));
+    } else if (active) {
+        wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+    }
+    if constexpr (kDiffuseType) {
+        // NEE records, one light class at a time in the reference's order
+        const uint32_t dest = survive ? (uint32_t)slot : (pid | kFinalBit);
+        uint32_t r2 = rngNee;
+        float a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f;
+        if (active) {
+            a0 = rngNext(r2); a1 = rngNext(r2);
+            if (sc.hasEnv) { e0 = rngNext(r2); e1 = rngNext(r2); }
+        }
+        for (int k = 0; k < wb.kinds; ++k) {
+            NeeRec rec;
+            rec.ok = false;
+            if (active) {
+                // a batch's light classes are the union over its scenes
+                if (k == wb.kindArea) rec = areaNEE<T>(sc, dIn, thrIn, its, frame, bsdf, a0, a1);
+                else if (k == wb.kindEnv) { if (sc.hasEnv) rec = envNEE<T>(sc, dIn, thrIn, its, frame, bsdf, e0, e1); }
+                else if (k - wb.kindSpot0 < sc.spotCount) rec = spotNEE<T>(sc, dIn, thrIn, its, frame, bsdf, k - wb.kindSpot0);
+            }
+            const int sl = wave_append(ctr + kCtrShadow + k, rec.ok);
+            if (rec.ok) {
+                const size_t idx = (size_t)k * (size_t)wb.cap + (size_t)sl;
+                wb.shO[idx] = make_float4(its.p.x, its.p.y, its.p.z, rec.tMax);
+                wb.shD[idx] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(dest));
+                wb.shC[idx] = make_float4(rec.c.x, rec.c.y, rec.c.z, __int_as_float(sid));
+            }
+        }
+    }
+}
+
+// Shade one category of hits (T = BsdfType; misses are shaded by k_categorize).
+template <int T, bool BATCH = false>
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, int bounce, int maxDepth,
+                                                   const DevScene* __restrict__ scenes) {
+    int* ctr = wb.counters + (size_t)bounce * kCtrStride;
+    const int count = ctr[kCtrCat + T];
     for (;;) {
         const int chunk = wave_grab(ctr + kCtrShadeWork + T, 64 * wb.grabRows);
         if (chunk >= count) break;
         for (int row = 0; row < wb.grabRows; ++row) {
-        const int li = chunk + row * 64 + lane_id();
-        const bool active = li < count;
-        V3 o = {0, 0, 0}, d = {0, 0, 0}, thr = {0, 0, 0}, rad = {0, 0, 0};
-        float brdfPdf = 0.0f, eta = 0.0f;
-        uint32_t rng = 0, pid = 0, aovBit = 0;
-        bool spec = false;
-        float4 h = make_float4(0, 0, 0, 0);
-        if (active) {
-            const int qi = list[li];
-            const float4 ro = wb.rayO[cur][qi], rd = wb.rayD[cur][qi], th = wb.thr[cur][qi], ra = wb.rad[cur][qi];
-            const uint2 st = wb.st[cur][qi];
-            h = wb.hit[qi];
-            o = xyz(ro); brdfPdf = ro.w;
-            d = xyz(rd); eta = rd.w;
-            thr = xyz(th);
-            rad = xyz(ra);
-            rng = st.x;
-            pid = st.y & kPidMask;
-            aovBit = st.y & kAovBit;
-            spec = (st.y & kSpecBit) != 0;
-            if (wb.hitCapture && bounce < wb.maxRec) wb.hitCapture[(size_t)pid * wb.maxRec + bounce] = __float_as_int(h.x);
-        }
-        bool survive = false;
-        Its its;
-        its.tri = -1; its.p = {0, 0, 0}; its.n = {0, 0, 0};
-        Bsdf bsdf;
-        bsdf.type = T;
-        Frame frame;
-        frame.t = frame.b = frame.n = V3{0, 0, 0};
-        const V3 dIn = d, thrIn = thr;
-        const uint32_t rngNee = rng;
-        if (active) {
-            its = makeIts(sc, o, d, h);
-            bsdf = loadBsdf(sc, its.tri);
-            if (sc.triFlags && sc.triFlags[its.tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
-            spec = !kDiffuseType;
-            if (kDiffuseType) {
-                rngNext(rng); rngNext(rng);                       // area NEE draws (always)
-                if (sc.hasEnv) { rngNext(rng); rngNext(rng); }    // env NEE draws
-            }
-            if (kDiffuseType && wb.aovAlb && !aovBit) {
-                // first non-delta hit: base colour and shading normal
-                wb.aovAlb[pid] = make_float4(bsdf.p0.x, bsdf.p0.y, bsdf.p0.z, 0.0f);
-                wb.aovNrm[pid] = make_float4(its.n.x, its.n.y, its.n.z, 0.0f);
-                aovBit = kAovBit;
-            }
-            frame = makeFrame(its.n);
-            scatter<T>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
-            survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
-        }
-        const int slot = wave_append(ctrNext + kCtrRays, survive);
-        if (survive) {
-            wb.rayO[nxt][slot] = make_float4(o.x, o.y, o.z, brdfPdf);
-            wb.rayD[nxt][slot] = make_float4(d.x, d.y, d.z, eta);
-            wb.thr[nxt][slot] = make_float4(thr.x, thr.y, thr.z, 0.0f);
-            wb.rad[nxt][slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
-            wb.st[nxt][slot] = make_uint2(rng, pid | aovBit | (spec ? kSpecBit : 0u));
-        } else if (active) {
-            wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
-        }
-        if constexpr (kDiffuseType) {
-            // NEE records, one light class at a time in the reference's order
-            const uint32_t dest = survive ? (uint32_t)slot : (pid | kFinalBit);
-            uint32_t r2 = rngNee;
-            float a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f;
-            if (active) {
-                a0 = rngNext(r2); a1 = rngNext(r2);
-                if (sc.hasEnv) { e0 = rngNext(r2); e1 = rngNext(r2); }
-            }
-            for (int k = 0; k < wb.kinds; ++k) {
-                NeeRec rec;
-                rec.ok = false;
-                if (active) {
-                    if (k == wb.kindArea) rec = areaNEE<T>(sc, dIn, thrIn, its, frame, bsdf, a0, a1);
-                    else if (k == wb.kindEnv) rec = envNEE<T>(sc, dIn, thrIn, its, frame, bsdf, e0, e1);
-                    else rec = spotNEE<T>(sc, dIn, thrIn, its, frame, bsdf, k - wb.kindSpot0);
-                }
-                const int sl = wave_append(ctr + kCtrShadow + k, rec.ok);
-                if (rec.ok) {
-                    const size_t idx = (size_t)k * (size_t)wb.cap + (size_t)sl;
-                    wb.shO[idx] = make_float4(its.p.x, its.p.y, its.p.z, rec.tMax);
-                    wb.shD[idx] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(dest));
-                    wb.shC[idx] = make_float4(rec.c.x, rec.c.y, rec.c.z, 0.0f);
+            const int li = chunk + row * 64 + lane_id();
+            if constexpr (!BATCH) {
+                shadeRow<T>(sc, wb, bounce, maxDepth, li, li < count, 0);
+            } else {
+                bool pending = li < count;
+                const int* list = wb.catQ + (size_t)T * (size_t)wb.cap;
+                const int sid = pending ? sceneOfPath(wb, wb.st[bounce & 1][list[li]].y & kPidMask) : 0;
+                while (__ballot(pending) != 0ull) {
+                    bool mine;
+                    const int s = waterfallScene(pending, sid, mine);
+                    shadeRow<T>(fetchScene(scenes, s), wb, bounce, maxDepth, li, mine, s);
+                    if (mine) pending = false;
                 }
             }
-        }
         }
     }
 }
@@ -1026,7 +1161,27 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
 // Any-hit visibility of one light class's shadow rays; the unoccluded ones add
 // their contribution to the destination path (src/wavefront/ray_connect.cu:52-72).
 template <bool COUNT, int L, int W>
-__global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb, int bounce, int kind) {
+BTE_DEV void connectOne(const DevScene& sc, const WaveBuffers& wb, int nxt, size_t si, const StackRef<L>& stk,
+                        uint32_t* nodes, uint32_t* tris) {
+    const float4 so = wb.shO[si];
+    const float4 sd = wb.shD[si];
+    float t, u, v;
+    const int occ = traverseW<W, true, COUNT, L>(sc, xyz(so), xyz(sd), RT_EPS, so.w, stk, t, u, v, nodes, tris);
+    if (!occ) {
+        const float4 c = wb.shC[si];
+        const uint32_t dest = __float_as_uint(sd.w);
+        float4* r = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.rad[nxt] + dest);
+        float4 v4 = *r;
+        v4.x = v4.x + c.x;
+        v4.y = v4.y + c.y;
+        v4.z = v4.z + c.z;
+        *r = v4;
+    }
+}
+
+template <bool COUNT, int L, int W, bool BATCH = false>
+__global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb, int bounce, int kind,
+                                                     const DevScene* __restrict__ scenes) {
     __shared__ int lds[L * kBlock];
     const int nxt = (bounce & 1) ^ 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
@@ -1039,23 +1194,22 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
         const int chunk = wave_grab(ctr + kCtrConnectWork + kind, 64 * wb.grabRows);
         if (chunk >= count) break;
         for (int row = 0; row < wb.grabRows; ++row) {
-        const int si = chunk + row * 64 + lane_id();
-        if (si < count) {
-            const float4 so = wb.shO[off + si];
-            const float4 sd = wb.shD[off + si];
-            float t, u, v;
-            const int occ = traverseW<W, true, COUNT, L>(sc, xyz(so), xyz(sd), RT_EPS, so.w, stk, t, u, v, &nodes, &tris);
-            if (!occ) {
-                const float4 c = wb.shC[off + si];
-                const uint32_t dest = __float_as_uint(sd.w);
-                float4* r = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.rad[nxt] + dest);
-                float4 v4 = *r;
-                v4.x = v4.x + c.x;
-                v4.y = v4.y + c.y;
-                v4.z = v4.z + c.z;
-                *r = v4;
+            const int si = chunk + row * 64 + lane_id();
+            if constexpr (!BATCH) {
+                if (si < count) connectOne<COUNT, L, W>(sc, wb, nxt, off + si, stk, &nodes, &tris);
+            } else {
+                // the shading pass stored the scene id in the contribution's w
+                bool pending = si < count;
+                const int sid = pending ? __float_as_int(wb.shC[off + si].w) : 0;
+                while (__ballot(pending) != 0ull) {
+                    bool mine;
+                    const int s = waterfallScene(pending, sid, mine);
+                    if (mine) {
+                        connectOne<COUNT, L, W>(fetchScene(scenes, s), wb, nxt, off + si, stk, &nodes, &tris);
+                        pending = false;
+                    }
+                }
             }
-        }
         }
     }
     if (COUNT) {
@@ -1064,40 +1218,56 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_present(WaveBuffers wb, int pixels, int frames, float4* accum,
-                                                     double* sum64, uint32_t* fb) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= pixels) return;
-    const int frameBase = *wb.frameBase;
+// Welford update of one pixel's running means with `frames` new samples: path
+// ids src + f * stride, output pixel `out`.
+BTE_DEV void presentOne(const WaveBuffers& wb, int frames, int frameBase, size_t src, size_t stride, size_t out,
+                        float4* accum, double* sum64, uint32_t* fb) {
     if (wb.aovAlb) {
         // running means of the AOVs, same Welford update as the radiance
-        float4 ma = wb.aovAccAlb[p], mn = wb.aovAccNrm[p];
+        float4 ma = wb.aovAccAlb[out], mn = wb.aovAccNrm[out];
         for (int f = 0; f < frames; ++f) {
-            const float4 a = wb.aovAlb[(size_t)f * pixels + p], nr = wb.aovNrm[(size_t)f * pixels + p];
+            const float4 a = wb.aovAlb[src + (size_t)f * stride], nr = wb.aovNrm[src + (size_t)f * stride];
             const float n = (float)((uint32_t)(frameBase + f) + 1u);
             ma.x = ma.x + (a.x - ma.x) / n; ma.y = ma.y + (a.y - ma.y) / n; ma.z = ma.z + (a.z - ma.z) / n;
             mn.x = mn.x + (nr.x - mn.x) / n; mn.y = mn.y + (nr.y - mn.y) / n; mn.z = mn.z + (nr.z - mn.z) / n;
         }
-        wb.aovAccAlb[p] = ma;
-        wb.aovAccNrm[p] = mn;
+        wb.aovAccAlb[out] = ma;
+        wb.aovAccNrm[out] = mn;
     }
-    float4 mean = accum[p];
+    float4 mean = accum[out];
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int f = 0; f < frames; ++f) {
-        const float4 x = wb.finalRad[(size_t)f * pixels + p];
+        const float4 x = wb.finalRad[src + (size_t)f * stride];
         const float n = (float)((uint32_t)(frameBase + f) + 1u);
         mean.x = mean.x + (x.x - mean.x) / n;
         mean.y = mean.y + (x.y - mean.y) / n;
         mean.z = mean.z + (x.z - mean.z) / n;
         sx += (double)x.x; sy += (double)x.y; sz += (double)x.z;
     }
-    accum[p] = mean;
+    accum[out] = mean;
     if (sum64) {
-        sum64[3 * (size_t)p] += sx;
-        sum64[3 * (size_t)p + 1] += sy;
-        sum64[3 * (size_t)p + 2] += sz;
+        sum64[3 * out] += sx;
+        sum64[3 * out + 1] += sy;
+        sum64[3 * out + 2] += sz;
     }
-    if (fb) fb[p] = packGamma(V3{mean.x, mean.y, mean.z});
+    if (fb) fb[out] = packGamma(V3{mean.x, mean.y, mean.z});
+}
+
+__global__ __launch_bounds__(kBlock) void k_present(WaveBuffers wb, int pixels, int frames, float4* accum,
+                                                     double* sum64, uint32_t* fb) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= pixels) return;
+    presentOne(wb, frames, *wb.frameBase, (size_t)p, (size_t)pixels, (size_t)p, accum, sum64, fb);
+}
+
+// Batch: blockIdx.y = scene; its pixels land at its pixelBase.
+__global__ __launch_bounds__(kBlock) void k_present_batch(WaveBuffers wb, int frames, float4* accum, double* sum64,
+                                                           uint32_t* fb) {
+    const BatchSlot b = wb.slots[blockIdx.y];
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= b.pixels) return;
+    presentOne(wb, frames, *wb.frameBase, (size_t)(b.pathBase + p), (size_t)b.pixels, (size_t)(b.pixelBase + p),
+               accum, sum64, fb);
 }
 
 // Megakernel (src/optix_programs.cu:54-108): one pixel per lane, spp paths
@@ -1295,58 +1465,86 @@ void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths,
     hipLaunchKernelGGL(k_generate, dim3(blocksFor(paths, kBlock)), dim3(kBlock), 0, s, sc, wb, paths, pixels);
 }
 
-template <int L, int W>
+template <int L, int W, bool B>
 static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth,
-                            const LaunchShape& ls, hipStream_t s, hipEvent_t* ev) {
-    const bool count = wb.travCtr != nullptr;
+                            const LaunchShape& ls, hipStream_t s, hipEvent_t* ev, const DevScene* scenes) {
+    // traversal counters are a single-scene instrument
+    const bool count = !B && wb.travCtr != nullptr;
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (count) hipLaunchKernelGGL((k_extend<true, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
-    else hipLaunchKernelGGL((k_extend<false, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    if (count)
+        hipLaunchKernelGGL((k_extend<true, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+    else
+        hipLaunchKernelGGL((k_extend<false, L, W, B>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+                           scenes);
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(k_categorize, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    hipLaunchKernelGGL(k_categorize<B>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     // a BSDF category is shaded only when the scene uses that type
     if (wb.catMask & (1 << BTE_BSDF_DIFFUSE))
-        hipLaunchKernelGGL(k_shade<BTE_BSDF_DIFFUSE>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
-                           maxDepth);
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIFFUSE, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+                           maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_DIELECTRIC))
-        hipLaunchKernelGGL(k_shade<BTE_BSDF_DIELECTRIC>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
-                           maxDepth);
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIELECTRIC, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb,
+                           bounce, maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_MIRROR))
-        hipLaunchKernelGGL(k_shade<BTE_BSDF_MIRROR>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
-                           maxDepth);
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_MIRROR, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+                           maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_MICROFACET))
-        hipLaunchKernelGGL(k_shade<BTE_BSDF_MICROFACET>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
-                           maxDepth);
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_MICROFACET, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb,
+                           bounce, maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_DISNEY))
-        hipLaunchKernelGGL(k_shade<BTE_BSDF_DISNEY>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
-                           maxDepth);
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_DISNEY, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+                           maxDepth, scenes);
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int k = 0; k < wb.kinds; ++k) {
         if (count)
-            hipLaunchKernelGGL((k_connect<true, L, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+            hipLaunchKernelGGL((k_connect<true, L, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k,
+                               scenes);
         else
-            hipLaunchKernelGGL((k_connect<false, L, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+            hipLaunchKernelGGL((k_connect<false, L, W, B>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb,
+                               bounce, k, scenes);
     }
     if (ev) (void)hipEventRecord(ev[3], s);
 }
 
-void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
-                   hipStream_t s, hipEvent_t* ev) {
-    const bool w4 = ls.bvhWidth == 4 && sc.nodes4 != nullptr;
+template <bool B>
+static void launch_bounce_b(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth,
+                            const LaunchShape& ls, hipStream_t s, hipEvent_t* ev, const DevScene* scenes, bool w4) {
     switch (ls.ldsStack) {
         case 8:
-            if (w4) launch_bounce_l<8, 4>(sc, wb, bounce, maxDepth, ls, s, ev);
-            else launch_bounce_l<8, 2>(sc, wb, bounce, maxDepth, ls, s, ev);
+            if (w4) launch_bounce_l<8, 4, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
+            else launch_bounce_l<8, 2, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
             break;
         case 16:
-            if (w4) launch_bounce_l<16, 4>(sc, wb, bounce, maxDepth, ls, s, ev);
-            else launch_bounce_l<16, 2>(sc, wb, bounce, maxDepth, ls, s, ev);
+            if (w4) launch_bounce_l<16, 4, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
+            else launch_bounce_l<16, 2, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
             break;
         default:
-            if (w4) launch_bounce_l<32, 4>(sc, wb, bounce, maxDepth, ls, s, ev);
-            else launch_bounce_l<32, 2>(sc, wb, bounce, maxDepth, ls, s, ev);
+            if (w4) launch_bounce_l<32, 4, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
+            else launch_bounce_l<32, 2, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
             break;
     }
+}
+
+void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                   hipStream_t s, hipEvent_t* ev, const DevScene* scenes) {
+    // a batch's scenes all carry both BVH layouts
+    const bool w4 = ls.bvhWidth == 4 && (scenes != nullptr || sc.nodes4 != nullptr);
+    if (scenes) launch_bounce_b<true>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, w4);
+    else launch_bounce_b<false>(sc, wb, bounce, maxDepth, ls, s, ev, nullptr, w4);
+}
+
+void launch_generate_batch(const DevScene* scenes, const WaveBuffers& wb, int maxScenePaths, long long totalSlots,
+                           hipStream_t s) {
+    if (maxScenePaths <= 0 || wb.sceneCount <= 0) return;
+    hipLaunchKernelGGL(k_generate_batch, dim3(blocksFor(maxScenePaths, kBlock), wb.sceneCount), dim3(kBlock), 0, s,
+                       scenes, wb, totalSlots);
+}
+
+void launch_present_batch(const WaveBuffers& wb, int maxScenePixels, int frames, float4* accum, double* sum64,
+                          uint32_t* fb, hipStream_t s) {
+    if (maxScenePixels <= 0 || wb.sceneCount <= 0) return;
+    hipLaunchKernelGGL(k_present_batch, dim3(blocksFor(maxScenePixels, kBlock), wb.sceneCount), dim3(kBlock), 0, s,
+                       wb, frames, accum, sum64, fb);
 }
 
 void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, float4* accum,

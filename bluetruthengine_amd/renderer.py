@@ -123,6 +123,18 @@ API = {
     "bte_scene_upload": [_p],
     "bte_scene_free": [_p],
     "bte_read_aov": [_p, _p, ctypes.c_size_t],
+    "bte_batch_add_scene": [_i, _i],
+    "bte_batch_autotune": [_i, _p],
+    "bte_batch_scene_count": [],
+    "bte_batch_render": [_i],
+    "bte_batch_reset": [],
+    "bte_batch_clear": [],
+    "bte_batch_frame_index": [],
+    "bte_batch_read_accum": [_i, _p, _sz],
+    "bte_batch_read_framebuffer": [_i, _p, _sz],
+    "bte_batch_read_accum64": [_i, _p, _sz],
+    "bte_batch_denoise": [],
+    "bte_batch_read_denoised": [_i, _p, _sz],
     "bte_denoise": [],
     "bte_read_denoised": [_p, _sz],
     "bte_denoiser_param_count": [],
@@ -260,6 +272,23 @@ def cudaCleanup():
 
 
 # ---- headless renderer -----------------------------------------------------------
+def _upload(scene):
+    """The reference's upload sequence (src/main.cpp:846-935) for one SceneData."""
+    cudaInitScene(scene.triangles, np.zeros((1, 3), np.float32), scene.tri_material_ids, scene.width,
+                  scene.height)
+    cudaInitCamera(scene.camera)
+    cudaInitTriangleEmission(scene.tri_emission)
+    cudaInitTriangleEmitterFlags(scene.tri_emitter_flags)
+    cudaInitEmitterTable(scene.emitters, scene.emitter_tri_indices, scene.emitter_tri_cdf, scene.scene_emitter_cdf)
+    cudaInitBsdfs(scene.bsdfs, scene.tri_bsdf_ids)
+    cudaInitSpotlights(scene.spotlights)
+    cudaInitEnvMap(scene.env_rgba, scene.env_intensity, scene.env_rotation_deg)
+
+
+def _rgb(flat, w, h):
+    return flat.reshape(h, w, 3)
+
+
 class Renderer:
     """Uploads a SceneData through the C ABI and renders into device buffers."""
 
@@ -268,16 +297,7 @@ class Renderer:
         self.scene = scene
         lib()
         cudaCleanup()
-        cudaInitScene(scene.triangles, np.zeros((1, 3), np.float32), scene.tri_material_ids, scene.width,
-                      scene.height)
-        cudaInitCamera(scene.camera)
-        cudaInitTriangleEmission(scene.tri_emission)
-        cudaInitTriangleEmitterFlags(scene.tri_emitter_flags)
-        cudaInitEmitterTable(scene.emitters, scene.emitter_tri_indices, scene.emitter_tri_cdf,
-                             scene.scene_emitter_cdf)
-        cudaInitBsdfs(scene.bsdfs, scene.tri_bsdf_ids)
-        cudaInitSpotlights(scene.spotlights)
-        cudaInitEnvMap(scene.env_rgba, scene.env_intensity, scene.env_rotation_deg)
+        _upload(scene)
         self.set_config(max_depth if max_depth is not None else scene.max_depth, frames_in_flight,
                         memory_budget_mb, megakernel_spp, traversal_stack, graphs, work_chunk,
                         traversal_blocks_per_cu, bvh_width)
@@ -380,6 +400,91 @@ class Renderer:
         total = ctypes.c_float(0)
         _call("bte_read_env_cdf", _ptr(marg), _ptr(cond), ctypes.byref(total))
         return marg, cond, np.float32(total.value)
+
+
+class Batch:
+    """Scenes of uneven size and resolution traced together (bte_batch_*): every
+    bounce of every scene runs in one persistent wavefront launch; each slot
+    accumulates exactly what a Renderer of that scene alone would."""
+
+    def __init__(self, max_depth=10, frames_in_flight=0, memory_budget_mb=0, traversal_stack=0, graphs=1,
+                 work_chunk=0, traversal_blocks_per_cu=0, bvh_width=4, aov=False):
+        lib()
+        cudaCleanup()
+        cfg = BteConfig(int(max_depth), 5, int(frames_in_flight), int(memory_budget_mb), int(traversal_stack),
+                        int(graphs), int(work_chunk), int(traversal_blocks_per_cu), int(bvh_width))
+        _call("bte_set_config", ctypes.byref(cfg))
+        cudaSetRenderMode(BTE_MODE_WAVEFRONT)
+        _call("bte_enable_aov", 1 if aov else 0)
+        self.sizes = []
+
+    def add(self, scene):
+        """Uploads `scene` and moves it into a new slot; returns the slot index."""
+        _upload(scene)
+        slot = _call("bte_batch_add_scene", int(scene.width), int(scene.height))
+        self.sizes.append((int(scene.width), int(scene.height)))
+        return slot
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def render(self, frames=1):
+        _call("bte_batch_render", int(frames))
+
+    def autotune(self, frames):
+        """Launch-shape search over `frames`-frame batch renders (resets the accumulation); returns
+        (best ms, config dict)."""
+        ms = ctypes.c_double(0.0)
+        _call("bte_batch_autotune", int(frames), ctypes.byref(ms))
+        cfg = BteConfig()
+        _call("bte_get_config", ctypes.byref(cfg))
+        return ms.value, {k: getattr(cfg, k) for k, _ in cfg._fields_}
+
+    def reset(self):
+        _call("bte_batch_reset")
+
+    def clear(self):
+        _call("bte_batch_clear")
+        self.sizes = []
+
+    def frame_index(self):
+        return _call("bte_batch_frame_index")
+
+    def stats(self):
+        s = BteStats()
+        _call("bte_get_stats", ctypes.byref(s))
+        return s
+
+    def accum(self, slot):
+        w, h = self.sizes[slot]
+        out = np.empty(w * h * 3, np.float32)
+        _call("bte_batch_read_accum", int(slot), _ptr(out), out.size)
+        return _rgb(out, w, h)
+
+    def enable_accum64(self, on=True):
+        """Binary64 sample sums per slot (takes effect from the next render, which then restarts the batch)."""
+        _call("bte_enable_accum64", int(on))
+
+    def accum64(self, slot):
+        w, h = self.sizes[slot]
+        out = np.empty(w * h * 3, np.float64)
+        _call("bte_batch_read_accum64", int(slot), _ptr(out), out.size)
+        return _rgb(out, w, h)
+
+    def framebuffer(self, slot):
+        w, h = self.sizes[slot]
+        out = np.empty(w * h, np.uint32)
+        _call("bte_batch_read_framebuffer", int(slot), _ptr(out), out.size)
+        return out.reshape(h, w)
+
+    def denoise(self):
+        _call("bte_batch_denoise")
+
+    def denoised(self, slot):
+        w, h = self.sizes[slot]
+        out = np.empty(w * h * 3, np.float32)
+        _call("bte_batch_read_denoised", int(slot), _ptr(out), out.size)
+        return _rgb(out, w, h)
 
 
 def trace_rays(rays, tmin=1e-4, tmax=1e30, any_hit=False, tmax_per_ray=None):

@@ -45,18 +45,20 @@ def make_scene(kind, seed, rng):
 
 
 def render_pairs(pairs, input_spp, target_spp, envlit_frac, seed):
+    """input_spp: a list; each pair's input takes one of its values at random."""
     rng = np.random.default_rng(seed)
     denoiser.dataset_reserve(pairs, CROP, CROP)
     t0 = time.perf_counter()
     for i in range(pairs):
         kind = "envlit" if rng.uniform() < envlit_frac else "terrain"
         sd = make_scene(kind, 10_000 + i, rng)
+        spp = int(rng.choice(input_spp)) if len(input_spp) > 1 else input_spp[0]
         r = renderer.Renderer(sd)
         r.enable_aov(True)
         r.build_accel()
-        r.render(input_spp)
+        r.render(spp)
         denoiser.capture(i, 0)
-        left = target_spp - input_spp
+        left = target_spp - spp
         while left > 0:
             n = min(left, 1024)
             r.render(n)
@@ -69,7 +71,8 @@ def render_pairs(pairs, input_spp, target_spp, envlit_frac, seed):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--pairs", type=int, default=256)
-    ap.add_argument("--input-spp", type=int, default=4)
+    ap.add_argument("--input-spp", default="4",
+                    help="input samples per pixel, or a comma-separated list to draw each pair's from")
     ap.add_argument("--target-spp", type=int, default=4096)
     ap.add_argument("--envlit-frac", type=float, default=0.75)
     ap.add_argument("--steps", type=int, default=4000)
@@ -90,7 +93,8 @@ def main(argv=None):
     renderer.lib()
     denoiser.init(args.seed)
     t0 = time.perf_counter()
-    render_pairs(args.pairs, args.input_spp, args.target_spp, args.envlit_frac, args.seed)
+    input_spp = [int(x) for x in str(args.input_spp).split(",") if x]
+    render_pairs(args.pairs, input_spp, args.target_spp, args.envlit_frac, args.seed)
     t_render = time.perf_counter() - t0
     log(f"rendered {args.pairs} pairs in {t_render:.0f} s")
     if args.init:
@@ -132,7 +136,7 @@ def main(argv=None):
         denoiser.set_loss(denoiser.LOSS_L1_LOG)
     if args.log_json:
         with open(args.log_json, "w") as fh:
-            json.dump({"pairs": args.pairs, "input_spp": args.input_spp, "target_spp": args.target_spp,
+            json.dump({"pairs": args.pairs, "input_spp": input_spp, "target_spp": args.target_spp,
                        "steps": done, "batch": args.batch, "lr": args.lr, "weight_decay": args.weight_decay,
                        "render_s": t_render, "train_s": t_train,
                        "loss": args.loss, "init": args.init,

@@ -212,6 +212,33 @@ int bte_set_hit_capture(int maxRecord);
 int bte_read_hit_capture(int* hostTris, size_t count);
 /* Env-map CDF arrays as built on the GPU (marginal H+1, conditional H*(W+1)). */
 int bte_read_env_cdf(float* marginal, float* conditional, float* totalPower);
+/* ---- multi-scene batches ------------------------------------------------ */
+/* No counterpart in the reference (its host renders one scene per process,
+ * src/main.cpp:846-956): scenes of uneven size and resolution traced together,
+ * every bounce of every scene in one persistent wavefront launch.
+ * bte_batch_add_scene moves the scene set up through the bte_init_* calls
+ * (triangles, BSDFs, emitters, spotlights, env map, camera; the BVH is built
+ * here) into a new batch slot rendered at width x height, leaves the current
+ * scene empty for the next one, resets every slot's accumulation and returns
+ * the slot index (at most 255 slots). */
+int bte_batch_add_scene(int width, int height);
+int bte_batch_scene_count(void);
+/* nframes frames (1 sample per pixel each) of every slot, accumulated per slot
+ * exactly as bte_render_frames would accumulate that scene alone (same seeds,
+ * same pixels); the current BteConfig (maxDepth, framesInFlight, memory budget,
+ * launch shape, graphs) applies to the whole batch.  Wavefront mode only. */
+int bte_batch_render(int nframes);
+int bte_batch_reset(void);          /* zero every slot's accumulation; frame index 0 */
+int bte_batch_clear(void);          /* free every slot (bte_cleanup does too) */
+int bte_batch_frame_index(void);
+/* Per slot, as bte_read_accum / bte_read_framebuffer / bte_read_accum64. */
+int bte_batch_read_accum(int slot, float* hostRGB, size_t count);
+int bte_batch_read_framebuffer(int slot, uint32_t* hostRGBA8, size_t count);
+int bte_batch_read_accum64(int slot, double* hostRGB, size_t count);
+/* The U-Net over every slot (AOVs enabled before the batch's first render),
+ * into each slot's display buffer and its denoised linear image. */
+int bte_batch_denoise(void);
+int bte_batch_read_denoised(int slot, float* hostRGB, size_t count);
 /* ---- learned denoiser (the slot of src/host/denoiser.cu:72-109) --------- */
 /* ≙ denoiseAndPresent (denoiser.cu:72-109): a U-Net (4 levels, 32/64/128/128
  * channels, bf16 MFMA convolutions) over log(1+RGB) of the accumulation and
@@ -304,6 +331,9 @@ void bte_scene_free(BteScene* scene);
  * configuration and resets the accumulation.  Images do not depend on these settings.  msBest
  * (optional) receives the best render time. */
 int bte_autotune(int width, int height, int frames, double* msBest);
+/* The same search over bte_batch_render(frames) of the current batch (whose
+ * accumulation it resets). */
+int bte_batch_autotune(int frames, double* msBest);
 
 /* Trace caller rays with the BVH2 kernels: rays = n x (ox,oy,oz,dx,dy,dz).
  * anyHit = 0: closest hit in [tmin, tmax] -> tri (-1 miss), t, u, v.
