@@ -64,6 +64,17 @@ BTE_DEV int wave_grab(int* ctr, int n = 64) {
     return __shfl(base, 0, 64);
 }
 
+// Work chunks of a persistent wave: the first one is fixed by the wave's index
+// in the grid, later ones come from the counter (offset past the fixed ones).
+// Device-wide atomics on one address serialise, so a wave whose fixed chunk
+// is already past the end leaves without touching the counter: a bounce with
+// little work costs no atomic storm.
+BTE_DEV int first_chunk(int n) { return (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * n; }
+BTE_DEV int next_chunk(int* ctr, int n, int count) {
+    const int fixedEnd = (int)(gridDim.x * (blockDim.x >> 6)) * n;
+    return fixedEnd >= count ? count : fixedEnd + wave_grab(ctr, n);
+}
+
 // ---------------------------------------------------------------------------
 // multi-scene batches
 // A batch keeps one DevScene per scene in device memory.  A wave whose lanes
@@ -917,9 +928,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     uint32_t nodes = 0, tris = 0;
-    for (;;) {
-        const int chunk = wave_grab(ctr + kCtrExtendWork, 64 * wb.grabRows);
-        if (chunk >= count) break;
+    const int grab = 64 * wb.grabRows;
+    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(ctr + kCtrExtendWork, grab, count)) {
         for (int row = 0; row < wb.grabRows; ++row) {
             const int qi = chunk + row * 64 + lane_id();
             if constexpr (!BATCH) {
@@ -1136,9 +1146,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
                                                    const DevScene* __restrict__ scenes) {
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
     const int count = ctr[kCtrCat + T];
-    for (;;) {
-        const int chunk = wave_grab(ctr + kCtrShadeWork + T, 64 * wb.grabRows);
-        if (chunk >= count) break;
+    const int grab = 64 * wb.grabRows;
+    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(ctr + kCtrShadeWork + T, grab, count)) {
         for (int row = 0; row < wb.grabRows; ++row) {
             const int li = chunk + row * 64 + lane_id();
             if constexpr (!BATCH) {
@@ -1190,9 +1199,8 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
     const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     const size_t off = (size_t)kind * (size_t)wb.cap;
     uint32_t nodes = 0, tris = 0;
-    for (;;) {
-        const int chunk = wave_grab(ctr + kCtrConnectWork + kind, 64 * wb.grabRows);
-        if (chunk >= count) break;
+    const int grab = 64 * wb.grabRows;
+    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(ctr + kCtrConnectWork + kind, grab, count)) {
         for (int row = 0; row < wb.grabRows; ++row) {
             const int si = chunk + row * 64 + lane_id();
             if constexpr (!BATCH) {

@@ -86,6 +86,8 @@ def main(argv=None):
     ap.add_argument("--finetune-steps", type=int, default=0,
                     help="after the main run, this many more steps with the relMSE loss at lr/3 (saved to --finetune-out)")
     ap.add_argument("--finetune-out", default="")
+    ap.add_argument("--swa-from", type=float, default=0.0,
+                    help="average the weights of every 250-step snapshot after this fraction of the steps (0 = off)")
     ap.add_argument("--out", default=denoiser.WEIGHTS_PATH)
     ap.add_argument("--log-json", default="")
     args = ap.parse_args(argv)
@@ -106,6 +108,7 @@ def main(argv=None):
     history = []
     t1 = time.perf_counter()
     done = 0
+    swa_sum, swa_n = None, 0
     for frac, lr in phases:
         steps = max(1, int(round(args.steps * frac)))
         chunk = 250
@@ -117,7 +120,16 @@ def main(argv=None):
             steps -= n
             log(f"step {done}: lr {lr:.1e}, mean {args.loss} loss of the last {n}: {losses.mean():.5f}, "
                 f"{time.perf_counter() - t1:.0f} s")
+            if args.swa_from > 0 and done >= args.swa_from * args.steps:
+                w = denoiser.get_weights().astype(np.float64)
+                swa_sum = w if swa_sum is None else swa_sum + w
+                swa_n += 1
     t_train = time.perf_counter() - t1
+    if swa_n:
+        # weight averaging over the snapshots of the last part of the run
+        denoiser.save(args.out.replace(".npy", "_last.npy"))
+        denoiser.set_weights((swa_sum / swa_n).astype(np.float32))
+        log(f"averaged {swa_n} weight snapshots from step {int(args.swa_from * args.steps)} on")
     denoiser.save(args.out)
     log(f"saved {args.out} ({denoiser.param_count()} parameters); training {t_train:.0f} s")
     ft_history = []
@@ -139,7 +151,7 @@ def main(argv=None):
             json.dump({"pairs": args.pairs, "input_spp": input_spp, "target_spp": args.target_spp,
                        "steps": done, "batch": args.batch, "lr": args.lr, "weight_decay": args.weight_decay,
                        "render_s": t_render, "train_s": t_train,
-                       "loss": args.loss, "init": args.init,
+                       "loss": args.loss, "init": args.init, "swa_snapshots": swa_n,
                        "loss_first_100": float(np.mean(history[:100])),
                        "loss_last_100": float(np.mean(history[-100:])),
                        "finetune_relmse_steps": args.finetune_steps,
