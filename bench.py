@@ -568,7 +568,8 @@ def main():
                    "bvh_width": cfg["bvhWidth"],
                    "hip_graphs": bool(cfg["graphs"]), "autotuned": not args.no_autotune},
         "mrays_per_s": round(rays * args.steps / elapsed / 1e6, 1) if rays else None,
-        "rays_per_step": {"extension": st["extensionRays"], "shadow": st["shadowRays"], "hits": st["hits"]},
+        "rays_per_step": {"extension": st["extensionRays"], "shadow": st["shadowRays"], "hits": st["hits"],
+                          "extension_per_bounce": st["raysPerBounce"]},
         "roofline": roofline, "kernels": kernel, "cpu_baseline": cpu, "time_to_relmse": relmse_rec,
     }
     print(json.dumps(out), flush=True)

@@ -89,7 +89,7 @@ struct Context : SceneData {
     long long batchPixels = 0;
     uint32_t batchFrame = 0;
     DBuf dBatchScenes, dBatchSlots, dBatchChunk, dBatchAccum, dBatchFb, dBatchSum64, dBatchAovAlb, dBatchAovNrm,
-        dBatchDenoised;
+        dBatchDenoised, dBatchSegCount, dBatchSegRows;
     long long batchLayoutKey = -1;            // frames per pass the slot table was built for
     unsigned long long batchLayoutGen = ~0ull;
     bool batchDenoised = false;
@@ -104,7 +104,7 @@ struct Context : SceneData {
 
     // wavefront buffers
     DBuf dRayO[2], dRayD[2], dThr[2], dRad[2], dSt[2], dHit, dCatQ, dCat, dFinal, dShO, dShD, dShC, dCounters, dHitCap, dOvf;
-    DBuf dFrameBase;
+    DBuf dFrameBase, dWork;
     bool aov = false;                 // albedo / normal AOVs of the first non-delta hit
     DBuf dAovAlb, dAovNrm, dAovAccAlb, dAovAccNrm;
 
@@ -312,6 +312,7 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     CHECK_RC(c.dShD.ensure(K * P * sizeof(float4)));
     CHECK_RC(c.dShC.ensure(K * P * sizeof(float4)));
     CHECK_RC(c.dCounters.ensure((size_t)kCtrStride * (size_t)(maxDepth + 1) * sizeof(int)));
+    CHECK_RC(c.dWork.ensure((size_t)kWorkStride * (size_t)(maxDepth + 1) * sizeof(int)));
     CHECK_RC(c.dFrameBase.ensure(sizeof(int)));
     if (c.aov) {
         CHECK_RC(c.dAovAlb.ensure(P * sizeof(float4)));
@@ -337,6 +338,7 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     wb.shD = c.dShD.as<float4>();
     wb.shC = c.dShC.as<float4>();
     wb.counters = c.dCounters.as<int>();
+    wb.work = c.dWork.as<int>();
     wb.frameBase = c.dFrameBase.as<int>();
     wb.aovAlb = c.aov ? c.dAovAlb.as<float4>() : nullptr;
     wb.aovNrm = c.aov ? c.dAovNrm.as<float4>() : nullptr;
@@ -493,6 +495,7 @@ int renderWavefront(int nframes) {
             CHECK_RC(runSegment(useGraph, key, [&]() {
                 if (b0 == 0) {
                     launch_fill_i32(wb.counters, (long long)ctrInts, 0, c.stream);
+                    launch_fill_i32(wb.work, (long long)kWorkStride * (maxDepth + 1), 0, c.stream);
                     if (wb.hitCapture) launch_fill_i32(wb.hitCapture, paths * wb.maxRec, -2, c.stream);
                     launch_generate(sc, wb, paths, N, c.stream);
                 }
@@ -608,7 +611,8 @@ void batchClear(Context& c) {
     c.batchFrame = 0;
     ++c.batchGen;
     DBuf* bufs[] = {&c.dBatchScenes, &c.dBatchSlots, &c.dBatchChunk, &c.dBatchAccum, &c.dBatchFb,
-                    &c.dBatchSum64, &c.dBatchAovAlb, &c.dBatchAovNrm, &c.dBatchDenoised};
+                    &c.dBatchSum64, &c.dBatchAovAlb, &c.dBatchAovNrm, &c.dBatchDenoised,
+                    &c.dBatchSegCount, &c.dBatchSegRows};
     for (DBuf* b : bufs) b->release();
     c.batchLayoutKey = -1;
     c.batchDenoised = false;
@@ -727,6 +731,10 @@ int renderBatch(int nframes) {
     const bool useGraph = c.cfg.graphs != 0;
     const size_t ctrInts = (size_t)kCtrStride * (size_t)(maxDepth + 1);
     std::vector<int> ctrHost(ctrInts);
+    const size_t segInts = (size_t)(maxDepth + 1) * kSegQueues * (size_t)S;
+    const size_t rowInts = (size_t)(maxDepth + 1) * kSegQueues * (size_t)(S + 1);
+    CHECK_RC(c.dBatchSegCount.ensure(segInts * sizeof(int)));
+    CHECK_RC(c.dBatchSegRows.ensure(rowInts * sizeof(int)));
     const DevScene* scenesDev = c.dBatchScenes.as<DevScene>();
     DevScene none;
     std::memset(&none, 0, sizeof none);
@@ -762,7 +770,10 @@ int renderBatch(int nframes) {
         wb.aovAccNrm = c.aov ? c.dBatchAovNrm.as<float4>() : nullptr;
         wb.sceneOfChunk = c.dBatchChunk.as<uint8_t>();
         wb.slots = c.dBatchSlots.as<BatchSlot>();
+        wb.segCount = c.dBatchSegCount.as<int>();
+        wb.segRows = c.dBatchSegRows.as<int>();
         wb.sceneCount = S;
+        wb.paddedSlots = (int)padded;
         launch_fill_i32(c.dFrameBase.as<int>(), 1, (int)c.batchFrame, c.stream);
         constexpr int kSegment = 16;
         int b = 0;
@@ -785,6 +796,8 @@ int renderBatch(int nframes) {
             CHECK_RC(runSegment(useGraph, key, [&]() {
                 if (b0 == 0) {
                     launch_fill_i32(wb.counters, (long long)ctrInts, 0, c.stream);
+                    launch_fill_i32(wb.work, (long long)kWorkStride * (maxDepth + 1), 0, c.stream);
+                    launch_fill_i32(wb.segCount, (long long)segInts, 0, c.stream);
                     launch_generate_batch(scenesDev, wb, maxPaths, slots, c.stream);
                 }
                 for (int bb = b0; bb < b1; ++bb) launch_bounce(none, wb, bb, maxDepth, ls, c.stream, nullptr, scenesDev);
@@ -1158,6 +1171,7 @@ int bte_cleanup(void) {
     denoiser_release();
     batchClear(c);
     c.dFrameBase.release();
+    c.dWork.release();
     c.dAovAlb.release(); c.dAovNrm.release(); c.dAovAccAlb.release(); c.dAovAccNrm.release();
     c.aov = false;
     DBuf* bufs[] = {&c.dTris, &c.dMaterials, &c.dTriMat, &c.dTriBsdf, &c.dBsdfs, &c.dEmission, &c.dFlags,

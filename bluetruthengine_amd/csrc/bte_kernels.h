@@ -51,14 +51,42 @@ constexpr int kCtrCat = 4 + 2 * kMaxShadowKinds;     // + category: rays queued 
 constexpr int kCtrShadeWork = kCtrCat + kNumCats;    // + category
 constexpr int kCtrStride = kCtrShadeWork + kNumCats;
 
-// Multi-scene batch: where each scene's paths and pixels live.  Path ids are
-// laid out scene by scene, each scene's range starting at a multiple of
-// kBatchChunk, so the scene of a path id is one byte-table lookup.
+// Per-scene queue counts of a batch, per bounce: the next bounce's rays, the
+// category lists, the shadow records of each light class.
+constexpr int kSegRays = 0;
+constexpr int kSegCat = 1;                            // + category
+constexpr int kSegShadow = kSegCat + kNumCats;        // + light class
+constexpr int kSegQueues = kSegShadow + kMaxShadowKinds;
+// A batch queue is walked in compact rows of 64 entries, each scene's rows
+// rounded up to kSegRowAlign so that neither a work chunk (at most 8 rows) nor
+// a categoriser block (16 rows) spans two scenes.
+constexpr int kSegRowAlign = 16;
+
+// Work counters of the persistent stages, per bounce: one group per stage
+// (extend, shade per category, connect per light class) of kWorkXcds
+// counters, each on its own 64-B line.  A block draws from counter
+// blockIdx.x % kWorkXcds: workgroups are dealt to the 8 XCDs in turn, so each
+// counter is hit by one XCD's waves and device-wide atomics spread over 8
+// addresses instead of serialising on one.
+constexpr int kWorkXcds = 8;
+constexpr int kWorkPad = 16;
+constexpr int kWorkExtend = 0;
+constexpr int kWorkShade = 1;                        // + category
+constexpr int kWorkConnect = kWorkShade + kNumCats;  // + light class
+constexpr int kWorkGroups = kWorkConnect + kMaxShadowKinds;
+constexpr int kWorkStride = kWorkGroups * kWorkXcds * kWorkPad;   // ints per bounce
+
+// Multi-scene batch: where each scene's paths and pixels live.  Path ids and
+// queue slots share one layout: scene by scene, each scene's range starting at
+// a multiple of kBatchChunk, so the scene of a slot is one byte-table lookup
+// and a 64-slot row never holds two scenes.  Every queue (rays, category
+// lists, shadow records) keeps each scene's entries at the start of that
+// scene's range, with a count per (queue, scene).
 constexpr int kBatchChunkShift = 12;
 constexpr long long kBatchChunk = 1ll << kBatchChunkShift;
 struct BatchSlot {
     long long pathBase;   // first path id of the scene (multiple of kBatchChunk)
-    long long slotBase;   // first queue slot of the scene in the generated queue
+    long long slotBase;   // first slot of the scene in a densely packed queue (unused by the kernels)
     int paths;            // pixels * frames of this pass
     int pixels;
     int pixelBase;        // first pixel of the scene in the batch's output buffers
@@ -81,6 +109,7 @@ struct WaveBuffers {
     float4* shD;          // [kinds * cap]: direction xyz, w = destination bits
     float4* shC;          // [kinds * cap]: contribution xyz
     int* counters;        // kCtrStride * (maxDepth + 1)
+    int* work;            // kWorkStride * (maxDepth + 1), zero at the start of a pass
     int* hitCapture;      // optional [P * maxRec]
     int* ovfStack;        // traversal stack overflow, per persistent thread
     unsigned long long* travCtr;  // optional: extend nodes, extend tris, connect nodes, connect tris
@@ -96,9 +125,13 @@ struct WaveBuffers {
     float4* aovAccNrm;
     const int* frameBase; // device int: frame index of the batch's first frame (set before each batch)
     // multi-scene batches only (null / 0 for a single scene)
-    const uint8_t* sceneOfChunk;   // scene of each kBatchChunk-aligned block of path ids
+    const uint8_t* sceneOfChunk;   // scene of each kBatchChunk-aligned block of slots / path ids
     const BatchSlot* slots;        // [sceneCount]
+    int* segCount;                 // [(maxDepth + 1) * kSegQueues * sceneCount]
+    int* segRows;                  // [(maxDepth + 1) * kSegQueues * (sceneCount + 1)]: first compact row of
+                                   // each scene (its rows rounded up to kSegRowAlign), total at the end
     int sceneCount;
+    int paddedSlots;               // end of the last scene's range
 };
 
 struct LaunchShape {
@@ -118,10 +151,13 @@ void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int ma
                    hipStream_t s, hipEvent_t* ev, const DevScene* scenes = nullptr);
 void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, float4* accum,
                     double* sum64, uint32_t* fb, hipStream_t s);
-// Multi-scene pass: paths of every scene (wb.slots) into queue slots
-// [0, totalSlots); present writes each scene's pixels at its pixelBase.
+// Multi-scene pass: paths of every scene (wb.slots) into its own slot range;
+// present writes each scene's pixels at its pixelBase.
 void launch_generate_batch(const DevScene* scenes, const WaveBuffers& wb, int maxScenePaths, long long totalSlots,
                            hipStream_t s);
+// Row starts of queues [q, q + n) of bounce b from their per-scene counts; the
+// ray and shadow queues' totals also go to the plain counters the host reads.
+void launch_segment_rows(const WaveBuffers& wb, int bounce, int q, int n, hipStream_t s);
 void launch_present_batch(const WaveBuffers& wb, int maxScenePixels, int frames, float4* accum, double* sum64,
                           uint32_t* fb, hipStream_t s);
 void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex, float4* accum, double* sum64,

@@ -69,23 +69,65 @@ BTE_DEV int wave_grab(int* ctr, int n = 64) {
 // Device-wide atomics on one address serialise, so a wave whose fixed chunk
 // is already past the end leaves without touching the counter: a bounce with
 // little work costs no atomic storm.
+// Later chunks are dealt out by one counter per XCD (group = kWorkXcds
+// counters): counter x hands out chunks fixedEnd + (i * nx + x) * n, so every
+// chunk goes to exactly one wave whatever the block-to-XCD placement.
 BTE_DEV int first_chunk(int n) { return (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * n; }
-BTE_DEV int next_chunk(int* ctr, int n, int count) {
+BTE_DEV int next_chunk(int* group, int n, int count) {
     const int fixedEnd = (int)(gridDim.x * (blockDim.x >> 6)) * n;
-    return fixedEnd >= count ? count : fixedEnd + wave_grab(ctr, n);
+    if (fixedEnd >= count) return count;
+    const int nx = min((int)gridDim.x, kWorkXcds);
+    const int x = (int)blockIdx.x % nx;
+    const int i = wave_grab(group + x * kWorkPad, 1);
+    return fixedEnd + (i * nx + x) * n;
+}
+BTE_DEV int* work_group(const WaveBuffers& wb, int bounce, int g) {
+    return wb.work + (size_t)bounce * kWorkStride + (size_t)g * (kWorkXcds * kWorkPad);
 }
 
 // ---------------------------------------------------------------------------
 // multi-scene batches
-// A batch keeps one DevScene per scene in device memory.  A wave whose lanes
-// hold paths of several scenes handles them one scene per round (a
-// "waterfall"): within a round the scene index is wave-uniform, so its
-// descriptor arrives through scalar loads, as a kernel argument would.  The
-// rounds are uniform loops (every lane iterates) so that the round body may
-// hold wave-wide operations.
+// A batch keeps one DevScene per scene in device memory.  Every queue keeps
+// each scene's entries at the start of that scene's own slot range and is
+// walked in compact rows (scene after scene, empty ranges skipped), so a work
+// chunk never spans two scenes: the scene index is wave-uniform and its
+// descriptor arrives through scalar loads, as a kernel argument would.
 // ---------------------------------------------------------------------------
-BTE_DEV int sceneOfPath(const WaveBuffers& wb, uint32_t pid) {
-    return (int)wb.sceneOfChunk[pid >> kBatchChunkShift];
+// Per-scene counts and compact row starts of the batch's queues.
+BTE_DEV int* segCounts(const WaveBuffers& wb, int bounce, int q) {
+    return wb.segCount + ((size_t)bounce * kSegQueues + (size_t)q) * (size_t)wb.sceneCount;
+}
+BTE_DEV int* segRowStart(const WaveBuffers& wb, int bounce, int q) {
+    return wb.segRows + ((size_t)bounce * kSegQueues + (size_t)q) * (size_t)(wb.sceneCount + 1);
+}
+// Scene of compact row r (r wave-uniform): the last s with start[s] <= r, so
+// scenes without rows are skipped.
+BTE_DEV int segSceneOfRow(const int* start, int sceneCount, int r) {
+    int lo = 0, hi = sceneCount - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= r) lo = mid;
+        else hi = mid - 1;
+    }
+    return __builtin_amdgcn_readfirstlane(lo);
+}
+// A batch work chunk (compact slot index c, wave-uniform): its scene s, the
+// queue position of its first entry and the number of entries left in s.
+struct SegChunk {
+    int s, first, left;
+};
+BTE_DEV SegChunk segChunk(const WaveBuffers& wb, int bounce, int q, int c) {
+    const int* start = segRowStart(wb, bounce, q);
+    const int r = c >> 6;
+    SegChunk k;
+    k.s = segSceneOfRow(start, wb.sceneCount, r);
+    const int local = (r - start[k.s]) * 64;
+    k.first = (int)wb.slots[k.s].pathBase + local;
+    k.left = segCounts(wb, bounce, q)[k.s] - local;
+    return k;
+}
+BTE_DEV int segSlots(const WaveBuffers& wb, int bounce, int q) {
+    return segRowStart(wb, bounce, q)[wb.sceneCount] * 64;
 }
 
 // A pointer read from memory is generic to the compiler; rebuilt from its
@@ -119,18 +161,6 @@ BTE_DEV DevScene fetchScene(const DevScene* __restrict__ scenes, int s) {
     sc.envMargGuide = globalPtr(sc.envMargGuide);
     sc.envCondGuide = globalPtr(sc.envCondGuide);
     return sc;
-}
-
-// One waterfall round: the scene of the first pending lane (wave-uniform) and
-// whether this lane belongs to it.  The membership test goes through a ballot
-// so that the compiler cannot substitute the lane's own (divergent) scene id
-// for the uniform one inside the round.
-BTE_DEV int waterfallScene(bool pending, int sid, bool& mine) {
-    const unsigned long long pm = __ballot(pending);
-    const int s = __builtin_amdgcn_readfirstlane(__shfl(sid, __ffsll((long long)pm) - 1, 64));
-    const unsigned long long match = __ballot(pending && sid == s);
-    mine = ((match >> lane_id()) & 1ull) != 0;
-    return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -882,9 +912,9 @@ __global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb
     if (i == 0) wb.counters[kCtrRays] = (int)paths;
 }
 
-// Batch: blockIdx.y = scene.  Scene s's paths take path ids from its pathBase
-// and queue slots from its slotBase; pixel and frame seeds are the scene's own,
-// so every scene renders exactly as it would alone.
+// Batch: blockIdx.y = scene.  Scene s's paths take the path ids and queue
+// slots of its range; pixel and frame seeds are the scene's own, so every
+// scene renders exactly as it would alone.
 __global__ __launch_bounds__(kBlock) void k_generate_batch(const DevScene* __restrict__ scenes, WaveBuffers wb,
                                                             long long totalSlots) {
     const int s = blockIdx.y;
@@ -893,8 +923,9 @@ __global__ __launch_bounds__(kBlock) void k_generate_batch(const DevScene* __res
     if (i >= b.paths) return;
     const DevScene sc = fetchScene(scenes, s);
     const int frame0 = *wb.frameBase;
-    generateOne(sc, wb, b.slotBase + i, (uint32_t)(b.pathBase + i), (int)(i % b.pixels),
+    generateOne(sc, wb, b.pathBase + i, (uint32_t)(b.pathBase + i), (int)(i % b.pixels),
                 frame0 + (int)(i / b.pixels));
+    if (i == 0) segCounts(wb, 0, kSegRays)[s] = b.paths;
     if (s == 0 && i == 0) wb.counters[kCtrRays] = (int)totalSlots;
 }
 
@@ -924,27 +955,25 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
     __shared__ int lds[L * kBlock];
     const int cur = bounce & 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
-    const int count = ctr[kCtrRays];
+    const int count = BATCH ? segSlots(wb, bounce, kSegRays) : ctr[kCtrRays];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     uint32_t nodes = 0, tris = 0;
     const int grab = 64 * wb.grabRows;
-    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(ctr + kCtrExtendWork, grab, count)) {
-        for (int row = 0; row < wb.grabRows; ++row) {
-            const int qi = chunk + row * 64 + lane_id();
-            if constexpr (!BATCH) {
+    int* wk = work_group(wb, bounce, kWorkExtend);
+    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(wk, grab, count)) {
+        if constexpr (!BATCH) {
+            for (int row = 0; row < wb.grabRows; ++row) {
+                const int qi = chunk + row * 64 + lane_id();
                 if (qi < count) extendOne<COUNT, L, W>(sc, wb, cur, qi, stk, &nodes, &tris);
-            } else {
-                bool pending = qi < count;
-                const int sid = pending ? sceneOfPath(wb, wb.st[cur][qi].y & kPidMask) : 0;
-                while (__ballot(pending) != 0ull) {
-                    bool mine;
-                    const int s = waterfallScene(pending, sid, mine);
-                    if (mine) {
-                        extendOne<COUNT, L, W>(fetchScene(scenes, s), wb, cur, qi, stk, &nodes, &tris);
-                        pending = false;
-                    }
-                }
+            }
+        } else {
+            const SegChunk k = segChunk(wb, bounce, kSegRays, chunk);
+            if (k.left <= 0) continue;
+            const DevScene S = fetchScene(scenes, k.s);
+            for (int row = 0; row < wb.grabRows; ++row) {
+                const int e = row * 64 + lane_id();
+                if (e < k.left) extendOne<COUNT, L, W>(S, wb, cur, k.first + e, stk, &nodes, &tris);
             }
         }
     }
@@ -982,31 +1011,29 @@ __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers 
     __shared__ int sCnt[kCatSub][kBlock / 64][kNumCats];
     __shared__ int sBase[kNumCats];
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
-    const int count = ctr[kCtrRays];
+    const int count = BATCH ? segSlots(wb, bounce, kSegRays) : ctr[kCtrRays];
     const int wave = threadIdx.x >> 6, lane = lane_id();
     const int per = kCatSub * kBlock;
     for (int base = blockIdx.x * per; base < count; base += gridDim.x * per) {
+        // batch: a 1024-entry block of compact rows lies in one scene
+        int first = base, left = count - base, begin = 0, s = 0;
+        if constexpr (BATCH) {
+            const SegChunk k = segChunk(wb, bounce, kSegRays, base);
+            if (k.left <= 0) continue;
+            s = k.s;
+            first = k.first;
+            left = k.left;
+            begin = (int)wb.slots[s].pathBase;
+        }
         int c[kCatSub];
         for (int j = 0; j < kCatSub; ++j) {
-            const int i = base + j * kBlock + wave * 64 + lane;
-            c[j] = (i < count) ? (int)wb.cat[i] : -1;
-            if constexpr (!BATCH) {
-                if (c[j] == kCatMiss) {
-                    shadeMiss(sc, wb, i, bounce);
-                    c[j] = -1;
-                }
-            } else {
-                bool pending = c[j] == kCatMiss;
-                const int sid = pending ? sceneOfPath(wb, wb.st[bounce & 1][i].y & kPidMask) : 0;
-                if (pending) c[j] = -1;
-                while (__ballot(pending) != 0ull) {
-                    bool mine;
-                    const int s = waterfallScene(pending, sid, mine);
-                    if (mine) {
-                        shadeMiss(fetchScene(scenes, s), wb, i, bounce);
-                        pending = false;
-                    }
-                }
+            const int e = j * kBlock + wave * 64 + lane;
+            const int i = first + e;
+            c[j] = (e < left) ? (int)wb.cat[i] : -1;
+            if (c[j] == kCatMiss) {
+                if constexpr (BATCH) shadeMiss(fetchScene(scenes, s), wb, i, bounce);
+                else shadeMiss(sc, wb, i, bounce);
+                c[j] = -1;
             }
             for (int k = 1; k < kNumCats; ++k) {
                 const unsigned long long m = __ballot(c[j] == k);
@@ -1023,12 +1050,13 @@ __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers 
                     sCnt[j][w][k] = run;
                     run += n;
                 }
-            sBase[k] = run ? atomicAdd(ctr + kCtrCat + k, run) : 0;
+            int* catCtr = BATCH ? segCounts(wb, bounce, kSegCat + k) + s : ctr + kCtrCat + k;
+            sBase[k] = run ? begin + atomicAdd(catCtr, run) : 0;
             if (run) atomicAdd(ctr + kCtrHits, run);
         }
         __syncthreads();
         for (int j = 0; j < kCatSub; ++j) {
-            const int i = base + j * kBlock + wave * 64 + lane;
+            const int i = first + j * kBlock + wave * 64 + lane;
             for (int k = 1; k < kNumCats; ++k) {
                 const unsigned long long m = __ballot(c[j] == k);
                 if (c[j] == k) {
@@ -1042,15 +1070,23 @@ __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers 
     }
 }
 
+// Where a shading row appends: surviving paths at survBase + (*survCtr)++,
+// shadow records of light class k at shBase + (*(shCtr + k * shStride))++.
+struct ShadeSinks {
+    int* survCtr;
+    int survBase;
+    int* shCtr;
+    int shStride;
+    int shBase;
+};
+
 // Shades one 64-slot row of a category queue: lanes with `active` set hold
 // the hits at list[li].  Every lane of the wave must call it (wave-wide
-// appends).  sid is stored with the shadow records for the connect pass.
+// appends).
 template <int T>
 BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, int li, bool active,
-                      int sid) {
+                      const ShadeSinks& out) {
     const int cur = bounce & 1, nxt = cur ^ 1;
-    int* ctr = wb.counters + (size_t)bounce * kCtrStride;
-    int* ctrNext = wb.counters + (size_t)(bounce + 1) * kCtrStride;
     const int* list = wb.catQ + (size_t)T * (size_t)wb.cap;
     constexpr bool kDiffuseType = (T == BTE_BSDF_DIFFUSE || T == BTE_BSDF_MICROFACET || T == BTE_BSDF_DISNEY);
     V3 o = {0, 0, 0}, d = {0, 0, 0}, thr = {0, 0, 0}, rad = {0, 0, 0};
@@ -1101,7 +1137,7 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
         scatter<T>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
         survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
     }
-    const int slot = wave_append(ctrNext + kCtrRays, survive);
+    const int slot = out.survBase + wave_append(out.survCtr, survive);
     if (survive) {
         wb.rayO[nxt][slot] = make_float4(o.x, o.y, o.z, brdfPdf);
         wb.rayD[nxt][slot] = make_float4(d.x, d.y, d.z, eta);
@@ -1129,12 +1165,12 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
                 else if (k == wb.kindEnv) { if (sc.hasEnv) rec = envNEE<T>(sc, dIn, thrIn, its, frame, bsdf, e0, e1); }
                 else if (k - wb.kindSpot0 < sc.spotCount) rec = spotNEE<T>(sc, dIn, thrIn, its, frame, bsdf, k - wb.kindSpot0);
             }
-            const int sl = wave_append(ctr + kCtrShadow + k, rec.ok);
+            const int sl = out.shBase + wave_append(out.shCtr + k * out.shStride, rec.ok);
             if (rec.ok) {
                 const size_t idx = (size_t)k * (size_t)wb.cap + (size_t)sl;
                 wb.shO[idx] = make_float4(its.p.x, its.p.y, its.p.z, rec.tMax);
                 wb.shD[idx] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(dest));
-                wb.shC[idx] = make_float4(rec.c.x, rec.c.y, rec.c.z, __int_as_float(sid));
+                wb.shC[idx] = make_float4(rec.c.x, rec.c.y, rec.c.z, 0.0f);
             }
         }
     }
@@ -1145,30 +1181,32 @@ template <int T, bool BATCH = false>
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, int bounce, int maxDepth,
                                                    const DevScene* __restrict__ scenes) {
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
-    const int count = ctr[kCtrCat + T];
+    int* ctrNext = wb.counters + (size_t)(bounce + 1) * kCtrStride;
+    const int count = BATCH ? segSlots(wb, bounce, kSegCat + T) : ctr[kCtrCat + T];
     const int grab = 64 * wb.grabRows;
-    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(ctr + kCtrShadeWork + T, grab, count)) {
-        for (int row = 0; row < wb.grabRows; ++row) {
-            const int li = chunk + row * 64 + lane_id();
-            if constexpr (!BATCH) {
-                shadeRow<T>(sc, wb, bounce, maxDepth, li, li < count, 0);
-            } else {
-                bool pending = li < count;
-                const int* list = wb.catQ + (size_t)T * (size_t)wb.cap;
-                const int sid = pending ? sceneOfPath(wb, wb.st[bounce & 1][list[li]].y & kPidMask) : 0;
-                while (__ballot(pending) != 0ull) {
-                    bool mine;
-                    const int s = waterfallScene(pending, sid, mine);
-                    shadeRow<T>(fetchScene(scenes, s), wb, bounce, maxDepth, li, mine, s);
-                    if (mine) pending = false;
-                }
+    int* wk = work_group(wb, bounce, kWorkShade + T);
+    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(wk, grab, count)) {
+        if constexpr (!BATCH) {
+            const ShadeSinks out{ctrNext + kCtrRays, 0, ctr + kCtrShadow, 1, 0};
+            for (int row = 0; row < wb.grabRows; ++row) {
+                const int li = chunk + row * 64 + lane_id();
+                shadeRow<T>(sc, wb, bounce, maxDepth, li, li < count, out);
+            }
+        } else {
+            const SegChunk k = segChunk(wb, bounce, kSegCat + T, chunk);
+            if (k.left <= 0) continue;
+            const int begin = (int)wb.slots[k.s].pathBase;
+            const DevScene S = fetchScene(scenes, k.s);
+            const ShadeSinks out{segCounts(wb, bounce + 1, kSegRays) + k.s, begin,
+                                 segCounts(wb, bounce, kSegShadow) + k.s, wb.sceneCount, begin};
+            for (int row = 0; row < wb.grabRows; ++row) {
+                const int e = row * 64 + lane_id();
+                shadeRow<T>(S, wb, bounce, maxDepth, k.first + e, e < k.left, out);
             }
         }
     }
 }
 
-// Any-hit visibility of one light class's shadow rays; the unoccluded ones add
-// their contribution to the destination path (src/wavefront/ray_connect.cu:52-72).
 template <bool COUNT, int L, int W>
 BTE_DEV void connectOne(const DevScene& sc, const WaveBuffers& wb, int nxt, size_t si, const StackRef<L>& stk,
                         uint32_t* nodes, uint32_t* tris) {
@@ -1194,29 +1232,26 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
     __shared__ int lds[L * kBlock];
     const int nxt = (bounce & 1) ^ 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
-    const int count = ctr[kCtrShadow + kind];
+    const int count = BATCH ? segSlots(wb, bounce, kSegShadow + kind) : ctr[kCtrShadow + kind];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     const size_t off = (size_t)kind * (size_t)wb.cap;
     uint32_t nodes = 0, tris = 0;
     const int grab = 64 * wb.grabRows;
-    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(ctr + kCtrConnectWork + kind, grab, count)) {
-        for (int row = 0; row < wb.grabRows; ++row) {
-            const int si = chunk + row * 64 + lane_id();
-            if constexpr (!BATCH) {
+    int* wk = work_group(wb, bounce, kWorkConnect + kind);
+    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(wk, grab, count)) {
+        if constexpr (!BATCH) {
+            for (int row = 0; row < wb.grabRows; ++row) {
+                const int si = chunk + row * 64 + lane_id();
                 if (si < count) connectOne<COUNT, L, W>(sc, wb, nxt, off + si, stk, &nodes, &tris);
-            } else {
-                // the shading pass stored the scene id in the contribution's w
-                bool pending = si < count;
-                const int sid = pending ? __float_as_int(wb.shC[off + si].w) : 0;
-                while (__ballot(pending) != 0ull) {
-                    bool mine;
-                    const int s = waterfallScene(pending, sid, mine);
-                    if (mine) {
-                        connectOne<COUNT, L, W>(fetchScene(scenes, s), wb, nxt, off + si, stk, &nodes, &tris);
-                        pending = false;
-                    }
-                }
+            }
+        } else {
+            const SegChunk k = segChunk(wb, bounce, kSegShadow + kind, chunk);
+            if (k.left <= 0) continue;
+            const DevScene S = fetchScene(scenes, k.s);
+            for (int row = 0; row < wb.grabRows; ++row) {
+                const int e = row * 64 + lane_id();
+                if (e < k.left) connectOne<COUNT, L, W>(S, wb, nxt, off + k.first + e, stk, &nodes, &tris);
             }
         }
     }
@@ -1459,6 +1494,26 @@ __global__ __launch_bounds__(256) void k_env_guides(const float* marg, const flo
         if (!(cdf[i + 1] >= cdf[i])) atomicAnd(monotone, 0);   // NaN clears it too
 }
 
+// Batch: row starts of queues [q0, q0 + n) of one bounce from their per-scene
+// counts (one thread per queue); the totals of the ray and shadow queues also
+// go to the plain counters the host reads.
+__global__ void k_segment_rows(WaveBuffers wb, int bounce, int q0, int n) {
+    if ((int)threadIdx.x >= n) return;
+    const int q = q0 + (int)threadIdx.x;
+    const int* cnt = segCounts(wb, bounce, q);
+    int* start = segRowStart(wb, bounce, q);
+    int rows = 0, sum = 0;
+    for (int s = 0; s < wb.sceneCount; ++s) {
+        start[s] = rows;
+        const int c = cnt[s];
+        sum += c;
+        rows += (c + 64 * kSegRowAlign - 1) / (64 * kSegRowAlign) * kSegRowAlign;
+    }
+    start[wb.sceneCount] = rows;
+    if (q == kSegRays) wb.counters[(size_t)bounce * kCtrStride + kCtrRays] = sum;
+    if (q >= kSegShadow) wb.counters[(size_t)bounce * kCtrStride + kCtrShadow + (q - kSegShadow)] = sum;
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -1486,6 +1541,7 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
                            scenes);
     if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(k_categorize<B>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+    if (B) launch_segment_rows(wb, bounce, kSegCat, kNumCats, s);
     // a BSDF category is shaded only when the scene uses that type
     if (wb.catMask & (1 << BTE_BSDF_DIFFUSE))
         hipLaunchKernelGGL((k_shade<BTE_BSDF_DIFFUSE, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
@@ -1502,6 +1558,10 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     if (wb.catMask & (1 << BTE_BSDF_DISNEY))
         hipLaunchKernelGGL((k_shade<BTE_BSDF_DISNEY, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
                            maxDepth, scenes);
+    if (B) {
+        launch_segment_rows(wb, bounce + 1, kSegRays, 1, s);
+        launch_segment_rows(wb, bounce, kSegShadow, wb.kinds, s);
+    }
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int k = 0; k < wb.kinds; ++k) {
         if (count)
@@ -1546,6 +1606,12 @@ void launch_generate_batch(const DevScene* scenes, const WaveBuffers& wb, int ma
     if (maxScenePaths <= 0 || wb.sceneCount <= 0) return;
     hipLaunchKernelGGL(k_generate_batch, dim3(blocksFor(maxScenePaths, kBlock), wb.sceneCount), dim3(kBlock), 0, s,
                        scenes, wb, totalSlots);
+    launch_segment_rows(wb, 0, kSegRays, 1, s);
+}
+
+void launch_segment_rows(const WaveBuffers& wb, int bounce, int q, int n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_segment_rows, dim3(1), dim3(64), 0, s, wb, bounce, q, n);
 }
 
 void launch_present_batch(const WaveBuffers& wb, int maxScenePixels, int frames, float4* accum, double* sum64,

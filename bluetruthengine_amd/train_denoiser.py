@@ -2,8 +2,9 @@
 
 Each training pair is a 256 x 256 crop (a camera aimed at a random window of
 a 1920 x 1080 view, so objects keep their full-resolution pixel footprint):
-the input is the 4-spp accumulation with its albedo / normal AOVs, the target
-the same crop at 4096 spp (log(1 + RGB)).  Scenes come from this package's
+the input is the accumulation after a few samples per pixel (4 by default, or
+drawn per pair from a list such as 4,8,16,32,64) with its albedo / normal
+AOVs, the target the same crop at 4096 spp (log(1 + RGB)).  Scenes come from this package's
 generators with seeds other than the benchmark scenes': env-lit fields of
 displaced spheres (the 1080p workload's generator: sphere layout, metal and
 glass materials, sky noise and sun position redrawn) and terrain rooms with
@@ -12,6 +13,10 @@ captures go straight from the accumulation buffers into the device-side
 dataset, and AdamW steps run on device.
 
     python -m bluetruthengine_amd.train_denoiser --pairs 256 --steps 4000
+
+The shipped weights (bluetruthengine_amd/weights/unet_c4.npy) come from
+    python -m bluetruthengine_amd.train_denoiser --pairs 512 --steps 12000 \
+        --input-spp 4,8,16,32,64 --envlit-frac 0.85 --seed 3
 """
 import argparse
 import json
