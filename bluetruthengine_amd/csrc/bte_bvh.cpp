@@ -229,34 +229,99 @@ Child2 childOf(const BvhHostNode& n, int k) {
     return c;
 }
 
-struct Collapser {
+// Collapse by surface-area cost (dynamic programming over the BVH2, after
+// Ylitie, Karras and Laine, "Efficient incoherent ray traversal on GPUs
+// through compressed wide BVHs", HPG 2017): for every BVH2 inner node and a
+// budget of 1..4 child slots, the cheapest way to represent its subtree,
+// either as one BVH4 node (one slot) or spread over the slots through its two
+// children.  A node visit costs kNodeCost per unit of box area, a leaf
+// kTriCost per triangle per unit area; BVH2 leaves are kept as they are.
+struct SahCollapser {
+    static constexpr float kNodeCost = 1.0f;
+    static constexpr float kTriCost = 0.3f;
     const std::vector<BvhHostNode>& n2;
+    std::vector<float> cost;     // [node * 5 + slots], slots 1..4
+    std::vector<int8_t> split;   // [node * 5 + slots]: slots given to child 0 (0 = one node), slot 0: node's own split
     std::vector<Bvh4HostNode> out;
     int maxDepth = 0;
 
+    explicit SahCollapser(const std::vector<BvhHostNode>& nodes) : n2(nodes) {}
+
+    float childCost(const Child2& c, int slots) const {
+        if (c.ref < 0) {
+            const int cnt = ((~c.ref) & (kMaxLeafTris - 1)) + 1;
+            return std::max(c.area(), 0.0f) * kTriCost * (float)cnt;
+        }
+        return cost[(size_t)c.ref * 5 + (size_t)slots];
+    }
+    float distribute(int i, int slots, int& bestK) const {
+        const Child2 a = childOf(n2[(size_t)i], 0), b = childOf(n2[(size_t)i], 1);
+        float best = __builtin_huge_valf();
+        bestK = 1;
+        for (int k = 1; k < slots; ++k) {
+            const float c = childCost(a, k) + childCost(b, slots - k);
+            if (c < best) {
+                best = c;
+                bestK = k;
+            }
+        }
+        return best;
+    }
+    void solve() {
+        const size_t n = n2.size();
+        cost.assign(n * 5, 0.0f);
+        split.assign(n * 5, 0);
+        // parents precede their children in the BVH2 array
+        for (size_t i = n; i-- > 0;) {
+            const BvhHostNode& nd = n2[i];
+            if (nd.child[0] == nd.child[1]) {   // single-leaf root: its second box is the far point
+                cost[i * 5 + 1] = cost[i * 5 + 2] = cost[i * 5 + 3] = cost[i * 5 + 4] = 0.0f;
+                continue;
+            }
+            const Child2 a = childOf(nd, 0), b = childOf(nd, 1);
+            Child2 u = a;
+            for (int ax = 0; ax < 3; ++ax) {
+                u.lo[ax] = std::min(a.lo[ax], b.lo[ax]);
+                u.hi[ax] = std::max(a.hi[ax], b.hi[ax]);
+            }
+            int k4 = 1;
+            const float asNode = std::max(u.area(), 0.0f) * kNodeCost + distribute((int)i, 4, k4);
+            cost[i * 5 + 1] = asNode;
+            split[i * 5 + 0] = (int8_t)k4;
+            for (int slots = 2; slots <= 4; ++slots) {
+                int k = 1;
+                const float spread = distribute((int)i, slots, k);
+                cost[i * 5 + (size_t)slots] = std::min(asNode, spread);
+                split[i * 5 + (size_t)slots] = spread < asNode ? (int8_t)k : (int8_t)0;
+            }
+        }
+    }
+    void collect(const Child2& c, int slots, Child2* acc, int& cnt) const {
+        if (c.ref < 0 || slots == 1 || split[(size_t)c.ref * 5 + (size_t)slots] == 0) {
+            acc[cnt++] = c;
+            return;
+        }
+        const int k = split[(size_t)c.ref * 5 + (size_t)slots];
+        const BvhHostNode& nd = n2[(size_t)c.ref];
+        collect(childOf(nd, 0), k, acc, cnt);
+        collect(childOf(nd, 1), slots - k, acc, cnt);
+    }
     // BVH4 node for BVH2 inner node `i`; returns its index
     int make(int i, int depth) {
         maxDepth = std::max(maxDepth, depth);
         Child2 ch[4];
         int cnt = 0;
         const BvhHostNode& root = n2[(size_t)i];
-        ch[cnt++] = childOf(root, 0);
-        ch[cnt++] = childOf(root, 1);
-        if (root.child[0] == root.child[1]) cnt = 1;   // single-leaf root (second box is the far point)
-        while (cnt < 4) {
-            int pick = -1;
-            float bestArea = -2.0f;
-            for (int k = 0; k < cnt; ++k)
-                if (ch[k].ref >= 0 && ch[k].area() > bestArea) { bestArea = ch[k].area(); pick = k; }
-            if (pick < 0) break;
-            const BvhHostNode& inner = n2[(size_t)ch[pick].ref];
-            ch[pick] = childOf(inner, 0);
-            ch[cnt++] = childOf(inner, 1);
+        if (root.child[0] == root.child[1]) {
+            ch[cnt++] = childOf(root, 0);
+        } else {
+            const int k = split[(size_t)i * 5 + 0];
+            collect(childOf(root, 0), k, ch, cnt);
+            collect(childOf(root, 1), 4 - k, ch, cnt);
         }
         const int idx = (int)out.size();
         out.push_back(Bvh4HostNode{});
-        int refs[4];
-        for (int k = 0; k < 4; ++k) refs[k] = 0;
+        int refs[4] = {0, 0, 0, 0};
         for (int k = 0; k < cnt; ++k) refs[k] = ch[k].ref >= 0 ? make(ch[k].ref, depth + 1) : ch[k].ref;
         Bvh4HostNode& nd = out[(size_t)idx];
         for (int k = 0; k < 4; ++k) {
@@ -275,7 +340,8 @@ struct Collapser {
 }  // namespace
 
 std::vector<Bvh4HostNode> collapse_bvh4(const std::vector<BvhHostNode>& nodes2, int& depth4) {
-    Collapser c{nodes2, {}, 0};
+    SahCollapser c(nodes2);
+    c.solve();
     c.out.reserve(nodes2.size() / 2 + 1);
     c.make(0, 1);
     depth4 = c.maxDepth;

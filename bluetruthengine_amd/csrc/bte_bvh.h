@@ -15,9 +15,10 @@ struct BvhBuild {
 
 BvhBuild build_bvh2(const BteTriangle* tris, int n);
 
-// Collapses a BVH2 into a BVH4: each BVH4 node takes the children of one
-// BVH2 inner node and keeps opening its largest-area inner child until it has
-// four.  Node 0 is the root; depth4 receives the BVH4 depth (<= BVH2 depth).
+// Collapses a BVH2 into a BVH4 by surface-area cost: each BVH4 node takes
+// up to four subtrees of one BVH2 inner node, chosen by dynamic programming
+// over the BVH2 (leaves and triangle order are kept).  Node 0 is the root;
+// depth4 receives the BVH4 depth (<= BVH2 depth).
 std::vector<Bvh4HostNode> collapse_bvh4(const std::vector<BvhHostNode>& nodes2, int& depth4);
 
 }  // namespace bte
