@@ -58,6 +58,8 @@ struct SceneData {
 
     // device scene
     DBuf dTris, dMaterials, dTriMat, dTriBsdf, dBsdfs, dEmission, dFlags, dEmitterOfTri;
+    DBuf dTexels, dTexInfo, dMrTexels, dMrInfo;   // texture maps (rt_material.cuh:26-56)
+    int texCount = 0, mrCount = 0;
     DBuf dEmitters, dEmTriIdx, dEmTriCdf, dSceneCdf, dSpots;
     DBuf dNodes, dNodes4, dLeafTris;
     int bvh4Depth = 0;
@@ -216,6 +218,15 @@ DevScene makeScene(const SceneData& c, int width, int height) {
     s.envCond = c.dEnvCond.as<float>();
     s.envMargGuide = c.envGuides ? c.dEnvMargGuide.as<int>() : nullptr;
     s.envCondGuide = c.envGuides ? c.dEnvCondGuide.as<int>() : nullptr;
+    if (c.texCount > 0) {
+        s.triMat = c.dTriMat.as<int>();
+        s.texels = c.dTexels.as<uint32_t>();
+        s.texInfo = c.dTexInfo.as<int4>();
+        s.texCount = c.texCount;
+        s.mrTexels = c.mrCount > 0 ? c.dMrTexels.as<uint32_t>() : nullptr;
+        s.mrInfo = c.mrCount > 0 ? c.dMrInfo.as<int4>() : nullptr;
+        s.mrCount = c.mrCount;
+    }
     s.cam = c.camera;
     s.emitterCount = c.emitterCount;
     s.spotCount = c.spotCount;
@@ -289,7 +300,7 @@ int kindsInUse(int& kindArea, int& kindSpot0, int& kindEnv) {
 }
 
 size_t bytesPerPath(int kinds) {
-    return 2 * (4 * sizeof(float4) + sizeof(uint2)) + 2 * sizeof(float4) + kNumCats * sizeof(int) + 1 +
+    return 2 * (4 * sizeof(float4) + sizeof(uint2)) + 2 * sizeof(float4) + kNumCats * sizeof(int) + sizeof(uint32_t) +
            (size_t)kinds * 3 * sizeof(float4) + (ctx().aov ? 2 * sizeof(float4) : 0);
 }
 
@@ -305,7 +316,7 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     }
     CHECK_RC(c.dHit.ensure(P * sizeof(float4)));
     CHECK_RC(c.dCatQ.ensure((size_t)kNumCats * P * sizeof(int)));
-    CHECK_RC(c.dCat.ensure(P));
+    CHECK_RC(c.dCat.ensure(P * sizeof(uint32_t)));
     CHECK_RC(c.dFinal.ensure(P * sizeof(float4)));
     const size_t K = (size_t)std::max(kinds, 1);
     CHECK_RC(c.dShO.ensure(K * P * sizeof(float4)));
@@ -332,7 +343,7 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     }
     wb.hit = c.dHit.as<float4>();
     wb.catQ = c.dCatQ.as<int>();
-    wb.cat = c.dCat.as<uint8_t>();
+    wb.cat = c.dCat.as<uint32_t>();
     wb.finalRad = c.dFinal.as<float4>();
     wb.shO = c.dShO.as<float4>();
     wb.shD = c.dShD.as<float4>();
@@ -1079,19 +1090,46 @@ int bte_init_spotlights(const BteSpotlight* spotlights, int spotlightCount) {
     return BTE_OK;
 }
 
-static int initTextureLayer(const uint8_t* const* pixels, int textureCount) {
-    if (!pixels || textureCount <= 0) return BTE_OK;
-    for (int i = 0; i < textureCount; ++i)
-        if (pixels[i]) return fail(BTE_ERR_UNSUPPORTED, "texture maps are not supported by this build");
+// One texture set (src/host/scene_upload.cu:213-271): entry i serves material
+// i; a null entry has no texture.  The RGBA8 texels of all entries go back to
+// back into one buffer, with (offset, width, height) per entry.
+static int uploadTextureSet(const uint8_t* const* pixels, const int* widths, const int* heights, int count,
+                            DBuf& texels, DBuf& info, int& outCount, const char* what) {
+    texels.release();
+    info.release();
+    outCount = 0;
+    if (!pixels || count <= 0) return BTE_OK;
+    std::vector<int> tab((size_t)count * 4, 0);
+    size_t total = 0;
+    for (int i = 0; i < count; ++i) {
+        if (!pixels[i]) continue;
+        if (!widths || !heights || widths[i] <= 0 || heights[i] <= 0 || widths[i] > 65536 || heights[i] > 65536)
+            return fail(BTE_ERR_INVALID, "%s %d: invalid size", what, i);
+        tab[4 * (size_t)i] = (int)total;
+        tab[4 * (size_t)i + 1] = widths[i];
+        tab[4 * (size_t)i + 2] = heights[i];
+        total += (size_t)widths[i] * (size_t)heights[i];
+        if (total >= (1ull << 31)) return fail(BTE_ERR_INVALID, "%s: more than 2^31 texels", what);
+    }
+    std::vector<uint32_t> all(std::max<size_t>(total, 1), 0u);
+    for (int i = 0; i < count; ++i)
+        if (pixels[i])
+            std::memcpy(all.data() + tab[4 * (size_t)i], pixels[i], (size_t)widths[i] * (size_t)heights[i] * 4);
+    CHECK_RC(texels.upload(all.data(), all.size() * sizeof(uint32_t)));
+    CHECK_RC(info.upload(tab.data(), tab.size() * sizeof(int)));
+    outCount = count;
     return BTE_OK;
 }
 int bte_init_textures(const uint8_t* const* pixels, const int* widths, const int* heights, int textureCount) {
-    (void)widths; (void)heights;
-    return initTextureLayer(pixels, textureCount);
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    return uploadTextureSet(pixels, widths, heights, textureCount, c.dTexels, c.dTexInfo, c.texCount, "texture");
 }
 int bte_init_mr_textures(const uint8_t* const* pixels, const int* widths, const int* heights, int textureCount) {
-    (void)widths; (void)heights;
-    return initTextureLayer(pixels, textureCount);
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    return uploadTextureSet(pixels, widths, heights, textureCount, c.dMrTexels, c.dMrInfo, c.mrCount,
+                            "metallic-roughness map");
 }
 
 int bte_init_env_map(const float* hdriPixelsRGBA, int width, int height, float intensity, float rotationDeg) {
@@ -1174,6 +1212,8 @@ int bte_cleanup(void) {
     c.dWork.release();
     c.dAovAlb.release(); c.dAovNrm.release(); c.dAovAccAlb.release(); c.dAovAccNrm.release();
     c.aov = false;
+    c.dTexels.release(); c.dTexInfo.release(); c.dMrTexels.release(); c.dMrInfo.release();
+    c.texCount = c.mrCount = 0;
     DBuf* bufs[] = {&c.dTris, &c.dMaterials, &c.dTriMat, &c.dTriBsdf, &c.dBsdfs, &c.dEmission, &c.dFlags,
                     &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes, &c.dNodes4,
                     &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dEnvMargGuide,

@@ -29,6 +29,12 @@ struct DevScene {
     const float* envCond;
     const int* envMargGuide;      // guide tables of the CDF inversion, null = plain binary search
     const int* envCondGuide;
+    const int* triMat;            // material id per triangle (texture index); null without textures
+    const uint32_t* texels;       // RGBA8 texels of every albedo texture, back to back
+    const int4* texInfo;          // [texCount]: texel offset, width, height (width 0 = no texture)
+    const uint32_t* mrTexels;     // metallic-roughness maps, same layout
+    const int4* mrInfo;           // [mrCount]
+    int texCount, mrCount;
     BteCamera cam;
     int emitterCount;
     int spotCount;
@@ -103,7 +109,7 @@ struct WaveBuffers {
     uint2* st[2];         // x = rng state, y = path id | specular << 31
     float4* hit;          // (triangle id bits, t, u, v) per queue slot
     int* catQ;            // [kNumCats * cap]: queue slots grouped by shading category
-    uint8_t* cat;         // [cap]: shading category of each queue slot
+    uint32_t* cat;        // [cap]: shading category (bits 0-2) | BSDF table index << 3, per queue slot
     float4* finalRad;     // per path id
     float4* shO;          // [kinds * cap]: origin xyz, w = tMax
     float4* shD;          // [kinds * cap]: direction xyz, w = destination bits

@@ -160,6 +160,11 @@ BTE_DEV DevScene fetchScene(const DevScene* __restrict__ scenes, int s) {
     sc.envCond = globalPtr(sc.envCond);
     sc.envMargGuide = globalPtr(sc.envMargGuide);
     sc.envCondGuide = globalPtr(sc.envCondGuide);
+    sc.triMat = globalPtr(sc.triMat);
+    sc.texels = globalPtr(sc.texels);
+    sc.texInfo = globalPtr(sc.texInfo);
+    sc.mrTexels = globalPtr(sc.mrTexels);
+    sc.mrInfo = globalPtr(sc.mrInfo);
     return sc;
 }
 
@@ -457,13 +462,77 @@ BTE_DEV int traverseW(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, co
 // ---------------------------------------------------------------------------
 // Scene helpers
 // ---------------------------------------------------------------------------
-BTE_DEV Bsdf loadBsdf(const DevScene& sc, int tri) {            // rt_material.cuh:15-24
-    const int id = sc.triBsdf[tri];
+BTE_DEV Bsdf loadBsdfId(const DevScene& sc, int id) {
     const float4* p = sc.bsdfs + 4 * (size_t)id;
     Bsdf b;
     b.p0 = p[0]; b.p1 = p[1]; b.p2 = p[2]; b.p3 = p[3];
     b.type = __float_as_int(b.p3.w);
     return b;
+}
+BTE_DEV Bsdf loadBsdf(const DevScene& sc, int tri) {            // rt_material.cuh:15-24
+    return loadBsdfId(sc, sc.triBsdf[tri]);
+}
+
+// RGBA8 texture lookup at normalized (u, v): bilinear, wrap addressing in both
+// axes, texel centres at (i + 0.5) / size, bytes read as byte / 255 (the
+// texture unit's normalized-float mode, evaluated in exact binary32).
+BTE_DEV float4 texBilinear(const uint32_t* texels, int4 info, float u, float v) {
+    const int W = info.y, H = info.z;
+    u = u - floorf(u);
+    v = v - floorf(v);
+    if (!(u == u)) u = 0.0f;
+    if (!(v == v)) v = 0.0f;
+    const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    const float fx = floorf(x), fy = floorf(y);
+    const float a = x - fx, b = y - fy;
+    int ix0 = (int)fx, iy0 = (int)fy;
+    if (ix0 < 0) ix0 += W;
+    if (iy0 < 0) iy0 += H;
+    if (ix0 >= W) ix0 -= W;
+    if (iy0 >= H) iy0 -= H;
+    const int ix1 = ix0 + 1 == W ? 0 : ix0 + 1, iy1 = iy0 + 1 == H ? 0 : iy0 + 1;
+    const uint32_t* t = texels + info.x;
+    const uint32_t t00 = t[(size_t)iy0 * W + ix0], t10 = t[(size_t)iy0 * W + ix1];
+    const uint32_t t01 = t[(size_t)iy1 * W + ix0], t11 = t[(size_t)iy1 * W + ix1];
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    float r[4];
+    for (int c = 0; c < 4; ++c) {
+        const float c00 = (float)((t00 >> (8 * c)) & 0xffu) / 255.0f, c10 = (float)((t10 >> (8 * c)) & 0xffu) / 255.0f;
+        const float c01 = (float)((t01 >> (8 * c)) & 0xffu) / 255.0f, c11 = (float)((t11 >> (8 * c)) & 0xffu) / 255.0f;
+        r[c] = c00 * w00 + c10 * w10 + c01 * w01 + c11 * w11;
+    }
+    return make_float4(r[0], r[1], r[2], r[3]);
+}
+
+// Texture override of a hit's BSDF (rt_material.cuh:26-56): the albedo map
+// replaces p0.xyz (microfacet ks recomputed from it), a metallic-roughness
+// map scales a Disney BSDF's roughness (p0.w, by G) and metallic (p1.x, by B).
+// (u, v) are the hit's barycentrics; the texture coordinate is interpolated
+// from the triangle's (src/optix_programs.cu:137-140).
+BTE_DEV void applyTextures(const DevScene& sc, int tri, float bu, float bv, Bsdf& b) {
+    if (!sc.triMat || sc.texCount <= 0) return;
+    const int m = sc.triMat[tri];
+    if (m < 0 || m >= sc.texCount) return;
+    const float* tp = reinterpret_cast<const float*>(sc.tris + tri) + 18;
+    const float w0 = 1.0f - bu - bv;
+    const float tu = w0 * tp[0] + bu * tp[2] + bv * tp[4];
+    const float tv = w0 * tp[1] + bu * tp[3] + bv * tp[5];
+    const int4 ti = sc.texInfo[m];
+    if (ti.y > 0) {
+        const float4 s = texBilinear(sc.texels, ti, tu, tv);
+        b.p0.x = s.x;
+        b.p0.y = s.y;
+        b.p0.z = s.z;
+        if (b.type == BTE_BSDF_MICROFACET) b.p1.w = 1.0f - fmaxf(b.p0.x, fmaxf(b.p0.y, b.p0.z));
+    }
+    if (m < sc.mrCount && b.type == BTE_BSDF_DISNEY) {
+        const int4 mi = sc.mrInfo[m];
+        if (mi.y > 0) {
+            const float4 mr = texBilinear(sc.mrTexels, mi, tu, tv);
+            b.p0.w = b.p0.w * mr.y;
+            b.p1.x = b.p1.x * mr.z;
+        }
+    }
 }
 
 // Hit attributes — src/optix_programs.cu:127-141 (hit point O + tD, normalized
@@ -929,10 +998,13 @@ __global__ __launch_bounds__(kBlock) void k_generate_batch(const DevScene* __res
     if (s == 0 && i == 0) wb.counters[kCtrRays] = (int)totalSlots;
 }
 
-// Shading category of a finished extension ray: 0 = miss, else the BsdfType.
-BTE_DEV int shadeCategory(const DevScene& sc, int tri) {
+// Category word of a finished extension ray: the shading category (0 = miss,
+// else the BsdfType) in bits 0-2 and the hit triangle's BSDF index above, so
+// the shading pass fetches the BSDF without going through the triangle.
+BTE_DEV uint32_t shadeCategory(const DevScene& sc, int tri) {
     if (tri < 0) return kCatMiss;
-    return __float_as_int(sc.bsdfs[4 * (size_t)sc.triBsdf[tri] + 3].w);
+    const int id = sc.triBsdf[tri];
+    return ((uint32_t)id << 3) | (uint32_t)__float_as_int(sc.bsdfs[4 * (size_t)id + 3].w);
 }
 
 // Closest hit of every queued ray (src/wavefront/ray_extend.cu:49-72).  Waves
@@ -946,7 +1018,7 @@ BTE_DEV void extendOne(const DevScene& sc, const WaveBuffers& wb, int cur, int q
     float t = 0.0f, u = 0.0f, v = 0.0f;
     const int tri = traverseW<W, false, COUNT, L>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, nodes, tris);
     wb.hit[qi] = make_float4(__int_as_float(tri), t, u, v);
-    wb.cat[qi] = (uint8_t)shadeCategory(sc, tri);
+    wb.cat[qi] = shadeCategory(sc, tri);
 }
 
 template <bool COUNT, int L, int W, bool BATCH = false>
@@ -1029,7 +1101,7 @@ __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers 
         for (int j = 0; j < kCatSub; ++j) {
             const int e = j * kBlock + wave * 64 + lane;
             const int i = first + e;
-            c[j] = (e < left) ? (int)wb.cat[i] : -1;
+            c[j] = (e < left) ? (int)(wb.cat[i] & 7u) : -1;
             if (c[j] == kCatMiss) {
                 if constexpr (BATCH) shadeMiss(fetchScene(scenes, s), wb, i, bounce);
                 else shadeMiss(sc, wb, i, bounce);
@@ -1094,11 +1166,13 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
     uint32_t rng = 0, pid = 0, aovBit = 0;
     bool spec = false;
     float4 h = make_float4(0, 0, 0, 0);
+    int bsdfId = 0;
     if (active) {
         const int qi = list[li];
         const float4 ro = wb.rayO[cur][qi], rd = wb.rayD[cur][qi], th = wb.thr[cur][qi], ra = wb.rad[cur][qi];
         const uint2 st = wb.st[cur][qi];
         h = wb.hit[qi];
+        bsdfId = (int)(wb.cat[qi] >> 3);
         o = xyz(ro); brdfPdf = ro.w;
         d = xyz(rd); eta = rd.w;
         thr = xyz(th);
@@ -1120,7 +1194,8 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
     const uint32_t rngNee = rng;
     if (active) {
         its = makeIts(sc, o, d, h);
-        bsdf = loadBsdf(sc, its.tri);
+        bsdf = loadBsdfId(sc, bsdfId);
+        applyTextures(sc, its.tri, h.z, h.w, bsdf);
         if (sc.triFlags && sc.triFlags[its.tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
         spec = !kDiffuseType;
         if (kDiffuseType) {
@@ -1343,7 +1418,8 @@ __global__ __launch_bounds__(kBlock) void k_megakernel(DevScene sc, int spp, int
                 const int tri = traverse<false>(sc, o, d, RT_EPS, 1e30f, stk, t, u, v);
                 if (tri < 0) { envHit(sc, d, spec, brdfPdf, thr, rad); break; }
                 const Its its = makeIts(sc, o, d, make_float4(__int_as_float(tri), t, u, v));
-                const Bsdf bsdf = loadBsdf(sc, tri);
+                Bsdf bsdf = loadBsdf(sc, tri);
+                applyTextures(sc, tri, u, v, bsdf);
                 if (sc.triFlags && sc.triFlags[tri] != 0) emitterHit(sc, o, d, spec, brdfPdf, thr, its, rad);
                 const Frame frame = makeFrame(its.n);
                 if (bsdfIsDiffuse(bsdf.type)) {

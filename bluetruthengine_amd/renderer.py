@@ -243,6 +243,26 @@ def cudaInitEnvMap(rgba, intensity=1.0, rotation_deg=0.0):
     return _call("bte_init_env_map", _ptr(e), W, H, float(intensity), float(rotation_deg))
 
 
+def _texture_set(name, images):
+    """cudaInitTextures / cudaInitMrTextures: one (H, W, 4) uint8 RGBA array or None per material."""
+    if not images:
+        return _call(name, None, None, None, 0)
+    n = len(images)
+    keep = [None if im is None else np.ascontiguousarray(im, np.uint8) for im in images]
+    ptrs = (ctypes.c_void_p * n)(*[None if im is None else im.ctypes.data for im in keep])
+    ws = np.array([0 if im is None else im.shape[1] for im in keep], np.int32)
+    hs = np.array([0 if im is None else im.shape[0] for im in keep], np.int32)
+    return _call(name, ptrs, _ptr(ws), _ptr(hs), n)
+
+
+def cudaInitTextures(images):
+    return _texture_set("bte_init_textures", images)
+
+
+def cudaInitMrTextures(images):
+    return _texture_set("bte_init_mr_textures", images)
+
+
 def cudaInitOptix():
     return _call("bte_build_accel")
 
@@ -283,6 +303,8 @@ def _upload(scene):
     cudaInitBsdfs(scene.bsdfs, scene.tri_bsdf_ids)
     cudaInitSpotlights(scene.spotlights)
     cudaInitEnvMap(scene.env_rgba, scene.env_intensity, scene.env_rotation_deg)
+    cudaInitTextures(scene.textures)
+    cudaInitMrTextures(scene.mr_textures)
 
 
 def _rgb(flat, w, h):

@@ -49,6 +49,9 @@ class SceneData:
     env_intensity: float = 1.0
     env_rotation_deg: float = 0.0
     camera: np.ndarray = None
+    # texture maps per material (tri_material_ids): (H, W, 4) uint8 RGBA or None
+    textures: list = None
+    mr_textures: list = None
     extra: dict = field(default_factory=dict)
 
     def digest(self):

@@ -139,8 +139,13 @@ int bte_init_emitter_table(const BteEmitter* emitters, int emitterCount,
 int bte_init_bsdfs(const BteBsdf* bsdfs, int bsdfCount, const int* triangleBsdfIds, int triangleCount);
 /* ≙ cudaInitSpotlights (render_kernel.h:148); null/0 clears (at most 14). */
 int bte_init_spotlights(const BteSpotlight* spotlights, int spotlightCount);
-/* ≙ cudaInitTextures / cudaInitMrTextures (render_kernel.h:155,163).
- * Null/0 clears; texture data returns BTE_ERR_UNSUPPORTED in this build. */
+/* ≙ cudaInitTextures / cudaInitMrTextures (render_kernel.h:155,163): entry i
+ * (RGBA8, row-major, widths[i] x heights[i]) serves material i of
+ * bte_init_scene's triangleMaterialIds; a null entry has none.  Bilinear, wrap
+ * addressing, bytes read as byte / 255.  The albedo map replaces BsdfData p0.xyz
+ * (microfacet ks recomputed); the metallic-roughness map (G = roughness,
+ * B = metallic) scales a Disney BSDF's p0.w and p1.x, for materials that also
+ * lie within the albedo set (rt_material.cuh:26-56).  Null/0 clears. */
 int bte_init_textures(const uint8_t* const* pixels, const int* widths, const int* heights, int textureCount);
 int bte_init_mr_textures(const uint8_t* const* pixels, const int* widths, const int* heights, int textureCount);
 /* ≙ cudaInitEnvMap (render_kernel.h:173).  The marginal/conditional CDFs are

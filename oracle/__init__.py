@@ -44,6 +44,8 @@ def lib():
         L.ora_math_batch.argtypes = [_i, _p, _p, _p, _l]
         L.ora_hash.restype = _u32
         L.ora_hash.argtypes = [_u32]
+        L.ora_texture_sample.argtypes = [_p, _p, _p, _l, _p]
+        L.ora_texture_sample.restype = None
         L.ora_rng_u32.argtypes = [_u32, _p, _i]
         L.ora_rng_f01.argtypes = [_u32, _p, _i]
         L.ora_pixel_seed.restype = _u32
@@ -154,7 +156,36 @@ class _SceneDesc(ctypes.Structure):
                 ("envIntensity", _f), ("envRotationDeg", _f),
                 ("envMarginalCdf", _p), ("envConditionalCdf", _p),
                 ("envTotalPower", _f),
-                ("camera", ctypes.c_byte * 56)]
+                ("camera", ctypes.c_byte * 56),
+                ("triangleMaterialIds", _p),
+                ("texels", _p), ("texInfo", _p), ("texCount", _i),
+                ("mrTexels", _p), ("mrInfo", _p), ("mrCount", _i)]
+
+
+def pack_textures(images):
+    """A texture set (list of (H, W, 4) uint8 arrays or None, one per material)
+    as the back-to-back RGBA8 texels (uint32) and (offset, width, height, 0) rows."""
+    info = np.zeros((len(images), 4), np.int32)
+    parts, off = [], 0
+    for i, im in enumerate(images):
+        if im is None:
+            continue
+        im = np.ascontiguousarray(im, np.uint8)
+        h, w = im.shape[:2]
+        info[i] = (off, w, h, 0)
+        parts.append(im.reshape(-1, 4).view(np.uint32).reshape(-1))
+        off += w * h
+    texels = np.concatenate(parts) if parts else np.zeros(1, np.uint32)
+    return texels, info
+
+
+def texture_sample(image, uv):
+    """Bilinear wrap lookup of one (H, W, 4) uint8 texture at (n, 2) normalized uv -> (n, 4)."""
+    texels, info = pack_textures([image])
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    out = np.empty((uv.shape[0], 4), np.float32)
+    lib().ora_texture_sample(_ptr(texels), _ptr(info), _ptr(uv), uv.shape[0], _ptr(out))
+    return out
 
 
 class OracleScene:
@@ -217,6 +248,15 @@ class OracleScene:
             d.envTotalPower = float(total)
         cam = keep(np.asarray(sd.camera, pod.CAMERA).reshape(()))
         ctypes.memmove(d.camera, cam.ctypes.data, 56)
+        textures = getattr(sd, "textures", None)
+        if textures:
+            d.triangleMaterialIds = _ptr(keep(sd.tri_material_ids, np.int32)).value
+            tx, ti = pack_textures(textures)
+            d.texels, d.texInfo, d.texCount = _ptr(keep(tx)).value, _ptr(keep(ti)).value, len(textures)
+            mr = getattr(sd, "mr_textures", None)
+            if mr:
+                mx, mi = pack_textures(mr)
+                d.mrTexels, d.mrInfo, d.mrCount = _ptr(keep(mx)).value, _ptr(keep(mi)).value, len(mr)
         self._desc = d
         self.handle = L.ora_scene_create(ctypes.byref(d))
         self.sd = sd

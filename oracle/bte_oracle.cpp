@@ -1225,6 +1225,69 @@ struct SampleTrace {   // optional per-bounce record for parity debugging
                        // non-delta hit (zeros when the path ends before one)
 };
 
+// RGBA8 texture lookup (the texture unit's bilinear filter with wrap
+// addressing and normalized-float reads, src/host/scene_upload.cu:250-259),
+// evaluated in exact binary32: texel centres at (i + 0.5) / size, bytes / 255.
+static inline void texBilinear(const uint32_t* texels, const int* info, float u, float v, float out[4]) {
+    const int W = info[1], H = info[2];
+    u = u - std::floor(u);
+    v = v - std::floor(v);
+    if (!(u == u)) u = 0.0f;
+    if (!(v == v)) v = 0.0f;
+    const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+    const float fx = std::floor(x), fy = std::floor(y);
+    const float a = x - fx, b = y - fy;
+    int ix0 = (int)fx, iy0 = (int)fy;
+    if (ix0 < 0) ix0 += W;
+    if (iy0 < 0) iy0 += H;
+    if (ix0 >= W) ix0 -= W;
+    if (iy0 >= H) iy0 -= H;
+    const int ix1 = ix0 + 1 == W ? 0 : ix0 + 1, iy1 = iy0 + 1 == H ? 0 : iy0 + 1;
+    const uint32_t* t = texels + info[0];
+    const uint32_t t00 = t[(size_t)iy0 * W + ix0], t10 = t[(size_t)iy0 * W + ix1];
+    const uint32_t t01 = t[(size_t)iy1 * W + ix0], t11 = t[(size_t)iy1 * W + ix1];
+    const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+    for (int c = 0; c < 4; ++c) {
+        const float c00 = (float)((t00 >> (8 * c)) & 0xffu) / 255.0f, c10 = (float)((t10 >> (8 * c)) & 0xffu) / 255.0f;
+        const float c01 = (float)((t01 >> (8 * c)) & 0xffu) / 255.0f, c11 = (float)((t11 >> (8 * c)) & 0xffu) / 255.0f;
+        out[c] = c00 * w00 + c10 * w10 + c01 * w01 + c11 * w11;
+    }
+}
+
+// loadBsdfForTriangle (include/rt_material.cuh:15-59): the BSDF by value, with
+// the albedo map replacing p0.xyz (microfacet ks recomputed) and the
+// metallic-roughness map scaling a Disney BSDF's p0.w (by G) and p1.x (by B).
+static inline OraBsdf loadBsdf(const Scene& s, const Hit& h) {
+    OraBsdf b = s.d.bsdfs[s.d.triangleBsdfIds[h.tri]];
+    if (!s.d.triangleMaterialIds || s.d.texCount <= 0) return b;
+    const int m = s.d.triangleMaterialIds[h.tri];
+    if (m < 0 || m >= s.d.texCount) return b;
+    const OraTriangle& tri = s.d.triangles[h.tri];
+    const float u = h.u, v = h.v;
+    const float w0 = 1.0f - u - v;
+    const float tu = w0 * tri.uv0.x + u * tri.uv1.x + v * tri.uv2.x;    // src/optix_programs.cu:137-140
+    const float tv = w0 * tri.uv0.y + u * tri.uv1.y + v * tri.uv2.y;
+    const int* ti = s.d.texInfo + 4 * (size_t)m;
+    if (ti[1] > 0) {
+        float t[4];
+        texBilinear(s.d.texels, ti, tu, tv, t);
+        b.p0.x = t[0];
+        b.p0.y = t[1];
+        b.p0.z = t[2];
+        if (b.type == BSDF_Microfacet) b.p1.w = 1.0f - std::max(b.p0.x, std::max(b.p0.y, b.p0.z));
+    }
+    if (m < s.d.mrCount && b.type == BSDF_Disney) {
+        const int* mi = s.d.mrInfo + 4 * (size_t)m;
+        if (mi[1] > 0) {
+            float t[4];
+            texBilinear(s.d.mrTexels, mi, tu, tv, t);
+            b.p0.w = b.p0.w * t[1];
+            b.p1.x = b.p1.x * t[2];
+        }
+    }
+    return b;
+}
+
 static inline Its makeIts(const Scene& s, const PathState& ps, const Hit& h) {
     Its its;
     its.tri = h.tri;
@@ -1275,7 +1338,7 @@ static F3 tracePath(const Scene& s, int px, int py, int W, int H, Rng& rng, int 
             break;
         }
         const Its its = makeIts(s, ps, h);
-        const OraBsdf& bsdf = s.d.bsdfs[s.d.triangleBsdfIds[its.tri]];
+        const OraBsdf bsdf = loadBsdf(s, h);
         if (s.d.triangleEmitterFlags && s.d.triangleEmitterFlags[its.tri] != 0) accumulateEmitterHit(s, ps, its);
         if (bsdfIsDiffuse(bsdf)) {
             if (!aovDone) {
@@ -1355,6 +1418,10 @@ void ora_math_batch(int fn, const float* x, const float* y, float* out, long n) 
     }
 }
 uint32_t ora_hash(uint32_t x) { return hashUint(x); }
+// Texture lookups for tests: info = (offset, width, height, 0); out4 per (u, v).
+void ora_texture_sample(const uint32_t* texels, const int* info, const float* uv, long n, float* out4) {
+    for (long i = 0; i < n; ++i) texBilinear(texels, info, uv[2 * i], uv[2 * i + 1], out4 + 4 * i);
+}
 // Writes n successive rngNextU32 outputs from makeRng(seed).
 void ora_rng_u32(uint32_t seed, uint32_t* out, int n) {
     Rng r = makeRng(seed);

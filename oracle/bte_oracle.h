@@ -79,6 +79,10 @@ struct OraSceneDesc {
     const float* envConditionalCdf;          // height*(width+1)
     float envTotalPower;
     OraCamera camera;
+    // texture maps (include/rt_material.cuh:26-56); null / 0 = none
+    const int* triangleMaterialIds;
+    const uint32_t* texels; const int* texInfo; int texCount;      // texInfo: 4 ints per entry
+    const uint32_t* mrTexels; const int* mrInfo; int mrCount;      // (offset, width, height, 0)
 };
 
 }  // extern "C"

@@ -147,6 +147,60 @@ def _spot_scene():
     return sd
 
 
+def _textured_scene():
+    """Cornell room with texture maps (rt_material.cuh:26-56): albedo maps on a
+    diffuse wall pair and a microfacet wall (ks recomputed from the map), albedo
+    and metallic-roughness maps on a Disney wall, texture coordinates well
+    outside [0, 1] (wrap addressing), untextured materials in between."""
+    from bluetruthengine_amd import pod
+    sd = scenes.cornell_c1(width=80, height=80, spp=2, max_depth=8)
+    tri = sd.triangles.copy()
+    pos = np.stack([tri["v0"], tri["v1"], tri["v2"]], 1)
+    uv = np.stack([pos[..., 0] * 1.7 + pos[..., 2] * 0.3, pos[..., 1] * 1.3 - pos[..., 2] * 0.9], -1)
+    tri["uv0"], tri["uv1"], tri["uv2"] = uv[:, 0], uv[:, 1], uv[:, 2]
+    sd.triangles = tri
+    mat = np.full(tri.shape[0], 3, np.int32)     # the room's 5 quads are the first 10 triangles
+    mat[0:2], mat[2:4], mat[4:6], mat[6:8] = 0, 3, 1, 2
+    sd.tri_material_ids = mat
+    n0 = sd.bsdfs.shape[0]
+    extra = np.array([pod.bsdf_microfacet((0.6, 0.5, 0.4), alpha=0.3),
+                      pod.bsdf_disney(base_color=(0.8, 0.7, 0.6), roughness=0.5, metallic=0.8)], pod.BSDF)
+    sd.bsdfs = np.concatenate([sd.bsdfs, extra])
+    ids = sd.tri_bsdf_ids.copy()
+    ids[6:8], ids[4:6] = n0, n0 + 1
+    sd.tri_bsdf_ids = ids
+    rs = np.random.default_rng(8)
+    checker = np.zeros((16, 16, 4), np.uint8)
+    checker[(np.add.outer(np.arange(16), np.arange(16)) // 4) % 2 == 0] = (230, 60, 40, 255)
+    checker[(np.add.outer(np.arange(16), np.arange(16)) // 4) % 2 == 1] = (40, 90, 220, 255)
+    sd.textures = [checker, rs.integers(0, 256, (5, 9, 4), dtype=np.uint8),
+                   rs.integers(0, 256, (32, 8, 4), dtype=np.uint8), None]
+    sd.mr_textures = [None, rs.integers(0, 256, (6, 6, 4), dtype=np.uint8), None, None]
+    sd.name = "cornell_textured"
+    return sd
+
+
+def test_textured_samples_match_oracle():
+    _say("case: Cornell with albedo / metallic-roughness texture maps 80x80, 2 frames, depth 8")
+    sd = _textured_scene()
+    _, _, exact = _sample_parity(sd, 2)
+    assert min(exact) == 1.0
+
+
+def test_textures_change_the_image():
+    # the same scene without maps renders differently where the maps are seen
+    sd = _textured_scene()
+    r = renderer.Renderer(sd)
+    r.render(2)
+    with_maps = r.accum().copy()
+    sd.textures = sd.mr_textures = None
+    r = renderer.Renderer(sd)
+    r.render(2)
+    diff = np.abs(with_maps - r.accum()).max(axis=-1)
+    _say(f"pixels changed by the maps: {(diff > 1e-3).mean():.3f}")
+    assert (diff > 1e-3).mean() > 0.2
+
+
 def test_spotlight_samples_match_oracle():
     _say("case: Cornell + 2 spotlights + microfacet walls 80x80, 2 frames, depth 8")
     _, _, exact = _sample_parity(_spot_scene(), 2)
