@@ -13,13 +13,19 @@
 //   triangle / emitter tables       src/main.cpp:669-758 (triangleAreaHost :469-483)
 //   spotlights                      src/main.cpp:879-904
 //   environment map                 src/main.cpp:907-930 (stbi_loadf of an RGBE file)
+//   albedo textures                 src/main.cpp:407-474, 600-612, 822-837
+//                                   (loadTextureAverage, loadImageRGBA, one
+//                                   image per material, no metallic-roughness
+//                                   maps for XML materials)
 // Differences by design: errors return a status and message instead of
 // exit(); relative paths resolve against a caller-given directory; <model>
-// (glTF) entries and albedo textures that exist on disk are reported as
-// unsupported (a texture path that does not exist falls back to the BSDF
-// colour, as in the reference).  Numbers are parsed with strtod and rounded
-// to float (OBJ: the vendored tinyobjloader's fast_float path rounds the same
-// way; XML: std::stof).
+// (glTF) entries are reported as unsupported.  Textures are decoded by the
+// baseline JPEG decoder in bte_image.cpp: a texture path that does not exist
+// falls back to the BSDF colour (white average) as in the reference, while a
+// file that exists but is not a baseline JPEG (PNG, progressive JPEG, ...)
+// is an error rather than a silently untextured material.  Numbers are
+// parsed with strtod and rounded to float (OBJ: the vendored tinyobjloader's
+// fast_float path rounds the same way; XML: std::stof).
 #include <cerrno>
 #include <cmath>
 #include <cstdarg>
@@ -30,6 +36,7 @@
 #include <vector>
 
 #include "../../include/bte_api.h"
+#include "bte_image.h"
 
 extern "C" {
 struct BteScene {
@@ -48,6 +55,10 @@ struct BteScene {
     float envIntensity = 0.0f, envRotation = 0.0f;
     BteCamera camera{};
     int width = 0, height = 0;
+    // one albedo image per material (RGBA8, row 0 = top); empty = no texture
+    std::vector<std::vector<uint8_t>> albedo;
+    std::vector<const uint8_t*> albedoPtrs;
+    std::vector<int> albedoW, albedoH;
 };
 }
 
@@ -611,14 +622,35 @@ int parse(const std::string& path, const std::string& baseDir, BteScene& sc) {
     if (mats.empty()) return sceneFail("no <material ...> elements found");
     if (bsdfs.empty()) return sceneFail("no <bsdf ...> elements found");
 
-    // materials: the texture average is stored but never read by the integrator;
-    // an existing texture file is not supported, a missing one falls back to white
+    // materials (src/main.cpp:600-612): the texture's average colour, which the
+    // integrator never reads, and the RGBA8 image itself, which overrides the
+    // BSDF colour at shading time; a missing file falls back to white / none
     for (const MaterialDesc& m : mats) {
-        if (!m.albedoTexture.empty() && fileExists(joinPath(baseDir, m.albedoTexture)))
-            return sceneFail("material \"%s\": albedo textures are not supported (\"%s\")", m.name.c_str(),
-                             m.albedoTexture.c_str());
-        sc.materials.push_back({1.0f, 1.0f, 1.0f});
+        std::vector<uint8_t> px;
+        int tw = 1, th = 1;
+        BteFloat3 avg = {1.0f, 1.0f, 1.0f};
+        const std::string tpath = joinPath(baseDir, m.albedoTexture);
+        if (!m.albedoTexture.empty() && fileExists(tpath)) {
+            std::string bytes, err;
+            if (!readFile(tpath, bytes))
+                return sceneFail("material \"%s\": cannot read albedo texture \"%s\"", m.name.c_str(),
+                                 tpath.c_str());
+            if (!bte::decode_jpeg(reinterpret_cast<const uint8_t*>(bytes.data()), bytes.size(), tw, th, px, err))
+                return sceneFail("material \"%s\": albedo texture \"%s\": %s", m.name.c_str(), tpath.c_str(),
+                                 err.c_str());
+            // loadTextureAverage (src/main.cpp:407-440): binary64 sums of byte/255
+            double acc[3] = {0.0, 0.0, 0.0};
+            const size_t n = (size_t)tw * (size_t)th;
+            for (size_t i = 0; i < n; ++i)
+                for (int k = 0; k < 3; ++k) acc[k] += px[4 * i + (size_t)k] / 255.0;
+            avg = {(float)(acc[0] / (double)n), (float)(acc[1] / (double)n), (float)(acc[2] / (double)n)};
+        }
+        sc.materials.push_back(avg);
+        sc.albedo.push_back(std::move(px));
+        sc.albedoW.push_back(tw);
+        sc.albedoH.push_back(th);
     }
+    for (const std::vector<uint8_t>& px : sc.albedo) sc.albedoPtrs.push_back(px.empty() ? nullptr : px.data());
     // BSDF table (src/main.cpp:630-665)
     for (const BsdfDesc& b : bsdfs) {
         BteBsdf d;
@@ -773,6 +805,9 @@ int bte_scene_arrays(const BteScene* sc, BteSceneArrays* a) {
     a->sceneEmitterCdf = sc->sceneCdf.data();
     a->spotlights = sc->spots.data();
     a->envRGBA = sc->env.empty() ? nullptr : sc->env.data();
+    a->albedoTextures = sc->albedoPtrs.data();
+    a->albedoTextureWidths = sc->albedoW.data();
+    a->albedoTextureHeights = sc->albedoH.data();
     return BTE_OK;
 }
 
@@ -797,6 +832,16 @@ int bte_scene_upload(const BteScene* sc) {
                              (int)sc->triBsdfIds.size())))
         return rc;
     if ((rc = bte_init_spotlights(sc->spots.empty() ? nullptr : sc->spots.data(), (int)sc->spots.size()))) return rc;
+    // src/main.cpp:822-837: one albedo entry per material (null = none), no
+    // metallic-roughness maps
+    bool anyTexture = false;
+    for (const uint8_t* p : sc->albedoPtrs) anyTexture = anyTexture || p != nullptr;
+    if (anyTexture)
+        rc = bte_init_textures(sc->albedoPtrs.data(), sc->albedoW.data(), sc->albedoH.data(),
+                               (int)sc->albedoPtrs.size());
+    else
+        rc = bte_init_textures(nullptr, nullptr, nullptr, 0);
+    if (rc || (rc = bte_init_mr_textures(nullptr, nullptr, nullptr, 0))) return rc;
     if (sc->env.empty()) return bte_init_env_map(nullptr, 0, 0, 0.0f, 0.0f);
     return bte_init_env_map(sc->env.data(), sc->envW, sc->envH, sc->envIntensity, sc->envRotation);
 }

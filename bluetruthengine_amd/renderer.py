@@ -57,7 +57,7 @@ class BteSceneArrays(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "triangles", "materials", "triangleMaterialIds", "triangleBsdfIds", "bsdfs", "triangleEmission",
         "triangleEmitterFlags", "emitters", "emitterTriIndices", "emitterTriCdf", "sceneEmitterCdf", "spotlights",
-        "envRGBA")]
+        "envRGBA", "albedoTextures", "albedoTextureWidths", "albedoTextureHeights")]
 
 
 class BteStats(ctypes.Structure):

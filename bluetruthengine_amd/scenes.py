@@ -595,6 +595,15 @@ def load_xml(path, base_dir=None, spp=1, max_depth=8, name=None):
             sd.env_intensity = float(info.envIntensity)
             sd.env_rotation_deg = float(info.envRotationDeg)
         sd.extra["materials"] = take(arr.materials, np.float32, 3 * info.materialCount, (info.materialCount, 3))
+        M = info.materialCount
+        if M and arr.albedoTextures:
+            ptrs = (ctypes.c_void_p * M).from_address(arr.albedoTextures)
+            ws = take(arr.albedoTextureWidths, np.int32, M)
+            hs = take(arr.albedoTextureHeights, np.int32, M)
+            tex = [take(ptrs[i], np.uint8, int(hs[i]) * int(ws[i]) * 4, (int(hs[i]), int(ws[i]), 4))
+                   if ptrs[i] else None for i in range(M)]
+            if any(t is not None for t in tex):
+                sd.textures = tex
         return sd
     finally:
         L.bte_scene_free(handle)

@@ -19,11 +19,30 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def test_loaded_scene_matches_oracle_and_library_upload(tmp_path):
-    _say("case: synthetic scene.xml (OBJ quads/tris, emitters, spotlight, RLE .hdr sky), 64x48, 3 frames, depth 6")
+def _add_jpeg_textures(tmp_path):
+    """Baseline JPEG albedo maps on the floor quad (4:4:4) and on the
+    microfacet triangles (4:2:0), written with the test encoder."""
+    from test_jpeg_decoder import QUANT, _image, _ycc, encode
+    y, cb, cr = _ycc(_image(32, 48, 2))
+    (tmp_path / "floor.jpg").write_bytes(encode([y, cb, cr], [(1, 1)] * 3, QUANT)[0])
+    y, cb, cr = _ycc(_image(16, 16, 3))
+    half = lambda p: p.reshape(8, 2, 8, 2).mean((1, 3))
+    (tmp_path / "mf.jpg").write_bytes(encode([y, half(cb), half(cr)], [(2, 2), (1, 1), (1, 1)], QUANT)[0])
+    xml = (tmp_path / "scene.xml").read_text().replace("missing.png", "floor.jpg")
+    xml = xml.replace('name="mMf" bsdf', 'name="mMf" albedoTexture="mf.jpg" bsdf')
+    (tmp_path / "scene.xml").write_text(xml)
+
+
+@pytest.mark.parametrize("textured", [False, True])
+def test_loaded_scene_matches_oracle_and_library_upload(tmp_path, textured):
+    _say(f"case: synthetic scene.xml (OBJ quads/tris, emitters, spotlight, RLE .hdr sky"
+         f"{', JPEG albedo maps' if textured else ''}), 64x48, 3 frames, depth 6")
     _synthetic(tmp_path, rle=True)
+    if textured:
+        _add_jpeg_textures(tmp_path)
     xml = str(tmp_path / "scene.xml")
     sd = scenes.load_xml(xml, base_dir=str(tmp_path), max_depth=6)
+    assert (sd.textures is not None) == textured
     r = renderer.Renderer(sd)
     r.set_hit_capture(3)
     r.render(3)

@@ -205,13 +205,57 @@ def test_loader_errors_are_reported(tmp_path):
     (tmp_path / "bad.xml").write_text(bad)
     with pytest.raises(ValueError, match="nope"):
         scenes.load_xml(str(tmp_path / "bad.xml"), base_dir=str(tmp_path))
-    (tmp_path / "tex.png").write_bytes(b"x")
+    # a texture file that exists but is not a baseline JPEG is an error (a
+    # missing one is no texture, as in the reference)
+    (tmp_path / "tex.png").write_bytes(b"\x89PNG\r\n\x1a\n")
     tex = (tmp_path / "scene.xml").read_text().replace("missing.png", "tex.png")
     (tmp_path / "tex.xml").write_text(tex)
-    with pytest.raises(ValueError, match="textures are not supported"):
+    with pytest.raises(ValueError, match="tex.png.*not a JPEG"):
         scenes.load_xml(str(tmp_path / "tex.xml"), base_dir=str(tmp_path))
     with pytest.raises(ValueError, match="unreadable"):
         scenes.load_xml(str(tmp_path / "none.xml"))
+
+
+def test_jpeg_albedo_texture_is_decoded_and_averaged(tmp_path):
+    from test_jpeg_decoder import QUANT, _image, _to_rgb, _ycc, encode
+    _synthetic(tmp_path)
+    H, W = 24, 40
+    y, cb, cr = _ycc(_image(H, W, 5))
+    data, recon = encode([y, cb, cr], [(1, 1)] * 3, QUANT)
+    (tmp_path / "wall.jpg").write_bytes(data)
+    xml = (tmp_path / "scene.xml").read_text().replace('name="mMf" bsdf', 'name="mMf" albedoTexture="wall.jpg" bsdf')
+    (tmp_path / "t.xml").write_text(xml)
+    sd = scenes.load_xml(str(tmp_path / "t.xml"), base_dir=str(tmp_path))
+    # one entry per material, only mMf (material 2) has an image
+    assert len(sd.textures) == 5 and [t is None for t in sd.textures] == [True, True, False, True, True]
+    img = sd.textures[2]
+    want = _to_rgb(recon, [(1, 1)] * 3, H, W)
+    assert img.shape == (H, W, 4) and np.all(img[..., 3] == 255)
+    assert np.abs(img[..., :3].astype(np.int64) - want.astype(np.int64)).max() <= 1
+    # loadTextureAverage (src/main.cpp:407-440): mean of byte/255 per channel
+    mats = sd.extra["materials"]
+    avg = (img[..., :3].astype(np.float64) / 255.0).reshape(-1, 3).mean(0)
+    assert np.allclose(mats[2], avg.astype(np.float32), rtol=0, atol=2e-7)
+    assert np.array_equal(np.delete(mats, 2, 0), np.ones((4, 3), np.float32))
+
+
+@pytest.mark.skipif(not os.path.isfile(os.path.join(REF_ASSETS, "checkerboard.jpg")),
+                    reason="reference assets not present")
+def test_reference_scene_xml_loads_its_floor_texture(tmp_path):
+    # assets/scene.xml without its <model> entries: the .glb files they name
+    # are not shipped (the reference exits on them, src/gltf_loader.cpp:784-791)
+    xml = re.sub(r"<model\b[^>]*/>", "", open(os.path.join(REF_ASSETS, "scene.xml")).read())
+    (tmp_path / "scene.xml").write_text(xml)
+    sd = scenes.load_xml(str(tmp_path / "scene.xml"), base_dir=os.path.dirname(REF_ASSETS))
+    mats = re.findall(r'<material name="([^"]+)"([^>]*)>', xml)
+    textured = [i for i, (_, attrs) in enumerate(mats) if "albedoTexture" in attrs]
+    assert textured and sd.textures is not None
+    for i, t in enumerate(sd.textures):
+        assert (t is not None) == (i in textured)
+    img = sd.textures[textured[0]]
+    assert img.shape == (2048, 2048, 4)
+    avg = (img[..., :3].astype(np.float64) / 255.0).reshape(-1, 3).mean(0)
+    assert np.allclose(sd.extra["materials"][textured[0]], avg, atol=1e-6)
 
 
 def _obj_faces(path):
