@@ -281,6 +281,9 @@ def relmse_bench(args):
            "denoised_curve": {str(k): round(v, 5) for k, v in sorted(dn_curve.items())},
            "input_curve": {str(k): round(v, 5) for k, v in sorted(noisy_curve.items())}}
     out["denoised_4spp"]["error_by_reference_luminance"] = relmse_by_luminance(dn4_img, ref)
+    out["denoised_4spp"]["flips"] = denoiser.get_flips()
+    if args.windows > 0:
+        out["denoised_4spp"]["windows"] = denoise_windows(r, denoiser, ref, args.windows, W, H)
     # relMSE(N) = a / N + b for independent samples: b is the reference's own
     # noise, a / 0.01 the spp a noise-free reference would need
     ks = sorted(k for k in curve if k >= 1024)
@@ -299,6 +302,36 @@ def relmse_bench(args):
         fit["denoised_4spp_relmse_minus_b"] = round(dn_curve[4] - b, 5)
         out["extrapolation"] = fit
     print(json.dumps(out), flush=True)
+
+
+def denoise_windows(r, denoiser, ref, n, W, H, spp=4):
+    """relMSE of 4 spp + the denoiser over n independent renders of the frame
+    (frames 4k..4k+3 of one accumulation: the window's radiance from the
+    binary64 sums, its AOV means from the running means), each denoised from
+    caller buffers (bte_denoise_image)."""
+    r.reset()
+    r.enable_accum64(True)
+    prev = None
+    errs, noisy = [], []
+    for k in range(n):
+        r.render(spp)
+        s64 = r.accum64().astype(np.float64)
+        a, m = (v.astype(np.float64) for v in r.aov())
+        f = spp * (k + 1)
+        if prev is None:
+            rgb, alb, nrm = s64 / spp, a, m
+        else:
+            ps, pa, pm = prev
+            rgb, alb, nrm = (s64 - ps) / spp, (a * f - pa * (f - spp)) / spp, (m * f - pm * (f - spp)) / spp
+        prev = (s64, a, m)
+        img = denoiser.denoise_image(rgb.astype(np.float32), alb.astype(np.float32), nrm.astype(np.float32))
+        errs.append(relmse(img, ref))
+        noisy.append(relmse(rgb, ref))
+        log(f"window {k} (frames {f - spp}..{f - 1}): denoised relMSE {errs[-1]:.5f} (input {noisy[-1]:.3f})")
+    r.enable_accum64(False)
+    return {"n": n, "spp": spp, "relmse": [round(e, 5) for e in errs], "mean": round(float(np.mean(errs)), 5),
+            "max": round(float(np.max(errs)), 5), "reaching_0.01": int(sum(e <= 0.01 for e in errs)),
+            "input_relmse": [round(e, 3) for e in noisy]}
 
 
 def serve_scenes(n):
@@ -441,6 +474,8 @@ def main():
                     help="only measure time to relMSE<=0.01: plain spp vs 4 spp + denoiser")
     ap.add_argument("--ref-spp", type=int, default=65536, help="samples per pixel of the --relmse reference")
     ap.add_argument("--skip-plain", action="store_true", help="--relmse: only the denoised side")
+    ap.add_argument("--windows", type=int, default=8,
+                    help="--relmse: also denoise this many independent 4-spp renders (0 = off)")
     ap.add_argument("--ref-cache", default="", metavar="NPZ",
                     help="--relmse: read the reference image from this file if it holds --ref-spp samples, "
                          "else render it and write it there (float32)")

@@ -22,6 +22,9 @@ typedef __bf16 bf16;
 constexpr int kInChannels = 9;     // log(1+RGB), albedo, normal
 constexpr int kInPad = 32;         // stored input channels
 constexpr int kOutChannels = 3;
+// Input firefly suppression: a pixel whose luminance exceeds this multiple of
+// the largest luminance among its in-image 8 neighbours is scaled down to it.
+constexpr float kFireflyRatio = 6.0f;
 
 // One 3x3 convolution (stride 1, zero padding 1):
 //   dst[p][co] = epilogue( sum_{tap, ci} src(p + tap)[ci] * w[co][tap][ci] )
@@ -88,13 +91,18 @@ void launch_adamw(const AdamArgs& a, hipStream_t s);
 // with wt16 of shape [cin][9][coutT] (coutT >= cout, zero-filled beyond cout).
 void launch_weight_copies(const float* w, int coutPad, int cin, int coutT, bf16* w16, bf16* wt16, hipStream_t s);
 
-// Render buffers -> network input: x[n][y][x][0..31] = {log(1+max(rgb,0)),
-// albedo, normal, 0...} for y < h, x < w and 0 in the padding up to (hp, wp).
+// Render buffers -> network input: x[b][y][x][0..31] = {log(1+rgb'),
+// albedo, normal, 0...} for y < h, x < w and 0 in the padding up to (hp, wp),
+// rgb' = max(rgb, 0) after firefly suppression (kFireflyRatio),
+// for b < flips (1, 2 or 4): image b is mirrored left-right when bit 0 of b is
+// set, top-bottom when bit 1 is set (a flip ensemble at inference).
 void launch_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
-                     bf16* x, hipStream_t s);
-// Network output (log space, fp32 [hp][wp][3]) -> linear float4 image and an
-// optional gamma-packed RGBA8 buffer (the present step of the denoised image).
-void launch_output(const float* o, int wp, int w, int h, float4* out, uint32_t* fb, hipStream_t s);
+                     int flips, bf16* x, hipStream_t s);
+// Network output (log space, fp32 [flips][hp][wp][3]) -> the mean over the
+// mirrored-back images in log space -> linear float4 image and an optional
+// gamma-packed RGBA8 buffer (the present step of the denoised image).
+void launch_output(const float* o, int wp, int hp, int w, int h, int flips, float4* out, uint32_t* fb,
+                   hipStream_t s);
 // Targets for training: t[p][c] = log(1 + max(rgb, 0)).
 void launch_log_target(const float4* rgb, long long pixels, float* t, hipStream_t s);
 void launch_fill_f32(float* p, long long n, float v, hipStream_t s);

@@ -509,18 +509,53 @@ void launch_weight_copies(const float* w, int coutPad, int cin, int coutT, bf16*
 // ---------------------------------------------------------------------------
 // render buffers <-> network tensors
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ float fireflyLum(float4 c) {
+    return (0.2126f * c.x + 0.7152f * c.y) + 0.0722f * c.z;
+}
+
+// Image b of the `flips` images (1, 2 or 4) is the frame mirrored left-right
+// when bit 0 of b is set and top-bottom when bit 1 is set, in the top-left
+// w x h corner of its wp x hp tensor (zero padding right and below).
 __global__ void k_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
-                           bf16* x) {
+                           int flips, bf16* x) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long long)wp * hp) return;
-    const int px = (int)(i % wp), py = (int)(i / wp);
+    const long long plane = (long long)wp * hp;
+    if (i >= plane * flips) return;
+    const int b = (int)(i / plane);
+    const long long j = i - (long long)b * plane;
+    const int px = (int)(j % wp), py = (int)(j / wp);
     float v[kInChannels] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (px < w && py < h) {
-        const size_t q = (size_t)py * w + px;
-        const float4 c = rgb[q];
-        v[0] = log1pf(fmaxf(c.x, 0.0f));
-        v[1] = log1pf(fmaxf(c.y, 0.0f));
-        v[2] = log1pf(fmaxf(c.z, 0.0f));
+        const int sx = (b & 1) ? w - 1 - px : px, sy = (b & 2) ? h - 1 - py : py;
+        const size_t q = (size_t)sy * w + sx;
+        float4 c = rgb[q];
+        c.x = fmaxf(c.x, 0.0f);
+        c.y = fmaxf(c.y, 0.0f);
+        c.z = fmaxf(c.z, 0.0f);
+        // firefly suppression: a pixel brighter than kFireflyRatio times the
+        // brightest of its in-image 8 neighbours is scaled down to that bound
+        const float lum = fireflyLum(c);
+        float nb = 0.0f;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int nx = sx + dx, ny = sy + dy;
+                if ((dx | dy) == 0 || nx < 0 || ny < 0 || nx >= w || ny >= h) continue;
+                float4 cn = rgb[(size_t)ny * w + nx];
+                cn.x = fmaxf(cn.x, 0.0f);
+                cn.y = fmaxf(cn.y, 0.0f);
+                cn.z = fmaxf(cn.z, 0.0f);
+                nb = fmaxf(nb, fireflyLum(cn));
+            }
+        const float bound = kFireflyRatio * nb;
+        if (lum > bound) {
+            const float k = bound / lum;
+            c.x = c.x * k;
+            c.y = c.y * k;
+            c.z = c.z * k;
+        }
+        v[0] = log1pf(c.x);
+        v[1] = log1pf(c.y);
+        v[2] = log1pf(c.z);
         if (alb) {
             const float4 a = alb[q], n = nrm[q];
             v[3] = a.x; v[4] = a.y; v[5] = a.z;
@@ -538,25 +573,37 @@ __global__ void k_features(const float4* rgb, const float4* alb, const float4* n
 }
 
 void launch_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
-                     bf16* x, hipStream_t s) {
-    const long long total = (long long)wp * hp;
+                     int flips, bf16* x, hipStream_t s) {
+    const long long total = (long long)wp * hp * flips;
     hipLaunchKernelGGL(k_features, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, rgb, alb, nrm, w, h, wp,
-                       hp, x);
+                       hp, flips, x);
 }
 
-__global__ void k_output(const float* o, int wp, int w, int h, float4* out, uint32_t* fb) {
+// Mean of the `flips` outputs in log space, each mirrored back (image b as in
+// k_features), summed in order b = 0, 1, ... so the result is deterministic.
+__global__ void k_output(const float* o, int wp, int hp, int w, int h, int flips, float4* out, uint32_t* fb) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)w * h) return;
     const int px = (int)(i % w), py = (int)(i / w);
-    const float* src = o + ((size_t)py * wp + px) * 3;
-    const float r = fmaxf(expm1f(src[0]), 0.0f), g = fmaxf(expm1f(src[1]), 0.0f), b = fmaxf(expm1f(src[2]), 0.0f);
-    out[i] = make_float4(r, g, b, 0.0f);
-    if (fb) fb[i] = packGamma(V3{r, g, b});
+    float l0 = 0.0f, l1 = 0.0f, l2 = 0.0f;
+    for (int b = 0; b < flips; ++b) {
+        const int sx = (b & 1) ? w - 1 - px : px, sy = (b & 2) ? h - 1 - py : py;
+        const float* src = o + (((size_t)b * hp + sy) * wp + sx) * 3;
+        l0 += src[0];
+        l1 += src[1];
+        l2 += src[2];
+    }
+    const float inv = 1.0f / (float)flips;
+    const float r = fmaxf(expm1f(l0 * inv), 0.0f), g = fmaxf(expm1f(l1 * inv), 0.0f), bl = fmaxf(expm1f(l2 * inv), 0.0f);
+    out[i] = make_float4(r, g, bl, 0.0f);
+    if (fb) fb[i] = packGamma(V3{r, g, bl});
 }
 
-void launch_output(const float* o, int wp, int w, int h, float4* out, uint32_t* fb, hipStream_t s) {
+void launch_output(const float* o, int wp, int hp, int w, int h, int flips, float4* out, uint32_t* fb,
+                   hipStream_t s) {
     const long long total = (long long)w * h;
-    hipLaunchKernelGGL(k_output, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, o, wp, w, h, out, fb);
+    hipLaunchKernelGGL(k_output, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, o, wp, hp, w, h, flips, out,
+                       fb);
 }
 
 __global__ void k_log_target(const float4* rgb, long long pixels, float* t) {

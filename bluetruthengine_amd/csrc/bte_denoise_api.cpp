@@ -89,6 +89,7 @@ struct Net {
     int outW = 0, outH = 0;
     bool enabled = false;
     int lossMode = 0;   // 0: L1 in log space, 1: relative MSE (linear)
+    int flips = 4;      // inference: mean over 1, 2 or 4 mirrored copies (log space)
 };
 
 Net& net() {
@@ -399,10 +400,11 @@ int denoise_view(const RenderView& v, float4* out) {
     if (!N.ready) return fail(BTE_ERR_STATE, "denoiser weights not loaded");
     if (!v.aovAlb) return fail(BTE_ERR_STATE, "the denoiser needs the albedo/normal AOVs (bte_enable_aov)");
     const int hp = (v.height + 7) / 8 * 8, wp = (v.width + 7) / 8 * 8;
-    CHECK_RC(ensureWorkspace(N, 1, hp, wp, false));
-    dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, wp, hp, T(N, T_X), v.stream);
+    // the flip ensemble runs as one batch of N.flips mirrored images
+    CHECK_RC(ensureWorkspace(N, N.flips, hp, wp, false));
+    dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, wp, hp, N.flips, T(N, T_X), v.stream);
     forward(N, v.stream);
-    dn::launch_output(N.o.as<float>(), wp, v.width, v.height, out, v.fb, v.stream);
+    dn::launch_output(N.o.as<float>(), wp, hp, v.width, v.height, N.flips, out, v.fb, v.stream);
     return BTE_OK;
 }
 
@@ -638,6 +640,35 @@ int bte_denoiser_read_tensor(int id, float* out, long long count) {
     return BTE_OK;
 }
 
+int bte_denoise_image(const float* rgb, const float* albedo, const float* normal, int width, int height,
+                      float* out) {
+    CHECK_RC(ensure_device());
+    if (!rgb || !albedo || !normal || !out || width <= 0 || height <= 0)
+        return fail(BTE_ERR_INVALID, "bte_denoise_image: null buffer or empty size");
+    for (const void* p : {(const void*)rgb, (const void*)albedo, (const void*)normal, (const void*)out})
+        if (reinterpret_cast<uintptr_t>(p) & 15u)
+            return fail(BTE_ERR_INVALID, "bte_denoise_image: buffers must be 16-byte aligned float4 arrays");
+    RenderView v;
+    v.stream = main_stream();
+    v.width = width;
+    v.height = height;
+    v.accum = reinterpret_cast<const float4*>(rgb);
+    v.aovAlb = reinterpret_cast<const float4*>(albedo);
+    v.aovNrm = reinterpret_cast<const float4*>(normal);
+    CHECK_RC(denoise_view(v, reinterpret_cast<float4*>(out)));
+    HIPCHK(hipStreamSynchronize(v.stream));
+    HIPCHK(hipGetLastError());
+    return BTE_OK;
+}
+
+int bte_denoiser_set_flips(int flips) {
+    if (flips != 1 && flips != 2 && flips != 4) return fail(BTE_ERR_INVALID, "flips must be 1, 2 or 4 (got %d)", flips);
+    net().flips = flips;
+    return BTE_OK;
+}
+
+int bte_denoiser_get_flips(void) { return net().flips; }
+
 int bte_denoiser_set_loss(int mode) {
     if (mode != 0 && mode != 1) return fail(BTE_ERR_INVALID, "loss mode must be 0 (L1, log space) or 1 (relMSE)");
     net().lossMode = mode;
@@ -671,7 +702,7 @@ int bte_denoiser_capture(int slot, int which) {
     const size_t P = (size_t)N.dsW * N.dsH;
     if (which == 0) {
         if (!v.aovAlb) return fail(BTE_ERR_STATE, "capture of inputs needs the AOVs (bte_enable_aov)");
-        dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, v.width, v.height,
+        dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, v.width, v.height, 1,
                             N.dsX.as<bf16>() + (size_t)slot * P * dn::kInPad, v.stream);
     } else {
         dn::launch_log_target(v.accum, (long long)P, N.dsT.as<float>() + (size_t)slot * P * 3, v.stream);

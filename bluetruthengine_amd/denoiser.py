@@ -97,6 +97,54 @@ def set_loss(mode):
     _call("bte_denoiser_set_loss", int(mode))
 
 
+def set_flips(flips):
+    """Inference flip ensemble: 1, 2 or 4 mirrored copies averaged in log space."""
+    _call("bte_denoiser_set_flips", int(flips))
+
+
+def get_flips():
+    return _call("bte_denoiser_get_flips")
+
+
+def _hip():
+    """The HIP runtime libbte is linked against (already loaded with it), for
+    the device buffers of denoise_image."""
+    lib()
+    h = ctypes.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    return h
+
+
+def denoise_image(rgb, albedo, normal):
+    """Denoises host images (H, W, 3 each, row 0 = bottom as Renderer.accum)
+    through bte_denoise_image (device float4 buffers)."""
+    H, W, _ = np.shape(rgb)
+    hip = _hip()
+    nbytes = H * W * 16
+    bufs = []
+    try:
+        for a in (rgb, albedo, normal, None):
+            p = ctypes.c_void_p()
+            if hip.hipMalloc(ctypes.byref(p), nbytes) != 0:
+                raise MemoryError("hipMalloc failed")
+            bufs.append(p)
+            if a is not None:
+                t = np.zeros((H, W, 4), np.float32)
+                t[..., :3] = a
+                if hip.hipMemcpy(p, t.ctypes.data_as(ctypes.c_void_p), nbytes, 1) != 0:
+                    raise RuntimeError("hipMemcpy to the device failed")
+        _call("bte_denoise_image", bufs[0], bufs[1], bufs[2], W, H, bufs[3])
+        out = np.empty((H, W, 4), np.float32)
+        if hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), bufs[3], nbytes, 2) != 0:
+            raise RuntimeError("hipMemcpy from the device failed")
+        return out[..., :3].copy()
+    finally:
+        for p in bufs:
+            hip.hipFree(p)
+
+
 def dataset_reserve(slots, width, height):
     _call("bte_denoiser_dataset_reserve", int(slots), int(width), int(height))
 

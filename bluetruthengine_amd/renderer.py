@@ -145,6 +145,9 @@ API = {
     "bte_denoiser_get_weights": [_p, _ll],
     "bte_denoiser_get_grads": [_p, _ll],
     "bte_denoiser_set_loss": [_i],
+    "bte_denoiser_set_flips": [_i],
+    "bte_denoise_image": [_p, _p, _p, _i, _i, _p],
+    "bte_denoiser_get_flips": [],
     "bte_denoiser_dataset_reserve": [_i, _i, _i],
     "bte_denoiser_capture": [_i, _i],
     "bte_denoiser_train": [_i, _i, _f, _f, ctypes.c_uint, _p],

@@ -273,6 +273,20 @@ int bte_denoiser_get_grads(float* grads, long long count);   /* of the last trai
  * linear radiance, (y - t)^2 / (y^2 + 0.01) with the prediction y held
  * constant in the denominator (the relMSE used to judge images). */
 int bte_denoiser_set_loss(int mode);
+/* Inference flip ensemble (no counterpart in the reference, whose OptiX
+ * denoiser is a single pass): bte_denoise / cudaToggleDenoiser run the network
+ * on `flips` copies of the frame (1: as is; 2: and mirrored left-right; 4: and
+ * mirrored top-bottom, both ways) as one batch and average the outputs in log
+ * space.  Default 4; the time of one denoise grows about in proportion. */
+int bte_denoiser_set_flips(int flips);
+/* ≙ optixDenoiserInvoke on caller-given layers (src/host/denoiser.cu:72-102):
+ * denoises device buffers of width x height float4 (RGB + unused w; row 0 =
+ * bottom, as bte_read_accum) -- radiance, albedo and normal -- into `out`
+ * (float4, the same layout), synchronously on the library's stream.  Needs
+ * loaded weights; the buffers must be 16-byte aligned. */
+int bte_denoise_image(const float* rgb, const float* albedo, const float* normal, int width, int height,
+                      float* out);
+int bte_denoiser_get_flips(void);
 int bte_denoiser_dataset_reserve(int slots, int width, int height);
 int bte_denoiser_capture(int slot, int which);
 int bte_denoiser_train(int steps, int batch, float lr, float weightDecay, unsigned seed, float* losses);

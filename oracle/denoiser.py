@@ -40,6 +40,37 @@ def bf16_round(x):
     return np.where(nan, np.float32(np.nan), out)
 
 
+FIREFLY_RATIO = np.float32(6.0)
+
+
+def features(rgb, alb, nrm):
+    """Render buffers (H, W, 3 each, float32) -> the network input (1, H, W, 9):
+    log(1 + rgb'), albedo, normal, where rgb' = max(rgb, 0) with firefly
+    suppression: a pixel whose luminance 0.2126 r + 0.7152 g + 0.0722 b
+    exceeds FIREFLY_RATIO times the largest luminance among its in-image 8
+    neighbours is scaled by bound / luminance (binary32 as in k_features)."""
+    f = np.float32
+    c = np.maximum(np.asarray(rgb, f), f(0))
+    H, W, _ = c.shape
+    lum = (f(0.2126) * c[..., 0] + f(0.7152) * c[..., 1]) + f(0.0722) * c[..., 2]
+    nb = np.zeros((H, W), f)
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            if dx == 0 and dy == 0:
+                continue
+            sh = np.zeros((H, W), f)      # out-of-image neighbours count as 0 (luminance is >= 0)
+            ys, yd = (slice(dy, H), slice(0, H - dy)) if dy >= 0 else (slice(0, H + dy), slice(-dy, H))
+            xs, xd = (slice(dx, W), slice(0, W - dx)) if dx >= 0 else (slice(0, W + dx), slice(-dx, W))
+            sh[yd, xd] = lum[ys, xs]
+            nb = np.maximum(nb, sh)
+    bound = FIREFLY_RATIO * nb
+    hit = lum > bound
+    k = np.where(hit, bound / np.where(hit, lum, f(1)), f(1)).astype(f)
+    c = np.where(hit[..., None], c * k[..., None], c).astype(f)
+    x = np.concatenate([np.log1p(c), np.asarray(alb, f), np.asarray(nrm, f)], -1)
+    return x[None].astype(f)
+
+
 def param_count():
     return sum(co * 9 * cr + co for _, cr, _, co, _ in LAYERS)
 

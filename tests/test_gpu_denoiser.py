@@ -179,16 +179,36 @@ def test_render_then_denoise_runs_and_matches_oracle():
     params, flat = _params(3)
     dn.set_weights(flat)
     r.render(4)
-    dn.denoise()
-    out = dn.denoised(64, 64)
-    assert np.all(np.isfinite(out)) and np.all(out >= 0)
     alb, nrm = r.aov()
     acc = r.accum()
-    x9 = np.concatenate([np.log1p(np.maximum(acc, 0)), alb, nrm], -1)[None]
-    ref = np.maximum(np.expm1(od.Net(params, bf16=True).forward(x9)["o"][0]), 0)
-    err = np.abs(np.log1p(out) - np.log1p(ref)).max()
-    _say(f"denoised vs oracle: max log-space err {err:.3g}")
-    assert err < 2e-2
+    x9 = od.features(acc, alb, nrm)
+    suppressed = int((x9[0, ..., :3] != np.log1p(np.maximum(acc, 0))).any(-1).sum())
+    _say(f"4-spp Cornell: {suppressed} pixels above the firefly bound")
+    assert suppressed > 0
+    net = od.Net(params, bf16=True)
+    assert dn.get_flips() == 4
+    for flips in (1, 2, 4):
+        # flip ensemble: the mean in log space of the outputs for the frame
+        # mirrored left-right (bit 0) and top-bottom (bit 1), mirrored back
+        dn.set_flips(flips)
+        dn.denoise()
+        out = dn.denoised(64, 64)
+        assert np.all(np.isfinite(out)) and np.all(out >= 0)
+        logs = []
+        for b in range(flips):
+            xb = x9[:, :, ::-1] if b & 1 else x9
+            xb = xb[:, ::-1] if b & 2 else xb
+            o = net.forward(np.ascontiguousarray(xb))["o"][0]
+            o = o[:, ::-1] if b & 1 else o
+            logs.append(o[::-1] if b & 2 else o)
+        ref = np.maximum(np.expm1(np.mean(logs, 0)), 0)
+        err = np.abs(np.log1p(out) - np.log1p(ref)).max()
+        _say(f"flips {flips}: denoised vs oracle: max log-space err {err:.3g}")
+        assert err < 2e-2
+        # the same buffers handed over by the caller (bte_denoise_image)
+        assert np.array_equal(dn.denoise_image(acc, alb, nrm), out)
+    with pytest.raises(renderer.BteError):
+        dn.set_flips(3)
     assert dn.toggle() is True
     r.render(1)                      # render + denoise-after-render
     out2 = dn.denoised(64, 64)

@@ -153,3 +153,43 @@ def test_planted_backward_mistakes_are_caught(monkeypatch):
     # 2: ReLU mask dropped
     monkeypatch.setattr(od.Net, "_dgrad", lambda self, name, g, mask=None: orig(self, name, g, None))
     assert worst(od.Net(params, bf16=False)) > 1e-3
+
+
+def _nb_max_torch(lum):
+    """largest luminance among the in-image 8 neighbours, through
+    torch.nn.functional.unfold (zero padding; luminance is >= 0)"""
+    t = torch.tensor(lum, dtype=torch.float64)[None, None]
+    cols = F.unfold(t, 3, padding=1)[0]                   # (9, H*W), row 4 = the pixel itself
+    return torch.cat([cols[:4], cols[5:]]).max(0).values.reshape(lum.shape).numpy()
+
+
+def test_features_firefly_suppression():
+    rs = np.random.default_rng(4)
+    H, W = 12, 17
+    rgb = np.full((H, W, 3), 0.5, np.float32) + rs.uniform(0, 0.1, (H, W, 3)).astype(np.float32)
+    alb = rs.uniform(0, 1, (H, W, 3)).astype(np.float32)
+    nrm = rs.normal(size=(H, W, 3)).astype(np.float32)
+    rgb[5, 7] = [100.0, 50.0, 25.0]          # isolated firefly
+    rgb[0, 0] = [40.0, 40.0, 40.0]           # corner firefly: 3 neighbours only
+    rgb[9, 3] = rgb[9, 4] = [30.0, 30.0, 30.0]   # a 2-pixel highlight is no firefly
+    rgb[2, 12] = [-1.0, 0.2, 0.3]            # negative radiance clamps to 0
+    x = od.features(rgb, alb, nrm)
+    assert x.shape == (1, H, W, 9) and x.dtype == np.float32
+    assert np.array_equal(x[0, ..., 3:6], alb) and np.array_equal(x[0, ..., 6:], nrm)
+    out = np.expm1(x[0, ..., :3].astype(np.float64))
+    c = np.maximum(rgb.astype(np.float64), 0)
+    lum = c @ [0.2126, 0.7152, 0.0722]
+    nb = _nb_max_torch(lum)
+    for (y, xx) in ((5, 7), (0, 0)):
+        lo = out[y, xx] @ [0.2126, 0.7152, 0.0722]
+        assert np.isclose(lo, 6.0 * nb[y, xx], rtol=1e-5), (y, xx)
+        assert np.allclose(out[y, xx] / out[y, xx].sum(), c[y, xx] / c[y, xx].sum(), rtol=1e-5)   # colour kept
+    # every other pixel is unchanged (up to log1p / expm1 rounding)
+    keep = lum <= 6.0 * nb
+    assert keep.sum() == H * W - 2
+    assert np.allclose(out[keep], c[keep], rtol=1e-5, atol=1e-7)
+    assert np.allclose(out[9, 3], 30.0, rtol=1e-5) and out[2, 12, 0] == 0.0
+    # exactly at the bound stays (strict inequality)
+    g = np.ones((3, 3, 3), np.float32)
+    g[1, 1] = 6.0
+    assert np.allclose(np.expm1(od.features(g, g, g)[0, 1, 1, :3]), 6.0, rtol=1e-6)
