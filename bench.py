@@ -94,16 +94,22 @@ def denoiser_bench(args):
         denoiser.load()
     else:
         denoiser.init(0)
-    for _ in range(2):
-        denoiser.denoise()
-    times = []
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        denoiser.denoise()
-        times.append(time.perf_counter() - t0)
-    inf_ms = float(np.median(times)) * 1e3
+    flips = denoiser.get_flips()
+
+    def timed_denoise(n_flips):
+        denoiser.set_flips(n_flips)
+        for _ in range(2):
+            denoiser.denoise()
+        times = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            denoiser.denoise()
+            times.append(time.perf_counter() - t0)
+        return float(np.median(times)) * 1e3
+    single_ms = timed_denoise(1)
+    inf_ms = timed_denoise(flips)       # the default: the flip ensemble as one batch
     hp, wp = (sd.height + 7) // 8 * 8, (sd.width + 7) // 8 * 8
-    fwd_flops = denoiser.flops_per_pixel() * hp * wp
+    fwd_flops = denoiser.flops_per_pixel() * hp * wp * flips
     # training step: batch 8 of 256x256 captured pairs
     tsd = scenes.small_envlit(width=256, height=256)
     t = renderer.Renderer(tsd)
@@ -120,11 +126,13 @@ def denoiser_bench(args):
     train_ms = (time.perf_counter() - t0) / n_train * 1e3
     train_flops = 3.0 * denoiser.flops_per_pixel() * 8 * 256 * 256   # forward + input grads + weight grads
     peak = 2500.0
-    out = {"metric": "denoiser", "inference_ms_1080p": round(inf_ms, 3), "frame": f"{sd.width}x{sd.height}",
+    out = {"metric": "denoiser", "inference_ms_1080p": round(inf_ms, 3), "flips": flips,
+           "single_pass_ms_1080p": round(single_ms, 3), "frame": f"{sd.width}x{sd.height}",
            "forward_tflop": round(fwd_flops / 1e12, 4),
            "roofline": {"bound": "mfma", "achieved": round(fwd_flops / (inf_ms * 1e-3) / 1e12, 2), "peak": peak,
                         "unit": "TFLOP/s", "frac": round(fwd_flops / (inf_ms * 1e-3) / 1e12 / peak, 4),
-                        "kernel": "U-Net forward (15 bf16 MFMA convs + pooling, features, output)"},
+                        "kernel": f"U-Net forward over the {flips}-image flip ensemble (15 bf16 MFMA convs + "
+                                  "pooling, features, output)"},
            "train_step_ms_b8_256": round(train_ms, 3),
            "train_tflops": round(train_flops / (train_ms * 1e-3) / 1e12, 2), "dtype": "bf16 (fp32 accumulate)"}
     print(json.dumps(out), flush=True)
@@ -372,6 +380,8 @@ def serve_bench(args):
             denoiser.load()
         else:
             denoiser.init(0)
+        if args.flips:
+            denoiser.set_flips(args.flips)
 
     # one scene after another
     cold, warm, dn_ms = [], [], []
@@ -443,7 +453,7 @@ def serve_bench(args):
            "pixels": int(sum(sd.width * sd.height for sd in sds)),
            "triangles": int(sum(sd.triangles.shape[0] for sd in sds)),
            "p50_ms": batch["frame_p50_ms"], "p99_ms": batch["frame_p99_ms"],
-           "batch": batch, "sequential": sequential,
+           "batch": batch, "sequential": sequential, "denoiser_flips": denoiser.get_flips(),
            "weights": "trained" if have_weights else "untrained (random init)"}
     print(json.dumps(out), flush=True)
 
@@ -481,6 +491,7 @@ def main():
                          "else render it and write it there (float32)")
     ap.add_argument("--serve", action="store_true", help="only run the 64-scene serving benchmark")
     ap.add_argument("--serve-scenes", type=int, default=64)
+    ap.add_argument("--flips", type=int, default=0, help="--serve: denoiser flip ensemble 1/2/4 (0 = library default)")
     ap.add_argument("--weights", default="", help="--relmse: comma-separated denoiser weight files to compare "
                                                   "(default: the shipped weights)")
     args = ap.parse_args()
