@@ -17,13 +17,16 @@
 //                                   (loadTextureAverage, loadImageRGBA, one
 //                                   image per material, no metallic-roughness
 //                                   maps for XML materials)
+//   glTF / GLB <model> entries      src/scene.cpp:274-316, src/main.cpp:760-818
+//                                   (loader in bte_gltf.cpp)
 // Differences by design: errors return a status and message instead of
-// exit(); relative paths resolve against a caller-given directory; <model>
-// (glTF) entries are reported as unsupported.  Textures are decoded by the
-// baseline JPEG decoder in bte_image.cpp: a texture path that does not exist
-// falls back to the BSDF colour (white average) as in the reference, while a
-// file that exists but is not a baseline JPEG (PNG, progressive JPEG, ...)
-// is an error rather than a silently untextured material.  Numbers are
+// exit(); relative paths resolve against a caller-given directory.  Textures
+// are decoded by the
+// PNG and baseline JPEG decoders in bte_png.cpp / bte_image.cpp: a texture
+// path that does not exist falls back to the BSDF colour (white average) as
+// in the reference, while a file that exists but cannot be decoded (another
+// format, progressive JPEG, corrupt data) is an error rather than a silently
+// untextured material.  Numbers are
 // parsed with strtod and rounded to float (OBJ: the vendored tinyobjloader's
 // fast_float path rounds the same way; XML: std::stof).
 #include <cerrno>
@@ -36,6 +39,7 @@
 #include <vector>
 
 #include "../../include/bte_api.h"
+#include "bte_gltf.h"
 #include "bte_image.h"
 
 extern "C" {
@@ -55,10 +59,11 @@ struct BteScene {
     float envIntensity = 0.0f, envRotation = 0.0f;
     BteCamera camera{};
     int width = 0, height = 0;
-    // one albedo image per material (RGBA8, row 0 = top); empty = no texture
-    std::vector<std::vector<uint8_t>> albedo;
-    std::vector<const uint8_t*> albedoPtrs;
-    std::vector<int> albedoW, albedoH;
+    // one albedo and one metallic-roughness image per material (RGBA8, row 0
+    // = top); empty = no texture
+    std::vector<std::vector<uint8_t>> albedo, mr;
+    std::vector<const uint8_t*> albedoPtrs, mrPtrs;
+    std::vector<int> albedoW, albedoH, mrW, mrH;
 };
 }
 
@@ -526,10 +531,26 @@ int parse(const std::string& path, const std::string& baseDir, BteScene& sc) {
         if (m.material.empty()) return sceneFail("<mesh filename=\"%s\"> missing required attribute `material`", m.filename.c_str());
         meshes.push_back(m);
     }
-    {
-        size_t from = 0;
-        if (nextTag(xml, "model", from, tag))
-            return sceneFail("<model> (glTF) entries are not supported by this loader");
+    // <model> (src/scene.cpp:274-316): glTF / GLB drop-ins
+    struct ModelDesc {
+        std::string filename;
+        bte::ModelPlacement place;
+    };
+    std::vector<ModelDesc> models;
+    for (size_t from = 0; nextTag(xml, "model", from, tag);) {
+        ModelDesc m;
+        m.filename = attr(tag, "filename");
+        bool ok = true;
+        auto rf = [&](const char* a, float& v) {
+            const std::string t = attr(tag, a);
+            if (!t.empty() && !toFloat(t, v)) ok = false;
+        };
+        rf("posX", m.place.pos[0]); rf("posY", m.place.pos[1]); rf("posZ", m.place.pos[2]);
+        rf("rotX", m.place.rotDeg[0]); rf("rotY", m.place.rotDeg[1]); rf("rotZ", m.place.rotDeg[2]);
+        rf("scaleX", m.place.scale[0]); rf("scaleY", m.place.scale[1]); rf("scaleZ", m.place.scale[2]);
+        if (!ok) return sceneFail("<model filename=\"%s\"> invalid float attribute", m.filename.c_str());
+        if (m.filename.empty()) return sceneFail("<model> missing required attribute `filename`");
+        models.push_back(m);
     }
     auto matIndex = [&](const std::string& n) {
         for (size_t i = 0; i < mats.size(); ++i)
@@ -618,9 +639,10 @@ int parse(const std::string& path, const std::string& baseDir, BteScene& sc) {
         sc.camera.fovYRadians = v[9] * 0.01745329251994329576923690768489f;
         sc.camera.aspect = (float)sc.width / (float)sc.height;
     }
-    if (meshes.empty()) return sceneFail("no <mesh> elements found");
-    if (mats.empty()) return sceneFail("no <material ...> elements found");
-    if (bsdfs.empty()) return sceneFail("no <bsdf ...> elements found");
+    // src/scene.cpp:498-512: materials and BSDFs are required by OBJ meshes only
+    if (meshes.empty() && models.empty()) return sceneFail("no <mesh> or <model> elements found");
+    if (!meshes.empty() && mats.empty()) return sceneFail("no <material ...> elements found");
+    if (!meshes.empty() && bsdfs.empty()) return sceneFail("no <bsdf ...> elements found");
 
     // materials (src/main.cpp:600-612): the texture's average colour, which the
     // integrator never reads, and the RGBA8 image itself, which overrides the
@@ -635,7 +657,7 @@ int parse(const std::string& path, const std::string& baseDir, BteScene& sc) {
             if (!readFile(tpath, bytes))
                 return sceneFail("material \"%s\": cannot read albedo texture \"%s\"", m.name.c_str(),
                                  tpath.c_str());
-            if (!bte::decode_jpeg(reinterpret_cast<const uint8_t*>(bytes.data()), bytes.size(), tw, th, px, err))
+            if (!bte::decode_image(reinterpret_cast<const uint8_t*>(bytes.data()), bytes.size(), tw, th, px, err))
                 return sceneFail("material \"%s\": albedo texture \"%s\": %s", m.name.c_str(), tpath.c_str(),
                                  err.c_str());
             // loadTextureAverage (src/main.cpp:407-440): binary64 sums of byte/255
@@ -649,8 +671,10 @@ int parse(const std::string& path, const std::string& baseDir, BteScene& sc) {
         sc.albedo.push_back(std::move(px));
         sc.albedoW.push_back(tw);
         sc.albedoH.push_back(th);
+        sc.mr.emplace_back();   // XML materials have no metallic-roughness map
+        sc.mrW.push_back(1);
+        sc.mrH.push_back(1);
     }
-    for (const std::vector<uint8_t>& px : sc.albedo) sc.albedoPtrs.push_back(px.empty() ? nullptr : px.data());
     // BSDF table (src/main.cpp:630-665)
     for (const BsdfDesc& b : bsdfs) {
         BteBsdf d;
@@ -726,6 +750,71 @@ int parse(const std::string& path, const std::string& baseDir, BteScene& sc) {
             sc.emitters.push_back(e);
         }
     }
+    // glTF / GLB models after every OBJ mesh (src/main.cpp:760-818): their
+    // materials, BSDFs and maps are appended, and each emissive material is
+    // one emitter over the model's triangles that use it
+    for (const ModelDesc& md : models) {
+        bte::GltfModel g;
+        std::string err;
+        const std::string mpath = joinPath(baseDir, md.filename);
+        if (!bte::load_gltf(mpath, md.place, g, err))
+            return sceneFail("<model filename=\"%s\">: %s", mpath.c_str(), err.c_str());
+        const int matOffset = (int)sc.materials.size(), bsdfOffset = (int)sc.bsdfs.size();
+        const int firstTri = (int)sc.tris.size();
+        for (bte::GltfSlot& slot : g.materials) {
+            sc.bsdfs.push_back(slot.bsdf);
+            sc.materials.push_back({1.0f, 1.0f, 1.0f});
+            sc.albedoW.push_back(slot.albedo.width > 0 ? slot.albedo.width : 1);
+            sc.albedoH.push_back(slot.albedo.height > 0 ? slot.albedo.height : 1);
+            sc.albedo.push_back(std::move(slot.albedo.px));
+            sc.mrW.push_back(slot.metallicRoughness.width > 0 ? slot.metallicRoughness.width : 1);
+            sc.mrH.push_back(slot.metallicRoughness.height > 0 ? slot.metallicRoughness.height : 1);
+            sc.mr.push_back(std::move(slot.metallicRoughness.px));
+        }
+        for (size_t i = 0; i < g.triangles.size(); ++i) {
+            const int local = g.materialIds[i];
+            const bte::GltfSlot& slot = g.materials[(size_t)local];
+            sc.tris.push_back(g.triangles[i]);
+            sc.triMaterialIds.push_back(matOffset + local);
+            sc.triBsdfIds.push_back(bsdfOffset + local);
+            sc.triEmitterFlags.push_back(slot.isEmitter ? 1 : 0);
+            sc.triEmission.push_back(slot.isEmitter ? slot.emission : BteFloat3{0.0f, 0.0f, 0.0f});
+        }
+        for (size_t mi = 0; mi < g.materials.size(); ++mi) {
+            const bte::GltfSlot& slot = g.materials[mi];
+            if (!slot.isEmitter) continue;
+            const int triOffset = (int)sc.emTriIdx.size(), cdfOffset = (int)sc.emTriCdf.size();
+            sc.emTriCdf.push_back(0.0f);
+            float areaSum = 0.0f;
+            const int matId = matOffset + (int)mi;
+            for (int ti = firstTri; ti < (int)sc.tris.size(); ++ti) {
+                if (sc.triMaterialIds[(size_t)ti] != matId) continue;
+                sc.emTriIdx.push_back(ti);
+                areaSum += triangleArea(sc.tris[(size_t)ti]);
+                sc.emTriCdf.push_back(areaSum);
+            }
+            // a material with no area leaves its CDF start behind, as in the reference
+            if (areaSum > 0.0f) {
+                const BteFloat3 emit = slot.emission;
+                const float lum = 0.2126f * emit.x + 0.7152f * emit.y + 0.0722f * emit.z;
+                const float weight = areaSum * lum;
+                weightSum += weight;
+                sc.sceneCdf.push_back(weightSum);
+                BteEmitter e;
+                std::memset(&e, 0, sizeof e);
+                e.triIndexOffset = triOffset;
+                e.triCount = (int)sc.emTriIdx.size() - triOffset;
+                e.cdfOffset = cdfOffset;
+                e.radiance = emit;
+                e.areaSum = areaSum;
+                e.powerWeight = weight;
+                sc.emitters.push_back(e);
+            }
+        }
+    }
+    if (sc.tris.empty()) return sceneFail("the scene has no triangles after loading meshes / models");
+    for (const std::vector<uint8_t>& px : sc.albedo) sc.albedoPtrs.push_back(px.empty() ? nullptr : px.data());
+    for (const std::vector<uint8_t>& px : sc.mr) sc.mrPtrs.push_back(px.empty() ? nullptr : px.data());
     // spotlights (src/main.cpp:879-904)
     for (const Spot& s : spots) {
         BteSpotlight sp;
@@ -808,6 +897,9 @@ int bte_scene_arrays(const BteScene* sc, BteSceneArrays* a) {
     a->albedoTextures = sc->albedoPtrs.data();
     a->albedoTextureWidths = sc->albedoW.data();
     a->albedoTextureHeights = sc->albedoH.data();
+    a->mrTextures = sc->mrPtrs.data();
+    a->mrTextureWidths = sc->mrW.data();
+    a->mrTextureHeights = sc->mrH.data();
     return BTE_OK;
 }
 
@@ -834,14 +926,24 @@ int bte_scene_upload(const BteScene* sc) {
     if ((rc = bte_init_spotlights(sc->spots.empty() ? nullptr : sc->spots.data(), (int)sc->spots.size()))) return rc;
     // src/main.cpp:822-837: one albedo entry per material (null = none), no
     // metallic-roughness maps
-    bool anyTexture = false;
-    for (const uint8_t* p : sc->albedoPtrs) anyTexture = anyTexture || p != nullptr;
-    if (anyTexture)
+    auto any = [](const std::vector<const uint8_t*>& v) {
+        for (const uint8_t* p : v)
+            if (p) return true;
+        return false;
+    };
+    // the reference uploads both per-material tables (null entries included);
+    // metallic-roughness maps are applied through the albedo table's count
+    if (any(sc->albedoPtrs) || any(sc->mrPtrs))
         rc = bte_init_textures(sc->albedoPtrs.data(), sc->albedoW.data(), sc->albedoH.data(),
                                (int)sc->albedoPtrs.size());
     else
         rc = bte_init_textures(nullptr, nullptr, nullptr, 0);
-    if (rc || (rc = bte_init_mr_textures(nullptr, nullptr, nullptr, 0))) return rc;
+    if (rc) return rc;
+    if (any(sc->mrPtrs))
+        rc = bte_init_mr_textures(sc->mrPtrs.data(), sc->mrW.data(), sc->mrH.data(), (int)sc->mrPtrs.size());
+    else
+        rc = bte_init_mr_textures(nullptr, nullptr, nullptr, 0);
+    if (rc) return rc;
     if (sc->env.empty()) return bte_init_env_map(nullptr, 0, 0, 0.0f, 0.0f);
     return bte_init_env_map(sc->env.data(), sc->envW, sc->envH, sc->envIntensity, sc->envRotation);
 }

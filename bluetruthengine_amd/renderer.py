@@ -57,7 +57,8 @@ class BteSceneArrays(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "triangles", "materials", "triangleMaterialIds", "triangleBsdfIds", "bsdfs", "triangleEmission",
         "triangleEmitterFlags", "emitters", "emitterTriIndices", "emitterTriCdf", "sceneEmitterCdf", "spotlights",
-        "envRGBA", "albedoTextures", "albedoTextureWidths", "albedoTextureHeights")]
+        "envRGBA", "albedoTextures", "albedoTextureWidths", "albedoTextureHeights", "mrTextures", "mrTextureWidths",
+        "mrTextureHeights")]
 
 
 class BteStats(ctypes.Structure):

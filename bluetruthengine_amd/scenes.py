@@ -596,14 +596,21 @@ def load_xml(path, base_dir=None, spp=1, max_depth=8, name=None):
             sd.env_rotation_deg = float(info.envRotationDeg)
         sd.extra["materials"] = take(arr.materials, np.float32, 3 * info.materialCount, (info.materialCount, 3))
         M = info.materialCount
-        if M and arr.albedoTextures:
-            ptrs = (ctypes.c_void_p * M).from_address(arr.albedoTextures)
-            ws = take(arr.albedoTextureWidths, np.int32, M)
-            hs = take(arr.albedoTextureHeights, np.int32, M)
+
+        def images(p_arr, p_w, p_h):
+            if not (M and p_arr):
+                return None
+            ptrs = (ctypes.c_void_p * M).from_address(p_arr)
+            ws, hs = take(p_w, np.int32, M), take(p_h, np.int32, M)
             tex = [take(ptrs[i], np.uint8, int(hs[i]) * int(ws[i]) * 4, (int(hs[i]), int(ws[i]), 4))
                    if ptrs[i] else None for i in range(M)]
-            if any(t is not None for t in tex):
-                sd.textures = tex
+            return tex if any(t is not None for t in tex) else None
+        sd.textures = images(arr.albedoTextures, arr.albedoTextureWidths, arr.albedoTextureHeights)
+        sd.mr_textures = images(arr.mrTextures, arr.mrTextureWidths, arr.mrTextureHeights)
+        if sd.mr_textures is not None and sd.textures is None:
+            # metallic-roughness maps apply through the per-material texture table
+            # (rt_material.cuh:26-56), which the reference always uploads
+            sd.textures = [None] * M
         return sd
     finally:
         L.bte_scene_free(handle)

@@ -305,11 +305,14 @@ int bte_denoiser_read_tensor(int id, float* out, long long count);
 /* ---- scene host layer: scene.xml / OBJ / .hdr (src/scene.cpp, src/main.cpp) */
 /* ≙ loadSceneDescription (src/scene.cpp:83-523) plus the set-up of
  * src/main.cpp:524-930: parses a scene.xml, imports its OBJ meshes (normal
- * and planar-UV fallbacks, Scale->Rx->Ry->Rz->Translate transforms), decodes
- * the materials' albedo textures (baseline JPEG; a missing file means no
- * texture, as in the reference; any other format is an error), packs the
- * BSDF table, builds the emitter tables, the camera basis, spotlights and the
- * Radiance .hdr environment.  Relative paths resolve against baseDir (the
+ * and planar-UV fallbacks, Scale->Rx->Ry->Rz->Translate transforms) and its
+ * glTF / GLB models (loadGltfModel, src/gltf_loader.cpp:768-846: node graph,
+ * Disney materials, albedo and metallic-roughness maps, emissive materials,
+ * backdrop / outline / outlier pruning), decodes the materials' albedo
+ * textures (PNG or baseline JPEG; a missing file means no texture, as in the
+ * reference; an undecodable one is an error), packs the BSDF table, builds
+ * the emitter tables, the camera basis, spotlights and the Radiance .hdr
+ * environment.  Relative paths resolve against baseDir (the
  * reference resolves them against its working directory).  Host only: no
  * device needed until bte_scene_upload.  Errors: bte_scene_last_error(). */
 typedef struct BteScene BteScene;      /* opaque */
@@ -340,6 +343,11 @@ typedef struct {                       /* views into the scene's arrays (valid u
     const uint8_t* const* albedoTextures;
     const int* albedoTextureWidths;
     const int* albedoTextureHeights;
+    /* metallic-roughness maps, materialCount entries (glTF materials only,
+     * src/main.cpp:776-781): G = roughness, B = metallic */
+    const uint8_t* const* mrTextures;
+    const int* mrTextureWidths;
+    const int* mrTextureHeights;
 } BteSceneArrays;
 int bte_scene_load_xml(const char* path, const char* baseDir, BteScene** scene);
 const char* bte_scene_last_error(void);

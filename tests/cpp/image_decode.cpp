@@ -1,6 +1,7 @@
-// Decodes a JPEG file with the library's host decoder (bte_image.cpp) and
-// writes "width height" and the RGBA8 pixels (row 0 = top) to a raw file:
-//     jpeg_decode in.jpg out.rgba
+// Decodes a PNG or JPEG file with the library's host decoders (bte_image.cpp,
+// bte_png.cpp) and writes the width, the height (int32) and the RGBA8 pixels
+// (row 0 = top) to a raw file:
+//     image_decode in.png|in.jpg out.rgba
 #include <cstdio>
 #include <vector>
 
@@ -17,7 +18,7 @@ int main(int argc, char** argv) {
     int w = 0, h = 0;
     std::vector<uint8_t> px;
     std::string err;
-    if (!bte::decode_jpeg(data.data(), data.size(), w, h, px, err)) {
+    if (!bte::decode_image(data.data(), data.size(), w, h, px, err)) {
         std::printf("error: %s\n", err.c_str());
         return 1;
     }

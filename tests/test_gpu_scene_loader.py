@@ -33,16 +33,57 @@ def _add_jpeg_textures(tmp_path):
     (tmp_path / "scene.xml").write_text(xml)
 
 
-@pytest.mark.parametrize("textured", [False, True])
-def test_loaded_scene_matches_oracle_and_library_upload(tmp_path, textured):
-    _say(f"case: synthetic scene.xml (OBJ quads/tris, emitters, spotlight, RLE .hdr sky"
-         f"{', JPEG albedo maps' if textured else ''}), 64x48, 3 frames, depth 6")
+def _add_gltf_model(tmp_path):
+    """A GLB model in view: a quad with PNG albedo and metallic-roughness maps
+    (texture coordinates past [0, 1]), a triangle strip with only a
+    metallic-roughness map, and an emissive pair of triangles."""
+    from gltf_writer import Gltf
+    from test_png_decoder import encode_png
+    rs = np.random.default_rng(21)
+    g = Gltf()
+    t_alb = g.image(encode_png(rs.integers(0, 256, (8, 8, 4)), 8, 6))
+    t_mr = g.image(encode_png(rs.integers(0, 256, (4, 4, 4)), 8, 6))
+    t_mr2 = g.image(encode_png(rs.integers(0, 256, (2, 3, 4)), 8, 6))
+    g.material(name="tex", pbrMetallicRoughness={"baseColorFactor": [0.9, 0.8, 0.7, 1.0], "metallicFactor": 0.8,
+                                                 "roughnessFactor": 0.7, "baseColorTexture": {"index": t_alb},
+                                                 "metallicRoughnessTexture": {"index": t_mr}})
+    g.material(name="mronly", pbrMetallicRoughness={"baseColorFactor": [0.2, 0.6, 0.9, 1.0],
+                                                    "metallicRoughnessTexture": {"index": t_mr2}})
+    g.material(name="glow", emissiveFactor=[3.0, 3.0, 2.0])
+    f32 = np.float32
+    quad = np.array([[-0.6, 0.2, -0.4], [0.2, 0.2, -0.6], [0.2, 1.0, -0.6], [-0.6, 1.0, -0.4]], f32)
+    uv = np.array([[-0.5, -0.5], [1.5, -0.5], [1.5, 1.5], [-0.5, 1.5]], f32)
+    idx = g.accessor(np.array([0, 1, 2, 0, 2, 3], np.uint16))
+    strip = np.array([[0.3, 0.1, -0.2], [0.7, 0.1, -0.3], [0.3, 0.6, -0.2], [0.7, 0.6, -0.4], [0.4, 1.1, -0.3],
+                      [0.8, 1.0, -0.5]], f32)
+    suv = np.array([[0, 0], [1, 0], [0, 1], [1, 1], [0, 2], [1, 2]], f32)
+    lamp = np.array([[-0.2, 1.6, -0.2], [0.2, 1.6, -0.2], [0.2, 1.6, 0.2], [-0.2, 1.6, 0.2]], f32)
+    m = g.mesh([{"attributes": {"POSITION": g.accessor(quad, minmax=True), "TEXCOORD_0": g.accessor(uv)},
+                 "indices": idx, "material": 0},
+                {"attributes": {"POSITION": g.accessor(strip, minmax=True), "TEXCOORD_0": g.accessor(suv)},
+                 "mode": 5, "material": 1},
+                {"attributes": {"POSITION": g.accessor(lamp, minmax=True)}, "indices": idx, "material": 2}])
+    g.doc["scenes"] = [{"nodes": [g.node(mesh=m, translation=[0.0, 0.05, 0.0])]}]
+    (tmp_path / "model.glb").write_bytes(g.glb())
+    xml = (tmp_path / "scene.xml").read_text().replace("</scene>", '<model filename="model.glb" rotY="10" />\n</scene>')
+    (tmp_path / "scene.xml").write_text(xml)
+
+
+@pytest.mark.parametrize("variant", ["plain", "jpeg", "gltf"])
+def test_loaded_scene_matches_oracle_and_library_upload(tmp_path, variant):
+    extra = {"plain": "", "jpeg": ", JPEG albedo maps", "gltf": ", GLB model with PNG albedo/MR maps and an emitter"}
+    _say(f"case: synthetic scene.xml (OBJ quads/tris, emitters, spotlight, RLE .hdr sky{extra[variant]}), "
+         f"64x48, 3 frames, depth 6")
     _synthetic(tmp_path, rle=True)
-    if textured:
+    if variant == "jpeg":
         _add_jpeg_textures(tmp_path)
+    elif variant == "gltf":
+        _add_gltf_model(tmp_path)
     xml = str(tmp_path / "scene.xml")
     sd = scenes.load_xml(xml, base_dir=str(tmp_path), max_depth=6)
-    assert (sd.textures is not None) == textured
+    assert (sd.textures is not None) == (variant != "plain")
+    if variant == "gltf":
+        assert sd.emitters.shape[0] == 3 and sd.mr_textures is not None and sd.triangles.shape[0] == 10 + 2 + 4 + 2
     r = renderer.Renderer(sd)
     r.set_hit_capture(3)
     r.render(3)

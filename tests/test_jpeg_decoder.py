@@ -165,9 +165,13 @@ def _to_rgb(recon, sampling, H, W):
 
 @pytest.fixture(scope="module")
 def decoder(tmp_path_factory):
-    exe = str(tmp_path_factory.mktemp("jpeg") / "jpeg_decode")
-    subprocess.run(["g++", "-std=c++17", "-O2", "-I", CSRC, os.path.join(ROOT, "tests", "cpp", "jpeg_decode.cpp"),
-                    os.path.join(CSRC, "bte_image.cpp"), "-o", exe], check=True)
+    return build_decoder(tmp_path_factory)
+
+
+def build_decoder(tmp_path_factory):
+    exe = str(tmp_path_factory.mktemp("imgdec") / "image_decode")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-I", CSRC, os.path.join(ROOT, "tests", "cpp", "image_decode.cpp"),
+                    os.path.join(CSRC, "bte_image.cpp"), os.path.join(CSRC, "bte_png.cpp"), "-o", exe], check=True)
     return exe
 
 

@@ -205,15 +205,32 @@ def test_loader_errors_are_reported(tmp_path):
     (tmp_path / "bad.xml").write_text(bad)
     with pytest.raises(ValueError, match="nope"):
         scenes.load_xml(str(tmp_path / "bad.xml"), base_dir=str(tmp_path))
-    # a texture file that exists but is not a baseline JPEG is an error (a
-    # missing one is no texture, as in the reference)
+    # a texture file that exists but cannot be decoded is an error (a missing
+    # one is no texture, as in the reference)
     (tmp_path / "tex.png").write_bytes(b"\x89PNG\r\n\x1a\n")
     tex = (tmp_path / "scene.xml").read_text().replace("missing.png", "tex.png")
     (tmp_path / "tex.xml").write_text(tex)
-    with pytest.raises(ValueError, match="tex.png.*not a JPEG"):
+    with pytest.raises(ValueError, match="tex.png.*PNG without IHDR"):
         scenes.load_xml(str(tmp_path / "tex.xml"), base_dir=str(tmp_path))
+    (tmp_path / "tex.bmp").write_bytes(b"BM" + bytes(60))
+    (tmp_path / "bmp.xml").write_text(tex.replace("tex.png", "tex.bmp"))
+    with pytest.raises(ValueError, match="unsupported image format"):
+        scenes.load_xml(str(tmp_path / "bmp.xml"), base_dir=str(tmp_path))
     with pytest.raises(ValueError, match="unreadable"):
         scenes.load_xml(str(tmp_path / "none.xml"))
+
+
+def test_png_albedo_texture(tmp_path):
+    from test_png_decoder import encode_png
+    _synthetic(tmp_path)
+    px = np.random.default_rng(6).integers(0, 256, (5, 7, 4))
+    (tmp_path / "wall.png").write_bytes(encode_png(px, 8, 6))
+    xml = (tmp_path / "scene.xml").read_text().replace("missing.png", "wall.png")
+    (tmp_path / "p.xml").write_text(xml)
+    sd = scenes.load_xml(str(tmp_path / "p.xml"), base_dir=str(tmp_path))
+    assert np.array_equal(sd.textures[0], px.astype(np.uint8)) and all(t is None for t in sd.textures[1:])
+    avg = (px[..., :3] / 255.0).reshape(-1, 3).mean(0)
+    assert np.allclose(sd.extra["materials"][0], avg, atol=1e-7)
 
 
 def test_jpeg_albedo_texture_is_decoded_and_averaged(tmp_path):
