@@ -43,8 +43,10 @@ struct ConvArgs {
     bf16* dstB; float* dstF; int dstC;   // output (one of the two), channel stride dstC
 };
 void launch_conv3x3(const ConvArgs& a, hipStream_t s);
-// 0: each wave reads weights from L1/L2; 1 (default): weight tiles staged in LDS.
+// 0: each wave reads weights from L1/L2; 1: weight tiles staged in LDS per
+// K step; 2 (default): input tiles with halo and weight slices staged in LDS.
 void set_conv_variant(int v);
+int conv_variant();
 
 // dW[co][tap][ci] += sum_p g[p][co] * src(p + tap)[ci] (fp32 atomics), src as
 // in ConvArgs; dB[co] += sum_p g[p][co] when dB is non-null.

@@ -219,12 +219,422 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_lds(ConvArgs a) {
     }
 }
 
-static int g_convVariant = 1;   // 0: direct loads, 1: LDS-staged weights
+// The convolution as a spatial tile per block: 4 * RW image rows x 32 pixels
+// (each wave RW rows, i.e. 2 * RW M tiles of 16 pixels) x NT * 16 output
+// channels.  For every 32-channel slice of the input, the block stages in LDS
+// the tile with its one-pixel halo (zero outside the image; nearest
+// upsampling and the channel concatenation resolved while staging) and the
+// slice's weights for all 9 taps; the 9 taps then read their A fragments
+// from shifted windows of that one tile, so activations cross the memory
+// hierarchy once per slice instead of once per tap.  LDS rows (one pixel's or
+// one output channel's 32 channels) are padded to 80 bytes to spread the
+// 16-byte fragment reads over the banks.
+constexpr int kTileW = 32;   // output pixels per tile row
+template <int NT, int RW>
+__global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX, int tilesY) {
+    constexpr int TH = 4 * RW, MT = 2 * RW;
+    constexpr int TR = TH + 2, TC = kTileW + 2;
+    constexpr int kPiecesA = TR * TC * 4, kPerA = (kPiecesA + kBlock - 1) / kBlock;
+    constexpr int kPiecesW = 9 * NT * 16 * 4, kPerW = (kPiecesW + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) bf16 sA[TR * TC * kLdsRow];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * NT * 16 * kLdsRow];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, r16 = lane & 15;
+    int bid = blockIdx.x;
+    const int tx = bid % tilesX;
+    bid /= tilesX;
+    const int ty = bid % tilesY, n = bid / tilesY;
+    const int x0 = tx * kTileW, y0 = ty * TH;
+    const int coBase = blockIdx.y * NT * 16;
+    const int cin = a.c0 + a.c1;
+    const int hh = a.h >> 1, wh = a.w_ >> 1;
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    for (int c = 0; c < cin; c += 32) {
+        if (c) __syncthreads();                // every wave is done with the previous slice
+        {   // input tile: global -> registers -> LDS
+            bf16x8 va[kPerA];
+#pragma unroll
+            for (int j = 0; j < kPerA; ++j) {
+                const int piece = threadIdx.x + j * kBlock;
+                va[j] = zero8();
+                if (piece < kPiecesA) {
+                    const int sidx = piece >> 2, q = piece & 3;
+                    const int yy = y0 - 1 + sidx / TC, xx = x0 - 1 + sidx % TC;
+                    if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_) {
+                        const bf16* src;
+                        if (c < a.c0)
+                            src = a.up0 ? a.src0 + (((size_t)n * hh + (yy >> 1)) * wh + (xx >> 1)) * a.c0 + c
+                                        : a.src0 + (((size_t)n * a.h + yy) * a.w_ + xx) * a.c0 + c;
+                        else
+                            src = a.src1 + (((size_t)n * a.h + yy) * a.w_ + xx) * a.c1 + (c - a.c0);
+                        va[j] = *reinterpret_cast<const bf16x8*>(src + 8 * q);
+                    }
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kPerA; ++j) {
+                const int piece = threadIdx.x + j * kBlock;
+                if (piece < kPiecesA) *reinterpret_cast<bf16x8*>(sA + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = va[j];
+            }
+        }
+        {   // weight slice of all 9 taps: global -> registers -> LDS
+            bf16x8 vw[kPerW];
+#pragma unroll
+            for (int j = 0; j < kPerW; ++j) {
+                const int piece = threadIdx.x + j * kBlock;
+                if (piece < kPiecesW) {
+                    const int row = piece >> 2, q = piece & 3;
+                    const int t = row / (NT * 16), r = row % (NT * 16);
+                    vw[j] = *reinterpret_cast<const bf16x8*>(a.w + ((size_t)(coBase + r) * 9 + t) * cin + c + 8 * q);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < kPerW; ++j) {
+                const int piece = threadIdx.x + j * kBlock;
+                if (piece < kPiecesW) *reinterpret_cast<bf16x8*>(sW + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = vw[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int oy = t / 3, ox = t % 3;   // tap offset + 1: tile row / column shift
+            bf16x8 B[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                B[nt] = *reinterpret_cast<const bf16x8*>(sW + ((t * NT + nt) * 16 + r16) * kLdsRow + 8 * g);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int ry = wave * RW + (mt >> 1), col = (mt & 1) * 16 + r16;
+                const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int y = y0 + wave * RW + (mt >> 1);
+        if (y >= a.h) continue;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = x0 + (mt & 1) * 16 + 4 * g + i;
+            if (x >= a.w_) continue;
+            const size_t p = ((size_t)n * a.h + y) * a.w_ + x;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = coBase + nt * 16 + r16;
+                if (co >= a.cout) continue;
+                float v = acc[mt][nt][i];
+                if (a.bias) v += a.bias[co];
+                if (a.resid) v += f(a.resid[p * a.residC + co]);
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (a.mask && !(f(a.mask[p * a.maskC + co]) > 0.0f)) v = 0.0f;
+                if (a.dstF) a.dstF[p * a.dstC + co] = v;
+                else a.dstB[p * a.dstC + co] = (bf16)v;
+            }
+        }
+    }
+}
+
+// Single-slice layers (32 input channels): persistent blocks keep the weights
+// of all 9 taps in LDS for their whole run and walk tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ...; the next tile's input (with halo) is loaded
+// into registers while the current one is computed and stored.
+template <int NT, int RW>
+__global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX, int tilesY, int numTiles) {
+    constexpr int TH = 4 * RW, MT = 2 * RW;
+    constexpr int TR = TH + 2, TC = kTileW + 2;
+    constexpr int kPiecesA = TR * TC * 4, kPerA = (kPiecesA + kBlock - 1) / kBlock;
+    constexpr int kPiecesW = 9 * NT * 16 * 4, kPerW = (kPiecesW + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) bf16 sA[TR * TC * kLdsRow];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * NT * 16 * kLdsRow];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int coBase = blockIdx.y * NT * 16;
+    const int hh = a.h >> 1, wh = a.w_ >> 1;
+    {   // weights of the one 32-channel slice, all taps
+        bf16x8 vw[kPerW];
+#pragma unroll
+        for (int j = 0; j < kPerW; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            if (piece < kPiecesW) {
+                const int row = piece >> 2, q = piece & 3;
+                const int t = row / (NT * 16), r = row % (NT * 16);
+                vw[j] = *reinterpret_cast<const bf16x8*>(a.w + ((size_t)(coBase + r) * 9 + t) * 32 + 8 * q);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPerW; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            if (piece < kPiecesW) *reinterpret_cast<bf16x8*>(sW + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = vw[j];
+        }
+    }
+    // per-lane output channels' biases, read once (no global loads in the
+    // epilogue of bias-only layers, so the next tile's loads stay in flight)
+    float biasv[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int co = coBase + nt * 16 + r16;
+        biasv[nt] = (a.bias && co < a.cout) ? a.bias[co] : 0.0f;
+    }
+    bf16x8 va[kPerA];
+    auto loadTile = [&](int tile) {
+        const int tx = tile % tilesX, rest = tile / tilesX;
+        const int ty = rest % tilesY, n = rest / tilesY;
+        const int x0 = tx * kTileW, y0 = ty * TH;
+#pragma unroll
+        for (int j = 0; j < kPerA; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            va[j] = zero8();
+            if (piece < kPiecesA) {
+                const int sidx = piece >> 2, q = piece & 3;
+                const int yy = y0 - 1 + sidx / TC, xx = x0 - 1 + sidx % TC;
+                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_) {
+                    const bf16* src;
+                    if (a.c0 > 0)
+                        src = a.up0 ? a.src0 + (((size_t)n * hh + (yy >> 1)) * wh + (xx >> 1)) * 32
+                                    : a.src0 + (((size_t)n * a.h + yy) * a.w_ + xx) * 32;
+                    else
+                        src = a.src1 + (((size_t)n * a.h + yy) * a.w_ + xx) * 32;
+                    va[j] = *reinterpret_cast<const bf16x8*>(src + 8 * q);
+                }
+            }
+        }
+    };
+    int tile = blockIdx.x;
+    if (tile < numTiles) loadTile(tile);
+    for (; tile < numTiles; tile += gridDim.x) {
+        __syncthreads();                       // the previous tile's taps are done with sA
+#pragma unroll
+        for (int j = 0; j < kPerA; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            if (piece < kPiecesA) *reinterpret_cast<bf16x8*>(sA + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = va[j];
+        }
+        __syncthreads();
+        const int next = tile + (int)gridDim.x;
+        if (next < numTiles) loadTile(next);   // in flight while this tile is computed
+        f32x4 acc[MT][NT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int oy = t / 3, ox = t % 3;
+            bf16x8 B[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                B[nt] = *reinterpret_cast<const bf16x8*>(sW + ((t * NT + nt) * 16 + r16) * kLdsRow + 8 * g);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int ry = wave * RW + (mt >> 1), col = (mt & 1) * 16 + r16;
+                const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+        const int tx = tile % tilesX, rest = tile / tilesX;
+        const int ty = rest % tilesY, n = rest / tilesY;
+        const int x0 = tx * kTileW, y0 = ty * TH;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const int y = y0 + wave * RW + (mt >> 1);
+            if (y >= a.h) continue;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = x0 + (mt & 1) * 16 + 4 * g + i;
+                if (x >= a.w_) continue;
+                const size_t p = ((size_t)n * a.h + y) * a.w_ + x;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int co = coBase + nt * 16 + r16;
+                    if (co >= a.cout) continue;
+                    float v = acc[mt][nt][i];
+                    if (a.bias) v += biasv[nt];
+                    if (a.resid) v += f(a.resid[p * a.residC + co]);
+                    if (a.relu) v = fmaxf(v, 0.0f);
+                    if (a.mask && !(f(a.mask[p * a.maskC + co]) > 0.0f)) v = 0.0f;
+                    if (a.dstF) a.dstF[p * a.dstC + co] = v;
+                    else a.dstB[p * a.dstC + co] = (bf16)v;
+                }
+            }
+        }
+    }
+}
+
+// Multi-slice layers (64 or more input channels): persistent blocks walk
+// (tile, 32-channel slice) items in order; while one slice's 9 taps run from
+// LDS, the next item's input slice (with halo) and weight slice are loaded
+// into registers.  The output of a tile is written after its last slice.
+template <int NT, int RW>
+__global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX, int tilesY, int numTiles) {
+    constexpr int TH = 4 * RW, MT = 2 * RW;
+    constexpr int TR = TH + 2, TC = kTileW + 2;
+    constexpr int kPiecesA = TR * TC * 4, kPerA = (kPiecesA + kBlock - 1) / kBlock;
+    constexpr int kPiecesW = 9 * NT * 16 * 4, kPerW = (kPiecesW + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) bf16 sA[TR * TC * kLdsRow];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * NT * 16 * kLdsRow];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int coBase = blockIdx.y * NT * 16;
+    const int cin = a.c0 + a.c1, slices = cin / 32;
+    const int hh = a.h >> 1, wh = a.w_ >> 1;
+    float biasv[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+        const int co = coBase + nt * 16 + r16;
+        biasv[nt] = (a.bias && co < a.cout) ? a.bias[co] : 0.0f;
+    }
+    bf16x8 va[kPerA], vw[kPerW];
+    auto load = [&](int tile, int slice) {
+        const int tx = tile % tilesX, rest = tile / tilesX;
+        const int ty = rest % tilesY, n = rest / tilesY;
+        const int x0 = tx * kTileW, y0 = ty * TH, c = slice * 32;
+#pragma unroll
+        for (int j = 0; j < kPerA; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            va[j] = zero8();
+            if (piece < kPiecesA) {
+                const int sidx = piece >> 2, q = piece & 3;
+                const int yy = y0 - 1 + sidx / TC, xx = x0 - 1 + sidx % TC;
+                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_) {
+                    const bf16* src;
+                    if (c < a.c0)
+                        src = a.up0 ? a.src0 + (((size_t)n * hh + (yy >> 1)) * wh + (xx >> 1)) * a.c0 + c
+                                    : a.src0 + (((size_t)n * a.h + yy) * a.w_ + xx) * a.c0 + c;
+                    else
+                        src = a.src1 + (((size_t)n * a.h + yy) * a.w_ + xx) * a.c1 + (c - a.c0);
+                    va[j] = *reinterpret_cast<const bf16x8*>(src + 8 * q);
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kPerW; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            if (piece < kPiecesW) {
+                const int row = piece >> 2, q = piece & 3;
+                const int t = row / (NT * 16), r = row % (NT * 16);
+                vw[j] = *reinterpret_cast<const bf16x8*>(a.w + ((size_t)(coBase + r) * 9 + t) * cin + c + 8 * q);
+            }
+        }
+    };
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    int tile = blockIdx.x, slice = 0;
+    if (tile < numTiles) load(tile, 0);
+    while (tile < numTiles) {
+        __syncthreads();                       // the previous slice's taps are done with LDS
+#pragma unroll
+        for (int j = 0; j < kPerA; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            if (piece < kPiecesA) *reinterpret_cast<bf16x8*>(sA + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = va[j];
+        }
+#pragma unroll
+        for (int j = 0; j < kPerW; ++j) {
+            const int piece = threadIdx.x + j * kBlock;
+            if (piece < kPiecesW) *reinterpret_cast<bf16x8*>(sW + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = vw[j];
+        }
+        __syncthreads();
+        const bool last = slice + 1 == slices;
+        const int nTile = last ? tile + (int)gridDim.x : tile, nSlice = last ? 0 : slice + 1;
+        if (nTile < numTiles) load(nTile, nSlice);   // in flight while this slice is computed
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int oy = t / 3, ox = t % 3;
+            bf16x8 B[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                B[nt] = *reinterpret_cast<const bf16x8*>(sW + ((t * NT + nt) * 16 + r16) * kLdsRow + 8 * g);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int ry = wave * RW + (mt >> 1), col = (mt & 1) * 16 + r16;
+                const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+        if (last) {
+            const int tx = tile % tilesX, rest = tile / tilesX;
+            const int ty = rest % tilesY, n = rest / tilesY;
+            const int x0 = tx * kTileW, y0 = ty * TH;
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const int y = y0 + wave * RW + (mt >> 1);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int x = x0 + (mt & 1) * 16 + 4 * g + i;
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt) {
+                        const int co = coBase + nt * 16 + r16;
+                        float v = acc[mt][nt][i];
+                        acc[mt][nt][i] = 0.0f;
+                        if (y >= a.h || x >= a.w_ || co >= a.cout) continue;
+                        const size_t p = ((size_t)n * a.h + y) * a.w_ + x;
+                        if (a.bias) v += biasv[nt];
+                        if (a.resid) v += f(a.resid[p * a.residC + co]);
+                        if (a.relu) v = fmaxf(v, 0.0f);
+                        if (a.mask && !(f(a.mask[p * a.maskC + co]) > 0.0f)) v = 0.0f;
+                        if (a.dstF) a.dstF[p * a.dstC + co] = v;
+                        else a.dstB[p * a.dstC + co] = (bf16)v;
+                    }
+                }
+            }
+        }
+        tile = nTile;
+        slice = nSlice;
+    }
+}
+
+static int g_convVariant = 2;   // 0: direct loads, 1: LDS-staged weights, 2: LDS-staged tiles
 void set_conv_variant(int v) { g_convVariant = v; }
+int conv_variant() { return g_convVariant; }
 
 void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
     const long long P = (long long)a.n * a.h * a.w_;
     const int tiles = a.coutPad / 16;
+    if (g_convVariant == 2) {
+        // 8 x 32-pixel tiles; at most 64 output channels per block (LDS)
+        const int nt4 = (tiles % 4 == 0) ? 4 : (tiles % 2 == 0) ? 2 : 1;
+        const int tilesX = (a.w_ + kTileW - 1) / kTileW, tilesY = (a.h + 7) / 8;
+        const long long numTiles = (long long)tilesX * tilesY * a.n;
+        // persistent blocks: 512 in all (two per CU), split over the output-channel groups
+        auto persistent = [&](int groups) {
+            return dim3((unsigned)std::max<long long>(1, std::min<long long>(numTiles, 512 / groups)), (unsigned)groups);
+        };
+        if (a.c0 + a.c1 == 32) {
+            const dim3 grid = persistent(tiles / nt4);
+            switch (nt4) {
+                case 4: hipLaunchKernelGGL((k_conv3x3_tile1<4, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles); break;
+                case 2: hipLaunchKernelGGL((k_conv3x3_tile1<2, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles); break;
+                default: hipLaunchKernelGGL((k_conv3x3_tile1<1, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles); break;
+            }
+            return;
+        }
+        if ((a.c0 + a.c1) >= 192 && nt4 == 4) {
+            // many slices and >= 64 output channels: one pass over the input
+            // per 64 channels beats the prefetching kernel's 32-channel groups
+            const dim3 grid((unsigned)numTiles, (unsigned)(tiles / 4));
+            hipLaunchKernelGGL((k_conv3x3_tile<4, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY);
+            return;
+        }
+        // several slices: at most 32 output channels per block (registers for the prefetch)
+        const int nt2 = (tiles % 2 == 0) ? 2 : 1;
+        const dim3 grid = persistent(tiles / nt2);
+        if (nt2 == 2) hipLaunchKernelGGL((k_conv3x3_tileN<2, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles);
+        else hipLaunchKernelGGL((k_conv3x3_tileN<1, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles);
+        return;
+    }
     const int nt = (tiles % 8 == 0) ? 8 : (tiles % 4 == 0) ? 4 : (tiles % 2 == 0) ? 2 : 1;
     if (g_convVariant == 1) {
         // pixels per block: 4 waves x MT x 16 (MT = 2 for 128-wide N tiles)

@@ -661,6 +661,12 @@ int bte_denoise_image(const float* rgb, const float* albedo, const float* normal
     return BTE_OK;
 }
 
+int bte_denoiser_set_conv_kernel(int variant) {
+    if (variant < 0 || variant > 2) return fail(BTE_ERR_INVALID, "conv kernel variant must be 0, 1 or 2 (got %d)", variant);
+    dn::set_conv_variant(variant);
+    return BTE_OK;
+}
+
 int bte_denoiser_set_flips(int flips) {
     if (flips != 1 && flips != 2 && flips != 4) return fail(BTE_ERR_INVALID, "flips must be 1, 2 or 4 (got %d)", flips);
     net().flips = flips;

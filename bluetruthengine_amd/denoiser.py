@@ -102,6 +102,11 @@ def set_flips(flips):
     _call("bte_denoiser_set_flips", int(flips))
 
 
+def set_conv_kernel(variant):
+    """0: fragments from L1/L2, 1: LDS-staged weight tiles, 2 (default): LDS-staged input tiles."""
+    _call("bte_denoiser_set_conv_kernel", int(variant))
+
+
 def get_flips():
     return _call("bte_denoiser_get_flips")
 

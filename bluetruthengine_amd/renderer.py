@@ -147,6 +147,7 @@ API = {
     "bte_denoiser_get_grads": [_p, _ll],
     "bte_denoiser_set_loss": [_i],
     "bte_denoiser_set_flips": [_i],
+    "bte_denoiser_set_conv_kernel": [_i],
     "bte_denoise_image": [_p, _p, _p, _i, _i, _p],
     "bte_denoiser_get_flips": [],
     "bte_denoiser_dataset_reserve": [_i, _i, _i],

@@ -279,6 +279,12 @@ int bte_denoiser_set_loss(int mode);
  * mirrored top-bottom, both ways) as one batch and average the outputs in log
  * space.  Default 4; the time of one denoise grows about in proportion. */
 int bte_denoiser_set_flips(int flips);
+/* Convolution kernel of the denoiser (no counterpart in the reference; for
+ * tests and measurements): 0 = fragments from L1/L2, 1 = weight tiles staged
+ * in LDS per K step, 2 (default) = input tile with halo and weight slice
+ * staged in LDS per 32-channel slice.  All three compute the same
+ * convolution; sums may differ in the last fp32 bits. */
+int bte_denoiser_set_conv_kernel(int variant);
 /* ≙ optixDenoiserInvoke on caller-given layers (src/host/denoiser.cu:72-102):
  * denoises device buffers of width x height float4 (RGB + unused w; row 0 =
  * bottom, as bte_read_accum) -- radiance, albedo and normal -- into `out`

@@ -58,11 +58,16 @@ def _forward_case(seed, n, h, w):
     return params, x9
 
 
-def test_forward_kernels_match_oracle():
-    _say("case: U-Net forward, n=2 h=16 w=24, seed 0")
+@pytest.mark.parametrize("variant,n,h,w", [(2, 2, 16, 24), (2, 1, 40, 72), (1, 2, 16, 24), (0, 1, 40, 72)])
+def test_forward_kernels_match_oracle(variant, n, h, w):
+    _say(f"case: U-Net forward, conv kernel {variant}, n={n} h={h} w={w}, seed 0")
     renderer.cudaCleanup()
-    params, x9 = _forward_case(0, 2, 16, 24)
-    out = dn.forward(x9)
+    params, x9 = _forward_case(0, n, h, w)
+    dn.set_conv_kernel(variant)
+    try:
+        out = dn.forward(x9)
+    finally:
+        dn.set_conv_kernel(2)
     net = od.Net(params, bf16=True)
     g = {name: dn.tensor(FWD[name]).astype(np.float64) for name in od.FORWARD}
     assert np.array_equal(g["x"][..., :9], od.bf16_round(x9).astype(np.float64))
