@@ -4,6 +4,10 @@
 
 #include "bte_device.h"
 
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 namespace bte {
 namespace dn {
 
@@ -342,11 +346,56 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX,
     }
 }
 
+// Epilogue of a tile kernel: bias, residual, ReLU, mask and the store of one
+// wave's MT x 16 pixels x NT x 16 channels.  With EXTRA (a residual or a
+// mask), the residual / mask reads of an M tile are issued together before
+// any of them is used, so each M tile waits for global memory once; without
+// it the epilogue reads no global memory.
+template <int MT, int NT, int RW, bool EXTRA>
+__device__ __forceinline__ void tileEpilogue(const ConvArgs& a, f32x4 (&acc)[MT][NT], const float (&biasv)[NT],
+                                             int n, int x0, int y0, int wave, int g, int r16, int coBase) {
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int y = y0 + wave * RW + (mt >> 1);
+        float rv[4][NT];
+        bool keep[4][NT];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = x0 + (mt & 1) * 16 + 4 * g + i;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = coBase + nt * 16 + r16;
+                const bool in = y < a.h && x < a.w_ && co < a.cout;
+                const size_t p = in ? ((size_t)n * a.h + y) * a.w_ + x : 0;
+                rv[i][nt] = (EXTRA && a.resid && in) ? f(a.resid[p * a.residC + co]) : 0.0f;
+                keep[i][nt] = !(EXTRA && a.mask && in) || f(a.mask[p * a.maskC + co]) > 0.0f;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int x = x0 + (mt & 1) * 16 + 4 * g + i;
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int co = coBase + nt * 16 + r16;
+                if (y >= a.h || x >= a.w_ || co >= a.cout) continue;
+                const size_t p = ((size_t)n * a.h + y) * a.w_ + x;
+                float v = acc[mt][nt][i];
+                if (a.bias) v += biasv[nt];
+                if (EXTRA && a.resid) v += rv[i][nt];
+                if (a.relu) v = fmaxf(v, 0.0f);
+                if (EXTRA && !keep[i][nt]) v = 0.0f;
+                if (a.dstF) a.dstF[p * a.dstC + co] = v;
+                else a.dstB[p * a.dstC + co] = (bf16)v;
+            }
+        }
+    }
+}
+
 // Single-slice layers (32 input channels): persistent blocks keep the weights
 // of all 9 taps in LDS for their whole run and walk tiles blockIdx.x,
 // blockIdx.x + gridDim.x, ...; the next tile's input (with halo) is loaded
 // into registers while the current one is computed and stored.
-template <int NT, int RW>
+template <int NT, int RW, bool EXTRA>
 __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX, int tilesY, int numTiles) {
     constexpr int TH = 4 * RW, MT = 2 * RW;
     constexpr int TR = TH + 2, TC = kTileW + 2;
@@ -442,30 +491,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
         }
         const int tx = tile % tilesX, rest = tile / tilesX;
         const int ty = rest % tilesY, n = rest / tilesY;
-        const int x0 = tx * kTileW, y0 = ty * TH;
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            const int y = y0 + wave * RW + (mt >> 1);
-            if (y >= a.h) continue;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int x = x0 + (mt & 1) * 16 + 4 * g + i;
-                if (x >= a.w_) continue;
-                const size_t p = ((size_t)n * a.h + y) * a.w_ + x;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int co = coBase + nt * 16 + r16;
-                    if (co >= a.cout) continue;
-                    float v = acc[mt][nt][i];
-                    if (a.bias) v += biasv[nt];
-                    if (a.resid) v += f(a.resid[p * a.residC + co]);
-                    if (a.relu) v = fmaxf(v, 0.0f);
-                    if (a.mask && !(f(a.mask[p * a.maskC + co]) > 0.0f)) v = 0.0f;
-                    if (a.dstF) a.dstF[p * a.dstC + co] = v;
-                    else a.dstB[p * a.dstC + co] = (bf16)v;
-                }
-            }
-        }
+        tileEpilogue<MT, NT, RW, EXTRA>(a, acc, biasv, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
     }
 }
 
@@ -473,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
 // (tile, 32-channel slice) items in order; while one slice's 9 taps run from
 // LDS, the next item's input slice (with halo) and weight slice are loaded
 // into registers.  The output of a tile is written after its last slice.
-template <int NT, int RW>
+template <int NT, int RW, bool EXTRA>
 __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX, int tilesY, int numTiles) {
     constexpr int TH = 4 * RW, MT = 2 * RW;
     constexpr int TR = TH + 2, TC = kTileW + 2;
@@ -567,33 +593,32 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
         if (last) {
             const int tx = tile % tilesX, rest = tile / tilesX;
             const int ty = rest % tilesY, n = rest / tilesY;
-            const int x0 = tx * kTileW, y0 = ty * TH;
+            tileEpilogue<MT, NT, RW, EXTRA>(a, acc, biasv, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const int y = y0 + wave * RW + (mt >> 1);
+            for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int x = x0 + (mt & 1) * 16 + 4 * g + i;
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt) {
-                        const int co = coBase + nt * 16 + r16;
-                        float v = acc[mt][nt][i];
-                        acc[mt][nt][i] = 0.0f;
-                        if (y >= a.h || x >= a.w_ || co >= a.cout) continue;
-                        const size_t p = ((size_t)n * a.h + y) * a.w_ + x;
-                        if (a.bias) v += biasv[nt];
-                        if (a.resid) v += f(a.resid[p * a.residC + co]);
-                        if (a.relu) v = fmaxf(v, 0.0f);
-                        if (a.mask && !(f(a.mask[p * a.maskC + co]) > 0.0f)) v = 0.0f;
-                        if (a.dstF) a.dstF[p * a.dstC + co] = v;
-                        else a.dstB[p * a.dstC + co] = (bf16)v;
-                    }
-                }
-            }
+                for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         }
         tile = nTile;
         slice = nSlice;
     }
+}
+
+// Blocks of `kernel` (256 threads) resident on the whole device at once, from
+// the occupancy calculator; cached per kernel.
+static int residentBlocks(const void* kernel) {
+    static std::vector<std::pair<const void*, int>> cache;
+    for (const auto& kv : cache)
+        if (kv.first == kernel) return kv.second;
+    int perCU = 0, cus = 0, dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, kernel, kBlock, 0) != hipSuccess || perCU < 1 || cus < 1) {
+        perCU = 2;
+        cus = 256;
+    }
+    cache.emplace_back(kernel, perCU * cus);
+    return perCU * cus;
 }
 
 static int g_convVariant = 2;   // 0: direct loads, 1: LDS-staged weights, 2: LDS-staged tiles
@@ -608,31 +633,44 @@ void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
         const int nt4 = (tiles % 4 == 0) ? 4 : (tiles % 2 == 0) ? 2 : 1;
         const int tilesX = (a.w_ + kTileW - 1) / kTileW, tilesY = (a.h + 7) / 8;
         const long long numTiles = (long long)tilesX * tilesY * a.n;
-        // persistent blocks: 512 in all (two per CU), split over the output-channel groups
-        auto persistent = [&](int groups) {
-            return dim3((unsigned)std::max<long long>(1, std::min<long long>(numTiles, 512 / groups)), (unsigned)groups);
+        // persistent blocks: as many as are resident at once, split over the
+        // output-channel groups
+        auto persistent = [&](const void* kernel, int groups) {
+            const long long per = std::max(1, residentBlocks(kernel) / groups);
+            return dim3((unsigned)std::max<long long>(1, std::min<long long>(numTiles, per)), (unsigned)groups);
         };
         if (a.c0 + a.c1 == 32) {
-            const dim3 grid = persistent(tiles / nt4);
+            const bool extra = a.resid || a.mask;
+#define BTE_TILE1(NT_, EX_)                                                                                   \
+    hipLaunchKernelGGL((k_conv3x3_tile1<NT_, 2, EX_>),                                                         \
+                       persistent(reinterpret_cast<const void*>(&k_conv3x3_tile1<NT_, 2, EX_>), tiles / NT_), \
+                       dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles)
             switch (nt4) {
-                case 4: hipLaunchKernelGGL((k_conv3x3_tile1<4, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles); break;
-                case 2: hipLaunchKernelGGL((k_conv3x3_tile1<2, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles); break;
-                default: hipLaunchKernelGGL((k_conv3x3_tile1<1, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles); break;
+                case 4: if (extra) BTE_TILE1(4, true); else BTE_TILE1(4, false); break;
+                case 2: if (extra) BTE_TILE1(2, true); else BTE_TILE1(2, false); break;
+                default: if (extra) BTE_TILE1(1, true); else BTE_TILE1(1, false); break;
             }
+#undef BTE_TILE1
             return;
         }
         if ((a.c0 + a.c1) >= 192 && nt4 == 4) {
             // many slices and >= 64 output channels: one pass over the input
-            // per 64 channels beats the prefetching kernel's 32-channel groups
+            // per 64 channels (measured faster than the prefetching kernel's
+            // 32-channel groups and than 4-row tiles with 64 channels)
             const dim3 grid((unsigned)numTiles, (unsigned)(tiles / 4));
             hipLaunchKernelGGL((k_conv3x3_tile<4, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY);
             return;
         }
         // several slices: at most 32 output channels per block (registers for the prefetch)
         const int nt2 = (tiles % 2 == 0) ? 2 : 1;
-        const dim3 grid = persistent(tiles / nt2);
-        if (nt2 == 2) hipLaunchKernelGGL((k_conv3x3_tileN<2, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles);
-        else hipLaunchKernelGGL((k_conv3x3_tileN<1, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles);
+        const bool extra = a.resid || a.mask;
+#define BTE_TILEN(NT_, EX_)                                                                                   \
+    hipLaunchKernelGGL((k_conv3x3_tileN<NT_, 2, EX_>),                                                         \
+                       persistent(reinterpret_cast<const void*>(&k_conv3x3_tileN<NT_, 2, EX_>), tiles / NT_), \
+                       dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles)
+        if (nt2 == 2) { if (extra) BTE_TILEN(2, true); else BTE_TILEN(2, false); }
+        else { if (extra) BTE_TILEN(1, true); else BTE_TILEN(1, false); }
+#undef BTE_TILEN
         return;
     }
     const int nt = (tiles % 8 == 0) ? 8 : (tiles % 4 == 0) ? 4 : (tiles % 2 == 0) ? 2 : 1;
