@@ -219,6 +219,16 @@ struct Doc {
     }
 };
 
+// a non-negative whole byte offset / length / count that fits comfortably in size_t
+bool toSize(double v, size_t& out) {
+    if (!(v >= 0.0 && v <= 4503599627370496.0)) return false;   // also rejects NaN
+    out = (size_t)v;
+    return true;
+}
+
+// a JSON number used as an index: -1 (no such item) when outside int
+int toIndex(double v) { return (v >= 0.0 && v <= 2147483647.0) ? (int)v : -1; }
+
 int componentCount(const std::string& type) {
     if (type == "SCALAR") return 1;
     if (type == "VEC2") return 2;
@@ -300,10 +310,13 @@ bool accessor(const Doc& d, int index, View& v) {
     const size_t elem = (size_t)v.comps * (size_t)cs;
     const int byteStride = bv->integer("byteStride", 0);
     v.stride = byteStride > 0 ? (size_t)byteStride : elem;
-    const double cnt = acc->number("count", 0.0);
-    if (cnt < 0.0) return false;
-    v.count = (size_t)cnt;
-    const size_t offset = (size_t)bv->number("byteOffset", 0.0) + (size_t)acc->number("byteOffset", 0.0);
+    size_t viewOff = 0, accOff = 0;
+    if (!toSize(acc->number("count", 0.0), v.count) || !toSize(bv->number("byteOffset", 0.0), viewOff) ||
+        !toSize(acc->number("byteOffset", 0.0), accOff))
+        return false;
+    // every element occupies at least one byte of the buffer: bounds the products below
+    if (v.count > buf.size() || viewOff > buf.size() || accOff > buf.size()) return false;
+    const size_t offset = viewOff + accOff;
     if (offset + (v.count ? (v.count - 1) * v.stride + elem : 0) > buf.size()) return false;
     v.p = buf.data() + offset;
     const JValue* nz = acc->get("normalized");
@@ -426,7 +439,7 @@ bool gatherStats(const Doc& d, int ni, const M4& parent, std::vector<PrimStat>& 
             if (skipPrim(d, prim)) continue;
             const JValue* pos = attribute(prim, "POSITION");
             if (!pos) continue;
-            const JValue* acc = d.item("accessors", (int)pos->num);
+            const JValue* acc = d.item("accessors", toIndex(pos->num));
             if (!acc) continue;
             const std::vector<double> lo = acc->numbers("min"), hi = acc->numbers("max");
             if (lo.size() < 3 || hi.size() < 3) continue;
@@ -450,7 +463,7 @@ bool gatherStats(const Doc& d, int ni, const M4& parent, std::vector<PrimStat>& 
     const JValue* ch = node->get("children");
     if (ch && ch->isArray())
         for (const JValue& c : ch->arr)
-            if (c.isNumber() && !gatherStats(d, (int)c.num, world, stats, depth + 1)) return false;
+            if (c.isNumber() && !gatherStats(d, toIndex(c.num), world, stats, depth + 1)) return false;
     return true;
 }
 
@@ -486,12 +499,12 @@ std::set<uint64_t> outliers(const std::vector<PrimStat>& st) {
 void emitPrimitive(const Doc& d, const JValue& prim, const M4& world, int slot, GltfModel& out) {
     const JValue* pa = attribute(prim, "POSITION");
     if (!pa) return;
-    const std::vector<BteFloat3> pos = readVec3(d, (int)pa->num);
+    const std::vector<BteFloat3> pos = readVec3(d, toIndex(pa->num));
     if (pos.empty()) return;
     const JValue* na = attribute(prim, "NORMAL");
-    const std::vector<BteFloat3> nrm = na ? readVec3(d, (int)na->num) : std::vector<BteFloat3>();
+    const std::vector<BteFloat3> nrm = na ? readVec3(d, toIndex(na->num)) : std::vector<BteFloat3>();
     const JValue* ua = attribute(prim, "TEXCOORD_0");
-    const std::vector<BteFloat2> uvs = ua ? readVec2(d, (int)ua->num) : std::vector<BteFloat2>();
+    const std::vector<BteFloat2> uvs = ua ? readVec2(d, toIndex(ua->num)) : std::vector<BteFloat2>();
     const int mode = prim.integer("mode", 4) < 0 ? 4 : prim.integer("mode", 4);
     if (mode != 4 && mode != 5 && mode != 6) {
         std::fprintf(stderr, "[gltf] skipping a non-triangle primitive (mode %d)\n", mode);
@@ -558,7 +571,7 @@ bool walk(const Doc& d, int ni, const M4& parent, const std::vector<int>& slotOf
     const JValue* ch = node->get("children");
     if (ch && ch->isArray())
         for (const JValue& c : ch->arr)
-            if (c.isNumber() && !walk(d, (int)c.num, world, slotOf, defaultSlot, skip, out, depth + 1)) return false;
+            if (c.isNumber() && !walk(d, toIndex(c.num), world, slotOf, defaultSlot, skip, out, depth + 1)) return false;
     return true;
 }
 
@@ -748,8 +761,10 @@ bool load_gltf(const std::string& path, const ModelPlacement& place, GltfModel& 
             }
         } else if (const JValue* bv = d.item("bufferViews", im.integer("bufferView", -1))) {
             const int b = bv->integer("buffer", -1);
-            const size_t off = (size_t)bv->number("byteOffset", 0.0), len = (size_t)bv->number("byteLength", 0.0);
-            if (b < 0 || (size_t)b >= d.buffers.size() || off + len > d.buffers[(size_t)b].size()) {
+            size_t off = 0, len = 0;
+            if (b < 0 || (size_t)b >= d.buffers.size() || !toSize(bv->number("byteOffset", 0.0), off) ||
+                !toSize(bv->number("byteLength", 0.0), len) || off > d.buffers[(size_t)b].size() ||
+                len > d.buffers[(size_t)b].size() - off) {
                 err = "image " + std::to_string(i) + ": buffer view out of range";
                 return false;
             }
@@ -783,7 +798,7 @@ bool load_gltf(const std::string& path, const ModelPlacement& place, GltfModel& 
     std::vector<int> roots;
     const int sceneIndex = d.j.integer("scene", -1) >= 0 ? d.j.integer("scene", -1) : 0;
     if (const JValue* sc = d.item("scenes", sceneIndex)) {
-        for (double n : sc->numbers("nodes")) roots.push_back((int)n);
+        for (double n : sc->numbers("nodes")) roots.push_back(toIndex(n));
     } else {
         for (size_t i = 0; i < d.count("nodes"); ++i) roots.push_back((int)i);
     }

@@ -23,7 +23,9 @@ double JValue::number(const char* key, double def) const {
 
 int JValue::integer(const char* key, int def) const {
     const JValue* v = get(key);
-    return (v && v->type == Number) ? (int)v->num : def;
+    // non-numbers and values outside int (or not finite) read as the default
+    if (!v || v->type != Number || !(v->num >= -2147483648.0 && v->num <= 2147483647.0)) return def;
+    return (int)v->num;
 }
 
 std::string JValue::string(const char* key) const {

@@ -287,3 +287,15 @@ def test_gltf_errors(tmp_path):
     g.node(mesh=g.mesh([{"attributes": {"POSITION": acc}}]))
     with pytest.raises(ValueError, match="no triangles"):
         load("short.glb", g.glb())
+    # absurd numbers (counts and offsets past any buffer, negative, beyond int) read as absent
+    for key, val in (("count", 1e18), ("count", -3), ("byteOffset", -8), ("byteOffset", 1e300)):
+        g = Gltf()
+        acc = g.accessor(np.zeros((3, 3), F32))
+        g.doc["accessors"][acc][key] = val
+        g.node(mesh=g.mesh([{"attributes": {"POSITION": acc}}]))
+        with pytest.raises(ValueError, match="no triangles"):
+            load("absurd.glb", g.glb())
+    g = Gltf()
+    g.node(mesh=1e20, children=[-1, 2.5e12])
+    with pytest.raises(ValueError, match="no triangles"):
+        load("index.glb", g.glb())
