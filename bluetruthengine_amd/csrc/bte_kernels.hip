@@ -1055,127 +1055,232 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
     }
 }
 
-// Closest-hit extension on the BVH4 with ray replacement: every lane of a
-// persistent wave traces its own ray one node (or leaf) per iteration, and
-// lanes whose ray has finished take the next queue slots of the wave's pool
-// (its work chunks) once kRefill or more of them are idle, so the wave keeps
-// its SIMD lanes busy instead of waiting for its slowest ray row by row.
-// The per-node and per-leaf work is traverse4's; the result does not depend
-// on the order in which rays or nodes are visited.
+// ---------------------------------------------------------------------------
+// BVH4 passes with ray replacement.  Every lane of a persistent wave traces its
+// own ray one node (or leaf) per iteration; lanes whose ray has finished take
+// the next slots of the wave's pool (its work chunks) once kRefill or more of
+// them are idle, so the wave keeps its SIMD lanes busy instead of waiting for
+// its slowest ray row by row.  In a batch the pool holds one scene at a time:
+// a chunk of another scene is held until every lane has finished the current
+// one, so the scene stays wave-uniform.  Results do not depend on the order in
+// which rays or nodes are visited.
+// ---------------------------------------------------------------------------
 #ifndef BTE_REFILL
 #define BTE_REFILL 16   // measured against 8 and 32
 #endif
 constexpr int kRefill = BTE_REFILL;
+
+// One closest-hit step: an inner node (hit children ordered by entry distance,
+// the nearest next, the others pushed farthest first) or a leaf, then a pop.
+// Returns true when the stack is empty: the query is done.
 template <bool COUNT, int L>
-__global__ __launch_bounds__(kBlock) void k_extend_pool(DevScene sc, WaveBuffers wb, int bounce) {
+BTE_DEV bool closestStep(const DevScene& S, const WtRay& r, int& node, int& sp, const StackRef<L>& stk, float& best,
+                         int& bestTri, float& tO, float& uO, float& vO, uint32_t& nodes, uint32_t& tris) {
+    constexpr float kNone = __builtin_huge_valf();
+    if (node >= 0) {
+        if (COUNT) ++nodes;
+        const float4* np = S.nodes4 + 8 * (size_t)node;
+        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+        float f0, f1, f2, f3;
+        const float e0 = boxEnter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, RT_EPS, best, f0);
+        const float e1 = boxEnter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, RT_EPS, best, f1);
+        const float e2 = boxEnter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, RT_EPS, best, f2);
+        const float e3 = boxEnter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, RT_EPS, best, f3);
+        float k0 = e0 <= f0 ? e0 : kNone, k1 = e1 <= f1 ? e1 : kNone;
+        float k2 = e2 <= f2 ? e2 : kNone, k3 = e3 <= f3 ? e3 : kNone;
+        int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+        sortPair(k0, c0, k1, c1);
+        sortPair(k2, c2, k3, c3);
+        sortPair(k0, c0, k2, c2);
+        sortPair(k1, c1, k3, c3);
+        sortPair(k1, c1, k2, c2);
+        if (k0 != kNone) {
+            if (k3 != kNone) push(stk, sp, c3);
+            if (k2 != kNone) push(stk, sp, c2);
+            if (k1 != kNone) push(stk, sp, c1);
+            node = c0;
+            return false;
+        }
+    } else {
+        const int code = ~node;
+        const int start = code >> kLeafCountBits;
+        const int cnt = (code & (kMaxLeafTris - 1)) + 1;
+        for (int i = 0; i < cnt; ++i) {
+            const float4* tp = S.leafTris + 3 * (size_t)(start + i);
+            const float4 a = tp[0], b = tp[1], c = tp[2];
+            if (COUNT) ++tris;
+            float t, u, v;
+            if (hitTriangle(r, a, b, c, RT_EPS, best, t, u, v)) {
+                const int id = __float_as_int(a.w);
+                if (t < best || bestTri < 0 || id < bestTri) {
+                    best = t; bestTri = id; tO = t; uO = u; vO = v;
+                }
+            }
+        }
+    }
+    if (sp == 0) return true;
+    node = pop(stk, sp);
+    return false;
+}
+
+// One any-hit step (children in any order, a hit child first); returns 1 when
+// a triangle blocks the ray, 2 when the stack is empty (unblocked), else 0.
+template <bool COUNT, int L>
+BTE_DEV int anyStep(const DevScene& S, const WtRay& r, float tmax, int& node, int& sp, const StackRef<L>& stk,
+                    uint32_t& nodes, uint32_t& tris) {
+    constexpr float kNone = __builtin_huge_valf();
+    if (node >= 0) {
+        if (COUNT) ++nodes;
+        const float4* np = S.nodes4 + 8 * (size_t)node;
+        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+        float f0, f1, f2, f3;
+        const float e0 = boxEnter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, RT_EPS, tmax, f0);
+        const float e1 = boxEnter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, RT_EPS, tmax, f1);
+        const float e2 = boxEnter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, RT_EPS, tmax, f2);
+        const float e3 = boxEnter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, RT_EPS, tmax, f3);
+        float k0 = e0 <= f0 ? e0 : kNone, k1 = e1 <= f1 ? e1 : kNone;
+        float k2 = e2 <= f2 ? e2 : kNone, k3 = e3 <= f3 ? e3 : kNone;
+        int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+        if (k0 == kNone) {
+            if (k1 != kNone) { k0 = k1; c0 = c1; k1 = kNone; }
+            else if (k2 != kNone) { k0 = k2; c0 = c2; k2 = kNone; }
+            else if (k3 != kNone) { k0 = k3; c0 = c3; k3 = kNone; }
+        }
+        if (k0 != kNone) {
+            if (k3 != kNone) push(stk, sp, c3);
+            if (k2 != kNone) push(stk, sp, c2);
+            if (k1 != kNone) push(stk, sp, c1);
+            node = c0;
+            return 0;
+        }
+    } else {
+        const int code = ~node;
+        const int start = code >> kLeafCountBits;
+        const int cnt = (code & (kMaxLeafTris - 1)) + 1;
+        for (int i = 0; i < cnt; ++i) {
+            const float4* tp = S.leafTris + 3 * (size_t)(start + i);
+            const float4 a = tp[0], b = tp[1], c = tp[2];
+            if (COUNT) ++tris;
+            float t, u, v;
+            if (hitTriangle(r, a, b, c, RT_EPS, tmax, t, u, v)) return 1;
+        }
+    }
+    if (sp == 0) return 2;
+    node = pop(stk, sp);
+    return 0;
+}
+
+// The wave's pool of queue slots: [beg, end) of the current scene, plus one
+// grabbed chunk [nBeg, nEnd) of scene nS not yet started.  All fields are
+// wave-uniform.
+struct RayPool {
+    int beg, end, nBeg, nEnd, nS, curS, chunk;
+    bool held, exhausted;
+};
+
+BTE_DEV RayPool makePool(int grab) {
+    RayPool p;
+    p.beg = p.end = p.nBeg = p.nEnd = p.nS = 0;
+    p.curS = -1;
+    p.chunk = first_chunk(grab);
+    p.held = p.exhausted = false;
+    return p;
+}
+
+// Hands the idle lanes the pool's next slots: start(slot) prepares a lane's
+// query and returns whether it needs traversal (false: finished at once).
+// Returns the idle mask afterwards; all idle = nothing left to start.
+template <bool BATCH, class Start>
+BTE_DEV unsigned long long refill(RayPool& p, DevScene& S, const DevScene* __restrict__ scenes, const WaveBuffers& wb,
+                                  int bounce, int q, int* wk, int grab, int count, bool& active, Start start) {
+    unsigned long long idle = __ballot(!active);
+    while (idle != 0ull) {
+        if (p.beg >= p.end) {
+            if (!p.held) {
+                if (p.exhausted) break;
+                const int c = p.chunk >= 0 ? p.chunk : next_chunk(wk, grab, count);
+                p.chunk = -1;
+                if (c >= count) {
+                    p.exhausted = true;
+                    break;
+                }
+                if constexpr (BATCH) {
+                    const SegChunk k = segChunk(wb, bounce, q, c);
+                    if (k.left <= 0) continue;
+                    p.nBeg = k.first;
+                    p.nEnd = k.first + min(grab, k.left);
+                    p.nS = k.s;
+                } else {
+                    p.nBeg = c;
+                    p.nEnd = min(c + grab, count);
+                }
+                p.held = true;
+            }
+            if constexpr (BATCH) {
+                if (p.nS != p.curS) {
+                    if (idle != ~0ull) break;    // lanes still trace the current scene
+                    p.curS = p.nS;
+                    S = fetchScene(scenes, p.curS);
+                }
+            }
+            p.beg = p.nBeg;
+            p.end = p.nEnd;
+            p.held = false;
+        }
+        const unsigned lo = (unsigned)(idle & 0xffffffffull), hi = (unsigned)(idle >> 32);
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+        const int n = __popcll(idle);
+        if (!active && rank < p.end - p.beg) active = start(p.beg + rank);
+        p.beg = min(p.beg + n, p.end);
+        idle = __ballot(!active);
+    }
+    return idle;
+}
+
+template <bool COUNT, int L, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_extend_pool(DevScene sc, WaveBuffers wb, int bounce,
+                                                         const DevScene* __restrict__ scenes) {
     __shared__ int lds[L * kBlock];
     const int cur = bounce & 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
-    const int count = ctr[kCtrRays];
+    const int count = BATCH ? segSlots(wb, bounce, kSegRays) : ctr[kCtrRays];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     uint32_t nodes = 0, tris = 0;
     const int grab = 64 * wb.grabRows;
     int* wk = work_group(wb, bounce, kWorkExtend);
-    // the wave's pool of queue slots not yet started: [pBeg, pEnd)
-    int pBeg = first_chunk(grab), pEnd = min(pBeg + grab, count);
-    bool exhausted = pBeg >= count;
-    if (exhausted) pBeg = pEnd = count;
-    constexpr float kNone = __builtin_huge_valf();
+    RayPool pool = makePool(grab);
+    DevScene S = sc;
     bool active = false;
     int qi = 0, node = 0, sp = 0, bestTri = -1;
     float best = 0.0f, tO = 0.0f, uO = 0.0f, vO = 0.0f;
     WtRay r;
+    auto start = [&](int slot) {
+        qi = slot;
+        const float4 o = wb.rayO[cur][qi], d = wb.rayD[cur][qi];
+        r = makeWtRay(xyz(o), xyz(d));
+        best = 1e30f;
+        bestTri = -1;
+        tO = uO = vO = 0.0f;
+        node = 0;
+        sp = 0;
+        if (r.valid && S.triCount > 0) return true;
+        wb.hit[qi] = make_float4(__int_as_float(-1), 0.0f, 0.0f, 0.0f);   // nothing to traverse: a miss
+        wb.cat[qi] = shadeCategory(S, -1);
+        return false;
+    };
     for (;;) {
-        unsigned long long idle = __ballot(!active);
-        const int nIdle = __popcll(idle);
+        const int nIdle = __popcll(__ballot(!active));
         if (nIdle >= kRefill || nIdle == 64) {
-            while (idle != 0ull && !(pBeg >= pEnd && exhausted)) {
-                if (pBeg >= pEnd) {
-                    const int c = next_chunk(wk, grab, count);
-                    if (c >= count) {
-                        exhausted = true;
-                        break;
-                    }
-                    pBeg = c;
-                    pEnd = min(c + grab, count);
-                }
-                const unsigned lo = (unsigned)(idle & 0xffffffffull), hi = (unsigned)(idle >> 32);
-                const int rank = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-                const int n = __popcll(idle);
-                if (!active && rank < pEnd - pBeg) {
-                    qi = pBeg + rank;
-                    const float4 o = wb.rayO[cur][qi], d = wb.rayD[cur][qi];
-                    r = makeWtRay(xyz(o), xyz(d));
-                    best = 1e30f;
-                    bestTri = -1;
-                    tO = uO = vO = 0.0f;
-                    node = 0;
-                    sp = 0;
-                    if (r.valid && sc.triCount > 0) {
-                        active = true;
-                    } else {   // nothing to traverse: a miss
-                        wb.hit[qi] = make_float4(__int_as_float(-1), 0.0f, 0.0f, 0.0f);
-                        wb.cat[qi] = shadeCategory(sc, -1);
-                    }
-                }
-                pBeg = min(pBeg + n, pEnd);
-                idle = __ballot(!active);
-            }
-            if (idle == ~0ull) break;          // no ray in flight and none left to start
+            if (refill<BATCH>(pool, S, scenes, wb, bounce, kSegRays, wk, grab, count, active, start) == ~0ull) break;
         }
         if (!active) continue;
-        bool descended = false;
-        if (node >= 0) {
-            if (COUNT) ++nodes;
-            const float4* np = sc.nodes4 + 8 * (size_t)node;
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const int4 ch = *reinterpret_cast<const int4*>(np + 6);
-            float f0, f1, f2, f3;
-            const float e0 = boxEnter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, RT_EPS, best, f0);
-            const float e1 = boxEnter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, RT_EPS, best, f1);
-            const float e2 = boxEnter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, RT_EPS, best, f2);
-            const float e3 = boxEnter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, RT_EPS, best, f3);
-            float k0 = e0 <= f0 ? e0 : kNone, k1 = e1 <= f1 ? e1 : kNone;
-            float k2 = e2 <= f2 ? e2 : kNone, k3 = e3 <= f3 ? e3 : kNone;
-            int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
-            sortPair(k0, c0, k1, c1);
-            sortPair(k2, c2, k3, c3);
-            sortPair(k0, c0, k2, c2);
-            sortPair(k1, c1, k3, c3);
-            sortPair(k1, c1, k2, c2);
-            if (k0 != kNone) {
-                if (k3 != kNone) push(stk, sp, c3);
-                if (k2 != kNone) push(stk, sp, c2);
-                if (k1 != kNone) push(stk, sp, c1);
-                node = c0;
-                descended = true;
-            }
-        } else {
-            const int code = ~node;
-            const int start = code >> kLeafCountBits;
-            const int cnt = (code & (kMaxLeafTris - 1)) + 1;
-            for (int i = 0; i < cnt; ++i) {
-                const float4* tp = sc.leafTris + 3 * (size_t)(start + i);
-                const float4 a = tp[0], b = tp[1], c = tp[2];
-                if (COUNT) ++tris;
-                float t, u, v;
-                if (hitTriangle(r, a, b, c, RT_EPS, best, t, u, v)) {
-                    const int id = __float_as_int(a.w);
-                    if (t < best || bestTri < 0 || id < bestTri) {
-                        best = t; bestTri = id; tO = t; uO = u; vO = v;
-                    }
-                }
-            }
-        }
-        if (!descended) {
-            if (sp == 0) {
-                wb.hit[qi] = make_float4(__int_as_float(bestTri), tO, uO, vO);
-                wb.cat[qi] = shadeCategory(sc, bestTri);
-                active = false;
-            } else {
-                node = pop(stk, sp);
-            }
+        if (closestStep<COUNT, L>(S, r, node, sp, stk, best, bestTri, tO, uO, vO, nodes, tris)) {
+            wb.hit[qi] = make_float4(__int_as_float(bestTri), tO, uO, vO);
+            wb.cat[qi] = shadeCategory(S, bestTri);
+            active = false;
         }
     }
     if (COUNT) {
@@ -1430,27 +1535,24 @@ BTE_DEV void connectOne(const DevScene& sc, const WaveBuffers& wb, int nxt, size
     }
 }
 
-// Shadow-ray (any-hit) pass of one light class on the BVH4 with the ray
-// replacement of k_extend_pool: lanes whose query has ended, by a hit or by an
-// empty stack, take the next queue slots once kRefill or more are idle.  An
+// Shadow-ray (any-hit) pass of one light class with ray replacement.  An
 // unoccluded ray adds its contribution to its path (each path has at most one
 // shadow ray per light class, so no two lanes write the same path).
-template <bool COUNT, int L>
-__global__ __launch_bounds__(kBlock) void k_connect_pool(DevScene sc, WaveBuffers wb, int bounce, int kind) {
+template <bool COUNT, int L, bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_connect_pool(DevScene sc, WaveBuffers wb, int bounce, int kind,
+                                                          const DevScene* __restrict__ scenes) {
     __shared__ int lds[L * kBlock];
     const int nxt = (bounce & 1) ^ 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
-    const int count = ctr[kCtrShadow + kind];
+    const int count = BATCH ? segSlots(wb, bounce, kSegShadow + kind) : ctr[kCtrShadow + kind];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
     const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
     const size_t off = (size_t)kind * (size_t)wb.cap;
     uint32_t nodes = 0, tris = 0;
     const int grab = 64 * wb.grabRows;
     int* wk = work_group(wb, bounce, kWorkConnect + kind);
-    int pBeg = first_chunk(grab), pEnd = min(pBeg + grab, count);
-    bool exhausted = pBeg >= count;
-    if (exhausted) pBeg = pEnd = count;
-    constexpr float kNone = __builtin_huge_valf();
+    RayPool pool = makePool(grab);
+    DevScene S = sc;
     bool active = false;
     size_t si = 0;
     int node = 0, sp = 0;
@@ -1466,87 +1568,27 @@ __global__ __launch_bounds__(kBlock) void k_connect_pool(DevScene sc, WaveBuffer
         v4.z = v4.z + c.z;
         *rp = v4;
     };
+    auto start = [&](int slot) {
+        si = off + (size_t)slot;
+        const float4 so = wb.shO[si], sd = wb.shD[si];
+        r = makeWtRay(xyz(so), xyz(sd));
+        tmax = so.w;
+        node = 0;
+        sp = 0;
+        if (r.valid && S.triCount > 0) return true;
+        unoccluded(si);       // nothing can block it
+        return false;
+    };
     for (;;) {
-        unsigned long long idle = __ballot(!active);
-        const int nIdle = __popcll(idle);
+        const int nIdle = __popcll(__ballot(!active));
         if (nIdle >= kRefill || nIdle == 64) {
-            while (idle != 0ull && !(pBeg >= pEnd && exhausted)) {
-                if (pBeg >= pEnd) {
-                    const int c = next_chunk(wk, grab, count);
-                    if (c >= count) {
-                        exhausted = true;
-                        break;
-                    }
-                    pBeg = c;
-                    pEnd = min(c + grab, count);
-                }
-                const unsigned lo = (unsigned)(idle & 0xffffffffull), hi = (unsigned)(idle >> 32);
-                const int rank = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-                const int n = __popcll(idle);
-                if (!active && rank < pEnd - pBeg) {
-                    si = off + (size_t)(pBeg + rank);
-                    const float4 so = wb.shO[si], sd = wb.shD[si];
-                    r = makeWtRay(xyz(so), xyz(sd));
-                    tmax = so.w;
-                    node = 0;
-                    sp = 0;
-                    if (r.valid && sc.triCount > 0) active = true;
-                    else unoccluded(si);       // nothing can block it
-                }
-                pBeg = min(pBeg + n, pEnd);
-                idle = __ballot(!active);
-            }
-            if (idle == ~0ull) break;
+            if (refill<BATCH>(pool, S, scenes, wb, bounce, kSegShadow + kind, wk, grab, count, active, start) == ~0ull)
+                break;
         }
         if (!active) continue;
-        bool descended = false, occluded = false;
-        if (node >= 0) {
-            if (COUNT) ++nodes;
-            const float4* np = sc.nodes4 + 8 * (size_t)node;
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const int4 ch = *reinterpret_cast<const int4*>(np + 6);
-            float f0, f1, f2, f3;
-            const float e0 = boxEnter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, RT_EPS, tmax, f0);
-            const float e1 = boxEnter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, RT_EPS, tmax, f1);
-            const float e2 = boxEnter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, RT_EPS, tmax, f2);
-            const float e3 = boxEnter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, RT_EPS, tmax, f3);
-            float k0 = e0 <= f0 ? e0 : kNone, k1 = e1 <= f1 ? e1 : kNone;
-            float k2 = e2 <= f2 ? e2 : kNone, k3 = e3 <= f3 ? e3 : kNone;
-            int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
-            if (k0 == kNone) {   // order is irrelevant: only move a hit child to slot 0
-                if (k1 != kNone) { k0 = k1; c0 = c1; k1 = kNone; }
-                else if (k2 != kNone) { k0 = k2; c0 = c2; k2 = kNone; }
-                else if (k3 != kNone) { k0 = k3; c0 = c3; k3 = kNone; }
-            }
-            if (k0 != kNone) {
-                if (k3 != kNone) push(stk, sp, c3);
-                if (k2 != kNone) push(stk, sp, c2);
-                if (k1 != kNone) push(stk, sp, c1);
-                node = c0;
-                descended = true;
-            }
-        } else {
-            const int code = ~node;
-            const int start = code >> kLeafCountBits;
-            const int cnt = (code & (kMaxLeafTris - 1)) + 1;
-            for (int i = 0; i < cnt && !occluded; ++i) {
-                const float4* tp = sc.leafTris + 3 * (size_t)(start + i);
-                const float4 a = tp[0], b = tp[1], c = tp[2];
-                if (COUNT) ++tris;
-                float t, u, v;
-                occluded = hitTriangle(r, a, b, c, RT_EPS, tmax, t, u, v);
-            }
-        }
-        if (occluded) {
-            active = false;
-        } else if (!descended) {
-            if (sp == 0) {
-                unoccluded(si);
-                active = false;
-            } else {
-                node = pop(stk, sp);
-            }
-        }
+        const int res = anyStep<COUNT, L>(S, r, tmax, node, sp, stk, nodes, tris);
+        if (res == 2) unoccluded(si);
+        if (res != 0) active = false;
     }
     if (COUNT) {
         wave_add_u64(wb.travCtr + 2, nodes);
@@ -1863,9 +1905,9 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     // traversal counters are a single-scene instrument
     const bool count = !B && wb.travCtr != nullptr;
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (W == 4 && !B) {
-        if (count) hipLaunchKernelGGL((k_extend_pool<true, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
-        else hipLaunchKernelGGL((k_extend_pool<false, L>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+    if (W == 4) {
+        if (count) hipLaunchKernelGGL((k_extend_pool<true, L, false>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+        else hipLaunchKernelGGL((k_extend_pool<false, L, B>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     } else if (count) {
         hipLaunchKernelGGL((k_extend<true, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     } else {
@@ -1897,11 +1939,11 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     }
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int k = 0; k < wb.kinds; ++k) {
-        if (W == 4 && !B) {
+        if (W == 4) {
             if (count)
-                hipLaunchKernelGGL((k_connect_pool<true, L>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+                hipLaunchKernelGGL((k_connect_pool<true, L, false>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k, scenes);
             else
-                hipLaunchKernelGGL((k_connect_pool<false, L>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k);
+                hipLaunchKernelGGL((k_connect_pool<false, L, B>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k, scenes);
         } else if (count) {
             hipLaunchKernelGGL((k_connect<true, L, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k,
                                scenes);
@@ -2005,8 +2047,8 @@ static int travOccupancy() {
     int a = 0, b = 0, c = 0, d = 0;
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false, L, 2>, kBlock, 0);
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_connect<false, L, 2>, kBlock, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_extend_pool<false, L>, kBlock, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_connect_pool<false, L>, kBlock, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_extend_pool<false, L, false>, kBlock, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_connect_pool<false, L, false>, kBlock, 0);
     return std::max(1, std::min(std::min(a, b), std::min(c, d)));
 }
 
