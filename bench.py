@@ -470,7 +470,7 @@ def main():
     ap.add_argument("--traversal-stack", type=int, default=0, help="LDS stack entries per lane (8/16/32)")
     ap.add_argument("--work-chunk", type=int, default=0, help="64-slot rows per work grab (1/2/4/8)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident traversal blocks per CU (0 = max)")
-    ap.add_argument("--bvh-width", type=int, default=4, help="2 or 4: BVH traversed by extend/connect")
+    ap.add_argument("--bvh-width", type=int, default=4, help="2, 4 or 8: BVH traversed by extend/connect")
     ap.add_argument("--no-graphs", action="store_true", help="plain launches instead of captured HIP graphs")
     ap.add_argument("--frames-in-flight", type=int, default=0,
                     help="frames traced per wavefront pass (0 = as many as the memory budget allows)")
@@ -567,7 +567,7 @@ def main():
         r.set_instrumentation(0)
         launches = max(1, s1.extendLaunches)
         avg_ms = s1.extendMs / launches
-        node_bytes = NODE_BYTES * (2 if cfg["bvhWidth"] == 4 else 1)
+        node_bytes = NODE_BYTES * (cfg["bvhWidth"] // 2 if cfg["bvhWidth"] in (4, 8) else 1)
         bytes_total = EXTEND_RAY_BYTES * s2.extensionRays + node_bytes * s2.extendNodes + TRI_BYTES * s2.extendTris
         bytes_per_launch = bytes_total / launches
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9

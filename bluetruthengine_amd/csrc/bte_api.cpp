@@ -61,8 +61,8 @@ struct SceneData {
     DBuf dTexels, dTexInfo, dMrTexels, dMrInfo;   // texture maps (rt_material.cuh:26-56)
     int texCount = 0, mrCount = 0;
     DBuf dEmitters, dEmTriIdx, dEmTriCdf, dSceneCdf, dSpots;
-    DBuf dNodes, dNodes4, dLeafTris;
-    int bvh4Depth = 0;
+    DBuf dNodes, dNodes4, dNodes8, dLeafTris;
+    int bvh4Depth = 0, bvh8Depth = 0;
     DBuf dEnv, dEnvMarg, dEnvCond, dEnvRowSums, dEnvTotal, dEnvMargGuide, dEnvCondGuide, dEnvMono;
     bool envGuides = false;
     bool haveEmission = false, haveFlags = false;
@@ -187,6 +187,13 @@ int ensureAccel(SceneData& c) {
         const std::vector<Bvh4HostNode> n4 = collapse_bvh4(b.nodes, c.bvh4Depth);
         CHECK_RC(c.dNodes4.upload(n4.data(), n4.size() * sizeof(Bvh4HostNode)));
     }
+    {
+        // the BVH8 needs up to 7 stack entries per level; a tree too deep for
+        // the traversal stack is not kept (width 8 then runs on the BVH4)
+        const std::vector<Bvh8HostNode> n8 = collapse_bvh8(b.nodes, c.bvh8Depth);
+        if (7 * c.bvh8Depth + 1 <= kStackSize) CHECK_RC(c.dNodes8.upload(n8.data(), n8.size() * sizeof(Bvh8HostNode)));
+        else c.dNodes8.release();
+    }
     std::vector<float> lt = std::move(b.leafTris);
     if (lt.empty()) lt.assign(12, 0.0f);
     CHECK_RC(c.dLeafTris.upload(lt.data(), lt.size() * sizeof(float)));
@@ -201,6 +208,7 @@ DevScene makeScene(const SceneData& c, int width, int height) {
     std::memset(&s, 0, sizeof s);   // padding too: scenes are compared bytewise as graph keys
     s.nodes = c.dNodes.as<float4>();
     s.nodes4 = c.dNodes4.as<float4>();
+    s.nodes8 = c.dNodes8.as<float4>();
     s.leafTris = c.dLeafTris.as<float4>();
     s.tris = c.dTris.as<BteTriangle>();
     s.triBsdf = c.dTriBsdf.as<int>();
@@ -463,8 +471,9 @@ int renderWavefront(int nframes) {
     wb.kindSpot0 = kindSpot0;
     wb.kindEnv = kindEnv;
     const DevScene sc = makeScene(c, c.accW, c.accH);
-    const LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16,
-                                              c.cfg.traversalBlocksPerCU, c.cfg.workChunk, c.cfg.bvhWidth);
+    LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16,
+                                        c.cfg.traversalBlocksPerCU, c.cfg.workChunk, c.cfg.bvhWidth);
+    if (ls.bvhWidth == 8 && !c.dNodes8.p) ls.bvhWidth = 4;   // BVH8 too deep for the stack
     wb.grabRows = ls.grabRows;
     // instrumented runs record events between launches: those stay uncaptured
     const bool useGraph = c.cfg.graphs != 0 && c.instr == 0;
@@ -738,7 +747,9 @@ int renderBatch(int nframes) {
     LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16,
                                         c.cfg.traversalBlocksPerCU, c.cfg.workChunk, c.cfg.bvhWidth);
     for (const BatchScene& b : c.batch)
-        if (!b.data->dNodes4.p) ls.bvhWidth = 2;   // a scene without the compressed BVH4
+        if (ls.bvhWidth == 8 && !b.data->dNodes8.p) ls.bvhWidth = 4;   // a scene without a BVH8
+    for (const BatchScene& b : c.batch)
+        if (ls.bvhWidth == 4 && !b.data->dNodes4.p) ls.bvhWidth = 2;
     const bool useGraph = c.cfg.graphs != 0;
     const size_t ctrInts = (size_t)kCtrStride * (size_t)(maxDepth + 1);
     std::vector<int> ctrHost(ctrInts);
@@ -883,10 +894,11 @@ int autotune(F&& timed, double* msBest) {
             best = cand;
         }
     };
-    {
-        BteConfig cand = best;
-        cand.bvhWidth = best.bvhWidth == 4 ? 2 : 4;
-        tryCand(cand);
+    const BteConfig startWidth = best;
+    for (int w : {2, 4, 8}) {
+        BteConfig cand = startWidth;
+        cand.bvhWidth = w;
+        if (w != startWidth.bvhWidth) tryCand(cand);
     }
     for (int st : {8, 16, 32}) {
         BteConfig cand = best;
@@ -1215,7 +1227,7 @@ int bte_cleanup(void) {
     c.dTexels.release(); c.dTexInfo.release(); c.dMrTexels.release(); c.dMrInfo.release();
     c.texCount = c.mrCount = 0;
     DBuf* bufs[] = {&c.dTris, &c.dMaterials, &c.dTriMat, &c.dTriBsdf, &c.dBsdfs, &c.dEmission, &c.dFlags,
-                    &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes, &c.dNodes4,
+                    &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes, &c.dNodes4, &c.dNodes8,
                     &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dEnvMargGuide,
                     &c.dEnvCondGuide, &c.dEnvMono, &c.dAccum,
                     &c.dFb, &c.dSum64, &c.dHit, &c.dCatQ, &c.dCat, &c.dFinal, &c.dShO, &c.dShD, &c.dShC, &c.dCounters, &c.dHitCap,
@@ -1251,8 +1263,8 @@ int bte_set_config(const BteConfig* config) {
         return fail(BTE_ERR_INVALID, "workChunk must be 0, 1, 2, 4 or 8");
     if (config->traversalBlocksPerCU < 0 || config->traversalBlocksPerCU > 64)
         return fail(BTE_ERR_INVALID, "traversalBlocksPerCU must be in [0, 64]");
-    if (config->bvhWidth != 0 && config->bvhWidth != 2 && config->bvhWidth != 4)
-        return fail(BTE_ERR_INVALID, "bvhWidth must be 0, 2 or 4");
+    if (config->bvhWidth != 0 && config->bvhWidth != 2 && config->bvhWidth != 4 && config->bvhWidth != 8)
+        return fail(BTE_ERR_INVALID, "bvhWidth must be 0, 2, 4 or 8");
     ctx().cfg = *config;
     return BTE_OK;
 }

@@ -232,17 +232,19 @@ Child2 childOf(const BvhHostNode& n, int k) {
 // Collapse by surface-area cost (dynamic programming over the BVH2, after
 // Ylitie, Karras and Laine, "Efficient incoherent ray traversal on GPUs
 // through compressed wide BVHs", HPG 2017): for every BVH2 inner node and a
-// budget of 1..4 child slots, the cheapest way to represent its subtree,
-// either as one BVH4 node (one slot) or spread over the slots through its two
-// children.  A node visit costs kNodeCost per unit of box area, a leaf
+// budget of 1..W child slots, the cheapest way to represent its subtree,
+// either as one W-wide node (one slot) or spread over the slots through its
+// two children.  A node visit costs kNodeCost per unit of box area, a leaf
 // kTriCost per triangle per unit area; BVH2 leaves are kept as they are.
+template <int W, class Node>
 struct SahCollapser {
     static constexpr float kNodeCost = 1.0f;
     static constexpr float kTriCost = 0.3f;
+    static constexpr int kS = W + 1;
     const std::vector<BvhHostNode>& n2;
-    std::vector<float> cost;     // [node * 5 + slots], slots 1..4
-    std::vector<int8_t> split;   // [node * 5 + slots]: slots given to child 0 (0 = one node), slot 0: node's own split
-    std::vector<Bvh4HostNode> out;
+    std::vector<float> cost;     // [node * kS + slots], slots 1..W
+    std::vector<int8_t> split;   // [node * kS + slots]: slots given to child 0 (0 = one node), slot 0: node's own split
+    std::vector<Node> out;
     int maxDepth = 0;
 
     explicit SahCollapser(const std::vector<BvhHostNode>& nodes) : n2(nodes) {}
@@ -252,7 +254,7 @@ struct SahCollapser {
             const int cnt = ((~c.ref) & (kMaxLeafTris - 1)) + 1;
             return std::max(c.area(), 0.0f) * kTriCost * (float)cnt;
         }
-        return cost[(size_t)c.ref * 5 + (size_t)slots];
+        return cost[(size_t)c.ref * kS + (size_t)slots];
     }
     float distribute(int i, int slots, int& bestK) const {
         const Child2 a = childOf(n2[(size_t)i], 0), b = childOf(n2[(size_t)i], 1);
@@ -269,13 +271,13 @@ struct SahCollapser {
     }
     void solve() {
         const size_t n = n2.size();
-        cost.assign(n * 5, 0.0f);
-        split.assign(n * 5, 0);
+        cost.assign(n * kS, 0.0f);
+        split.assign(n * kS, 0);
         // parents precede their children in the BVH2 array
         for (size_t i = n; i-- > 0;) {
             const BvhHostNode& nd = n2[i];
             if (nd.child[0] == nd.child[1]) {   // single-leaf root: its second box is the far point
-                cost[i * 5 + 1] = cost[i * 5 + 2] = cost[i * 5 + 3] = cost[i * 5 + 4] = 0.0f;
+                for (int k = 1; k <= W; ++k) cost[i * kS + (size_t)k] = 0.0f;
                 continue;
             }
             const Child2 a = childOf(nd, 0), b = childOf(nd, 1);
@@ -284,47 +286,47 @@ struct SahCollapser {
                 u.lo[ax] = std::min(a.lo[ax], b.lo[ax]);
                 u.hi[ax] = std::max(a.hi[ax], b.hi[ax]);
             }
-            int k4 = 1;
-            const float asNode = std::max(u.area(), 0.0f) * kNodeCost + distribute((int)i, 4, k4);
-            cost[i * 5 + 1] = asNode;
-            split[i * 5 + 0] = (int8_t)k4;
-            for (int slots = 2; slots <= 4; ++slots) {
+            int kw = 1;
+            const float asNode = std::max(u.area(), 0.0f) * kNodeCost + distribute((int)i, W, kw);
+            cost[i * kS + 1] = asNode;
+            split[i * kS + 0] = (int8_t)kw;
+            for (int slots = 2; slots <= W; ++slots) {
                 int k = 1;
                 const float spread = distribute((int)i, slots, k);
-                cost[i * 5 + (size_t)slots] = std::min(asNode, spread);
-                split[i * 5 + (size_t)slots] = spread < asNode ? (int8_t)k : (int8_t)0;
+                cost[i * kS + (size_t)slots] = std::min(asNode, spread);
+                split[i * kS + (size_t)slots] = spread < asNode ? (int8_t)k : (int8_t)0;
             }
         }
     }
     void collect(const Child2& c, int slots, Child2* acc, int& cnt) const {
-        if (c.ref < 0 || slots == 1 || split[(size_t)c.ref * 5 + (size_t)slots] == 0) {
+        if (c.ref < 0 || slots == 1 || split[(size_t)c.ref * kS + (size_t)slots] == 0) {
             acc[cnt++] = c;
             return;
         }
-        const int k = split[(size_t)c.ref * 5 + (size_t)slots];
+        const int k = split[(size_t)c.ref * kS + (size_t)slots];
         const BvhHostNode& nd = n2[(size_t)c.ref];
         collect(childOf(nd, 0), k, acc, cnt);
         collect(childOf(nd, 1), slots - k, acc, cnt);
     }
-    // BVH4 node for BVH2 inner node `i`; returns its index
+    // W-wide node for BVH2 inner node `i`; returns its index
     int make(int i, int depth) {
         maxDepth = std::max(maxDepth, depth);
-        Child2 ch[4];
+        Child2 ch[W];
         int cnt = 0;
         const BvhHostNode& root = n2[(size_t)i];
         if (root.child[0] == root.child[1]) {
             ch[cnt++] = childOf(root, 0);
         } else {
-            const int k = split[(size_t)i * 5 + 0];
+            const int k = split[(size_t)i * kS + 0];
             collect(childOf(root, 0), k, ch, cnt);
-            collect(childOf(root, 1), 4 - k, ch, cnt);
+            collect(childOf(root, 1), W - k, ch, cnt);
         }
         const int idx = (int)out.size();
-        out.push_back(Bvh4HostNode{});
-        int refs[4] = {0, 0, 0, 0};
+        out.push_back(Node{});
+        int refs[W] = {};
         for (int k = 0; k < cnt; ++k) refs[k] = ch[k].ref >= 0 ? make(ch[k].ref, depth + 1) : ch[k].ref;
-        Bvh4HostNode& nd = out[(size_t)idx];
-        for (int k = 0; k < 4; ++k) {
+        Node& nd = out[(size_t)idx];
+        for (int k = 0; k < W; ++k) {
             const bool used = k < cnt;
             for (int a = 0; a < 3; ++a) {
                 nd.box[2 * a][k] = used ? ch[k].lo[a] : -3.0e38f;
@@ -340,11 +342,20 @@ struct SahCollapser {
 }  // namespace
 
 std::vector<Bvh4HostNode> collapse_bvh4(const std::vector<BvhHostNode>& nodes2, int& depth4) {
-    SahCollapser c(nodes2);
+    SahCollapser<4, Bvh4HostNode> c(nodes2);
     c.solve();
     c.out.reserve(nodes2.size() / 2 + 1);
     c.make(0, 1);
     depth4 = c.maxDepth;
+    return std::move(c.out);
+}
+
+std::vector<Bvh8HostNode> collapse_bvh8(const std::vector<BvhHostNode>& nodes2, int& depth8) {
+    SahCollapser<8, Bvh8HostNode> c(nodes2);
+    c.solve();
+    c.out.reserve(nodes2.size() / 4 + 1);
+    c.make(0, 1);
+    depth8 = c.maxDepth;
     return std::move(c.out);
 }
 

@@ -21,4 +21,8 @@ BvhBuild build_bvh2(const BteTriangle* tris, int n);
 // depth4 receives the BVH4 depth (<= BVH2 depth).
 std::vector<Bvh4HostNode> collapse_bvh4(const std::vector<BvhHostNode>& nodes2, int& depth4);
 
+// The same collapse into 8-wide nodes (up to eight subtrees per node).  A
+// traversal stack needs up to 7 entries per level: (7 * depth8 + 1) entries.
+std::vector<Bvh8HostNode> collapse_bvh8(const std::vector<BvhHostNode>& nodes2, int& depth8);
+
 }  // namespace bte

@@ -12,6 +12,7 @@ namespace bte {
 struct DevScene {
     const float4* nodes;          // BVH2, 4 float4 per node
     const float4* nodes4;         // BVH4, 8 float4 per node (null when not built)
+    const float4* nodes8;         // BVH8, 16 float4 per node (null when not built)
     const float4* leafTris;       // 3 float4 per leaf-ordered triangle (v0.w = triangle id bits)
     const BteTriangle* tris;      // original order, 96 B each
     const int* triBsdf;

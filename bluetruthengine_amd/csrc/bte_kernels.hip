@@ -143,6 +143,7 @@ BTE_DEV DevScene fetchScene(const DevScene* __restrict__ scenes, int s) {
     DevScene sc = scenes[s];
     sc.nodes = globalPtr(sc.nodes);
     sc.nodes4 = globalPtr(sc.nodes4);
+    sc.nodes8 = globalPtr(sc.nodes8);
     sc.leafTris = globalPtr(sc.leafTris);
     sc.tris = globalPtr(sc.tris);
     sc.triBsdf = globalPtr(sc.triBsdf);
@@ -1073,11 +1074,54 @@ constexpr int kRefill = BTE_REFILL;
 // One closest-hit step: an inner node (hit children ordered by entry distance,
 // the nearest next, the others pushed farthest first) or a leaf, then a pop.
 // Returns true when the stack is empty: the query is done.
-template <bool COUNT, int L>
+// Entry distances of the eight children of a BVH8 node (kNone: missed).
+BTE_DEV void enter8(const DevScene& S, const WtRay& r, int node, float best, float (&k)[8], int (&c)[8]) {
+    constexpr float kNone = __builtin_huge_valf();
+    const float4* np = S.nodes8 + 16 * (size_t)node;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const float4 lx = np[h], hx = np[2 + h], ly = np[4 + h], hy = np[6 + h], lz = np[8 + h], hz = np[10 + h];
+        const int4 ch = *reinterpret_cast<const int4*>(np + 12 + h);
+        float f0, f1, f2, f3;
+        const float e0 = boxEnter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, RT_EPS, best, f0);
+        const float e1 = boxEnter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, RT_EPS, best, f1);
+        const float e2 = boxEnter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, RT_EPS, best, f2);
+        const float e3 = boxEnter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, RT_EPS, best, f3);
+        k[4 * h + 0] = e0 <= f0 ? e0 : kNone;
+        k[4 * h + 1] = e1 <= f1 ? e1 : kNone;
+        k[4 * h + 2] = e2 <= f2 ? e2 : kNone;
+        k[4 * h + 3] = e3 <= f3 ? e3 : kNone;
+        c[4 * h + 0] = ch.x; c[4 * h + 1] = ch.y; c[4 * h + 2] = ch.z; c[4 * h + 3] = ch.w;
+    }
+}
+
+template <bool COUNT, int L, int W = 4>
 BTE_DEV bool closestStep(const DevScene& S, const WtRay& r, int& node, int& sp, const StackRef<L>& stk, float& best,
                          int& bestTri, float& tO, float& uO, float& vO, uint32_t& nodes, uint32_t& tris) {
     constexpr float kNone = __builtin_huge_valf();
-    if (node >= 0) {
+    if (W == 8 && node >= 0) {
+        if (COUNT) ++nodes;
+        float k[8];
+        int c[8];
+        enter8(S, r, node, best, k, c);
+        // 19-comparator sorting network (nearest first)
+        sortPair(k[0], c[0], k[2], c[2]); sortPair(k[1], c[1], k[3], c[3]);
+        sortPair(k[4], c[4], k[6], c[6]); sortPair(k[5], c[5], k[7], c[7]);
+        sortPair(k[0], c[0], k[4], c[4]); sortPair(k[1], c[1], k[5], c[5]);
+        sortPair(k[2], c[2], k[6], c[6]); sortPair(k[3], c[3], k[7], c[7]);
+        sortPair(k[0], c[0], k[1], c[1]); sortPair(k[2], c[2], k[3], c[3]);
+        sortPair(k[4], c[4], k[5], c[5]); sortPair(k[6], c[6], k[7], c[7]);
+        sortPair(k[2], c[2], k[4], c[4]); sortPair(k[3], c[3], k[5], c[5]);
+        sortPair(k[1], c[1], k[4], c[4]); sortPair(k[3], c[3], k[6], c[6]);
+        sortPair(k[1], c[1], k[2], c[2]); sortPair(k[3], c[3], k[4], c[4]); sortPair(k[5], c[5], k[6], c[6]);
+        if (k[0] != kNone) {
+#pragma unroll
+            for (int j = 7; j >= 1; --j)
+                if (k[j] != kNone) push(stk, sp, c[j]);
+            node = c[0];
+            return false;
+        }
+    } else if (node >= 0) {
         if (COUNT) ++nodes;
         const float4* np = S.nodes4 + 8 * (size_t)node;
         const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
@@ -1126,11 +1170,28 @@ BTE_DEV bool closestStep(const DevScene& S, const WtRay& r, int& node, int& sp, 
 
 // One any-hit step (children in any order, a hit child first); returns 1 when
 // a triangle blocks the ray, 2 when the stack is empty (unblocked), else 0.
-template <bool COUNT, int L>
+template <bool COUNT, int L, int W = 4>
 BTE_DEV int anyStep(const DevScene& S, const WtRay& r, float tmax, int& node, int& sp, const StackRef<L>& stk,
                     uint32_t& nodes, uint32_t& tris) {
     constexpr float kNone = __builtin_huge_valf();
-    if (node >= 0) {
+    if (W == 8 && node >= 0) {
+        if (COUNT) ++nodes;
+        float k[8];
+        int c[8];
+        enter8(S, r, node, tmax, k, c);
+        // any hit ends the search: descend into the first hit child, push the others
+        int first = -1;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+            if (first < 0 && k[j] != kNone) first = j;
+        if (first >= 0) {
+#pragma unroll
+            for (int j = 7; j >= 0; --j)
+                if (j != first && k[j] != kNone) push(stk, sp, c[j]);
+            node = c[first];
+            return 0;
+        }
+    } else if (node >= 0) {
         if (COUNT) ++nodes;
         const float4* np = S.nodes4 + 8 * (size_t)node;
         const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
@@ -1239,7 +1300,7 @@ BTE_DEV unsigned long long refill(RayPool& p, DevScene& S, const DevScene* __res
     return idle;
 }
 
-template <bool COUNT, int L, bool BATCH>
+template <bool COUNT, int L, bool BATCH, int W = 4>
 __global__ __launch_bounds__(kBlock) void k_extend_pool(DevScene sc, WaveBuffers wb, int bounce,
                                                          const DevScene* __restrict__ scenes) {
     __shared__ int lds[L * kBlock];
@@ -1277,7 +1338,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_pool(DevScene sc, WaveBuffers
             if (refill<BATCH>(pool, S, scenes, wb, bounce, kSegRays, wk, grab, count, active, start) == ~0ull) break;
         }
         if (!active) continue;
-        if (closestStep<COUNT, L>(S, r, node, sp, stk, best, bestTri, tO, uO, vO, nodes, tris)) {
+        if (closestStep<COUNT, L, W>(S, r, node, sp, stk, best, bestTri, tO, uO, vO, nodes, tris)) {
             wb.hit[qi] = make_float4(__int_as_float(bestTri), tO, uO, vO);
             wb.cat[qi] = shadeCategory(S, bestTri);
             active = false;
@@ -1538,7 +1599,7 @@ BTE_DEV void connectOne(const DevScene& sc, const WaveBuffers& wb, int nxt, size
 // Shadow-ray (any-hit) pass of one light class with ray replacement.  An
 // unoccluded ray adds its contribution to its path (each path has at most one
 // shadow ray per light class, so no two lanes write the same path).
-template <bool COUNT, int L, bool BATCH>
+template <bool COUNT, int L, bool BATCH, int W = 4>
 __global__ __launch_bounds__(kBlock) void k_connect_pool(DevScene sc, WaveBuffers wb, int bounce, int kind,
                                                           const DevScene* __restrict__ scenes) {
     __shared__ int lds[L * kBlock];
@@ -1586,7 +1647,7 @@ __global__ __launch_bounds__(kBlock) void k_connect_pool(DevScene sc, WaveBuffer
                 break;
         }
         if (!active) continue;
-        const int res = anyStep<COUNT, L>(S, r, tmax, node, sp, stk, nodes, tris);
+        const int res = anyStep<COUNT, L, W>(S, r, tmax, node, sp, stk, nodes, tris);
         if (res == 2) unoccluded(si);
         if (res != 0) active = false;
     }
@@ -1905,9 +1966,9 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     // traversal counters are a single-scene instrument
     const bool count = !B && wb.travCtr != nullptr;
     if (ev) (void)hipEventRecord(ev[0], s);
-    if (W == 4) {
-        if (count) hipLaunchKernelGGL((k_extend_pool<true, L, false>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
-        else hipLaunchKernelGGL((k_extend_pool<false, L, B>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+    if constexpr (W >= 4) {
+        if (count) hipLaunchKernelGGL((k_extend_pool<true, L, false, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+        else hipLaunchKernelGGL((k_extend_pool<false, L, B, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     } else if (count) {
         hipLaunchKernelGGL((k_extend<true, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     } else {
@@ -1939,11 +2000,11 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     }
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int k = 0; k < wb.kinds; ++k) {
-        if (W == 4) {
+        if constexpr (W >= 4) {
             if (count)
-                hipLaunchKernelGGL((k_connect_pool<true, L, false>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k, scenes);
+                hipLaunchKernelGGL((k_connect_pool<true, L, false, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k, scenes);
             else
-                hipLaunchKernelGGL((k_connect_pool<false, L, B>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k, scenes);
+                hipLaunchKernelGGL((k_connect_pool<false, L, B, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k, scenes);
         } else if (count) {
             hipLaunchKernelGGL((k_connect<true, L, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k,
                                scenes);
@@ -1955,31 +2016,34 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     if (ev) (void)hipEventRecord(ev[3], s);
 }
 
+template <bool B, int L>
+static void launch_bounce_w(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth,
+                            const LaunchShape& ls, hipStream_t s, hipEvent_t* ev, const DevScene* scenes, int width) {
+    if (width == 8) launch_bounce_l<L, 8, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
+    else if (width == 4) launch_bounce_l<L, 4, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
+    else launch_bounce_l<L, 2, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
+}
+
 template <bool B>
 static void launch_bounce_b(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth,
-                            const LaunchShape& ls, hipStream_t s, hipEvent_t* ev, const DevScene* scenes, bool w4) {
+                            const LaunchShape& ls, hipStream_t s, hipEvent_t* ev, const DevScene* scenes, int width) {
     switch (ls.ldsStack) {
-        case 8:
-            if (w4) launch_bounce_l<8, 4, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
-            else launch_bounce_l<8, 2, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
-            break;
-        case 16:
-            if (w4) launch_bounce_l<16, 4, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
-            else launch_bounce_l<16, 2, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
-            break;
-        default:
-            if (w4) launch_bounce_l<32, 4, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
-            else launch_bounce_l<32, 2, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
-            break;
+        case 8: launch_bounce_w<B, 8>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, width); break;
+        case 16: launch_bounce_w<B, 16>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, width); break;
+        default: launch_bounce_w<B, 32>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, width); break;
     }
 }
 
 void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
                    hipStream_t s, hipEvent_t* ev, const DevScene* scenes) {
-    // a batch's scenes all carry both BVH layouts
-    const bool w4 = ls.bvhWidth == 4 && (scenes != nullptr || sc.nodes4 != nullptr);
-    if (scenes) launch_bounce_b<true>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, w4);
-    else launch_bounce_b<false>(sc, wb, bounce, maxDepth, ls, s, ev, nullptr, w4);
+    // the host checks that every scene of a batch carries the chosen layout
+    int width = ls.bvhWidth;
+    if (!scenes) {
+        if (width == 8 && sc.nodes8 == nullptr) width = 4;
+        if (width == 4 && sc.nodes4 == nullptr) width = 2;
+    }
+    if (scenes) launch_bounce_b<true>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, width);
+    else launch_bounce_b<false>(sc, wb, bounce, maxDepth, ls, s, ev, nullptr, width);
 }
 
 void launch_generate_batch(const DevScene* scenes, const WaveBuffers& wb, int maxScenePaths, long long totalSlots,
@@ -2042,19 +2106,23 @@ void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, d
     hipLaunchKernelGGL(k_env_guides, dim3(H + 1), dim3(256), 0, s, marg, cond, W, H, margGuide, condGuide, monotone);
 }
 
-template <int L>
+// Resident blocks per CU of the traversal passes for one stack size and BVH width.
+template <int L, int W>
 static int travOccupancy() {
-    int a = 0, b = 0, c = 0, d = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false, L, 2>, kBlock, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_connect<false, L, 2>, kBlock, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_extend_pool<false, L, false>, kBlock, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&d, k_connect_pool<false, L, false>, kBlock, 0);
-    return std::max(1, std::min(std::min(a, b), std::min(c, d)));
+    int a = 0, b = 0;
+    if constexpr (W == 2) {
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend<false, L, 2>, kBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_connect<false, L, 2>, kBlock, 0);
+    } else {
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_extend_pool<false, L, false, W>, kBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_connect_pool<false, L, false, W>, kBlock, 0);
+    }
+    return std::max(1, std::min(a, b));
 }
 
 LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows, int bvhWidth) {
     static int cachedDev = -1;
-    static int cus = 1, occS = 1, occMT = 1, occL[3] = {1, 1, 1};
+    static int cus = 1, occS = 1, occMT = 1, occL[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};   // [width][stack]
     if (device != cachedDev) {
         hipDeviceProp_t prop;
         (void)hipGetDeviceProperties(&prop, device);
@@ -2066,17 +2134,23 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_megakernel, kBlock, 0);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&t, k_trace_rays, kBlock, 0);
         occMT = std::max(1, std::min(m, t));
-        occL[0] = travOccupancy<8>();
-        occL[1] = travOccupancy<16>();
-        occL[2] = travOccupancy<32>();
+        occL[0][0] = travOccupancy<8, 2>();
+        occL[0][1] = travOccupancy<16, 2>();
+        occL[0][2] = travOccupancy<32, 2>();
+        occL[1][0] = travOccupancy<8, 4>();
+        occL[1][1] = travOccupancy<16, 4>();
+        occL[1][2] = travOccupancy<32, 4>();
+        occL[2][0] = travOccupancy<8, 8>();
+        occL[2][1] = travOccupancy<16, 8>();
+        occL[2][2] = travOccupancy<32, 8>();
         cachedDev = device;
     }
     LaunchShape ls{};
     ls.ldsStack = (ldsStack == 8 || ldsStack == 16) ? ldsStack : 32;
-    int occ = occL[ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
+    ls.bvhWidth = (bvhWidth == 4 || bvhWidth == 8) ? bvhWidth : 2;
+    int occ = occL[ls.bvhWidth == 2 ? 0 : (ls.bvhWidth == 4 ? 1 : 2)][ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
     if (traversalBlocksPerCU > 0) occ = std::min(occ, traversalBlocksPerCU);
     ls.grabRows = (grabRows == 1 || grabRows == 2 || grabRows == 8) ? grabRows : 4;
-    ls.bvhWidth = bvhWidth == 4 ? 4 : 2;
     ls.extendBlocks = cus * occ;
     ls.connectBlocks = cus * occ;
     ls.shadeBlocks = cus * occS;

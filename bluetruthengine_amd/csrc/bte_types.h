@@ -39,6 +39,15 @@ struct Bvh4HostNode {
 };
 static_assert(sizeof(Bvh4HostNode) == 128, "BVH4 node must be 128 B");
 
+// BVH8 node: box[axis lo/hi][child] as in Bvh4HostNode, 8 children, padded to
+// two 128-B lines.
+struct Bvh8HostNode {
+    float box[6][8];
+    int child[8];
+    int pad[8];
+};
+static_assert(sizeof(Bvh8HostNode) == 256, "BVH8 node must be 256 B");
+
 // Shadow-slot kinds, traced in this order so contributions add in the
 // reference's order (area, spotlights, environment).
 constexpr int kMaxShadowKinds = 2 + 14;   // area + env + up to 14 spotlights

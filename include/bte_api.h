@@ -88,7 +88,7 @@ typedef struct {
     int graphs;              /* 1 = replay the bounce loop as captured HIP graphs (default), 0 = plain launches */
     int workChunk;           /* 64-slot queue rows a persistent wave takes per atomic: 1, 2, 4, 8 (0 = 4) */
     int traversalBlocksPerCU;/* resident traversal blocks per CU (0 = as many as occupancy allows) */
-    int bvhWidth;            /* 2: BVH2 or 4: BVH4 (collapsed from the BVH2) for extend / connect (default 4; 0 = 2) */
+    int bvhWidth;            /* 2: BVH2, 4: BVH4 or 8: BVH8 (both collapsed from the BVH2) for extend / connect (default 4; 0 = 2) */
 } BteConfig;
 
 /* Counters of the last bte_render / bte_render_frames call. */
@@ -365,7 +365,7 @@ void bte_scene_free(BteScene* scene);
 
 /* Launch-shape autotuner (no counterpart in the reference, whose launch
  * shape OptiX fixes).  Renders `frames` frames at width x height once per
- * candidate (BVH width 2/4, then traversal LDS stack 8/16/32, then work chunk
+ * candidate (BVH width 2/4/8, then traversal LDS stack 8/16/32, then work chunk
  * 1/2/4/8 rows, then traversal blocks per CU: all, 3/4, 1/2), by coordinate
  * descent from the current configuration, keeps the fastest in the
  * configuration and resets the accumulation.  Images do not depend on these settings.  msBest

@@ -1,9 +1,9 @@
-"""The BVH4 the kernels traverse (the BVH2 collapsed by surface-area cost) is
+"""The BVH4 and BVH8 the kernels traverse (the BVH2 collapsed by surface-area cost) are
 structurally sound: tests/cpp/bvh4_check.cpp, compiled here with g++ against
 the library's own bte_bvh.cpp, checks that every triangle is referenced once,
 that every box contains its subtree, and that the depth stays within the
 bound the traversal stack is sized for.  The GPU parity tests then show that
-traversing it gives the same hits as the BVH2 and the oracle."""
+traversing them gives the same hits as the BVH2 and the oracle."""
 import os
 import subprocess
 
@@ -19,4 +19,4 @@ def test_bvh4_structure(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     print(out.stdout, out.stderr)
     assert out.returncode == 0, out.stdout + out.stderr
-    assert out.stdout.count("BVH4") == 6
+    assert out.stdout.count("BVH4") == 6 and out.stdout.count("BVH8") == 6

@@ -96,6 +96,14 @@ def test_batch_uneven_passes_and_plain_launches():
         _check(_slots(b, len(sds)), sds, 10, f"uneven passes graphs={graphs}")
 
 
+def test_batch_bvh8_equals_bvh4_alone():
+    # batch passes over 8-wide nodes against each scene rendered alone over the default BVH4
+    _say("case: 4-scene batch, BVH8 extend/connect, 2 frames, vs scenes alone on BVH4")
+    sds = _scene_set()
+    b = _batched(sds, 2, bvh_width=8)
+    _check(_slots(b, len(sds)), sds, 2, "batch bvh8")
+
+
 def test_batch_reset_and_clear():
     _say("case: batch reset restarts accumulation; clear empties the batch")
     sds = _scene_set()[:2]
@@ -139,7 +147,7 @@ def test_batch_autotune_keeps_images():
     b = _batched(sds, 1)
     ms, cfg = b.autotune(2)
     _say(f"batch autotune: {ms:.2f} ms with {cfg}")
-    assert ms > 0 and cfg["bvhWidth"] in (2, 4) and cfg["traversalStack"] in (8, 16, 32)
+    assert ms > 0 and cfg["bvhWidth"] in (2, 4, 8) and cfg["traversalStack"] in (8, 16, 32)
     assert b.frame_index() == 0
     b.render(3)
     _check(_slots(b, len(sds)), sds, 3, "after autotune")

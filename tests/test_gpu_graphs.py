@@ -65,7 +65,8 @@ def test_deep_paths_run_in_segments_with_early_exit():
 
 @pytest.mark.parametrize("cfg", [dict(traversal_stack=8), dict(traversal_stack=32), dict(work_chunk=1),
                                  dict(work_chunk=8), dict(traversal_blocks_per_cu=2), dict(bvh_width=2),
-                                 dict(bvh_width=2, traversal_stack=8, work_chunk=1)])
+                                 dict(bvh_width=2, traversal_stack=8, work_chunk=1),
+                                 dict(bvh_width=8), dict(bvh_width=8, traversal_stack=8, work_chunk=2)])
 def test_launch_shape_does_not_change_result(cfg):
     _say(f"case: reduced env-lit 128x72 depth 12, 2 frames, launch shape {cfg} vs default")
     sd = scenes.small_envlit()
@@ -85,7 +86,7 @@ def test_autotune_keeps_result_and_valid_config():
     assert cfg["traversalStack"] in (8, 16, 32)
     assert cfg["workChunk"] in (1, 2, 4, 8)
     assert 0 <= cfg["traversalBlocksPerCU"] <= 64
-    assert cfg["bvhWidth"] in (2, 4)
+    assert cfg["bvhWidth"] in (2, 4, 8)
     assert r.frame_index() == 0
     r.render(2)
     ref = oracle.OracleScene(sd).render(128, 128, 0, 2, sd.max_depth)

@@ -115,6 +115,14 @@ def test_bvh2_traversal_samples_match_oracle():
         assert min(exact) == 1.0
 
 
+def test_bvh8_traversal_samples_match_oracle():
+    # 8-wide nodes collapsed from the same BVH2: closest and any hit through the 8-child sorting network
+    _say("case: BVH8 extend/connect, terrain freq 20 + reduced env-lit, 2 frames each")
+    for sd in (scenes.terrain_c2(width=96, height=54, spp=2, max_depth=10, freq=20), scenes.small_envlit()):
+        _, _, exact = _sample_parity(sd, 2, bvh_width=8)
+        assert min(exact) == 1.0
+
+
 def test_terrain_disney_samples_match_oracle():
     _say("case: terrain freq 20 (8000 tris) + 64 Disney slots + 4 lights, 96x54, 2 frames, depth 10")
     sd = scenes.terrain_c2(width=96, height=54, spp=2, max_depth=10, freq=20)
