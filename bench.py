@@ -580,7 +580,8 @@ def main():
                         traffic = int(line.split(":")[1])
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                    "kernel": f"k_extend (BVH{cfg['bvhWidth']} closest hit)", "launches": launches,
+                    "kernel": (f"k_extend_pool (BVH{cfg['bvhWidth']} closest hit)" if cfg["bvhWidth"] in (4, 8)
+                              else "k_extend (BVH2 closest hit)"), "launches": launches,
                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
         kernel = {"extend_ms": round(s1.extendMs, 2), "shade_ms": round(s1.shadeMs, 2),
                   "connect_ms": round(s1.connectMs, 2), "nodes_per_ray": round(s2.extendNodes / max(1, s2.extensionRays), 2),

@@ -1150,8 +1150,10 @@ BTE_DEV bool closestStep(const DevScene& S, const WtRay& r, int& node, int& sp, 
         const int code = ~node;
         const int start = code >> kLeafCountBits;
         const int cnt = (code & (kMaxLeafTris - 1)) + 1;
-        for (int i = 0; i < cnt; ++i) {
-            const float4* tp = S.leafTris + 3 * (size_t)(start + i);
+        // one triangle per step: an iteration of the wave costs one triangle
+        // test for its lanes at leaves, not the longest leaf's loop
+        {
+            const float4* tp = S.leafTris + 3 * (size_t)start;
             const float4 a = tp[0], b = tp[1], c = tp[2];
             if (COUNT) ++tris;
             float t, u, v;
@@ -1160,6 +1162,10 @@ BTE_DEV bool closestStep(const DevScene& S, const WtRay& r, int& node, int& sp, 
                 if (t < best || bestTri < 0 || id < bestTri) {
                     best = t; bestTri = id; tO = t; uO = u; vO = v;
                 }
+            }
+            if (cnt > 1) {   // the rest of the leaf is the next step
+                node = ~(((start + 1) << kLeafCountBits) | (cnt - 2));
+                return false;
             }
         }
     }
@@ -1220,12 +1226,18 @@ BTE_DEV int anyStep(const DevScene& S, const WtRay& r, float tmax, int& node, in
         const int code = ~node;
         const int start = code >> kLeafCountBits;
         const int cnt = (code & (kMaxLeafTris - 1)) + 1;
-        for (int i = 0; i < cnt; ++i) {
-            const float4* tp = S.leafTris + 3 * (size_t)(start + i);
+        // one triangle per step: an iteration of the wave costs one triangle
+        // test for its lanes at leaves, not the longest leaf's loop
+        {
+            const float4* tp = S.leafTris + 3 * (size_t)start;
             const float4 a = tp[0], b = tp[1], c = tp[2];
             if (COUNT) ++tris;
             float t, u, v;
             if (hitTriangle(r, a, b, c, RT_EPS, tmax, t, u, v)) return 1;
+            if (cnt > 1) {   // the rest of the leaf is the next step
+                node = ~(((start + 1) << kLeafCountBits) | (cnt - 2));
+                return 0;
+            }
         }
     }
     if (sp == 0) return 2;
