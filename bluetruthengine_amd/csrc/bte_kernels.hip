@@ -1057,8 +1057,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
 }
 
 // ---------------------------------------------------------------------------
-// BVH4 passes with ray replacement.  Every lane of a persistent wave traces its
-// own ray one node (or leaf) per iteration; lanes whose ray has finished take
+// BVH4 / BVH8 passes with ray replacement.  Every lane of a persistent wave
+// traces its own ray one node (or leaf triangle) per iteration; lanes whose ray has finished take
 // the next slots of the wave's pool (its work chunks) once kRefill or more of
 // them are idle, so the wave keeps its SIMD lanes busy instead of waiting for
 // its slowest ray row by row.  In a batch the pool holds one scene at a time:
@@ -1067,7 +1067,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
 // which rays or nodes are visited.
 // ---------------------------------------------------------------------------
 #ifndef BTE_REFILL
-#define BTE_REFILL 16   // measured against 8 and 32
+#define BTE_REFILL 24   // measured against 8, 16 and 32
 #endif
 constexpr int kRefill = BTE_REFILL;
 
