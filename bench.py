@@ -343,24 +343,11 @@ def denoise_windows(r, denoiser, ref, n, W, H, spp=4):
 
 
 def serve_scenes(n):
-    """64 uneven scenes (seeds 2000..2063): square frames of 256..1024 px
-    (multiples of 64), 10k..500k triangles (log-uniform), the terrain-room
-    generator's Disney materials and quad lights plus a procedural sky."""
+    """The 64 uneven serving scenes (scenes.serve_c5)."""
     from bluetruthengine_amd import scenes
-    rs = np.random.default_rng(2000)
-    out = []
-    for k in range(n):
-        side = int(rs.integers(4, 17)) * 64
-        tris = float(np.exp(rs.uniform(np.log(1e4), np.log(5e5))))
-        freq = max(4, int(round(np.sqrt(tris / 20.0))))
-        seed = 2000 + k
-        sd = scenes.terrain_c2(side, side, spp=8, max_depth=10, freq=freq, seed=seed)
-        srng = np.random.default_rng(seed)
-        sd.env_rgba = scenes.sky_env(512, 256, seed=seed, sun_elev_deg=srng.uniform(15, 75),
-                                     sun_azim_deg=srng.uniform(0, 360))
-        out.append(sd)
-        if k % 8 == 7:
-            log(f"generated {k + 1}/{n} scenes")
+    t0 = time.perf_counter()
+    out = scenes.serve_c5(n)
+    log(f"generated {n} scenes in {time.perf_counter() - t0:.1f} s")
     return out
 
 

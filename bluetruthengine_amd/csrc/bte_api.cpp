@@ -93,6 +93,7 @@ struct Context : SceneData {
     DBuf dBatchScenes, dBatchSlots, dBatchChunk, dBatchAccum, dBatchFb, dBatchSum64, dBatchAovAlb, dBatchAovNrm,
         dBatchDenoised, dBatchSegCount, dBatchSegRows;
     long long batchLayoutKey = -1;            // frames per pass the slot table was built for
+    int batchLastFrames = 0;                  // frames of the batch's last wavefront pass
     unsigned long long batchLayoutGen = ~0ull;
     bool batchDenoised = false;
 
@@ -337,8 +338,7 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
         CHECK_RC(c.dAovAlb.ensure(P * sizeof(float4)));
         CHECK_RC(c.dAovNrm.ensure(P * sizeof(float4)));
     }
-    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4, 2);
-    CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
+    CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints_max(c.device) * sizeof(int)));
     if (c.hitCaptureRec > 0) CHECK_RC(c.dHitCap.ensure(P * (size_t)c.hitCaptureRec * sizeof(int)));
     c.cap = cap;
     c.capKinds = kinds;
@@ -474,6 +474,8 @@ int renderWavefront(int nframes) {
     LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16,
                                         c.cfg.traversalBlocksPerCU, c.cfg.workChunk, c.cfg.bvhWidth);
     if (ls.bvhWidth == 8 && !c.dNodes8.p) ls.bvhWidth = 4;   // BVH8 too deep for the stack
+    if ((size_t)overflow_stack_ints(ls) * sizeof(int) > c.dOvf.bytes)
+        return fail(BTE_ERR_STATE, "traversal overflow stack smaller than the launch shape needs");
     wb.grabRows = ls.grabRows;
     // instrumented runs record events between launches: those stay uncaptured
     const bool useGraph = c.cfg.graphs != 0 && c.instr == 0;
@@ -585,8 +587,7 @@ int renderWavefront(int nframes) {
 int renderMegakernel(int nframes) {
     Context& c = ctx();
     const DevScene sc = makeScene(c, c.accW, c.accH);
-    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4, 2);
-    CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
+    CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints_max(c.device) * sizeof(int)));
     uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
     double* sum64 = c.accum64 ? c.dSum64.as<double>() : nullptr;
     const int spp = std::max(1, c.cfg.megakernelSpp);
@@ -635,6 +636,7 @@ void batchClear(Context& c) {
                     &c.dBatchSegCount, &c.dBatchSegRows};
     for (DBuf* b : bufs) b->release();
     c.batchLayoutKey = -1;
+    c.batchLastFrames = 0;
     c.batchDenoised = false;
 }
 
@@ -687,6 +689,14 @@ int batchKinds(const Context& c, int& kindArea, int& kindSpot0, int& kindEnv, in
 long long batchPaddedPaths(const Context& c, int F) {
     long long pb = 0;
     for (const BatchScene& b : c.batch) pb += ((long long)b.w * b.h * F + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
+    return pb;
+}
+
+// First path id of batch slot `slot` in a pass of F frames (the layout above).
+long long batchPathBase(const Context& c, int F, int slot) {
+    long long pb = 0;
+    for (int i = 0; i < slot; ++i)
+        pb += ((long long)c.batch[i].w * c.batch[i].h * F + kBatchChunk - 1) / kBatchChunk * kBatchChunk;
     return pb;
 }
 
@@ -774,13 +784,9 @@ int renderBatch(int nframes) {
         CHECK_RC(batchLayout(c, frames, padded, slots, maxPaths, maxPixels));
         WaveBuffers wb;
         std::memset(&wb, 0, sizeof wb);   // padding too: compared bytewise as graph keys
-        const int savedCapture = c.hitCaptureRec;
-        c.hitCaptureRec = 0;
-        const int rc = ensureWaveBuffers(padded, kinds, maxDepth, wb);
-        c.hitCaptureRec = savedCapture;
-        CHECK_RC(rc);
-        wb.hitCapture = nullptr;
-        wb.maxRec = 0;
+        CHECK_RC(ensureWaveBuffers(padded, kinds, maxDepth, wb));   // hit capture indexed by padded path id
+        if ((size_t)overflow_stack_ints(ls) * sizeof(int) > c.dOvf.bytes)
+            return fail(BTE_ERR_STATE, "traversal overflow stack smaller than the launch shape needs");
         wb.kinds = kinds;
         wb.travCtr = nullptr;
         wb.catMask = catMask;
@@ -820,6 +826,7 @@ int renderBatch(int nframes) {
                     launch_fill_i32(wb.counters, (long long)ctrInts, 0, c.stream);
                     launch_fill_i32(wb.work, (long long)kWorkStride * (maxDepth + 1), 0, c.stream);
                     launch_fill_i32(wb.segCount, (long long)segInts, 0, c.stream);
+                    if (wb.hitCapture) launch_fill_i32(wb.hitCapture, padded * wb.maxRec, -2, c.stream);
                     launch_generate_batch(scenesDev, wb, maxPaths, slots, c.stream);
                 }
                 for (int bb = b0; bb < b1; ++bb) launch_bounce(none, wb, bb, maxDepth, ls, c.stream, nullptr, scenesDev);
@@ -854,6 +861,7 @@ int renderBatch(int nframes) {
         c.stats.bounces = std::max(c.stats.bounces, b);
         c.stats.pathsTraced += slots;
         c.batchFrame += (uint32_t)frames;
+        c.batchLastFrames = frames;
         done += frames;
     }
     HIPCHK(hipEventRecord(c.ev1, c.stream));
@@ -1441,9 +1449,7 @@ int bte_read_hit_capture(int* hostTris, size_t count) {
     return BTE_OK;
 }
 
-int bte_read_env_cdf(float* marginal, float* conditional, float* totalPower) {
-    ENSURE_DEVICE();
-    Context& c = ctx();
+static int readEnvCdf(const SceneData& c, float* marginal, float* conditional, float* totalPower) {
     if (!c.hasEnv) return fail(BTE_ERR_STATE, "no environment map");
     if (marginal) HIPCHK(hipMemcpy(marginal, c.dEnvMarg.p, (size_t)(c.envH + 1) * sizeof(float), hipMemcpyDeviceToHost));
     if (conditional)
@@ -1451,6 +1457,18 @@ int bte_read_env_cdf(float* marginal, float* conditional, float* totalPower) {
                          hipMemcpyDeviceToHost));
     if (totalPower) *totalPower = c.envTotal;
     return BTE_OK;
+}
+
+int bte_read_env_cdf(float* marginal, float* conditional, float* totalPower) {
+    ENSURE_DEVICE();
+    return readEnvCdf(ctx(), marginal, conditional, totalPower);
+}
+
+int bte_batch_read_env_cdf(int slot, float* marginal, float* conditional, float* totalPower) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    CHECK_RC(batchSlot(c, slot));
+    return readEnvCdf(*c.batch[slot].data, marginal, conditional, totalPower);
 }
 
 int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, float tmin, float tmax, int anyHit,
@@ -1463,8 +1481,7 @@ int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, floa
     if (c.tris.empty()) return fail(BTE_ERR_STATE, "no triangles uploaded");
     CHECK_RC(ensureAccel(c));
     if (c.accW <= 0) { c.accW = std::max(1, c.sceneW); c.accH = std::max(1, c.sceneH); }
-    const LaunchShape ls = query_launch_shape(c.device, 8, 0, 4, 2);
-    CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints(ls) * sizeof(int)));
+    CHECK_RC(c.dOvf.ensure((size_t)overflow_stack_ints_max(c.device) * sizeof(int)));
     DBuf dr, dtm, dtri, dt, du, dv;
     CHECK_RC(dr.upload(rays, (size_t)n * 6 * sizeof(float)));
     if (anyHit && tmaxPerRay) CHECK_RC(dtm.upload(tmaxPerRay, (size_t)n * sizeof(float)));
@@ -1576,6 +1593,40 @@ int bte_batch_read_accum64(int slot, double* hostRGB, size_t count) {
     if (!c.accum64 || !c.dBatchSum64.p) return fail(BTE_ERR_STATE, "binary64 accumulation is off");
     HIPCHK(hipMemcpy(hostRGB, c.dBatchSum64.as<double>() + 3 * (size_t)b.pixelBase, 3 * N * sizeof(double),
                      hipMemcpyDeviceToHost));
+    return BTE_OK;
+}
+
+int bte_batch_read_sample_radiance(int slot, float* hostRGB, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    CHECK_RC(batchSlot(c, slot));
+    if (c.batchLastFrames <= 0 || !c.dFinal.p) return fail(BTE_ERR_STATE, "no batch pass recorded");
+    const BatchScene& b = c.batch[slot];
+    const size_t P = (size_t)b.w * b.h * (size_t)c.batchLastFrames;
+    if (!hostRGB || count < 3 * P) return fail(BTE_ERR_INVALID, "sample copy needs %zu floats", 3 * P);
+    const long long base = batchPathBase(c, c.batchLastFrames, slot);
+    std::vector<float> tmp(4 * P);
+    HIPCHK(hipMemcpy(tmp.data(), c.dFinal.as<float4>() + base, P * sizeof(float4), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < P; ++i) {
+        hostRGB[3 * i] = tmp[4 * i];
+        hostRGB[3 * i + 1] = tmp[4 * i + 1];
+        hostRGB[3 * i + 2] = tmp[4 * i + 2];
+    }
+    return BTE_OK;
+}
+
+int bte_batch_read_hit_capture(int slot, int* hostTris, size_t count) {
+    ENSURE_DEVICE();
+    Context& c = ctx();
+    CHECK_RC(batchSlot(c, slot));
+    if (c.batchLastFrames <= 0 || c.hitCaptureRec <= 0 || !c.dHitCap.p)
+        return fail(BTE_ERR_STATE, "no batch hit capture recorded");
+    const BatchScene& b = c.batch[slot];
+    const size_t rec = (size_t)c.hitCaptureRec;
+    const size_t n = (size_t)b.w * b.h * (size_t)c.batchLastFrames * rec;
+    if (!hostTris || count < n) return fail(BTE_ERR_INVALID, "hit capture copy needs %zu ints", n);
+    const long long base = batchPathBase(c, c.batchLastFrames, slot);
+    HIPCHK(hipMemcpy(hostTris, c.dHitCap.as<int>() + (size_t)base * rec, n * sizeof(int), hipMemcpyDeviceToHost));
     return BTE_OK;
 }
 

@@ -177,6 +177,9 @@ void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, d
 // traversalBlocksPerCU 0 = as many as occupancy allows; grabRows 1, 2, 4 or 8 (else 4).
 LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows, int bvhWidth);
 int persistent_threads(const LaunchShape& ls);
-int overflow_stack_ints(const LaunchShape& ls);   // size of the overflow-stack buffer (ints)
+int overflow_stack_ints(const LaunchShape& ls);   // overflow-stack ints a launch of shape ls indexes
+// Overflow-stack ints that cover every launch shape query_launch_shape can
+// return on `device` (all BVH widths and LDS stack sizes, full occupancy).
+long long overflow_stack_ints_max(int device);
 
 }  // namespace bte

@@ -26,10 +26,6 @@ constexpr int kOvfStack = kStackSize - kMinLdsStack;   // global overflow entrie
 constexpr uint32_t kSpecBit = 0x80000000u;
 constexpr uint32_t kAovBit = 0x40000000u;
 constexpr uint32_t kPidMask = 0x3fffffffu;
-[[maybe_unused]] constexpr int kTraversalDone = 0x7fffffff;   // sentinel: no node left to visit
-#ifndef BTE_TRAVERSAL_WHILE_WHILE
-#define BTE_TRAVERSAL_WHILE_WHILE 0
-#endif
 constexpr uint32_t kFinalBit = 0x80000000u;
 
 // ---------------------------------------------------------------------------
@@ -209,7 +205,27 @@ BTE_DEV WtRay makeWtRay(V3 o, V3 d) {
     return r;
 }
 
-// Returns true with (t, u, v) when the ray hits within [tmin, tmax].
+// Line interval of the box of three points, in the arithmetic of boxEnter;
+// false when it is empty.
+BTE_DEV bool triSpan(const WtRay& r, float4 a, float4 b, float4 c, float& lo, float& hi) {
+    const float x0 = (fminf(fminf(a.x, b.x), c.x) - r.o.x) * r.inv[0];
+    const float x1 = (fmaxf(fmaxf(a.x, b.x), c.x) - r.o.x) * r.inv[0];
+    const float y0 = (fminf(fminf(a.y, b.y), c.y) - r.o.y) * r.inv[1];
+    const float y1 = (fmaxf(fmaxf(a.y, b.y), c.y) - r.o.y) * r.inv[1];
+    const float z0 = (fminf(fminf(a.z, b.z), c.z) - r.o.z) * r.inv[2];
+    const float z1 = (fmaxf(fmaxf(a.z, b.z), c.z) - r.o.z) * r.inv[2];
+    lo = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+    hi = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+    return lo <= hi;
+}
+
+// Returns true with (t, u, v) when the ray hits within [tmin, tmax].  The
+// distance T/det of a ray nearly parallel to a large triangle is badly
+// conditioned, so it is clamped to the line interval of the triangle's own
+// box (a triangle whose interval is empty is missed).  Every BVH box holding
+// the triangle then admits that distance in boxEnter's arithmetic (rounding
+// is monotone), so a traversal finds exactly what a loop over all triangles
+// finds, whatever the tree.
 BTE_DEV bool hitTriangle(const WtRay& r, float4 a, float4 b, float4 c, float tmin, float tmax,
                          float& tOut, float& uOut, float& vOut) {
     const V3 A = sub3(xyz(a), r.o);
@@ -232,7 +248,9 @@ BTE_DEV bool hitTriangle(const WtRay& r, float4 a, float4 b, float4 c, float tmi
     const float Bz = r.Sz * Bz_;
     const float Cz = r.Sz * Cz_;
     const float T = U * Az + V * Bz + W * Cz;
-    const float t = T / det;
+    float lo, hi;
+    if (!triSpan(r, a, b, c, lo, hi)) return false;
+    const float t = fminf(fmaxf(T / det, lo), hi);
     if (!(t >= tmin && t <= tmax)) return false;
     tOut = t;
     uOut = V / det;
@@ -240,14 +258,17 @@ BTE_DEV bool hitTriangle(const WtRay& r, float4 a, float4 b, float4 c, float tmi
     return true;
 }
 
-// Conservative slab interval of one child box (far distance inflated).
+// Slab interval of one child box: entry distance returned, far distance
+// (limited by the query's current tmax / best hit) in tfar.  No widening is
+// needed: hitTriangle clamps a hit to its triangle's own interval, which any
+// enclosing box's computed interval contains.
 BTE_DEV float boxEnter(const WtRay& r, float lx, float hx, float ly, float hy, float lz, float hz, float tmin,
                        float tmax, float& tfar) {
     const float ax = (lx - r.o.x) * r.inv[0], bx = (hx - r.o.x) * r.inv[0];
     const float ay = (ly - r.o.y) * r.inv[1], by = (hy - r.o.y) * r.inv[1];
     const float az = (lz - r.o.z) * r.inv[2], bz = (hz - r.o.z) * r.inv[2];
     const float t0 = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), tmin));
-    const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)) * 1.0000005f;
+    const float t1 = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
     tfar = fminf(t1, tmax);
     return t0;
 }
@@ -282,55 +303,6 @@ BTE_DEV int traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, con
     int bestTri = -1;
     int node = 0;
     int sp = 0;
-#if BTE_TRAVERSAL_WHILE_WHILE
-    for (;;) {
-        // descend through inner nodes until this lane holds a leaf
-        while (node >= 0) {
-            if (COUNT) ++*nodeCount;
-            const float4* np = sc.nodes + 4 * (size_t)node;
-            const float4 n0 = np[0], n1 = np[1], n2 = np[2];
-            const int4 ch = *reinterpret_cast<const int4*>(np + 3);
-            float f0, f1;
-            const float e0 = boxEnter(r, n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, tmin, best, f0);
-            const float e1 = boxEnter(r, n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, tmin, best, f1);
-            const bool h0 = e0 <= f0, h1 = e1 <= f1;
-            if (h0 && h1) {
-                const bool first0 = e0 <= e1;
-                node = first0 ? ch.x : ch.y;
-                push(stk, sp, first0 ? ch.y : ch.x);
-            } else if (h0) {
-                node = ch.x;
-            } else if (h1) {
-                node = ch.y;
-            } else {
-                if (sp == 0) { node = kTraversalDone; break; }
-                node = pop(stk, sp);
-            }
-        }
-        if (node == kTraversalDone) break;
-        {
-            const int code = ~node;
-            const int start = code >> kLeafCountBits;
-            const int cnt = (code & (kMaxLeafTris - 1)) + 1;
-            for (int i = 0; i < cnt; ++i) {
-                const float4* tp = sc.leafTris + 3 * (size_t)(start + i);
-                const float4 a = tp[0], b = tp[1], c = tp[2];
-                if (COUNT) ++*triCount;
-                float t, u, v;
-                if (hitTriangle(r, a, b, c, tmin, best, t, u, v)) {
-                    if (ANY) return 1;
-                    const int id = __float_as_int(a.w);
-                    if (t < best || bestTri < 0 || id < bestTri) {
-                        best = t; bestTri = id; tOut = t; uOut = u; vOut = v;
-                    }
-                }
-            }
-        }
-        if (sp == 0) break;
-        node = pop(stk, sp);
-    }
-    return ANY ? 0 : bestTri;
-#else
     for (;;) {
         if (node >= 0) {
             if (COUNT) ++*nodeCount;
@@ -371,7 +343,6 @@ BTE_DEV int traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, con
         node = pop(stk, sp);
     }
     return ANY ? 0 : bestTri;
-#endif
 }
 
 BTE_DEV void sortPair(float& ka, int& ca, float& kb, int& cb) {
@@ -1721,7 +1692,14 @@ BTE_DEV void presentOne(const WaveBuffers& wb, int frames, int frameBase, size_t
         wb.aovAccNrm[out] = mn;
     }
     float4 mean = accum[out];
+    // binary64 sums take one frame at a time, in frame order, so they do not
+    // depend on how the frames were split into passes
     double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (sum64) {
+        sx = sum64[3 * out];
+        sy = sum64[3 * out + 1];
+        sz = sum64[3 * out + 2];
+    }
     for (int f = 0; f < frames; ++f) {
         const float4 x = wb.finalRad[src + (size_t)f * stride];
         const float n = (float)((uint32_t)(frameBase + f) + 1u);
@@ -1732,9 +1710,9 @@ BTE_DEV void presentOne(const WaveBuffers& wb, int frames, int frameBase, size_t
     }
     accum[out] = mean;
     if (sum64) {
-        sum64[3 * out] += sx;
-        sum64[3 * out + 1] += sy;
-        sum64[3 * out + 2] += sz;
+        sum64[3 * out] = sx;
+        sum64[3 * out + 1] = sy;
+        sum64[3 * out + 2] = sz;
     }
     if (fb) fb[out] = packGamma(V3{mean.x, mean.y, mean.z});
 }
@@ -2175,5 +2153,14 @@ int persistent_threads(const LaunchShape& ls) {
 }
 
 int overflow_stack_ints(const LaunchShape& ls) { return persistent_threads(ls) * kOvfStack + 64; }
+
+long long overflow_stack_ints_max(int device) {
+    // every traversal width and LDS stack size the configuration can select,
+    // at the largest grid any of them launches
+    int most = 0;
+    for (int w : {2, 4, 8})
+        for (int st : {8, 16, 32}) most = std::max(most, persistent_threads(query_launch_shape(device, st, 0, 4, w)));
+    return (long long)most * kOvfStack + 64;
+}
 
 }  // namespace bte

@@ -134,6 +134,9 @@ API = {
     "bte_batch_read_accum": [_i, _p, _sz],
     "bte_batch_read_framebuffer": [_i, _p, _sz],
     "bte_batch_read_accum64": [_i, _p, _sz],
+    "bte_batch_read_sample_radiance": [_i, _p, _sz],
+    "bte_batch_read_hit_capture": [_i, _p, _sz],
+    "bte_batch_read_env_cdf": [_i, _p, _p, _p],
     "bte_batch_denoise": [],
     "bte_batch_read_denoised": [_i, _p, _sz],
     "bte_denoise": [],
@@ -503,6 +506,32 @@ class Batch:
         out = np.empty(w * h, np.uint32)
         _call("bte_batch_read_framebuffer", int(slot), _ptr(out), out.size)
         return out.reshape(h, w)
+
+    def set_hit_capture(self, max_rec):
+        """Record the hit triangle of the first max_rec bounces of every path of the next pass (0 = off)."""
+        _call("bte_set_hit_capture", int(max_rec))
+
+    def sample_radiance(self, slot, frames):
+        """Per-sample radiance of slot's scene in the last pass: (frames, H, W, 3)."""
+        w, h = self.sizes[slot]
+        out = np.empty((frames, h, w, 3), np.float32)
+        _call("bte_batch_read_sample_radiance", int(slot), _ptr(out), out.size)
+        return out
+
+    def hit_capture(self, slot, frames, max_rec):
+        w, h = self.sizes[slot]
+        out = np.empty((frames, h * w, max_rec), np.int32)
+        _call("bte_batch_read_hit_capture", int(slot), _ptr(out), out.size)
+        return out
+
+    def env_cdf(self, slot, env_shape):
+        """(marginal, conditional, total) of slot's env map as built on the GPU; env_shape = (H, W)."""
+        H, W = env_shape
+        marg = np.empty(H + 1, np.float32)
+        cond = np.empty(H * (W + 1), np.float32)
+        total = ctypes.c_float(0)
+        _call("bte_batch_read_env_cdf", int(slot), _ptr(marg), _ptr(cond), ctypes.byref(total))
+        return marg, cond, np.float32(total.value)
 
     def denoise(self):
         _call("bte_batch_denoise")

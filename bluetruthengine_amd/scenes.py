@@ -537,7 +537,28 @@ def small_envlit(width=128, height=72, n_spheres=12, env_size=(256, 128)):
     return sd
 
 
-CONFIGS = {"c1": cornell_c1, "c2": terrain_c2, "c3": envlit_c3}
+def serve_c5(n=64):
+    """The batched-serving workload: n uneven scenes (seeds 2000..2000+n-1),
+    square frames of 256..1024 px (multiples of 64), 10k..500k triangles
+    (log-uniform), the terrain-room generator's Disney materials and quad
+    lights plus a 512x256 procedural sky with its own sun, 8 spp, depth 10."""
+    rs = np.random.default_rng(2000)
+    out = []
+    for k in range(n):
+        side = int(rs.integers(4, 17)) * 64
+        tris = float(np.exp(rs.uniform(np.log(1e4), np.log(5e5))))
+        freq = max(4, int(round(np.sqrt(tris / 20.0))))
+        seed = 2000 + k
+        sd = terrain_c2(side, side, spp=8, max_depth=10, freq=freq, seed=seed)
+        srng = np.random.default_rng(seed)
+        sd.env_rgba = sky_env(512, 256, seed=seed, sun_elev_deg=srng.uniform(15, 75),
+                              sun_azim_deg=srng.uniform(0, 360))
+        sd.name = f"c5_serve_{k}"
+        out.append(sd)
+    return out
+
+
+CONFIGS = {"c1": cornell_c1, "c2": terrain_c2, "c3": envlit_c3, "c5": serve_c5}
 
 
 # ---------------------------------------------------------------------------

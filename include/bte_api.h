@@ -240,6 +240,13 @@ int bte_batch_frame_index(void);
 int bte_batch_read_accum(int slot, float* hostRGB, size_t count);
 int bte_batch_read_framebuffer(int slot, uint32_t* hostRGBA8, size_t count);
 int bte_batch_read_accum64(int slot, double* hostRGB, size_t count);
+/* Per slot, as bte_read_sample_radiance / bte_read_hit_capture, for the
+ * batch's last wavefront pass: (frame - firstFrameOfPass) * W*H + pixel of
+ * that slot's scene (bte_set_hit_capture applies to batches too). */
+int bte_batch_read_sample_radiance(int slot, float* hostRGB, size_t count);
+int bte_batch_read_hit_capture(int slot, int* hostTris, size_t count);
+/* Per slot, as bte_read_env_cdf. */
+int bte_batch_read_env_cdf(int slot, float* marginal, float* conditional, float* totalPower);
 /* The U-Net over every slot (AOVs enabled before the batch's first render),
  * into each slot's display buffer and its denoised linear image. */
 int bte_batch_denoise(void);

@@ -775,7 +775,23 @@ static inline bool intersectTri(const WtRay& r, const OraTriangle& tri, float tm
     const float Bz = r.Sz * Bz_;
     const float Cz = r.Sz * Cz_;
     const float T = U * Az + V * Bz + W * Cz;
-    const float t = T / det;
+    // T/det is badly conditioned for a ray nearly parallel to a large
+    // triangle: the distance is clamped to the line interval of the
+    // triangle's own box, computed as hitBox computes a node's (an empty
+    // interval is a miss).  Rounding is monotone, so every node box holding
+    // the triangle admits the clamped distance and the BVH finds exactly what
+    // the loop over all triangles finds.
+    float lo = -INFINITY, hi = INFINITY;
+    const F3 P[3] = {tri.v0, tri.v1, tri.v2};
+    for (int k = 0; k < 3; ++k) {
+        const float o = comp(r.o, k);
+        const float a = (std::fmin(std::fmin(comp(P[0], k), comp(P[1], k)), comp(P[2], k)) - o) * r.inv[k];
+        const float b = (std::fmax(std::fmax(comp(P[0], k), comp(P[1], k)), comp(P[2], k)) - o) * r.inv[k];
+        lo = std::fmax(lo, std::fmin(a, b));
+        hi = std::fmin(hi, std::fmax(a, b));
+    }
+    if (!(lo <= hi)) return false;
+    const float t = std::fmin(std::fmax(T / det, lo), hi);
     if (!(t >= tmin && t <= tmax)) return false;
     tOut = t;
     uOut = V / det;
@@ -783,7 +799,11 @@ static inline bool intersectTri(const WtRay& r, const OraTriangle& tri, float tm
     return true;
 }
 
-// Conservative slab test: far distance inflated by 4 ulp-ish (Ize 2013).
+// Slab test of a node box against [tmin, tmax].  No widening is needed: a
+// triangle's hit distance is clamped to its own box interval (intersectTri),
+// which any enclosing node's computed interval contains, so the answer is the
+// minimum over all triangles whatever the tree (equal to the brute-force loop
+// bit for bit).
 static inline bool hitBox(const WtRay& r, const Node& n, float tmin, float tmax) {
     float t0 = tmin, t1 = tmax;
     const float o[3] = {r.o.x, r.o.y, r.o.z};
@@ -792,7 +812,7 @@ static inline bool hitBox(const WtRay& r, const Node& n, float tmin, float tmax)
         const float b = (n.hi[k] - o[k]) * r.inv[k];
         const float lo = std::fmin(a, b), hi = std::fmax(a, b);
         t0 = std::fmax(t0, lo);
-        t1 = std::fmin(t1, hi * 1.0000005f);
+        t1 = std::fmin(t1, hi);
     }
     return t0 <= t1;
 }

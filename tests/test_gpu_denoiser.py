@@ -259,3 +259,108 @@ def test_relmse_training_loss_matches_oracle():
     go = dn.tensor(GRD["go"])
     assert np.all(go[..., 3:] == 0)
     _close_bf16(go[..., :3], od.bf16_round(ref_go.astype(np.float32)), "relMSE loss gradient", min_exact=0.99)
+
+
+def _window(t, img, y0, x0, s, up=False):
+    """Rows y0-1..y0+s and columns x0-1..x0+s of image `img` of tensor t (of
+    its 2x nearest upsampling when `up`), zero outside the image: the input a
+    3x3 convolution needs for the s x s outputs at (y0, x0)."""
+    _, ht, wt, _ = t.shape
+    h, w = (2 * ht, 2 * wt) if up else (ht, wt)
+    ys, xs = np.arange(y0 - 1, y0 + s + 1), np.arange(x0 - 1, x0 + s + 1)
+    inside = ((ys >= 0) & (ys < h))[:, None] & ((xs >= 0) & (xs < w))[None, :]
+    yi, xi = np.clip(ys, 0, h - 1), np.clip(xs, 0, w - 1)
+    if up:
+        yi, xi = yi // 2, xi // 2
+    win = t[img][yi][:, xi].astype(np.float64) * inside[..., None]
+    return win[None]
+
+
+def _windows(rs, n, h, w, s=32):
+    """Corner windows (top-left, bottom-right) and one inside, each in a random image."""
+    return [(int(rs.integers(n)), 0, 0), (int(rs.integers(n)), h - s, w - s),
+            (int(rs.integers(n)), int(rs.integers(1, h - s)), int(rs.integers(1, w - s)))]
+
+
+# conv layers of the forward pass: (layer, output tensor, inputs as (tensor, upsampled), level)
+_FWD_CONVS = [("c0a", "e0a", [("x", False)], 0), ("c0b", "s0", [("e0a", False)], 0),
+              ("c1a", "e1a", [("p0", False)], 1), ("c1b", "s1", [("e1a", False)], 1),
+              ("c2a", "e2a", [("p1", False)], 2), ("c2b", "s2", [("e2a", False)], 2),
+              ("c3a", "e3a", [("p2", False)], 3), ("c3b", "b3", [("e3a", False)], 3),
+              ("d2a", "d2a", [("b3", True), ("s2", False)], 2), ("d2b", "d2b", [("d2a", False)], 2),
+              ("d1a", "d1a", [("d2b", True), ("s1", False)], 1), ("d1b", "d1b", [("d1a", False)], 1),
+              ("d0a", "d0a", [("d1b", True), ("s0", False)], 0), ("d0b", "d0b", [("d0a", False)], 0)]
+
+
+def test_forward_multi_tile_windows_match_oracle():
+    # 4 images of 1920x1088 (the flip ensemble's batch at 1080p): every layer
+    # holds far more 8x32 tiles than the persistent conv grid has blocks per
+    # output-channel group (level 0: 32,640 tiles; level 3: 544), so each block
+    # loops over tiles, prefetching the next tile or slice while computing, and
+    # reuses its LDS and accumulators from tile to tile.  Each conv layer is
+    # checked on three 32x32 windows (two at image corners) against the oracle
+    # conv of the GPU's own input window; the residual output layer in fp32.
+    _say("case: U-Net forward n=4 h=1088 w=1920 (multi-tile persistent conv grids), seed 5, 3 windows per layer")
+    renderer.cudaCleanup()
+    n, h, w = 4, 1088, 1920
+    params, x9 = _forward_case(5, n, h, w)
+    out = dn.forward(x9)
+    del x9
+    net = od.Net(params, bf16=True)
+    rs = np.random.default_rng(5)
+    last_use = {}
+    for i, (_, _, ins, _) in enumerate(_FWD_CONVS):
+        for name, _ in ins:
+            last_use[name] = i
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            t = dn.tensor(FWD[name])
+            cache[name] = t[..., :9] if name == "x" else t
+        return cache[name]
+    for i, (layer, oname, ins, lvl) in enumerate(_FWD_CONVS):
+        got = get(oname)
+        for img, y0, x0 in _windows(rs, n, h >> lvl, w >> lvl):
+            win = np.concatenate([_window(get(t), img, y0, x0, 32, up) for t, up in ins], -1)
+            ref = net.conv(layer, win, True)[0, 1:-1, 1:-1]
+            _close_bf16(got[img, y0:y0 + 32, x0:x0 + 32, :ref.shape[-1]], ref,
+                        f"{layer} window img {img} at ({y0}, {x0})")
+        for name, _ in ins:
+            if last_use[name] == i and name not in ("x", "d0b"):
+                cache.pop(name, None)
+    x, d0b = get("x"), get("d0b")
+    for img, y0, x0 in _windows(rs, n, h, w):
+        resid = _window(x, img, y0, x0, 32)[..., :3]
+        ref = net.conv("out", _window(d0b, img, y0, x0, 32), False, resid=resid)[0, 1:-1, 1:-1]
+        err = np.abs(out[img, y0:y0 + 32, x0:x0 + 32] - ref).max() / (np.abs(ref).max() + 1e-30)
+        _say(f"out window img {img} at ({y0}, {x0}) (fp32): max rel-to-max err {err:.3g}")
+        assert err < 1e-5
+
+
+def test_backward_multi_tile_windows_match_oracle():
+    # one 2048x1088 image (training rows are multiples of 256 wide): the
+    # input-gradient convolutions with a ReLU mask (the EXTRA epilogue) run
+    # with more tiles than blocks per group at levels 0-2 (level 1: 2,176
+    # tiles).  Three 32x32 windows per layer against the oracle's dgrad of the
+    # GPU's own output-gradient window.
+    _say("case: U-Net backward (lr 0) n=1 h=1088 w=2048, multi-tile dgrad with ReLU masks, seed 6")
+    renderer.cudaCleanup()
+    n, h, w = 1, 1088, 2048
+    params, x9 = _forward_case(6, n, h, w)
+    target = np.random.default_rng(9).normal(0, 1, (n, h, w, 3)).astype(np.float32)
+    dn.train_step(x9, target, lr=0.0)
+    del x9, target
+    net = od.Net(params, bf16=True)
+    rs = np.random.default_rng(6)
+    # (layer, gradient out, gradient in, mask tensor, level): tile1 (32 channels in) and tileN (64, 128)
+    for layer, gout, gin, mask, lvl in (("d0b", "gd0b", "gd0a", "d0a", 0), ("c1b", "gs1", "ge1a", "e1a", 1),
+                                        ("c2b", "gs2", "ge2a", "e2a", 2)):
+        g_out = dn.tensor(GRD[gout])
+        g_in = dn.tensor(GRD[gin])
+        m = dn.tensor(FWD[mask])
+        for img, y0, x0 in _windows(rs, n, h >> lvl, w >> lvl):
+            ref = net._dgrad(layer, _window(g_out, img, y0, x0, 32), _window(m, img, y0, x0, 32))[0, 1:-1, 1:-1]
+            _close_bf16(g_in[img, y0:y0 + 32, x0:x0 + 32, :ref.shape[-1]], ref,
+                        f"dgrad {layer} window at ({y0}, {x0})")
+        del g_out, g_in, m

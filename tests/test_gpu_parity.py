@@ -301,3 +301,21 @@ def test_aov_means_match_oracle():
         plain = renderer.Renderer(sd)
         plain.render(3)
         assert np.array_equal(r.accum(), plain.accum())     # AOVs leave the radiance untouched
+
+
+def test_accum64_matches_oracle_across_passes():
+    # the binary64 sums behind every converged reference image: 7 frames in
+    # passes of 3 (3 + 3 + 1) with graphs on, against the oracle's per-frame
+    # sums (rt_path_integrator.cuh:72-87 plus the plain binary64 sum)
+    for sd in (scenes.cornell_c1(width=64, height=64, spp=7, max_depth=8), scenes.small_envlit()):
+        _say(f"case: {sd.name} {sd.width}x{sd.height}, 7 frames in passes of 3, graphs on: binary64 sums")
+        r = renderer.Renderer(sd, frames_in_flight=3)
+        r.enable_accum64(True)
+        r.render(7)
+        assert r.stats().graphLaunches > 0
+        ref = oracle.OracleScene(sd, env_cdf=r.env_cdf() if sd.env_rgba is not None else None)
+        s64 = np.zeros((sd.height, sd.width, 3), np.float64)
+        acc = ref.render(sd.width, sd.height, 0, 7, sd.max_depth, sum64=s64)
+        same = np.array_equal(r.accum64(), s64), np.array_equal(r.accum(), acc)
+        _say(f"{sd.name}: binary64 sums equal {same[0]}, running means equal {same[1]}")
+        assert all(same)

@@ -112,3 +112,52 @@ def test_primary_ray_closed_form():
     sd1 = scenes.cornell_c1(width=33, height=33)
     L.ora_primary_ray(oracle._ptr(np.ascontiguousarray(sd1.camera)), 16, 16, 33, 33, 0.0, 0.0, oracle._ptr(out))
     assert np.allclose(out, sd1.camera["forward"], atol=1e-7)
+
+
+def _near_tie_scene():
+    """Triangles at nearly equal distances along many rays: 60 overlapping
+    coplanar quads at y = 1.98 (like the terrain room's ceiling lights) and 40
+    mutually intersecting icospheres resting on a floor quad."""
+    from bluetruthengine_amd import pod
+    sb = scenes.SceneBuilder()
+    b = sb.add_bsdf(pod.bsdf_diffuse((0.5, 0.5, 0.5)))
+    rs = np.random.default_rng(21)
+    for _ in range(60):
+        sx, sz = rs.uniform(0.2, 0.5, 2)
+        cx, cz = rs.uniform(-0.7, 0.7, 2)
+        sb.add_mesh(scenes.ceiling_light(cx, cz, sx, sz, 1.98), None, b)
+    sb.add_mesh(scenes.quad((-2, 0, -2), (-2, 0, 2), (2, 0, 2), (2, 0, -2)), None, b)
+    v, f = scenes.icosphere(2)
+    for _ in range(40):
+        r = rs.uniform(0.1, 0.3)
+        c = np.array([rs.uniform(-0.8, 0.8), r, rs.uniform(-0.8, 0.8)])
+        p, n = scenes.mesh_arrays(v * r + c, f, v)
+        sb.add_mesh(p, n, b)
+    cam = scenes.make_camera((0, 1, 3.6), (0, 1, 0), (0, 1, 0), 40.0, 16, 16)
+    return sb.build("near_ties", 16, 16, 1, 4, cam)
+
+
+def test_bvh_equals_brute_force_on_near_ties():
+    # the box test must not prune a box whose triangle's computed distance is
+    # within rounding of the box entry: with overlapping coplanar quads the
+    # closest of two nearly equal distances must not depend on the tree
+    sd = _near_tie_scene()
+    ref = oracle.OracleScene(sd)
+    rs = np.random.default_rng(22)
+    n = 200_000
+    o = rs.uniform((-0.9, 0.05, -0.9), (0.9, 1.2, 0.9), (n, 3))
+    tgt = np.stack([rs.uniform(-0.9, 0.9, n), np.full(n, 1.98), rs.uniform(-0.9, 0.9, n)], 1)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    up = np.concatenate([o, d], 1).astype(np.float32)
+    down = np.concatenate([o, rs.normal(size=(n, 3))], 1).astype(np.float32)
+    down[:, 3:] /= np.linalg.norm(down[:, 3:], axis=1, keepdims=True)
+    for rays in (up, down):
+        a = ref.trace_closest(rays)
+        b = ref.trace_closest(rays, brute=True)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+        # any hit with tMax equal to the closest distance: the closest triangle itself occludes
+        hit = a[0] >= 0
+        occ = ref.trace_occluded(rays[hit], a[1][hit])
+        assert occ.all()
