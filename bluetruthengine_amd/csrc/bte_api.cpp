@@ -1503,6 +1503,51 @@ int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, floa
     return BTE_OK;
 }
 
+int bte_trace_rays_noalloc(const float* rays, const float* tmaxPerRay, long long n, float tmin, float tmax,
+                           int anyHit, int* tri, float* t, float* u, float* v) {
+    Context& c = ctx();
+    // no set-up here: the device, the scene and its BVH come from earlier calls
+    if (!c.deviceReady) return fail(BTE_ERR_STATE, "bte_trace_rays_noalloc: no device set up (upload a scene first)");
+    if (n <= 0) return BTE_OK;
+    if (!rays || !tri || (!anyHit && (!t || !u || !v)))
+        return fail(BTE_ERR_INVALID, "bte_trace_rays_noalloc: null buffer");
+    constexpr long long kMaxRays = (1ll << 31) - (1ll << 20);   // the work counter is a 32-bit int
+    if (n > kMaxRays)
+        return fail(BTE_ERR_INVALID, "bte_trace_rays_noalloc: %lld rays exceed the maximum of %lld per call", n,
+                    kMaxRays);
+    if (c.tris.empty()) return fail(BTE_ERR_STATE, "bte_trace_rays_noalloc: no triangles uploaded");
+    if (c.bvhDirty)
+        return fail(BTE_ERR_STATE, "bte_trace_rays_noalloc: the BVH is not built (call bte_build_accel first; "
+                                   "this entry point builds nothing)");
+    if (c.bvhDepth > noalloc_max_bvh_depth())
+        return fail(BTE_ERR_UNSUPPORTED, "bte_trace_rays_noalloc: BVH depth %d exceeds the %d levels its fixed "
+                                         "traversal stacks hold", c.bvhDepth, noalloc_max_bvh_depth());
+    const void* bufs[] = {rays, (anyHit && tmaxPerRay) ? tmaxPerRay : nullptr, tri, anyHit ? nullptr : t,
+                          anyHit ? nullptr : u, anyHit ? nullptr : v};
+    for (const void* b : bufs) {
+        if (!b) continue;
+        hipPointerAttribute_t at{};
+        if (hipPointerGetAttributes(&at, b) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(BTE_ERR_INVALID, "bte_trace_rays_noalloc: buffer %p is not device memory", b);
+        }
+        if ((at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged) || at.device != c.device)
+            return fail(BTE_ERR_INVALID, "bte_trace_rays_noalloc: buffer %p is not memory of device %d", b, c.device);
+    }
+    int* scr = nullptr;
+    const hipError_t e = noalloc_scratch(&scr);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(BTE_ERR_DEVICE, "bte_trace_rays_noalloc: scratch address lookup failed: %s", hipGetErrorString(e));
+    }
+    const DevScene sc = makeScene(c, std::max(1, c.sceneW), std::max(1, c.sceneH));
+    launch_trace_rays_noalloc(sc, rays, (anyHit && tmaxPerRay) ? tmaxPerRay : nullptr, n, tmin, tmax, anyHit, tri,
+                              t, u, v, scr, c.stream);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    HIPCHK(hipGetLastError());
+    return BTE_OK;
+}
+
 // ---- multi-scene batches -------------------------------------------------
 int bte_batch_add_scene(int width, int height) {
     ENSURE_DEVICE();

@@ -172,6 +172,18 @@ void launch_megakernel(const DevScene& sc, int spp, int maxDepth, int frameIndex
 void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxPerRay, long long n, float tmin,
                        float tmax, int anyHit, int* tri, float* t, float* u, float* v, int* ovfStack,
                        hipStream_t s);
+// Allocation-free form of launch_trace_rays: scratchPtr is the device
+// address of the library's static scratch array (noalloc_scratch), whose
+// first int is cleared here; the BVH2 must be at most noalloc_max_bvh_depth()
+// levels deep.
+void launch_trace_rays_noalloc(const DevScene& sc, const float* rays, const float* tmaxPerRay, long long n,
+                               float tmin, float tmax, int anyHit, int* tri, float* t, float* u, float* v,
+                               int* scratchPtr, hipStream_t s);
+// Device address of the static scratch array on the current device (looked up
+// once per device with hipGetSymbolAddress).
+hipError_t noalloc_scratch(int** ptr);
+size_t noalloc_scratch_bytes();
+int noalloc_max_bvh_depth();
 void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
                     int* margGuide, int* condGuide, int* monotone, hipStream_t s);
 // traversalBlocksPerCU 0 = as many as occupancy allows; grabRows 1, 2, 4 or 8 (else 4).

@@ -1806,12 +1806,15 @@ __global__ __launch_bounds__(kBlock) void k_megakernel(DevScene sc, int spp, int
     }
 }
 
+// OVF: overflow-stack entries per thread beyond the kLdsStack in LDS (the
+// BVH2 traversal holds at most one entry per tree level).
+template <int OVF>
 __global__ __launch_bounds__(kBlock) void k_trace_rays(DevScene sc, const float* rays, const float* tmaxPer,
                                                         long long n, float tmin, float tmax, int anyHit, int* tri,
                                                         float* tt, float* uu, float* vv, int* ovf, int* work) {
     __shared__ int lds[kLdsStack * kBlock];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
-    const StackRef<kLdsStack> stk{lds + threadIdx.x, ovf + (size_t)gtid * kOvfStack};
+    const StackRef<kLdsStack> stk{lds + threadIdx.x, ovf + (size_t)gtid * OVF};
     for (;;) {
         const int base = wave_grab(work);
         if (base >= n) break;
@@ -2084,8 +2087,51 @@ void launch_trace_rays(const DevScene& sc, const float* rays, const float* tmaxP
     int dev = 0;
     (void)hipGetDevice(&dev);
     const LaunchShape ls = query_launch_shape(dev, 32, 0, 4, 2);
-    hipLaunchKernelGGL(k_trace_rays, dim3(ls.auxBlocks), dim3(kBlock), 0, s, sc, rays, tmaxPerRay, n, tmin, tmax,
+    hipLaunchKernelGGL(k_trace_rays<kOvfStack>, dim3(ls.auxBlocks), dim3(kBlock), 0, s, sc, rays, tmaxPerRay, n, tmin, tmax,
                        anyHit, tri, t, u, v, ovf, work);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free ray tracing: the work counter and the traversal stacks'
+// overflow live in one static device array, `scratch`, of a fixed size that
+// every device holds for the life of the process.  Host code never passes the
+// array's name to a kernel (on the host it is a stand-in, not a device
+// address): the address comes from hipGetSymbolAddress, once per device.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocBlocks = 512;            // persistent grid (256 threads per block)
+constexpr int kNoAllocOvf = 32;                // overflow entries per thread beyond the 32 in LDS
+constexpr int kNoAllocHeader = 64;             // ints before the overflow area; int 0 = work counter
+constexpr size_t kNoAllocScratchInts = (size_t)kNoAllocHeader + (size_t)kNoAllocBlocks * kBlock * kNoAllocOvf;
+static_assert(kNoAllocScratchInts * sizeof(int) <= (16u << 20) + 256u, "scratch stays at 16 MiB + 256 B");
+__device__ __align__(256) int scratch[kNoAllocScratchInts];
+
+constexpr int kMaxScratchDevices = 64;
+static int* g_scratchDev[kMaxScratchDevices] = {};
+
+hipError_t noalloc_scratch(int** ptr) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    if (!g_scratchDev[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchDev[dev] = static_cast<int*>(p);
+    }
+    *ptr = g_scratchDev[dev];
+    return hipSuccess;
+}
+
+size_t noalloc_scratch_bytes() { return kNoAllocScratchInts * sizeof(int); }
+int noalloc_max_bvh_depth() { return kLdsStack + kNoAllocOvf; }
+
+void launch_trace_rays_noalloc(const DevScene& sc, const float* rays, const float* tmaxPerRay, long long n,
+                               float tmin, float tmax, int anyHit, int* tri, float* t, float* u, float* v,
+                               int* scratchPtr, hipStream_t s) {
+    launch_fill_i32(scratchPtr, 1, 0, s);   // work counter (a kernel, not a memset node)
+    hipLaunchKernelGGL(k_trace_rays<kNoAllocOvf>, dim3(kNoAllocBlocks), dim3(kBlock), 0, s, sc, rays, tmaxPerRay, n,
+                       tmin, tmax, anyHit, tri, t, u, v, scratchPtr + kNoAllocHeader, scratchPtr);
 }
 
 void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
@@ -2122,7 +2168,7 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
         occS = std::max(1, o);
         int m = 0, t = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_megakernel, kBlock, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&t, k_trace_rays, kBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&t, k_trace_rays<kOvfStack>, kBlock, 0);
         occMT = std::max(1, std::min(m, t));
         occL[0][0] = travOccupancy<8, 2>();
         occL[0][1] = travOccupancy<16, 2>();

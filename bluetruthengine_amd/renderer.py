@@ -115,6 +115,7 @@ API = {
     "bte_read_hit_capture": [_p, _sz],
     "bte_read_env_cdf": [_p, _p, _p],
     "bte_trace_rays": [_p, _p, _ll, _f, _f, _i, _p, _p, _p, _p],
+    "bte_trace_rays_noalloc": [_p, _p, _ll, _f, _f, _i, _p, _p, _p, _p],
     "bte_autotune": [_i, _i, _i, _p],
     "bte_enable_aov": [_i],
     "bte_scene_load_xml": [ctypes.c_char_p, ctypes.c_char_p, _p],

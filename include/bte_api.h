@@ -388,6 +388,21 @@ int bte_batch_autotune(int frames, double* msBest);
 int bte_trace_rays(const float* rays, const float* tmaxPerRay, long long n, float tmin, float tmax,
                    int anyHit, int* tri, float* t, float* u, float* v);
 
+/* No counterpart in the reference: bte_trace_rays for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every buffer is device memory of the
+ * current device, owned by the caller: rays (n x 6 floats), tmaxPerRay
+ * (optional, any hit), tri and, for closest hits, t, u, v.  It traces the
+ * current scene's BVH, which bte_build_accel must already have built (it
+ * builds, uploads and allocates nothing: no hipMalloc, hipFree or allocating
+ * library call).  Its work counter and traversal stacks live in one static
+ * device array of 16 MiB + 256 B that each GPU the library runs on holds
+ * until the process ends; calls that overlap on one GPU race on it.  Limits:
+ * n <= 2^31 - 2^20 rays per call and a BVH of at most 64 levels (the builder
+ * guarantees 60); beyond them it returns an error.  Returns after the work. */
+int bte_trace_rays_noalloc(const float* rays, const float* tmaxPerRay, long long n, float tmin, float tmax,
+                           int anyHit, int* tri, float* t, float* u, float* v);
+
 #ifdef __cplusplus
 }
 #endif
