@@ -33,55 +33,86 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
-        for name in ("ora_sin", "ora_cos", "ora_tan", "ora_exp", "ora_log", "ora_acos"):
-            getattr(L, name).restype = _f
-            getattr(L, name).argtypes = [_f]
-        L.ora_pow.restype = _f
-        L.ora_pow.argtypes = [_f, _f]
-        L.ora_atan2.restype = _f
-        L.ora_atan2.argtypes = [_f, _f]
-        L.ora_math_batch.argtypes = [_i, _p, _p, _p, _l]
-        L.ora_hash.restype = _u32
-        L.ora_hash.argtypes = [_u32]
-        L.ora_texture_sample.argtypes = [_p, _p, _p, _l, _p]
-        L.ora_texture_sample.restype = None
-        L.ora_rng_u32.argtypes = [_u32, _p, _i]
-        L.ora_rng_f01.argtypes = [_u32, _p, _i]
-        L.ora_pixel_seed.restype = _u32
-        L.ora_pixel_seed.argtypes = [_u32, _u32]
-        L.ora_bsdf_eval.argtypes = [_p, _p, _p, _p]
-        L.ora_bsdf_pdf.restype = _f
-        L.ora_bsdf_pdf.argtypes = [_p, _p, _p]
-        L.ora_bsdf_sample.argtypes = [_p, _p, _f, _f, _f, _p]
-        L.ora_bsdf_batch_sample.argtypes = [_p, _p, _p, _l, _p]
-        L.ora_bsdf_batch_eval.argtypes = [_p, _p, _p, _l, _p]
-        L.ora_fresnel_dielectric.restype = _f
-        L.ora_fresnel_dielectric.argtypes = [_f, _f, _f]
-        L.ora_frame.argtypes = [_p, _p, _p, _p]
-        L.ora_primary_ray.argtypes = [_p, _i, _i, _i, _i, _f, _f, _p]
-        L.ora_env_build_cdf.restype = _f
-        L.ora_env_build_cdf.argtypes = [_p, _i, _i, _i, _p, _p]
-        L.ora_scene_create.restype = _p
-        L.ora_scene_create.argtypes = [_p]
-        L.ora_scene_destroy.argtypes = [_p]
-        L.ora_trace_closest.argtypes = [_p, _p, _l, _f, _f, _i, _p, _p, _p, _p]
-        L.ora_trace_occluded.argtypes = [_p, _p, _p, _l, _p]
-        L.ora_env_eval.argtypes = [_p, _p, _p]
-        L.ora_env_pdf.restype = _f
-        L.ora_env_pdf.argtypes = [_p, _p]
-        L.ora_env_sample.argtypes = [_p, _f, _f, _p]
-        L.ora_dir_to_equirect.argtypes = [_p, _f, _p]
-        L.ora_sample_emitter.argtypes = [_p, _p, _f, _f, _p]
-        L.ora_render.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p, _p, _i]
-        L.ora_trace_samples.argtypes = [_p, _p, _l, _i, _i, _i, _i, _p, _p, _i, _p]
-        L.ora_render_megakernel.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p]
-        L.ora_render_aov.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
-        L.ora_pack_gamma.restype = _u32
-        L.ora_pack_gamma.argtypes = [_f, _f, _f]
-        L.ora_num_threads.restype = _i
-        _lib = L
+        _lib = _load(_LIB_PATH)
     return _lib
+
+
+class using_library:
+    """Context manager: every oracle call inside uses the library at `path`
+    (a copy built with a planted mistake, oracle/Makefile `planted`); objects
+    created inside must be released inside."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = _load(self.path)
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
+
+
+def build_planted():
+    subprocess.run(["make", "-s", "-j", "5", "-C", _HERE, "planted"], check=True)
+    return [os.path.join(_HERE, "_build", "planted", f"libbte_oracle_plant{k}.so") for k in range(1, 6)]
+
+
+def _load(path):
+    """Loads an oracle library and declares its entry points."""
+    L = ctypes.CDLL(path)
+    for name in ("ora_sin", "ora_cos", "ora_tan", "ora_exp", "ora_log", "ora_acos"):
+        getattr(L, name).restype = _f
+        getattr(L, name).argtypes = [_f]
+    L.ora_pow.restype = _f
+    L.ora_pow.argtypes = [_f, _f]
+    L.ora_atan2.restype = _f
+    L.ora_atan2.argtypes = [_f, _f]
+    L.ora_math_batch.argtypes = [_i, _p, _p, _p, _l]
+    L.ora_hash.restype = _u32
+    L.ora_hash.argtypes = [_u32]
+    L.ora_texture_sample.argtypes = [_p, _p, _p, _l, _p]
+    L.ora_texture_sample.restype = None
+    L.ora_rng_u32.argtypes = [_u32, _p, _i]
+    L.ora_rng_f01.argtypes = [_u32, _p, _i]
+    L.ora_pixel_seed.restype = _u32
+    L.ora_pixel_seed.argtypes = [_u32, _u32]
+    L.ora_bsdf_eval.argtypes = [_p, _p, _p, _p]
+    L.ora_bsdf_pdf.restype = _f
+    L.ora_bsdf_pdf.argtypes = [_p, _p, _p]
+    L.ora_bsdf_sample.argtypes = [_p, _p, _f, _f, _f, _p]
+    L.ora_bsdf_batch_sample.argtypes = [_p, _p, _p, _l, _p]
+    L.ora_bsdf_batch_eval.argtypes = [_p, _p, _p, _l, _p]
+    L.ora_fresnel_dielectric.restype = _f
+    L.ora_fresnel_dielectric.argtypes = [_f, _f, _f]
+    L.ora_frame.argtypes = [_p, _p, _p, _p]
+    L.ora_primary_ray.argtypes = [_p, _i, _i, _i, _i, _f, _f, _p]
+    L.ora_env_build_cdf.restype = _f
+    L.ora_env_build_cdf.argtypes = [_p, _i, _i, _i, _p, _p]
+    L.ora_scene_create.restype = _p
+    L.ora_scene_create.argtypes = [_p]
+    L.ora_scene_destroy.argtypes = [_p]
+    L.ora_trace_closest.argtypes = [_p, _p, _l, _f, _f, _i, _p, _p, _p, _p]
+    L.ora_trace_occluded.argtypes = [_p, _p, _p, _l, _p]
+    L.ora_env_eval.argtypes = [_p, _p, _p]
+    L.ora_env_pdf.restype = _f
+    L.ora_env_pdf.argtypes = [_p, _p]
+    L.ora_env_sample.argtypes = [_p, _f, _f, _p]
+    L.ora_dir_to_equirect.argtypes = [_p, _f, _p]
+    L.ora_sample_emitter.argtypes = [_p, _p, _f, _f, _p]
+    L.ora_render.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p, _p, _i]
+    L.ora_trace_samples.argtypes = [_p, _p, _l, _i, _i, _i, _i, _p, _p, _i, _p]
+    L.ora_render_megakernel.argtypes = [_p, _i, _i, _i, _i, _i, _i, _p]
+    L.ora_render_aov.argtypes = [_p, _i, _i, _i, _i, _i, _p, _p]
+    L.ora_pack_gamma.restype = _u32
+    L.ora_pack_gamma.argtypes = [_f, _f, _f]
+    L.ora_num_threads.restype = _i
+    L.ora_spot_nee.argtypes = [_p, _p, _p, _p, _p, _p, _i, _p]
+    return L
 
 
 def _ptr(a):
@@ -304,6 +335,15 @@ class OracleScene:
         out = np.empty(11, np.float32)
         lib().ora_sample_emitter(self.handle, _ptr(o), u0, u1, _ptr(out))
         return out
+
+    def spot_nee(self, p, n, d_in, thr, bsdf, index):
+        """Spotlight NEE record: (ok, direction (3), tMax, contribution (3))."""
+        f = lambda a: np.ascontiguousarray(a, np.float32)
+        b = np.ascontiguousarray(np.asarray(bsdf, pod.BSDF).reshape(()))
+        out = np.empty(8, np.float32)
+        args = [f(p), f(n), f(d_in), f(thr)]
+        lib().ora_spot_nee(self.handle, *[_ptr(a) for a in args], _ptr(b), int(index), _ptr(out))
+        return bool(out[0]), out[1:4], out[4], out[5:8]
 
     def render(self, W, H, frame0, nframes, max_depth, accum=None, sum64=None, rgba8=False, threads=0):
         if accum is None:

@@ -277,6 +277,15 @@ static inline F3 sampleMicrofacet(const OraBsdf& b, BRec& r, float u1, float u2)
     return mul3(evalMicrofacet(b, r), r.wo.z / p);
 }
 
+// ORA_PLANT (test infrastructure): a nonzero value builds a copy of the oracle
+// with one deliberate mistake (1 sheen x2, 2 subsurface clamp dropped, 3 eta^2
+// dropped from the rough-glass Jacobian, 4 emitter u0 not rescaled, 5 linear
+// spotlight falloff), so the tests can show that their closed-form pins fail
+// on it.  The oracle proper is built with ORA_PLANT = 0.
+#ifndef ORA_PLANT
+#define ORA_PLANT 0
+#endif
+
 // ---- Disney (rt_bsdf_disney.cuh:16-892)
 // p0.xyz base color, p0.w roughness; p1 metallic, specular, specTrans, specTint;
 // p2 sheen, sheenTint, subsurface, anisotropic; p3 clearcoat, clearcoatGloss, eta.
@@ -297,7 +306,11 @@ static inline F3 evalDiffuseLobe(const OraBsdf& b, const BRec& r) {  // :60-102
     const float ssNinety = rough * hDotWo * hDotWo;
     const F3 colorOverPi = div3(dBase(b), RT_PI);
     const F3 baseDiffuse = mul3(colorOverPi, diffuseFresnel(ci, dfNinety) * diffuseFresnel(co, dfNinety) * co);
+#if ORA_PLANT == 2
+    const float ssDenom = ci + co;                      // planted: clamp dropped
+#else
     const float ssDenom = std::fmax(1e-2f, ci + co);
+#endif
     const F3 subsurface = mul3(colorOverPi,
         1.25f * (diffuseFresnel(ci, ssNinety) * diffuseFresnel(co, ssNinety) * ((1.0f / ssDenom) - 0.5f) + 0.5f) * co);
     const float ss = b.p2.z;
@@ -512,7 +525,11 @@ static inline float pdfGlassLobe(const OraBsdf& b, const BRec& r) {  // :536-583
     const float commonPDF = (dg * gg * std::fabs(hDotWi)) / std::fabs(ci);
     if (inNout > 0.0f) return (commonPDF * fg) / (4.0f * std::fabs(hDotWo));
     const float q = hDotWi + (eta * hDotWo);
+#if ORA_PLANT == 3
+    const float jacobian = std::fabs(hDotWo) / std::fmax(q * q, 1e-4f);   // planted: eta^2 dropped
+#else
     const float jacobian = (eta * eta * std::fabs(hDotWo)) / std::fmax(q * q, 1e-4f);
+#endif
     return (1.0f - fg) * commonPDF * jacobian;
 }
 static inline void sampleGlassLobe(const OraBsdf& b, BRec& r, float u1, float u2) {  // :586-674
@@ -551,7 +568,11 @@ static inline F3 evalSheenLobe(const OraBsdf& b, const BRec& r) {   // :679-711
     const F3 sheen = {(1.0f - sheenTint) + (sheenTint * tint.x),
                       (1.0f - sheenTint) + (sheenTint * tint.y),
                       (1.0f - sheenTint) + (sheenTint * tint.z)};
+#if ORA_PLANT == 1
+    return mul3(sheen, 2.0f * pow5(1.0f - hDotWo) * co);   // planted: sheen x2
+#else
     return mul3(sheen, pow5(1.0f - hDotWo) * co);
+#endif
 }
 static inline F3 evalDisney(const OraBsdf& b, const BRec& r) {      // :758-787
     const float specT = b.p1.z;
@@ -1018,7 +1039,9 @@ static inline int sampleCdfReuse(const float* cdf, int count, float& u, float& o
     const float c0 = cdf[idx], c1 = cdf[idx + 1];
     const float denom = c1 - c0;
     if (denom <= 0.0f) { u = 0.0f; outPdf = 0.0f; return idx; }
-    u = (x - c0) / denom;
+#if ORA_PLANT != 4
+    u = (x - c0) / denom;                               // (planted 4: u0 not rescaled)
+#endif
     outPdf = denom / sum;
     return idx;
 }
@@ -1155,7 +1178,11 @@ static inline bool prepareSpotNEE(const Scene& s, const PathState& ps, const Its
         spotLe = mul3(spot.radiance, spot.intensity / sqDist);
     } else {
         const float falloff = (lightCos - spot.outerConeCosine) / (spot.innerConeCosine - spot.outerConeCosine);
+#if ORA_PLANT == 5
+        const float sm = falloff;                       // planted: linear falloff
+#else
         const float sm = (falloff * falloff) * (3.0f - (2.0f * falloff));
+#endif
         spotLe = mul3(spot.radiance, (spot.intensity * sm) / sqDist);
     }
     BRec br;
@@ -1603,6 +1630,25 @@ void ora_sample_emitter(void* p, const float* origin, float u0, float u1, float*
     out11[0] = q.hitPoint.x; out11[1] = q.hitPoint.y; out11[2] = q.hitPoint.z;
     out11[3] = q.hitNormal.x; out11[4] = q.hitNormal.y; out11[5] = q.hitNormal.z;
     out11[6] = q.pdf; out11[7] = q.emitterPdf; out11[8] = le.x; out11[9] = le.y; out11[10] = le.z;
+}
+
+// Spotlight NEE record (prepareSpotlightNEE) at point p3 with shading normal
+// n3 for a path arriving along dIn3 with throughput thr3:
+// out8 = [ok, direction3, tMax, contribution3].
+void ora_spot_nee(void* p, const float* p3, const float* n3, const float* dIn3, const float* thr3,
+                  const OraBsdf* bsdf, int si, float* out8) {
+    const Scene& s = *static_cast<Scene*>(p);
+    PathState ps{};
+    ps.d = {dIn3[0], dIn3[1], dIn3[2]};
+    ps.throughput = {thr3[0], thr3[1], thr3[2]};
+    Its its{};
+    its.p = {p3[0], p3[1], p3[2]};
+    its.n = {n3[0], n3[1], n3[2]};
+    Shadow sh{};
+    const bool ok = si >= 0 && si < s.d.spotlightCount && prepareSpotNEE(s, ps, its, *bsdf, si, sh);
+    out8[0] = ok ? 1.0f : 0.0f;
+    out8[1] = sh.d.x; out8[2] = sh.d.y; out8[3] = sh.d.z; out8[4] = sh.tMax;
+    out8[5] = sh.contrib.x; out8[6] = sh.contrib.y; out8[7] = sh.contrib.z;
 }
 
 // Render frames [frame0, frame0+nframes) into accum (W*H*3, binary32 Welford

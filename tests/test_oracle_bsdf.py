@@ -13,9 +13,9 @@ Closed forms used:
     Disney eval returns f*cos_o (rt_bsdf_disney.cuh:239).
   * Disney clearcoat = GTR1 (Burley 2012) with alpha = lerp(0.1, 0.001, gloss),
     Fresnel F0 = 0.04, G with alpha 0.25.
-Remaining Disney numerics (diffuse/subsurface/sheen clamps, rough-glass lobe)
-are pinned only by the sample/eval/pdf identities below (parity vs the
-reference's own binary unpinned: the reference ships no vectors).
+The Disney diffuse, subsurface and sheen lobes and the rough-glass lobe are
+pinned in closed form in tests/test_oracle_closed_forms.py; here every BSDF
+also meets the sample/eval/pdf identities below.
 """
 import numpy as np
 import pytest
