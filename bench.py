@@ -7,12 +7,18 @@ the gamma-packed framebuffer.  Scene generation, BVH build and CDF build happen
 before the timed region (inputs resident in HBM).
 
 The JSON line also carries
-  roofline     : for the dominant kernel (BVH2 closest-hit `extend`), algorithmic
-                 bytes per launch / its average launch duration measured with HIP
-                 events on the library's stream (instrumented run outside the
-                 timed region), against MI355X's 8 TB/s HBM peak;
-  cpu_baseline : the CPU oracle (same estimator, binary32, OpenMP over all host
-                 threads) on a bounded random subset of the same frame's pixels.
+  roofline       : for the dominant kernel (the BVH4 closest-hit pass), the HBM
+                   bytes one launch moves (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+                   copy-calibrated, from profiles/) over its average launch
+                   duration measured here with HIP events on the library's
+                   stream, against MI355X's 8 TB/s; beside it the bytes the
+                   traversal requests and its measured lane utilisation;
+  cpu_baseline   : the CPU oracle (same estimator, binary32, OpenMP over all
+                   host threads) on a bounded random subset of the frame's pixels;
+  time_to_relmse : relMSE <= 0.01 against a 65,536-spp binary64 reference
+                   rendered in this run: plain spp (doubling up to 16,384 spp;
+                   a plain side that does not get there is a lower bound) vs
+                   4 spp + the learned denoiser.
 """
 import argparse
 import json
@@ -37,6 +43,18 @@ TRI_BYTES = 48
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def profile_value(name, key):
+    """A `key: value` line of a committed rocprofv3 summary under profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith(key + ":"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(sd, seconds):
@@ -138,91 +156,94 @@ def denoiser_bench(args):
     print(json.dumps(out), flush=True)
 
 
-def relmse_bench(args):
+def measure_relmse(r, sd, ref_spp, plain_max_spp, weight_files=None, windows=8, ref_cache="", skip_plain=False):
     """Time to relMSE <= 0.01 on the 1080p env-lit frame: plain path tracing
-    (smallest spp that reaches it) against 4 spp + the learned denoiser.  The
-    reference is the same estimator at args.ref_spp samples per pixel summed in
-    binary64, from frames 2^24 on (independent of the test renders' frames)."""
-    from bluetruthengine_amd import denoiser, renderer, scenes
-    sd = scenes.envlit_c3(spp=16)
+    (smallest spp that reaches it, if it does by plain_max_spp) against 4 spp
+    + the learned denoiser.  The reference is the same estimator at ref_spp
+    samples per pixel summed in binary64, from frames 2^24 on (a sample stream
+    independent of the test renders' frames 0..).  r is a Renderer of sd (its
+    launch shape already chosen); its accumulation is reset afterwards."""
+    from bluetruthengine_amd import denoiser
     W, H = sd.width, sd.height
-    r = renderer.Renderer(sd)
-    r.build_accel()
-    r.autotune(16)
+    t_all = time.perf_counter()
+    r.enable_aov(False)
     r.enable_accum64(True)
-    r.set_frame_index(1 << 24)          # reference samples independent of the test renders (frames 0..)
+    r.reset()
+    r.set_frame_index(1 << 24)
     ref = None
-    if args.ref_cache and os.path.exists(args.ref_cache):
-        with np.load(args.ref_cache) as z:
-            if int(z["spp"]) == args.ref_spp and z["ref"].shape == (H, W, 3):
+    if ref_cache and os.path.exists(ref_cache):
+        with np.load(ref_cache) as z:
+            if int(z["spp"]) == ref_spp and z["ref"].shape == (H, W, 3):
                 ref = z["ref"].astype(np.float64)
-                log(f"reference: {args.ref_spp} spp read from {args.ref_cache}")
+                log(f"reference: {ref_spp} spp read from {ref_cache}")
     if ref is None:
         t0 = time.perf_counter()
-        chunk = 256
         done = 0
-        while done < args.ref_spp:
-            n = min(chunk, args.ref_spp - done)
+        while done < ref_spp:
+            n = min(256, ref_spp - done)
             r.render(n)
             done += n
-            if done % 4096 == 0 or done == args.ref_spp:
-                log(f"reference: {done}/{args.ref_spp} spp, {time.perf_counter() - t0:.0f} s")
-        ref = r.accum64() / float(args.ref_spp)
-        if args.ref_cache:
-            np.savez(args.ref_cache, ref=ref.astype(np.float32), spp=np.int64(args.ref_spp))
+            if done % 4096 == 0 or done == ref_spp:
+                log(f"reference: {done}/{ref_spp} spp, {time.perf_counter() - t0:.0f} s")
+        ref = r.accum64() / float(ref_spp)
+        if ref_cache:
+            np.savez(ref_cache, ref=ref.astype(np.float32), spp=np.int64(ref_spp))
             ref = ref.astype(np.float32).astype(np.float64)   # same values as a later read of the file
+    ref_s = time.perf_counter() - t_all
     r.enable_accum64(False)
 
     # One progressive pass (a render of n spp equals the first n frames of a
-    # longer one): relMSE at powers of two until it reaches 0.01, then a second
-    # pass re-accumulates to the last power of two below and checks every 1/64
-    # of the final interval.
+    # longer one): relMSE at powers of two until it reaches 0.01 or
+    # plain_max_spp; when reached, a second pass re-accumulates to the power
+    # of two below and checks every 1/64 of the final interval.
     curve = {}
-    r.reset()
-    spp, done = 1, 0
-    while not args.skip_plain:
-        r.render(spp - done)
-        done = spp
-        curve[spp] = relmse(r.accum(), ref)
-        log(f"plain {spp} spp: relMSE {curve[spp]:.5f}")
-        if curve[spp] <= 0.01 or spp >= args.ref_spp // 4:
-            break
-        spp *= 2
-    plain_spp = spp
-    if not args.skip_plain and spp > 1 and curve[spp] <= 0.01:
-        lo = spp // 2
-        step = max(1, (spp - lo) // 64)
+    reached = False
+    plain_spp = None
+    if not skip_plain:
         r.reset()
-        r.render(lo)
-        done = lo
-        while done + step < spp:
-            r.render(step)
-            done += step
-            curve[done] = relmse(r.accum(), ref)
-            log(f"plain {done} spp: relMSE {curve[done]:.5f}")
-            if curve[done] <= 0.01:
-                plain_spp = done
+        spp, done = 1, 0
+        while True:
+            r.render(spp - done)
+            done = spp
+            curve[spp] = relmse(r.accum(), ref)
+            log(f"plain {spp} spp: relMSE {curve[spp]:.5f}")
+            if curve[spp] <= 0.01 or spp * 2 > plain_max_spp:
                 break
-        log(f"plain: {plain_spp} spp reach relMSE {curve[plain_spp]:.5f}")
+            spp *= 2
+        plain_spp = spp
+        reached = curve[spp] <= 0.01
+        if reached and spp > 1:
+            lo = spp // 2
+            step = max(1, (spp - lo) // 64)
+            r.reset()
+            r.render(lo)
+            done = lo
+            while done + step < spp:
+                r.render(step)
+                done += step
+                curve[done] = relmse(r.accum(), ref)
+                if curve[done] <= 0.01:
+                    plain_spp = done
+                    break
+        log(f"plain: {plain_spp} spp, relMSE {curve[plain_spp]:.5f} ({'reached' if reached else 'not reached'})")
 
     # render time is linear in spp (passes of frames in flight): time a
-    # bounded render and scale it to plain_spp
-    timed_spp = min(plain_spp, 1024)
-
-    def timed_plain():
-        r.reset()
-        t = time.perf_counter()
-        r.render(timed_spp)
-        dt = time.perf_counter() - t
-        log(f"plain timing: {timed_spp} spp in {dt * 1e3:.1f} ms")
-        return dt
+    # bounded render and scale it
     plain_ms = None
-    if not args.skip_plain:
+    if plain_spp:
+        timed_spp = min(plain_spp, 1024)
+
+        def timed_plain():
+            r.reset()
+            t = time.perf_counter()
+            r.render(timed_spp)
+            return time.perf_counter() - t
         timed_plain()
         plain_ms = min(timed_plain() for _ in range(2)) * 1e3 * plain_spp / timed_spp
 
+    # 4 spp + denoiser
     r.enable_aov(True)
-    weight_files = [w for w in args.weights.split(",") if w] or [denoiser.WEIGHTS_PATH]
+    weight_files = weight_files or [denoiser.WEIGHTS_PATH]
 
     def timed_dn(spp):
         r.reset()
@@ -230,9 +251,6 @@ def relmse_bench(args):
         r.render(spp)
         denoiser.denoise()
         return time.perf_counter() - t
-
-    # the denoised curve (4 spp is the metric; weights trained on several
-    # input spp improve with it) for every weight file
     curves, noisy_curve, dn4 = {}, {}, {}
     for wf in weight_files:
         denoiser.load(wf)
@@ -251,50 +269,35 @@ def relmse_bench(args):
             log(f"{os.path.basename(wf)}: denoised {spp} spp: relMSE {cur[spp]:.5f} (input {noisy_curve[spp]:.5f})")
     compared = {wf: curves[wf][4] for wf in weight_files}
     best = min(compared, key=compared.get)
-    dn4_img = dn4[best]
-    trained = True
     dn_curve = curves[best]
     denoiser.load(best)
     timed_dn(4)
     dn_ms = min(timed_dn(4) for _ in range(3)) * 1e3
-    # smallest spp at which some weight file reaches 0.01
-    reach, reach_wf = [], None
-    for spp in (4, 8, 16, 32, 64):
-        ok = [wf for wf in weight_files if curves[wf][spp] <= 0.01]
-        if ok:
-            reach, reach_wf = [spp], min(ok, key=lambda w: curves[w][spp])
-            break
-    dn_reach_ms = None
-    if reach:
-        denoiser.load(reach_wf)
-        dn_reach_ms = min(timed_dn(reach[0]) for _ in range(2)) * 1e3
-        denoiser.load(best)
-    out = {"metric": "time to relMSE<=0.01, 1080p env-lit", "reference_spp": args.ref_spp,
-           "plain": None if args.skip_plain else {
-               "spp": plain_spp, "relmse": round(curve[plain_spp], 5), "ms": round(plain_ms, 2),
-               "ms_from": f"{timed_spp}-spp render x {plain_spp / timed_spp:g}"},
-           "denoised_4spp": {"relmse": round(dn_curve[4], 5), "ms": round(dn_ms, 2),
-                             "input_relmse": round(noisy_curve[4], 5),
-                             "weights": os.path.basename(best) if trained else "untrained (random init)",
-                             "weights_compared": {os.path.basename(k): round(v, 5) for k, v in compared.items()}},
-           "speedup_4spp": round(plain_ms / dn_ms, 2) if (plain_ms and dn_curve[4] <= 0.01) else None,
-           "denoised_min_spp_reaching": reach[0] if reach else None,
-           "denoised_reach_weights": os.path.basename(reach_wf) if reach_wf else None,
-           "denoised_curves": {os.path.basename(w): {str(k): round(v, 5) for k, v in sorted(c.items())}
-                               for w, c in curves.items()},
-           "denoised_reach_ms": round(dn_reach_ms, 2) if dn_reach_ms else None,
-           "speedup_at_reach": round(plain_ms / dn_reach_ms, 2) if (plain_ms and dn_reach_ms) else None,
+    dn_ok = dn_curve[4] <= 0.01
+    out = {"reference_spp": ref_spp, "reference_s": round(ref_s, 1),
+           "plain": None if skip_plain else {
+               "reached": reached, "spp": plain_spp, "relmse": round(curve[plain_spp], 5), "ms": round(plain_ms, 2),
+               "ms_is_lower_bound": not reached, "max_spp_tried": plain_max_spp,
+               "ms_from": f"{min(plain_spp, 1024)}-spp render x {plain_spp / min(plain_spp, 1024):g}"},
+           "denoised_4spp": {"relmse": round(dn_curve[4], 5), "ms": round(dn_ms, 2), "reached": dn_ok,
+                             "input_relmse": round(noisy_curve[4], 5), "weights": os.path.basename(best),
+                             "flips": denoiser.get_flips()},
            "target_speedup": 5.0,
            "plain_curve": {str(k): round(v, 5) for k, v in sorted(curve.items())},
            "denoised_curve": {str(k): round(v, 5) for k, v in sorted(dn_curve.items())},
            "input_curve": {str(k): round(v, 5) for k, v in sorted(noisy_curve.items())}}
-    out["denoised_4spp"]["error_by_reference_luminance"] = relmse_by_luminance(dn4_img, ref)
-    out["denoised_4spp"]["flips"] = denoiser.get_flips()
-    if args.windows > 0:
-        out["denoised_4spp"]["windows"] = denoise_windows(r, denoiser, ref, args.windows, W, H)
+    if plain_ms and dn_ok:
+        out["speedup_4spp" if reached else "speedup_4spp_lower_bound"] = round(plain_ms / dn_ms, 2)
+    if len(weight_files) > 1:
+        out["denoised_4spp"]["weights_compared"] = {os.path.basename(k): round(v, 5) for k, v in compared.items()}
+        out["denoised_curves"] = {os.path.basename(w): {str(k): round(v, 5) for k, v in sorted(c.items())}
+                                  for w, c in curves.items()}
+    out["denoised_4spp"]["error_by_reference_luminance"] = relmse_by_luminance(dn4[best], ref)
+    if windows > 0:
+        out["denoised_4spp"]["windows"] = denoise_windows(r, denoiser, ref, windows, W, H)
     # relMSE(N) = a / N + b for independent samples: b is the reference's own
     # noise, a / 0.01 the spp a noise-free reference would need
-    ks = sorted(k for k in curve if k >= 1024)
+    ks = sorted(k for k in curve if k >= 1024 and (k & (k - 1)) == 0)
     if len(ks) >= 2 and plain_ms:
         n1, n2 = ks[-2], ks[-1]
         a = (curve[n1] - curve[n2]) / (1.0 / n1 - 1.0 / n2)
@@ -307,8 +310,27 @@ def relmse_bench(args):
             fit["plain_ms_needed_vs_this_reference"] = round(need * ms_per_spp, 1)
         fit["plain_spp_needed_noise_free_reference"] = int(np.ceil(a / 0.01))
         fit["plain_ms_needed_noise_free_reference"] = round(a / 0.01 * ms_per_spp, 1)
-        fit["denoised_4spp_relmse_minus_b"] = round(dn_curve[4] - b, 5)
+        if dn_ok:
+            fit["speedup_4spp_vs_extrapolated_plain"] = round(a / 0.01 * ms_per_spp / dn_ms, 1)
         out["extrapolation"] = fit
+    r.enable_aov(False)
+    r.reset()
+    out["wall_s"] = round(time.perf_counter() - t_all, 1)
+    return out
+
+
+def relmse_bench(args):
+    """--relmse: the time-to-relMSE measurement alone, with a larger plain
+    budget and optionally several weight files."""
+    from bluetruthengine_amd import renderer, scenes
+    sd = scenes.envlit_c3(spp=16)
+    r = renderer.Renderer(sd)
+    r.build_accel()
+    r.autotune(16)
+    weight_files = [w for w in args.weights.split(",") if w] or None
+    out = measure_relmse(r, sd, args.ref_spp, args.plain_max_spp or args.ref_spp // 2, weight_files, args.windows,
+                         args.ref_cache, args.skip_plain)
+    out["metric"] = "time to relMSE<=0.01, 1080p env-lit"
     print(json.dumps(out), flush=True)
 
 
@@ -469,7 +491,10 @@ def main():
     ap.add_argument("--denoiser", action="store_true", help="only time the U-Net denoiser (inference, training)")
     ap.add_argument("--relmse", action="store_true",
                     help="only measure time to relMSE<=0.01: plain spp vs 4 spp + denoiser")
-    ap.add_argument("--ref-spp", type=int, default=65536, help="samples per pixel of the --relmse reference")
+    ap.add_argument("--ref-spp", type=int, default=65536, help="samples per pixel of the relMSE reference")
+    ap.add_argument("--plain-max-spp", type=int, default=0,
+                    help="largest plain spp tried for relMSE <= 0.01 (0: 16384 in the default run, ref-spp/2 with --relmse)")
+    ap.add_argument("--no-relmse", action="store_true", help="skip the time-to-relMSE measurement")
     ap.add_argument("--skip-plain", action="store_true", help="--relmse: only the denoised side")
     ap.add_argument("--windows", type=int, default=8,
                     help="--relmse: also denoise this many independent 4-spp renders (0 = off)")
@@ -555,21 +580,29 @@ def main():
         launches = max(1, s1.extendLaunches)
         avg_ms = s1.extendMs / launches
         node_bytes = NODE_BYTES * (cfg["bvhWidth"] // 2 if cfg["bvhWidth"] in (4, 8) else 1)
-        bytes_total = EXTEND_RAY_BYTES * s2.extensionRays + node_bytes * s2.extendNodes + TRI_BYTES * s2.extendTris
-        bytes_per_launch = bytes_total / launches
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        traffic = None
-        tp = os.path.join(ROOT, "profiles", "extend_c3_hbm_traffic.txt")
-        if os.path.exists(tp):
-            with open(tp) as fh:
-                for line in fh:
-                    if line.startswith("bytes_per_launch:"):
-                        traffic = int(line.split(":")[1])
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+        request_total = EXTEND_RAY_BYTES * s2.extensionRays + node_bytes * s2.extendNodes + TRI_BYTES * s2.extendTris
+        request_per_launch = request_total / launches
+        # bytes that cross HBM per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE of
+        # this kernel on this workload, calibrated with a copy of known size
+        traffic = profile_value("extend_c3_hbm_traffic.txt", "bytes_per_launch")
+        achieved = traffic / (avg_ms * 1e-3) / 1e9 if traffic else None
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBPS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4) if achieved else None,
+                    "traffic": int(traffic) if traffic else None,
+                    "traffic_source": "profiles/extend_c3_hbm_traffic.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE per "
+                                      "launch, copy-calibrated)",
                     "kernel": (f"k_extend_pool (BVH{cfg['bvhWidth']} closest hit)" if cfg["bvhWidth"] in (4, 8)
-                              else "k_extend (BVH2 closest hit)"), "launches": launches,
-                    "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+                               else "k_extend (BVH2 closest hit)"), "launches": launches,
+                    "avg_launch_ms": round(avg_ms, 4),
+                    "request_bytes_per_launch": int(request_per_launch),
+                    "request_GBps": round(request_per_launch / (avg_ms * 1e-3) / 1e9, 1),
+                    "request_note": f"bytes the traversal asks the memory system for ({node_bytes} B per node visit, "
+                                    f"{TRI_BYTES} B per triangle test, {EXTEND_RAY_BYTES} B ray + hit); the L2 and "
+                                    "MALL serve all but the traffic above",
+                    "valu_utilization_pct": profile_value("c3_occupancy_divergence.md", "extend_valu_utilization"),
+                    "waves_per_cu": profile_value("c3_occupancy_divergence.md", "extend_waves_per_cu"),
+                    "limiter": "vector-ALU issue of divergent traversal waves (lanes busy per VALU instruction "
+                               "above), not HBM bandwidth"}
         kernel = {"extend_ms": round(s1.extendMs, 2), "shade_ms": round(s1.shadeMs, 2),
                   "connect_ms": round(s1.connectMs, 2), "nodes_per_ray": round(s2.extendNodes / max(1, s2.extensionRays), 2),
                   "tris_per_ray": round(s2.extendTris / max(1, s2.extensionRays), 2),
@@ -580,16 +613,11 @@ def main():
         cpu = cpu_baseline(sd, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
 
-    # second half of the metric (time to relMSE <= 0.01) from the committed
-    # --relmse measurement, which needs a 131072-spp reference (minutes)
+    # second half of the metric: time to relMSE <= 0.01, measured in this run
     relmse_rec = None
-    rp = os.path.join(ROOT, "profiles", "relmse_c3.json")
-    if os.path.exists(rp):
-        with open(rp) as fh:
-            rec = json.load(fh)
-        relmse_rec = {k: rec.get(k) for k in ("reference_spp", "plain", "denoised_4spp", "speedup_4spp",
-                                              "extrapolation")}
-        relmse_rec["source"] = "profiles/relmse_c3.json (bench.py --relmse --ref-spp 131072)"
+    if not args.no_relmse:
+        relmse_rec = measure_relmse(r, sd, args.ref_spp, args.plain_max_spp or 16384, windows=args.windows)
+        log(f"time to relMSE: {json.dumps({k: relmse_rec.get(k) for k in ('plain', 'speedup_4spp', 'speedup_4spp_lower_bound')})}")
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 2), "higher_is_better": True,

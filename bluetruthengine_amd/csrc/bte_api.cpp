@@ -106,7 +106,8 @@ struct Context : SceneData {
     int mode = BTE_MODE_WAVEFRONT;
 
     // wavefront buffers
-    DBuf dRayO[2], dRayD[2], dThr[2], dRad[2], dSt[2], dHit, dCatQ, dCat, dFinal, dShO, dShD, dShC, dCounters, dHitCap, dOvf;
+    DBuf dRayO[2], dRayD[2], dThr[2], dRad[2], dSt[2], dHit, dCatQ, dCat, dFinal, dShO, dShD, dShC, dShX, dCounters,
+        dHitCap, dOvf;
     DBuf dFrameBase, dWork;
     bool aov = false;                 // albedo / normal AOVs of the first non-delta hit
     DBuf dAovAlb, dAovNrm, dAovAccAlb, dAovAccNrm;
@@ -298,22 +299,21 @@ int resetAccum(int w, int h) {
     return BTE_OK;
 }
 
-int kindsInUse(int& kindArea, int& kindSpot0, int& kindEnv) {
+int kindsInUse(int& kindArea, int& kindSpot, int& kindEnv) {
     Context& c = ctx();
     int k = 0;
     kindArea = c.emitterCount > 0 ? k++ : -1;
-    kindSpot0 = k;
-    k += c.spotCount;
+    kindSpot = c.spotCount > 0 ? k++ : -1;
     kindEnv = c.hasEnv ? k++ : -1;
     return k;
 }
 
-size_t bytesPerPath(int kinds) {
+size_t bytesPerPath(int kinds, bool spots) {
     return 2 * (4 * sizeof(float4) + sizeof(uint2)) + 2 * sizeof(float4) + kNumCats * sizeof(int) + sizeof(uint32_t) +
-           (size_t)kinds * 3 * sizeof(float4) + (ctx().aov ? 2 * sizeof(float4) : 0);
+           (size_t)kinds * 3 * sizeof(float4) + (spots ? sizeof(float4) : 0) + (ctx().aov ? 2 * sizeof(float4) : 0);
 }
 
-int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
+int ensureWaveBuffers(long long cap, int kinds, bool spots, int maxDepth, WaveBuffers& wb) {
     Context& c = ctx();
     const size_t P = (size_t)cap;
     for (int i = 0; i < 2; ++i) {
@@ -331,6 +331,7 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     CHECK_RC(c.dShO.ensure(K * P * sizeof(float4)));
     CHECK_RC(c.dShD.ensure(K * P * sizeof(float4)));
     CHECK_RC(c.dShC.ensure(K * P * sizeof(float4)));
+    if (spots) CHECK_RC(c.dShX.ensure(P * sizeof(float4)));
     CHECK_RC(c.dCounters.ensure((size_t)kCtrStride * (size_t)(maxDepth + 1) * sizeof(int)));
     CHECK_RC(c.dWork.ensure((size_t)kWorkStride * (size_t)(maxDepth + 1) * sizeof(int)));
     CHECK_RC(c.dFrameBase.ensure(sizeof(int)));
@@ -356,6 +357,7 @@ int ensureWaveBuffers(long long cap, int kinds, int maxDepth, WaveBuffers& wb) {
     wb.shO = c.dShO.as<float4>();
     wb.shD = c.dShD.as<float4>();
     wb.shC = c.dShC.as<float4>();
+    wb.shX = spots ? c.dShX.as<float4>() : nullptr;
     wb.counters = c.dCounters.as<int>();
     wb.work = c.dWork.as<int>();
     wb.frameBase = c.dFrameBase.as<int>();
@@ -435,13 +437,13 @@ int runSegment(bool useGraph, const std::vector<unsigned char>& key, F&& emit) {
 int renderWavefront(int nframes) {
     Context& c = ctx();
     const int N = c.accW * c.accH;
-    int kindArea, kindSpot0, kindEnv;
-    const int kinds = kindsInUse(kindArea, kindSpot0, kindEnv);
+    int kindArea, kindSpot, kindEnv;
+    const int kinds = kindsInUse(kindArea, kindSpot, kindEnv);
     const int maxDepth = std::max(1, c.cfg.maxDepth);
     int F = c.cfg.framesInFlight;
     if (F <= 0) {
         const double budget = (double)(c.cfg.memoryBudgetMB > 0 ? c.cfg.memoryBudgetMB : 32768) * 1048576.0;
-        F = (int)std::max(1.0, std::floor(budget / ((double)N * (double)bytesPerPath(kinds))));
+        F = (int)std::max(1.0, std::floor(budget / ((double)N * (double)bytesPerPath(kinds, kindSpot >= 0))));
     }
     F = std::max(1, std::min(F, nframes));
     // path ids occupy 30 bits of the path word
@@ -449,7 +451,7 @@ int renderWavefront(int nframes) {
     if ((long long)N > (1ll << 30)) return fail(BTE_ERR_INVALID, "more than 2^30 pixels per frame");
     WaveBuffers wb;
     std::memset(&wb, 0, sizeof wb);   // padding too: compared bytewise as graph keys
-    CHECK_RC(ensureWaveBuffers((long long)N * F, kinds, maxDepth, wb));
+    CHECK_RC(ensureWaveBuffers((long long)N * F, kinds, kindSpot >= 0, maxDepth, wb));
     wb.kinds = kinds;
     wb.travCtr = nullptr;
     if (c.instr & BTE_INSTR_TRAVERSAL_COUNTS) {
@@ -468,7 +470,7 @@ int renderWavefront(int nframes) {
     wb.catMask = 0;
     for (const BteBsdf& b : c.bsdfsHost) wb.catMask |= 1 << b.type;
     wb.kindArea = kindArea;
-    wb.kindSpot0 = kindSpot0;
+    wb.kindSpot = kindSpot;
     wb.kindEnv = kindEnv;
     const DevScene sc = makeScene(c, c.accW, c.accH);
     LaunchShape ls = query_launch_shape(c.device, c.cfg.traversalStack > 0 ? c.cfg.traversalStack : 16,
@@ -544,7 +546,7 @@ int renderWavefront(int nframes) {
             const int* cb = ctrHost.data() + (size_t)i * kCtrStride;
             c.stats.extensionRays += cb[kCtrRays];
             c.stats.hits += cb[kCtrHits];
-            for (int k = 0; k < kinds; ++k) c.stats.shadowRays += cb[kCtrShadow + k];
+            for (int k = 0; k < kinds; ++k) c.stats.shadowRays += (k == kindSpot) ? cb[kCtrSpotRays] : cb[kCtrShadow + k];
             if (i < 64) c.stats.raysPerBounce[i] += cb[kCtrRays];
         }
         c.stats.bounces = std::max(c.stats.bounces, b);
@@ -666,20 +668,18 @@ int batchResetAccum(Context& c) {
 
 // Light classes of a batch: the union over its scenes (a scene without a
 // class simply makes no records of it).
-int batchKinds(const Context& c, int& kindArea, int& kindSpot0, int& kindEnv, int& catMask) {
-    bool area = false, env = false;
-    int spots = 0;
+int batchKinds(const Context& c, int& kindArea, int& kindSpot, int& kindEnv, int& catMask) {
+    bool area = false, env = false, spots = false;
     catMask = 0;
     for (const BatchScene& b : c.batch) {
         area = area || b.data->emitterCount > 0;
         env = env || b.data->hasEnv;
-        spots = std::max(spots, b.data->spotCount);
+        spots = spots || b.data->spotCount > 0;
         for (const BteBsdf& x : b.data->bsdfsHost) catMask |= 1 << x.type;
     }
     int k = 0;
     kindArea = area ? k++ : -1;
-    kindSpot0 = k;
-    k += spots;
+    kindSpot = spots ? k++ : -1;
     kindEnv = env ? k++ : -1;
     return k;
 }
@@ -741,13 +741,13 @@ int renderBatch(int nframes) {
         for (int i = 0; i < S; ++i) scenes[i] = makeScene(*c.batch[i].data, c.batch[i].w, c.batch[i].h);
         CHECK_RC(c.dBatchScenes.upload(scenes.data(), scenes.size() * sizeof(DevScene)));
     }
-    int kindArea, kindSpot0, kindEnv, catMask;
-    const int kinds = batchKinds(c, kindArea, kindSpot0, kindEnv, catMask);
+    int kindArea, kindSpot, kindEnv, catMask;
+    const int kinds = batchKinds(c, kindArea, kindSpot, kindEnv, catMask);
     const int maxDepth = std::max(1, c.cfg.maxDepth);
     int F = c.cfg.framesInFlight;
     if (F <= 0) {
         const double budget = (double)(c.cfg.memoryBudgetMB > 0 ? c.cfg.memoryBudgetMB : 32768) * 1048576.0;
-        F = (int)std::max(1.0, std::floor(budget / ((double)c.batchPixels * (double)bytesPerPath(kinds))));
+        F = (int)std::max(1.0, std::floor(budget / ((double)c.batchPixels * (double)bytesPerPath(kinds, kindSpot >= 0))));
     }
     F = std::max(1, std::min(F, nframes));
     // path ids (padded layout) occupy 30 bits of the path word
@@ -784,14 +784,14 @@ int renderBatch(int nframes) {
         CHECK_RC(batchLayout(c, frames, padded, slots, maxPaths, maxPixels));
         WaveBuffers wb;
         std::memset(&wb, 0, sizeof wb);   // padding too: compared bytewise as graph keys
-        CHECK_RC(ensureWaveBuffers(padded, kinds, maxDepth, wb));   // hit capture indexed by padded path id
+        CHECK_RC(ensureWaveBuffers(padded, kinds, kindSpot >= 0, maxDepth, wb));   // hit capture by padded path id
         if ((size_t)overflow_stack_ints(ls) * sizeof(int) > c.dOvf.bytes)
             return fail(BTE_ERR_STATE, "traversal overflow stack smaller than the launch shape needs");
         wb.kinds = kinds;
         wb.travCtr = nullptr;
         wb.catMask = catMask;
         wb.kindArea = kindArea;
-        wb.kindSpot0 = kindSpot0;
+        wb.kindSpot = kindSpot;
         wb.kindEnv = kindEnv;
         wb.grabRows = ls.grabRows;
         wb.aovAccAlb = c.aov ? c.dBatchAovAlb.as<float4>() : nullptr;
@@ -855,7 +855,7 @@ int renderBatch(int nframes) {
             const int* cb = ctrHost.data() + (size_t)i * kCtrStride;
             c.stats.extensionRays += cb[kCtrRays];
             c.stats.hits += cb[kCtrHits];
-            for (int k = 0; k < kinds; ++k) c.stats.shadowRays += cb[kCtrShadow + k];
+            for (int k = 0; k < kinds; ++k) c.stats.shadowRays += (k == kindSpot) ? cb[kCtrSpotRays] : cb[kCtrShadow + k];
             if (i < 64) c.stats.raysPerBounce[i] += cb[kCtrRays];
         }
         c.stats.bounces = std::max(c.stats.bounces, b);
@@ -1103,8 +1103,6 @@ int bte_init_spotlights(const BteSpotlight* spotlights, int spotlightCount) {
         c.dSpots.release();
         return BTE_OK;
     }
-    if (spotlightCount > kMaxShadowKinds - 2)
-        return fail(BTE_ERR_UNSUPPORTED, "bte_init_spotlights: at most %d spotlights", kMaxShadowKinds - 2);
     CHECK_RC(c.dSpots.upload(spotlights, (size_t)spotlightCount * sizeof(BteSpotlight)));
     c.spotCount = spotlightCount;
     return BTE_OK;
@@ -1238,7 +1236,7 @@ int bte_cleanup(void) {
                     &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes, &c.dNodes4, &c.dNodes8,
                     &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dEnvMargGuide,
                     &c.dEnvCondGuide, &c.dEnvMono, &c.dAccum,
-                    &c.dFb, &c.dSum64, &c.dHit, &c.dCatQ, &c.dCat, &c.dFinal, &c.dShO, &c.dShD, &c.dShC, &c.dCounters, &c.dHitCap,
+                    &c.dFb, &c.dSum64, &c.dHit, &c.dCatQ, &c.dCat, &c.dFinal, &c.dShO, &c.dShD, &c.dShC, &c.dShX, &c.dCounters, &c.dHitCap,
                     &c.dOvf};
     for (DBuf* b : bufs) b->release();
     c.dTrav.release();

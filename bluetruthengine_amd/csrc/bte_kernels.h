@@ -56,7 +56,8 @@ constexpr int kNumCats = 6;                          // shading categories: miss
 constexpr int kCatMiss = 0;
 constexpr int kCtrCat = 4 + 2 * kMaxShadowKinds;     // + category: rays queued for that category
 constexpr int kCtrShadeWork = kCtrCat + kNumCats;    // + category
-constexpr int kCtrStride = kCtrShadeWork + kNumCats;
+constexpr int kCtrSpotRays = kCtrShadeWork + kNumCats;  // spotlight shadow rays traced by the spot pass
+constexpr int kCtrStride = kCtrSpotRays + 1;
 
 // Per-scene queue counts of a batch, per bounce: the next bounce's rays, the
 // category lists, the shadow records of each light class.
@@ -115,6 +116,10 @@ struct WaveBuffers {
     float4* shO;          // [kinds * cap]: origin xyz, w = tMax
     float4* shD;          // [kinds * cap]: direction xyz, w = destination bits
     float4* shC;          // [kinds * cap]: contribution xyz
+    // The spotlight kind holds one request per path instead: shO = (hit point,
+    // destination bits), shD = (shading normal, triangle bits), shC =
+    // (incoming direction, u), shX = (throughput, v).
+    float4* shX;          // [cap], null without spotlights
     int* counters;        // kCtrStride * (maxDepth + 1)
     int* work;            // kWorkStride * (maxDepth + 1), zero at the start of a pass
     int* hitCapture;      // optional [P * maxRec]
@@ -123,7 +128,7 @@ struct WaveBuffers {
     long long cap;        // slots per buffer (paths in flight)
     int maxRec;
     int kinds;            // shadow kinds in use
-    int kindArea, kindSpot0, kindEnv;   // kind index of each light class (-1 absent)
+    int kindArea, kindSpot, kindEnv;    // kind index of each light class (-1 absent)
     int catMask;          // bit t set: the scene has BSDF type t (its shading kernel runs)
     int grabRows;         // 64-slot rows a persistent wave takes per work-counter atomic (1, 2, 4, 8)
     float4* aovAlb;       // optional, per path id: albedo at the first non-delta hit (0 if none)

@@ -1512,18 +1512,26 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
         for (int k = 0; k < wb.kinds; ++k) {
             NeeRec rec;
             rec.ok = false;
+            const bool spots = (k == wb.kindSpot);
             if (active) {
                 // a batch's light classes are the union over its scenes
                 if (k == wb.kindArea) rec = areaNEE<T>(sc, dIn, thrIn, its, frame, bsdf, a0, a1);
                 else if (k == wb.kindEnv) { if (sc.hasEnv) rec = envNEE<T>(sc, dIn, thrIn, its, frame, bsdf, e0, e1); }
-                else if (k - wb.kindSpot0 < sc.spotCount) rec = spotNEE<T>(sc, dIn, thrIn, its, frame, bsdf, k - wb.kindSpot0);
+                else if (spots) rec.ok = sc.spotCount > 0;     // a request: the spot pass evaluates every spotlight
             }
             const int sl = out.shBase + wave_append(out.shCtr + k * out.shStride, rec.ok);
             if (rec.ok) {
                 const size_t idx = (size_t)k * (size_t)wb.cap + (size_t)sl;
-                wb.shO[idx] = make_float4(its.p.x, its.p.y, its.p.z, rec.tMax);
-                wb.shD[idx] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(dest));
-                wb.shC[idx] = make_float4(rec.c.x, rec.c.y, rec.c.z, 0.0f);
+                if (spots) {
+                    wb.shO[idx] = make_float4(its.p.x, its.p.y, its.p.z, __uint_as_float(dest));
+                    wb.shD[idx] = make_float4(its.n.x, its.n.y, its.n.z, __int_as_float(its.tri));
+                    wb.shC[idx] = make_float4(dIn.x, dIn.y, dIn.z, h.z);
+                    wb.shX[sl] = make_float4(thrIn.x, thrIn.y, thrIn.z, h.w);
+                } else {
+                    wb.shO[idx] = make_float4(its.p.x, its.p.y, its.p.z, rec.tMax);
+                    wb.shD[idx] = make_float4(rec.d.x, rec.d.y, rec.d.z, __uint_as_float(dest));
+                    wb.shC[idx] = make_float4(rec.c.x, rec.c.y, rec.c.z, 0.0f);
+                }
             }
         }
     }
@@ -1669,6 +1677,83 @@ __global__ __launch_bounds__(kBlock) void k_connect(DevScene sc, WaveBuffers wb,
             }
         }
     }
+    if (COUNT) {
+        wave_add_u64(wb.travCtr + 2, nodes);
+        wave_add_u64(wb.travCtr + 3, tris);
+    }
+}
+
+// Spotlight pass (rt_direct_lighting.cuh:175-230, rt_optix_shading.cuh:163):
+// each request is one path's diffuse-type hit; the lane evaluates every
+// spotlight of the scene in index order and traces the shadow ray of each one
+// that lights the point, adding unoccluded contributions in that order (the
+// megakernel's order), so any number of spotlights costs one record per path.
+template <bool COUNT, int L, int WS>
+BTE_DEV int spotOne(const DevScene& sc, const WaveBuffers& wb, int nxt, size_t si, size_t qi, const StackRef<L>& stk,
+                    uint32_t* nodes, uint32_t* tris) {
+    const float4 a = wb.shO[si], b = wb.shD[si], c = wb.shC[si], x = wb.shX[qi];
+    Its its;
+    its.p = xyz(a);
+    its.n = xyz(b);
+    its.tri = __float_as_int(b.w);
+    Bsdf bsdf = loadBsdf(sc, its.tri);
+    applyTextures(sc, its.tri, c.w, x.w, bsdf);
+    const Frame f = makeFrame(its.n);
+    const uint32_t dest = __float_as_uint(a.w);
+    float4* rp = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.rad[nxt] + dest);
+    int rays = 0;
+    for (int s = 0; s < sc.spotCount; ++s) {
+        const NeeRec rec = spotNEE<0>(sc, xyz(c), xyz(x), its, f, bsdf, s);
+        if (!rec.ok) continue;
+        ++rays;
+        float t, u, v;
+        if (!traverseW<WS, true, COUNT, L>(sc, its.p, rec.d, RT_EPS, rec.tMax, stk, t, u, v, nodes, tris)) {
+            float4 v4 = *rp;
+            v4.x = v4.x + rec.c.x;
+            v4.y = v4.y + rec.c.y;
+            v4.z = v4.z + rec.c.z;
+            *rp = v4;
+        }
+    }
+    return rays;
+}
+
+template <bool COUNT, int L, int W, bool BATCH = false>
+__global__ __launch_bounds__(kBlock) void k_connect_spots(DevScene sc, WaveBuffers wb, int bounce, int kind,
+                                                           const DevScene* __restrict__ scenes) {
+    constexpr int WS = W >= 4 ? 4 : 2;   // BVH8 scenes carry the BVH4 too
+    __shared__ int lds[L * kBlock];
+    const int nxt = (bounce & 1) ^ 1;
+    int* ctr = wb.counters + (size_t)bounce * kCtrStride;
+    const int count = BATCH ? segSlots(wb, bounce, kSegShadow + kind) : ctr[kCtrShadow + kind];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
+    const size_t off = (size_t)kind * (size_t)wb.cap;
+    uint32_t nodes = 0, tris = 0;
+    int rays = 0;
+    const int grab = 64 * wb.grabRows;
+    int* wk = work_group(wb, bounce, kWorkConnect + kind);
+    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(wk, grab, count)) {
+        if constexpr (!BATCH) {
+            for (int row = 0; row < wb.grabRows; ++row) {
+                const int q = chunk + row * 64 + lane_id();
+                if (q < count) rays += spotOne<COUNT, L, WS>(sc, wb, nxt, off + q, (size_t)q, stk, &nodes, &tris);
+            }
+        } else {
+            const SegChunk k = segChunk(wb, bounce, kSegShadow + kind, chunk);
+            if (k.left <= 0) continue;
+            const DevScene S = fetchScene(scenes, k.s);
+            for (int row = 0; row < wb.grabRows; ++row) {
+                const int e = row * 64 + lane_id();
+                if (e < k.left)
+                    rays += spotOne<COUNT, L, WS>(S, wb, nxt, off + k.first + e, (size_t)(k.first + e), stk, &nodes,
+                                                  &tris);
+            }
+        }
+    }
+    unsigned long long r = (unsigned)rays;
+    for (int o = 32; o > 0; o >>= 1) r += __shfl_xor(r, o, 64);
+    if (lane_id() == 0 && r) atomicAdd(ctr + kCtrSpotRays, (int)r);
     if (COUNT) {
         wave_add_u64(wb.travCtr + 2, nodes);
         wave_add_u64(wb.travCtr + 3, tris);
@@ -1993,6 +2078,15 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     }
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int k = 0; k < wb.kinds; ++k) {
+        if (k == wb.kindSpot) {
+            if (count)
+                hipLaunchKernelGGL((k_connect_spots<true, L, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb,
+                                   bounce, k, scenes);
+            else
+                hipLaunchKernelGGL((k_connect_spots<false, L, W, B>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc,
+                                   wb, bounce, k, scenes);
+            continue;
+        }
         if constexpr (W >= 4) {
             if (count)
                 hipLaunchKernelGGL((k_connect_pool<true, L, false, W>), dim3(ls.connectBlocks), dim3(kBlock), 0, s, sc, wb, bounce, k, scenes);

@@ -48,8 +48,9 @@ struct Bvh8HostNode {
 };
 static_assert(sizeof(Bvh8HostNode) == 256, "BVH8 node must be 256 B");
 
-// Shadow-slot kinds, traced in this order so contributions add in the
-// reference's order (area, spotlights, environment).
-constexpr int kMaxShadowKinds = 2 + 14;   // area + env + up to 14 spotlights
+// Shadow-record kinds (light classes), traced in this order so contributions
+// add in the reference's order: area lights, spotlights (one record per path;
+// its pass traces every spotlight in index order), environment.
+constexpr int kMaxShadowKinds = 3;
 
 }  // namespace bte

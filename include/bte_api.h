@@ -137,7 +137,7 @@ int bte_init_emitter_table(const BteEmitter* emitters, int emitterCount,
                            const float* sceneEmitterCdf);
 /* ≙ cudaInitBsdfs (render_kernel.h:143). */
 int bte_init_bsdfs(const BteBsdf* bsdfs, int bsdfCount, const int* triangleBsdfIds, int triangleCount);
-/* ≙ cudaInitSpotlights (render_kernel.h:148); null/0 clears (at most 14). */
+/* ≙ cudaInitSpotlights (render_kernel.h:148); any count; null/0 clears. */
 int bte_init_spotlights(const BteSpotlight* spotlights, int spotlightCount);
 /* ≙ cudaInitTextures / cudaInitMrTextures (render_kernel.h:155,163): entry i
  * (RGBA8, row-major, widths[i] x heights[i]) serves material i of

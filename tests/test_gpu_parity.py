@@ -319,3 +319,40 @@ def test_accum64_matches_oracle_across_passes():
         same = np.array_equal(r.accum64(), s64), np.array_equal(r.accum(), acc)
         _say(f"{sd.name}: binary64 sums equal {same[0]}, running means equal {same[1]}")
         assert all(same)
+
+
+def _many_spots(sd, n, seed):
+    """n spotlights near the ceiling aimed down at random (cone angles from
+    narrow to wide), in addition to the scene's other lights."""
+    from bluetruthengine_amd import pod
+    rs = np.random.default_rng(seed)
+    sp = np.zeros(n, pod.SPOTLIGHT)
+    for i in range(n):
+        d = rs.normal(size=3) * 0.5 + np.array([0.0, -1.0, 0.0])
+        sp[i]["position"] = rs.uniform((-0.9, 1.6, -0.9), (0.9, 1.95, 0.9))
+        sp[i]["direction"] = d / np.linalg.norm(d)
+        sp[i]["radiance"] = rs.uniform(0.3, 1.0, 3)
+        sp[i]["intensity"] = rs.uniform(0.5, 3.0)
+        inner, outer = np.sort(rs.uniform(5, 80, 2))
+        sp[i]["innerConeCosine"] = np.cos(np.radians(inner))
+        sp[i]["outerConeCosine"] = np.cos(np.radians(outer))
+    sd.spotlights = sp
+    sd.name += f"_{n}spots"
+    return sd
+
+
+def test_twenty_spotlights_match_oracle():
+    # any number of spotlights (cudaInitSpotlights takes any count): one spot
+    # pass traces each path's spotlights in index order; textured diffuse,
+    # microfacet and Disney walls, BVH4 and BVH2
+    _say("case: textured Cornell + ceiling light + 20 spotlights 64x64, 2 frames, depth 8, BVH4 and BVH2")
+    for width in (4, 2):
+        sd = _many_spots(_textured_scene(), 20, 17)
+        sd.width = sd.height = 64
+        _, _, exact = _sample_parity(sd, 2, bvh_width=width)
+        assert min(exact) == 1.0
+    r = renderer.Renderer(_many_spots(scenes.cornell_c1(width=32, height=32, spp=1, max_depth=8), 20, 18))
+    r.render(1)
+    st = r.stats()
+    _say(f"shadow rays {st.shadowRays} for {st.pathsTraced} paths")
+    assert st.shadowRays > 5 * st.pathsTraced      # counts every traced spotlight ray, not one per request
