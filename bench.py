@@ -380,6 +380,7 @@ def serve_bench(args):
     scene after another.  Scene generation, upload and BVH build are outside
     the timed regions; a first untimed round captures the graphs."""
     from bluetruthengine_amd import denoiser, renderer
+    renderer.set_accel_builder(args.accel_builder)
     sds = serve_scenes(args.serve_scenes)
     have_weights = os.path.exists(denoiser.WEIGHTS_PATH)
     samples = sum(sd.width * sd.height * 8 for sd in sds)
@@ -427,8 +428,10 @@ def serve_bench(args):
     t0 = time.perf_counter()
     b = renderer.Batch(max_depth=10, aov=True, graphs=0 if args.no_graphs else 1,
                        frames_in_flight=args.frames_in_flight, memory_budget_mb=args.memory_budget_mb)
+    build_ms = []
     for k, sd in enumerate(sds):
         b.add(sd)
+        build_ms.append(renderer.accel_stats()["buildMs"])
         if k % 8 == 7:
             log(f"batch: {k + 1}/{len(sds)} scenes uploaded, BVHs built ({time.perf_counter() - t0:.0f} s)")
     load_weights()
@@ -463,6 +466,9 @@ def serve_bench(args):
            "triangles": int(sum(sd.triangles.shape[0] for sd in sds)),
            "p50_ms": batch["frame_p50_ms"], "p99_ms": batch["frame_p99_ms"],
            "batch": batch, "sequential": sequential, "denoiser_flips": denoiser.get_flips(),
+           "bvh_build_ms": {"builder": "gpu" if args.accel_builder == 1 else "host",
+                            "all_scenes": round(float(np.sum(build_ms)), 1),
+                            "max_scene": round(float(np.max(build_ms)), 1)},
            "weights": "trained" if have_weights else "untrained (random init)"}
     print(json.dumps(out), flush=True)
 
@@ -481,6 +487,7 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident traversal blocks per CU (0 = max)")
     ap.add_argument("--bvh-width", type=int, default=4, help="2, 4 or 8: BVH traversed by extend/connect")
     ap.add_argument("--no-graphs", action="store_true", help="plain launches instead of captured HIP graphs")
+    ap.add_argument("--accel-builder", type=int, default=1, help="BVH builder: 1 on the GPU (default), 0 on the host")
     ap.add_argument("--frames-in-flight", type=int, default=0,
                     help="frames traced per wavefront pass (0 = as many as the memory budget allows)")
     ap.add_argument("--memory-budget-mb", type=int, default=0, help="path-state budget (0 = 32768 MB)")
@@ -530,6 +537,7 @@ def main():
         sys.exit(2)
 
     from bluetruthengine_amd import renderer, scenes
+    renderer.set_accel_builder(args.accel_builder)
     t0 = time.perf_counter()
     sd = scenes.envlit_c3(spp=args.spp)
     log(f"scene: {sd.triangles.shape[0]} triangles, {sd.bsdfs.shape[0]} bsdfs, generated in {time.perf_counter() - t0:.1f} s")
@@ -539,7 +547,8 @@ def main():
                           work_chunk=args.work_chunk, traversal_blocks_per_cu=args.blocks_per_cu,
                           bvh_width=args.bvh_width)
     r.build_accel()
-    log(f"upload + env CDF + BVH build: {time.perf_counter() - t0:.1f} s")
+    accel = r.accel_stats()
+    log(f"upload + env CDF + BVH build: {time.perf_counter() - t0:.1f} s (BVH: {accel})")
     W, H, spp = sd.width, sd.height, sd.spp
     if not args.no_autotune:
         t0 = time.perf_counter()
@@ -633,6 +642,9 @@ def main():
         "rays_per_step": {"extension": st["extensionRays"], "shadow": st["shadowRays"], "hits": st["hits"],
                           "extension_per_bounce": st["raysPerBounce"]},
         "roofline": roofline, "kernels": kernel, "cpu_baseline": cpu, "time_to_relmse": relmse_rec,
+        "bvh_build": {"builder": "gpu" if accel["builder"] == 1 else "host", "ms": round(accel["buildMs"], 1),
+                      "triangles": accel["triangles"], "bvh2_nodes": accel["nodes2"], "bvh4_nodes": accel["nodes4"],
+                      "depth": accel["depth2"]},
     }
     print(json.dumps(out), flush=True)
 

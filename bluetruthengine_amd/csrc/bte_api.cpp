@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "bte_bvh.h"
+#include "bte_bvh_gpu.h"
 #include "bte_internal.h"
 #include "bte_kernels.h"
 
@@ -68,6 +69,7 @@ struct SceneData {
     bool haveEmission = false, haveFlags = false;
     bool bvhDirty = true;
     int bvhNodes = 0, bvhDepth = 0;
+    BteAccelStats accel{};
     int envW = 0, envH = 0;
     float envIntensity = 0.0f, envRot = 0.0f, envTotal = 0.0f;
     bool hasEnv = false;
@@ -128,6 +130,8 @@ struct Context : SceneData {
     int lastPassFrames = 0;
 
     BteConfig cfg{200, 5, 0, 32768, 16, 1, 4, 0, 4};
+    int accelBuilder = 1;                      // 0: host binned SAH, 1: GPU clustering
+    BteAccelStats lastAccel{};                 // of the last BVH build (any scene)
     BteStats stats{};
     int instr = 0;
     DBuf dTrav;
@@ -182,12 +186,36 @@ int rebuildEmitterOfTri() {
 int ensureAccel(SceneData& c) {
     if (!c.bvhDirty) return BTE_OK;
     const auto t0 = std::chrono::steady_clock::now();
-    BvhBuild b = build_bvh2(c.tris.data(), (int)c.tris.size());
-    (void)t0;
+    const int T = (int)c.tris.size();
+    if (ctx().accelBuilder == 1 && T >= 2) {
+        GpuBvh g;
+        CHECK_RC(build_bvh_gpu(c.dTris.as<BteTriangle>(), T, ctx().stream, g));
+        // a tree too deep for the traversal stacks is built again on the host
+        if (g.depth2 <= kMaxBvhDepth && g.depth4 <= g.depth2) {
+            c.dNodes = std::move(g.dNodes2);
+            c.dNodes4 = std::move(g.dNodes4);
+            c.bvh4Depth = g.depth4;
+            c.bvh8Depth = g.depth8;
+            if (7 * g.depth8 + 1 <= kStackSize) c.dNodes8 = std::move(g.dNodes8);
+            else c.dNodes8.release();
+            c.dLeafTris = std::move(g.dLeafTris);
+            c.bvhNodes = g.nodes2;
+            c.bvhDepth = g.depth2;
+            c.bvhDirty = false;
+            c.accel = BteAccelStats{std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                                        .count(),
+                                    1, T, g.nodes2, g.nodes4, c.dNodes8.p ? g.nodes8 : 0, g.depth2, g.depth4,
+                                    g.depth8};
+            ctx().lastAccel = c.accel;
+            return BTE_OK;
+        }
+    }
+    BvhBuild b = build_bvh2(c.tris.data(), T);
     CHECK_RC(c.dNodes.upload(b.nodes.data(), b.nodes.size() * sizeof(BvhHostNode)));
     {
         const std::vector<Bvh4HostNode> n4 = collapse_bvh4(b.nodes, c.bvh4Depth);
         CHECK_RC(c.dNodes4.upload(n4.data(), n4.size() * sizeof(Bvh4HostNode)));
+        c.accel.nodes4 = (int)n4.size();
     }
     {
         // the BVH8 needs up to 7 stack entries per level; a tree too deep for
@@ -195,6 +223,7 @@ int ensureAccel(SceneData& c) {
         const std::vector<Bvh8HostNode> n8 = collapse_bvh8(b.nodes, c.bvh8Depth);
         if (7 * c.bvh8Depth + 1 <= kStackSize) CHECK_RC(c.dNodes8.upload(n8.data(), n8.size() * sizeof(Bvh8HostNode)));
         else c.dNodes8.release();
+        c.accel.nodes8 = c.dNodes8.p ? (int)n8.size() : 0;
     }
     std::vector<float> lt = std::move(b.leafTris);
     if (lt.empty()) lt.assign(12, 0.0f);
@@ -202,6 +231,14 @@ int ensureAccel(SceneData& c) {
     c.bvhNodes = (int)b.nodes.size();
     c.bvhDepth = b.depth;
     c.bvhDirty = false;
+    c.accel.buildMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    c.accel.builder = 0;
+    c.accel.triangles = T;
+    c.accel.nodes2 = c.bvhNodes;
+    c.accel.depth2 = c.bvhDepth;
+    c.accel.depth4 = c.bvh4Depth;
+    c.accel.depth8 = c.bvh8Depth;
+    ctx().lastAccel = c.accel;
     return BTE_OK;
 }
 
@@ -1252,6 +1289,20 @@ int bte_cleanup(void) {
     c.frameIndex = 0;
     c.cap = 0;
     c.externalFb = nullptr;
+    return BTE_OK;
+}
+
+int bte_set_accel_builder(int builder) {
+    if (builder != 0 && builder != 1) return fail(BTE_ERR_INVALID, "accel builder must be 0 (host) or 1 (GPU)");
+    Context& c = ctx();
+    if (c.accelBuilder != builder) c.bvhDirty = true;   // the next render builds with the new builder
+    c.accelBuilder = builder;
+    return BTE_OK;
+}
+
+int bte_get_accel_stats(BteAccelStats* stats) {
+    if (!stats) return fail(BTE_ERR_INVALID, "null pointer");
+    *stats = ctx().lastAccel;
     return BTE_OK;
 }
 

@@ -61,6 +61,14 @@ class BteSceneArrays(ctypes.Structure):
         "mrTextureHeights")]
 
 
+class BteAccelStats(ctypes.Structure):
+    _fields_ = [("buildMs", ctypes.c_double), ("builder", _i), ("triangles", _i), ("nodes2", _i), ("nodes4", _i),
+                ("nodes8", _i), ("depth2", _i), ("depth4", _i), ("depth8", _i)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class BteStats(ctypes.Structure):
     _fields_ = [("renderMs", ctypes.c_double), ("extensionRays", _ll), ("hits", _ll), ("shadowRays", _ll),
                 ("pathsTraced", _ll), ("bounces", _i), ("framesInFlight", _i), ("bvhNodes", _i), ("bvhDepth", _i),
@@ -90,6 +98,8 @@ API = {
     "bte_init_mr_textures": [_p, _p, _p, _i],
     "bte_init_env_map": [_p, _i, _i, _f, _f],
     "bte_build_accel": [],
+    "bte_set_accel_builder": [_i],
+    "bte_get_accel_stats": [_p],
     "bte_register_framebuffer": [_p],
     "bte_reset_accumulation": [_i, _i],
     "bte_render": [_i, _i],
@@ -357,6 +367,10 @@ class Renderer:
     def build_accel(self):
         cudaInitOptix()
 
+    def accel_stats(self):
+        """Builder, build time, node counts and depths of the last BVH build."""
+        return accel_stats()
+
     def reset(self):
         cudaResetAccumulation(self.scene.width, self.scene.height)
 
@@ -542,6 +556,18 @@ class Batch:
         out = np.empty(w * h * 3, np.float32)
         _call("bte_batch_read_denoised", int(slot), _ptr(out), out.size)
         return _rgb(out, w, h)
+
+
+def accel_stats():
+    """Builder, build time, node counts and depths of the last BVH build (bte_get_accel_stats)."""
+    s = BteAccelStats()
+    _call("bte_get_accel_stats", ctypes.byref(s))
+    return s.as_dict()
+
+
+def set_accel_builder(builder):
+    """0: binned SAH on the host, 1 (default): on the GPU (takes effect at the next BVH build)."""
+    _call("bte_set_accel_builder", int(builder))
 
 
 def trace_rays(rays, tmin=1e-4, tmax=1e30, any_hit=False, tmax_per_ray=None):

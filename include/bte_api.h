@@ -152,8 +152,24 @@ int bte_init_mr_textures(const uint8_t* const* pixels, const int* widths, const 
  * built on the GPU in binary64 and stored as binary32.  null/0 clears. */
 int bte_init_env_map(const float* hdriPixelsRGBA, int width, int height, float intensity, float rotationDeg);
 /* ≙ cudaInitOptix (render_kernel.h:103): builds the BVH2 over the uploaded
- * triangles.  Rendering builds it implicitly when the triangles changed. */
+ * triangles (and the BVH4 / BVH8 collapsed from it).  Rendering builds it
+ * implicitly when the triangles changed. */
 int bte_build_accel(void);
+/* Builder of bte_build_accel (no counterpart in the reference, whose builder
+ * is optixAccelBuild on the device): 1 (default) = on the GPU (clustering of
+ * Morton-sorted triangles, surface-area leaves and wide collapse); 0 = binned
+ * SAH on the host.  Hits do not depend on the tree.  A GPU tree deeper than
+ * the traversal stacks allow is built again on the host. */
+int bte_set_accel_builder(int builder);
+typedef struct {
+    double buildMs;            /* wall time of the last build, upload included */
+    int builder;               /* 0 host, 1 GPU */
+    int triangles;
+    int nodes2, nodes4, nodes8;/* nodes8 = 0 when the BVH8 is not kept */
+    int depth2, depth4, depth8;
+} BteAccelStats;
+/* Of the last BVH build (any scene, batch slots included). */
+int bte_get_accel_stats(BteAccelStats* stats);
 
 /* ---- framebuffer / accumulation / rendering: render_kernel.h:177-211 ---- */
 /* ≙ cudaRegisterPBO (render_kernel.h:177): a caller-owned device buffer of
