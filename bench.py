@@ -427,7 +427,8 @@ def serve_bench(args):
     # one batch
     t0 = time.perf_counter()
     b = renderer.Batch(max_depth=10, aov=True, graphs=0 if args.no_graphs else 1,
-                       frames_in_flight=args.frames_in_flight, memory_budget_mb=args.memory_budget_mb)
+                       frames_in_flight=args.frames_in_flight, memory_budget_mb=args.memory_budget_mb,
+                       tail_bounce=args.tail_bounce)
     build_ms = []
     for k, sd in enumerate(sds):
         b.add(sd)
@@ -438,7 +439,7 @@ def serve_bench(args):
     tuned = None
     if not args.no_autotune:
         ms, cfg = b.autotune(8)
-        tuned = {k: cfg[k] for k in ("traversalStack", "workChunk", "traversalBlocksPerCU", "bvhWidth")}
+        tuned = {k: cfg[k] for k in ("traversalStack", "workChunk", "traversalBlocksPerCU", "bvhWidth", "tailBounce")}
         log(f"batch autotune: best {ms:.1f} ms with {tuned}")
     rounds = []
     for it in range(1 + max(1, args.steps)):
@@ -486,6 +487,8 @@ def main():
     ap.add_argument("--work-chunk", type=int, default=0, help="64-slot rows per work grab (1/2/4/8)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="resident traversal blocks per CU (0 = max)")
     ap.add_argument("--bvh-width", type=int, default=4, help="2, 4 or 8: BVH traversed by extend/connect")
+    ap.add_argument("--tail-bounce", type=int, default=None,
+                    help="bounce from which one fused pass finishes the paths (0 = off; default: the library's)")
     ap.add_argument("--no-graphs", action="store_true", help="plain launches instead of captured HIP graphs")
     ap.add_argument("--accel-builder", type=int, default=1, help="BVH builder: 1 on the GPU (default), 0 on the host")
     ap.add_argument("--frames-in-flight", type=int, default=0,
@@ -545,7 +548,7 @@ def main():
     r = renderer.Renderer(sd, frames_in_flight=args.frames_in_flight, memory_budget_mb=args.memory_budget_mb,
                           traversal_stack=args.traversal_stack, graphs=0 if args.no_graphs else 1,
                           work_chunk=args.work_chunk, traversal_blocks_per_cu=args.blocks_per_cu,
-                          bvh_width=args.bvh_width)
+                          bvh_width=args.bvh_width, tail_bounce=args.tail_bounce)
     r.build_accel()
     accel = r.accel_stats()
     log(f"upload + env CDF + BVH build: {time.perf_counter() - t0:.1f} s (BVH: {accel})")
@@ -613,7 +616,9 @@ def main():
                     "limiter": "vector-ALU issue of divergent traversal waves (lanes busy per VALU instruction "
                                "above), not HBM bandwidth"}
         kernel = {"extend_ms": round(s1.extendMs, 2), "shade_ms": round(s1.shadeMs, 2),
-                  "connect_ms": round(s1.connectMs, 2), "nodes_per_ray": round(s2.extendNodes / max(1, s2.extensionRays), 2),
+                  "connect_ms": round(s1.connectMs, 2), "tail_ms": round(s1.tailMs, 2),
+                  "tail_from_bounce": cfg["tailBounce"] if s1.tailLaunches else None,
+                  "nodes_per_ray": round(s2.extendNodes / max(1, s2.extensionRays), 2),
                   "tris_per_ray": round(s2.extendTris / max(1, s2.extensionRays), 2),
                   "shadow_nodes_per_ray": round(s2.connectNodes / max(1, s2.shadowRays), 2)}
 
@@ -636,7 +641,7 @@ def main():
                    "width": W, "height": H, "spp": spp, "max_depth": sd.max_depth,
                    "frames_in_flight": st["framesInFlight"], "traversal_stack": cfg["traversalStack"],
                    "work_chunk_rows": cfg["workChunk"], "traversal_blocks_per_cu": cfg["traversalBlocksPerCU"],
-                   "bvh_width": cfg["bvhWidth"],
+                   "bvh_width": cfg["bvhWidth"], "tail_bounce": cfg["tailBounce"],
                    "hip_graphs": bool(cfg["graphs"]), "autotuned": not args.no_autotune},
         "mrays_per_s": round(rays * args.steps / elapsed / 1e6, 1) if rays else None,
         "rays_per_step": {"extension": st["extensionRays"], "shadow": st["shadowRays"], "hits": st["hits"],

@@ -82,6 +82,12 @@ struct BatchScene {
     long long pixelBase = 0;
 };
 
+// Library defaults (bte_get_default_config): reference depth and megakernel
+// paths per pixel, automatic frames in flight, 16-entry LDS stacks, graphs on,
+// 4-row work chunks, BVH4, tail pass from bounce 6 (measured on the 1080p
+// env-lit and 720p terrain workloads).
+constexpr BteConfig kDefaultConfig{200, 5, 0, 32768, 16, 1, 4, 0, 4, 6};
+
 struct Context : SceneData {
     bool deviceReady = false;
     int device = 0;
@@ -129,7 +135,7 @@ struct Context : SceneData {
     long long lastPassPaths = 0;
     int lastPassFrames = 0;
 
-    BteConfig cfg{200, 5, 0, 32768, 16, 1, 4, 0, 4};
+    BteConfig cfg = kDefaultConfig;
     int accelBuilder = 1;                      // 0: host binned SAH, 1: GPU clustering
     BteAccelStats lastAccel{};                 // of the last BVH build (any scene)
     BteStats stats{};
@@ -471,6 +477,12 @@ int runSegment(bool useGraph, const std::vector<unsigned char>& key, F&& emit) {
     return BTE_OK;
 }
 
+// Bounce from which the tail pass finishes the paths (maxDepth: never).
+int tailBounce(int maxDepth) {
+    const int t = ctx().cfg.tailBounce;
+    return (t > 0 && t < maxDepth) ? t : maxDepth;
+}
+
 int renderWavefront(int nframes) {
     Context& c = ctx();
     const int N = c.accW * c.accH;
@@ -518,6 +530,8 @@ int renderWavefront(int nframes) {
     wb.grabRows = ls.grabRows;
     // instrumented runs record events between launches: those stay uncaptured
     const bool useGraph = c.cfg.graphs != 0 && c.instr == 0;
+    // traversal counts are kept by the per-bounce passes only
+    const int tailAt = wb.travCtr ? maxDepth : tailBounce(maxDepth);
     uint32_t* fb = c.externalFb ? c.externalFb : c.dFb.as<uint32_t>();
     double* sum64 = c.accum64 ? c.dSum64.as<double>() : nullptr;
     const size_t ctrInts = (size_t)kCtrStride * (size_t)(maxDepth + 1);
@@ -538,18 +552,21 @@ int renderWavefront(int nframes) {
         launch_fill_i32(c.dFrameBase.as<int>(), 1, (int)c.frameIndex, c.stream);
         // Bounces run in segments of kSegment; between segments the host reads
         // the next bounce's ray count and stops early once no path is left.
+        // The segment holding the tail bounce ends with the tail pass.
         constexpr int kSegment = 16;
-        int b = 0;
+        int b = 0, bTimed = 0;
         bool presented = false;
         for (int b0 = 0; b0 < maxDepth; b0 += kSegment) {
             const int b1 = std::min(b0 + kSegment, maxDepth);
-            const bool last = b1 == maxDepth;
+            const bool tail = tailAt >= b0 && tailAt < b1;
+            const int bEnd = tail ? tailAt : b1;
+            const bool last = b1 == maxDepth || tail;
             std::vector<unsigned char> key;
             if (useGraph) {
                 appendKey(key, sc);
                 appendKey(key, wb);
                 appendKey(key, ls);
-                const long long params[] = {paths, N, frames, b0, b1, maxDepth, (long long)(uintptr_t)fb,
+                const long long params[] = {paths, N, frames, b0, b1, maxDepth, tailAt, (long long)(uintptr_t)fb,
                                             (long long)(uintptr_t)sum64, (long long)(uintptr_t)c.dAccum.p};
                 appendKey(key, params);
             }
@@ -560,11 +577,14 @@ int renderWavefront(int nframes) {
                     if (wb.hitCapture) launch_fill_i32(wb.hitCapture, paths * wb.maxRec, -2, c.stream);
                     launch_generate(sc, wb, paths, N, c.stream);
                 }
-                for (int bb = b0; bb < b1; ++bb)
+                for (int bb = b0; bb < bEnd; ++bb)
                     launch_bounce(sc, wb, bb, maxDepth, ls, c.stream, timed ? &c.events[4 * (size_t)bb] : nullptr);
+                if (tail)
+                    launch_tail(sc, wb, tailAt, maxDepth, ls, c.stream, timed ? &c.events[4 * (size_t)tailAt] : nullptr);
                 if (last) launch_present(sc, wb, N, frames, c.dAccum.as<float4>(), sum64, fb, c.stream);
             }));
-            b = b1;
+            b = tail ? maxDepth : b1;
+            bTimed = tail ? tailAt + 1 : b1;
             if (last) {
                 presented = true;
                 break;
@@ -588,17 +608,22 @@ int renderWavefront(int nframes) {
         }
         c.stats.bounces = std::max(c.stats.bounces, b);
         if (timed) {
-            for (int i = 0; i < b; ++i) {
+            for (int i = 0; i < bTimed; ++i) {
                 float e = 0.0f, sh = 0.0f, cn = 0.0f;
                 HIPCHK(hipEventElapsedTime(&e, c.events[4 * i], c.events[4 * i + 1]));
+                if (i == tailAt) {            // the tail pass: its own figure
+                    c.stats.tailMs += e;
+                    ++c.stats.tailLaunches;
+                    continue;
+                }
                 HIPCHK(hipEventElapsedTime(&sh, c.events[4 * i + 1], c.events[4 * i + 2]));
                 HIPCHK(hipEventElapsedTime(&cn, c.events[4 * i + 2], c.events[4 * i + 3]));
                 c.stats.extendMs += e;
                 c.stats.shadeMs += sh;
                 c.stats.connectMs += cn;
+                ++c.stats.extendLaunches;
+                c.stats.connectLaunches += kinds;
             }
-            c.stats.extendLaunches += b;
-            c.stats.connectLaunches += b * kinds;
         }
         c.stats.pathsTraced += paths;
         c.frameIndex += (uint32_t)frames;
@@ -798,6 +823,7 @@ int renderBatch(int nframes) {
     for (const BatchScene& b : c.batch)
         if (ls.bvhWidth == 4 && !b.data->dNodes4.p) ls.bvhWidth = 2;
     const bool useGraph = c.cfg.graphs != 0;
+    const int tailAt = tailBounce(maxDepth);
     const size_t ctrInts = (size_t)kCtrStride * (size_t)(maxDepth + 1);
     std::vector<int> ctrHost(ctrInts);
     const size_t segInts = (size_t)(maxDepth + 1) * kSegQueues * (size_t)S;
@@ -845,7 +871,9 @@ int renderBatch(int nframes) {
         bool presented = false;
         for (int b0 = 0; b0 < maxDepth; b0 += kSegment) {
             const int b1 = std::min(b0 + kSegment, maxDepth);
-            const bool last = b1 == maxDepth;
+            const bool tail = tailAt >= b0 && tailAt < b1;
+            const int bEnd = tail ? tailAt : b1;
+            const bool last = b1 == maxDepth || tail;
             std::vector<unsigned char> key;
             if (useGraph) {
                 const char tag[8] = {'b', 'a', 't', 'c', 'h', 0, 0, 0};
@@ -853,7 +881,7 @@ int renderBatch(int nframes) {
                 appendKey(key, c.batchGen);
                 appendKey(key, wb);
                 appendKey(key, ls);
-                const long long params[] = {padded, slots, maxPaths, maxPixels, frames, b0, b1, maxDepth,
+                const long long params[] = {padded, slots, maxPaths, maxPixels, frames, b0, b1, maxDepth, tailAt,
                                             (long long)(uintptr_t)scenesDev, (long long)(uintptr_t)sum64,
                                             (long long)(uintptr_t)c.dBatchAccum.p, (long long)(uintptr_t)c.dBatchFb.p};
                 appendKey(key, params);
@@ -866,12 +894,13 @@ int renderBatch(int nframes) {
                     if (wb.hitCapture) launch_fill_i32(wb.hitCapture, padded * wb.maxRec, -2, c.stream);
                     launch_generate_batch(scenesDev, wb, maxPaths, slots, c.stream);
                 }
-                for (int bb = b0; bb < b1; ++bb) launch_bounce(none, wb, bb, maxDepth, ls, c.stream, nullptr, scenesDev);
+                for (int bb = b0; bb < bEnd; ++bb) launch_bounce(none, wb, bb, maxDepth, ls, c.stream, nullptr, scenesDev);
+                if (tail) launch_tail(none, wb, tailAt, maxDepth, ls, c.stream, nullptr, scenesDev);
                 if (last)
                     launch_present_batch(wb, maxPixels, frames, c.dBatchAccum.as<float4>(), sum64,
                                          c.dBatchFb.as<uint32_t>(), c.stream);
             }));
-            b = b1;
+            b = tail ? maxDepth : b1;
             if (last) {
                 presented = true;
                 break;
@@ -912,7 +941,7 @@ int renderBatch(int nframes) {
 }
 
 // Coordinate descent over the launch shape (BVH width, LDS stack, work chunk,
-// traversal blocks per CU).  timed(ms) renders one candidate's workload from a
+// traversal blocks per CU, tail-pass bounce).  timed(ms) renders one candidate's workload from a
 // reset accumulation; each candidate runs twice (the first run captures its
 // graphs) and keeps the second time.  Leaves the best configuration set.
 template <class F>
@@ -965,6 +994,12 @@ int autotune(F&& timed, double* msBest) {
         BteConfig cand = afterChunk;
         cand.traversalBlocksPerCU = bpc;
         if (bpc != afterChunk.traversalBlocksPerCU && (bpc == 0 || bpc < occ)) tryCand(cand);
+    }
+    const BteConfig afterBlocks = best;
+    for (int tb : {0, 2, 3, 4, 5, 6, 8}) {
+        BteConfig cand = afterBlocks;
+        cand.tailBounce = tb;
+        if (tb != afterBlocks.tailBounce && tb < std::max(1, afterBlocks.maxDepth)) tryCand(cand);
     }
     c.instr = savedInstr;
     if (rc != BTE_OK) {
@@ -1322,7 +1357,13 @@ int bte_set_config(const BteConfig* config) {
         return fail(BTE_ERR_INVALID, "traversalBlocksPerCU must be in [0, 64]");
     if (config->bvhWidth != 0 && config->bvhWidth != 2 && config->bvhWidth != 4 && config->bvhWidth != 8)
         return fail(BTE_ERR_INVALID, "bvhWidth must be 0, 2, 4 or 8");
+    if (config->tailBounce < 0) return fail(BTE_ERR_INVALID, "tailBounce must not be negative");
     ctx().cfg = *config;
+    return BTE_OK;
+}
+int bte_get_default_config(BteConfig* config) {
+    if (!config) return fail(BTE_ERR_INVALID, "null config");
+    *config = kDefaultConfig;
     return BTE_OK;
 }
 int bte_get_config(BteConfig* config) {

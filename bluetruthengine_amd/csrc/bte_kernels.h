@@ -81,7 +81,8 @@ constexpr int kWorkPad = 16;
 constexpr int kWorkExtend = 0;
 constexpr int kWorkShade = 1;                        // + category
 constexpr int kWorkConnect = kWorkShade + kNumCats;  // + light class
-constexpr int kWorkGroups = kWorkConnect + kMaxShadowKinds;
+constexpr int kWorkTail = kWorkConnect + kMaxShadowKinds;
+constexpr int kWorkGroups = kWorkTail + 1;
 constexpr int kWorkStride = kWorkGroups * kWorkXcds * kWorkPad;   // ints per bounce
 
 // Multi-scene batch: where each scene's paths and pixels live.  Path ids and
@@ -148,6 +149,7 @@ struct WaveBuffers {
 
 struct LaunchShape {
     int extendBlocks, shadeBlocks, connectBlocks, auxBlocks;
+    int tailBlocks;       // resident grid of the tail pass (launch_tail)
     int ldsStack;         // traversal LDS stack entries per lane: 8, 16 or 32
     int grabRows;         // work chunk of the persistent kernels, in 64-slot rows
     int bvhWidth;         // 2 or 4: BVH traversed by extend / connect
@@ -163,6 +165,13 @@ void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int ma
                    hipStream_t s, hipEvent_t* ev, const DevScene* scenes = nullptr);
 void launch_present(const DevScene& sc, const WaveBuffers& wb, int pixels, int frames, float4* accum,
                     double* sum64, uint32_t* fb, hipStream_t s);
+// Tail pass: finishes every path queued for `bounce` (all its remaining
+// bounces with their shadow rays) in one persistent kernel, in place of
+// launch_bounce for bounce..maxDepth-1.  Its per-bounce ray, hit and shadow
+// counts land in the same counter blocks as launch_bounce's; ev (optional) as
+// for launch_bounce, the whole pass timed as extension.
+void launch_tail(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                 hipStream_t s, hipEvent_t* ev, const DevScene* scenes = nullptr);
 // Multi-scene pass: paths of every scene (wb.slots) into its own slot range;
 // present writes each scene's pixels at its pixelBase.
 void launch_generate_batch(const DevScene* scenes, const WaveBuffers& wb, int maxScenePaths, long long totalSlots,

@@ -1760,6 +1760,452 @@ __global__ __launch_bounds__(kBlock) void k_connect_spots(DevScene sc, WaveBuffe
     }
 }
 
+// ---------------------------------------------------------------------------
+// Tail pass.  Past the first few bounces few paths are left, and a bounce's
+// extend / categorise / shade / connect launches each cost more in launch and
+// drain time than in work.  From bounce `bounce` on, one persistent kernel
+// takes the queued paths and runs each to its end: closest hit, emission,
+// next-event estimation with the shadow rays traced at once (area light,
+// spotlights in index order, sky: the order in which the connect passes add
+// their contributions), scatter and Russian roulette, bounce after bounce.
+// A lane whose path has ended takes the next queued path, so lanes of a wave
+// may be at different bounces.  Every path is computed by the same functions
+// in the same order as in the per-bounce passes, so its radiance, hit
+// sequence and AOVs are bit-identical to theirs.
+// Ray, hit and shadow-ray counts are kept per bounce in LDS and added to each
+// bounce's counter block when the block ends.
+// ---------------------------------------------------------------------------
+constexpr int kTailStatBounces = 64;   // bounces past the first counted in LDS (later ones: global atomics)
+enum { kTsRays, kTsHits, kTsArea, kTsEnv, kTsSpot, kTailStats };
+
+BTE_DEV int tailCounter(const WaveBuffers& wb, int what) {
+    switch (what) {
+        case kTsRays: return kCtrRays;
+        case kTsHits: return kCtrHits;
+        case kTsArea: return kCtrShadow + wb.kindArea;
+        case kTsEnv: return kCtrShadow + wb.kindEnv;
+        default: return kCtrSpotRays;
+    }
+}
+
+// Closest hit of one ray by a whole traversal (BVH2 or BVH4).
+template <int WS, int L>
+BTE_DEV int closestHit(const DevScene& S, V3 o, V3 d, const StackRef<L>& stk, float& t, float& u, float& v) {
+    return traverseW<WS, false, false, L>(S, o, d, RT_EPS, 1e30f, stk, t, u, v, nullptr, nullptr);
+}
+template <int WS, int L>
+BTE_DEV bool occluded(const DevScene& S, V3 o, V3 d, float tMax, const StackRef<L>& stk) {
+    float t, u, v;
+    return traverseW<WS, true, false, L>(S, o, d, RT_EPS, tMax, stk, t, u, v, nullptr, nullptr) != 0;
+}
+
+#ifndef BTE_TAIL_WAVES
+#define BTE_TAIL_WAVES 3   // minimum resident waves per SIMD the register allocation must allow
+#endif
+template <int L, int WS, bool BATCH = false>
+__global__ __launch_bounds__(kBlock, BTE_TAIL_WAVES) void k_tail(DevScene sc, WaveBuffers wb, int bounce, int maxDepth,
+                                                  const DevScene* __restrict__ scenes) {
+    __shared__ int lds[L * kBlock];
+    __shared__ int sStat[kTailStatBounces * kTailStats];
+    for (int i = threadIdx.x; i < kTailStatBounces * kTailStats; i += kBlock) sStat[i] = 0;
+    __syncthreads();
+    const int cur = bounce & 1;
+    const int count = BATCH ? segSlots(wb, bounce, kSegRays) : wb.counters[(size_t)bounce * kCtrStride + kCtrRays];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
+    const int grab = 64 * wb.grabRows;
+    int* wk = work_group(wb, bounce, kWorkTail);
+    RayPool pool = makePool(grab);
+    DevScene S = sc;
+    auto stat = [&](int b, int what) {
+        const int j = b - bounce;
+        if (j < kTailStatBounces) atomicAdd(&sStat[j * kTailStats + what], 1);
+        else atomicAdd(wb.counters + (size_t)b * kCtrStride + tailCounter(wb, what), 1);
+    };
+    // the lane's path
+    bool active = false;
+    V3 o = {0, 0, 0}, d = {0, 0, 0}, thr = {0, 0, 0}, rad = {0, 0, 0};
+    float brdfPdf = 0.0f, eta = 1.0f;
+    uint32_t rng = 0, pid = 0, aovBit = 0;
+    bool spec = false;
+    int b = bounce;
+    auto start = [&](int slot) {
+        const float4 ro = wb.rayO[cur][slot], rd = wb.rayD[cur][slot], th = wb.thr[cur][slot], ra = wb.rad[cur][slot];
+        const uint2 st = wb.st[cur][slot];
+        o = xyz(ro); brdfPdf = ro.w;
+        d = xyz(rd); eta = rd.w;
+        thr = xyz(th);
+        rad = xyz(ra);
+        rng = st.x;
+        pid = st.y & kPidMask;
+        aovBit = st.y & kAovBit;
+        spec = (st.y & kSpecBit) != 0;
+        b = bounce;
+        return true;
+    };
+    for (;;) {
+        if (__ballot(!active) != 0ull) {
+            if (refill<BATCH>(pool, S, scenes, wb, bounce, kSegRays, wk, grab, count, active, start) == ~0ull) break;
+        }
+        if (!active) continue;
+        float t = 0.0f, u = 0.0f, v = 0.0f;
+        const int tri = closestHit<WS, L>(S, o, d, stk, t, u, v);
+        if (wb.hitCapture && b < wb.maxRec) wb.hitCapture[(size_t)pid * wb.maxRec + b] = tri;
+        if (tri < 0) {                                   // as shadeMiss
+            envHit(S, d, spec, brdfPdf, thr, rad);
+            wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+            active = false;
+            continue;
+        }
+        stat(b, kTsHits);
+        const Its its = makeIts(S, o, d, make_float4(__int_as_float(tri), t, u, v));
+        Bsdf bsdf = loadBsdf(S, tri);
+        applyTextures(S, tri, u, v, bsdf);
+        if (S.triFlags && S.triFlags[tri] != 0) emitterHit(S, o, d, spec, brdfPdf, thr, its, rad);
+        const Frame frame = makeFrame(its.n);
+        if (bsdfIsDiffuse(bsdf.type)) {                  // as shadeRow, then the connect passes in class order
+            spec = false;
+            const float a0 = rngNext(rng), a1 = rngNext(rng);
+            float e0 = 0.0f, e1 = 0.0f;
+            if (S.hasEnv) { e0 = rngNext(rng); e1 = rngNext(rng); }
+            if (wb.aovAlb && !aovBit) {
+                wb.aovAlb[pid] = make_float4(bsdf.p0.x, bsdf.p0.y, bsdf.p0.z, 0.0f);
+                wb.aovNrm[pid] = make_float4(its.n.x, its.n.y, its.n.z, 0.0f);
+                aovBit = kAovBit;
+            }
+            NeeRec r = areaNEE<0>(S, d, thr, its, frame, bsdf, a0, a1);
+            if (r.ok) {
+                stat(b, kTsArea);
+                if (!occluded<WS, L>(S, its.p, r.d, r.tMax, stk)) rad = add3(rad, r.c);
+            }
+            for (int s = 0; s < S.spotCount; ++s) {
+                r = spotNEE<0>(S, d, thr, its, frame, bsdf, s);
+                if (!r.ok) continue;
+                stat(b, kTsSpot);
+                if (!occluded<WS, L>(S, its.p, r.d, r.tMax, stk)) rad = add3(rad, r.c);
+            }
+            if (S.hasEnv) {
+                r = envNEE<0>(S, d, thr, its, frame, bsdf, e0, e1);
+                if (r.ok) {
+                    stat(b, kTsEnv);
+                    if (!occluded<WS, L>(S, its.p, r.d, r.tMax, stk)) rad = add3(rad, r.c);
+                }
+            }
+        } else {
+            spec = true;
+        }
+        scatter<0>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
+        if (roulette(b, rng, thr, eta) && b + 1 < maxDepth) {
+            ++b;
+            stat(b, kTsRays);
+        } else {
+            wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+            active = false;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTailStatBounces * kTailStats; i += kBlock) {
+        const int n = sStat[i];
+        if (n) atomicAdd(wb.counters + (size_t)(bounce + i / kTailStats) * kCtrStride + tailCounter(wb, i % kTailStats), n);
+    }
+}
+
+// Tail pass over a BVH4 or BVH8 as a step machine.  Each lane holds its path
+// and one query in flight (the path's closest hit, or one of its shadow rays).
+// An iteration advances every lane's query by one node or one triangle: closest
+// and shadow queries share one step (children sorted by entry distance in
+// both; a shadow ray stops at its first blocker), so lanes of a wave stay
+// converged whatever query they trace.  Lanes whose query has finished wait
+// until kTailService or more of the wave do (or none is running); then each
+// consumes its result (shading after a closest hit, the contribution of an
+// unblocked shadow ray) and starts its next query: next-event estimation of
+// the hit one light at a time (area light, spotlights in index order, sky),
+// then the next bounce.  Finished paths are replaced from the queue.
+// Shadow contributions are added in that order, after the emission of the
+// hit, as the connect passes add them.
+#ifndef BTE_TAIL_SERVICE
+#define BTE_TAIL_SERVICE 48   // measured against 8, 16, 32, 40 and 64
+#endif
+constexpr int kTailService = BTE_TAIL_SERVICE;
+
+// One step of a closest (any = false) or shadow (any = true) query: returns 1
+// when a shadow ray is blocked, 2 when the stack is empty (done), else 0.
+// tmax is the query's limit (a closest query's best hit so far).
+template <int L, int W>
+BTE_DEV int queryStep(const DevScene& S, const WtRay& r, bool any, float& tmax, int& node, int& sp,
+                      const StackRef<L>& stk, int& bestTri, float& tO, float& uO, float& vO) {
+    constexpr float kNone = __builtin_huge_valf();
+    if (W == 8 && node >= 0) {
+        float k[8];
+        int c[8];
+        enter8(S, r, node, tmax, k, c);
+        sortPair(k[0], c[0], k[2], c[2]); sortPair(k[1], c[1], k[3], c[3]);
+        sortPair(k[4], c[4], k[6], c[6]); sortPair(k[5], c[5], k[7], c[7]);
+        sortPair(k[0], c[0], k[4], c[4]); sortPair(k[1], c[1], k[5], c[5]);
+        sortPair(k[2], c[2], k[6], c[6]); sortPair(k[3], c[3], k[7], c[7]);
+        sortPair(k[0], c[0], k[1], c[1]); sortPair(k[2], c[2], k[3], c[3]);
+        sortPair(k[4], c[4], k[5], c[5]); sortPair(k[6], c[6], k[7], c[7]);
+        sortPair(k[2], c[2], k[4], c[4]); sortPair(k[3], c[3], k[5], c[5]);
+        sortPair(k[1], c[1], k[4], c[4]); sortPair(k[3], c[3], k[6], c[6]);
+        sortPair(k[1], c[1], k[2], c[2]); sortPair(k[3], c[3], k[4], c[4]); sortPair(k[5], c[5], k[6], c[6]);
+        if (k[0] != kNone) {
+#pragma unroll
+            for (int j = 7; j >= 1; --j)
+                if (k[j] != kNone) push(stk, sp, c[j]);
+            node = c[0];
+            return 0;
+        }
+    } else if (node >= 0) {
+        const float4* np = S.nodes4 + 8 * (size_t)node;
+        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+        float f0, f1, f2, f3;
+        const float e0 = boxEnter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, RT_EPS, tmax, f0);
+        const float e1 = boxEnter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, RT_EPS, tmax, f1);
+        const float e2 = boxEnter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, RT_EPS, tmax, f2);
+        const float e3 = boxEnter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, RT_EPS, tmax, f3);
+        float k0 = e0 <= f0 ? e0 : kNone, k1 = e1 <= f1 ? e1 : kNone;
+        float k2 = e2 <= f2 ? e2 : kNone, k3 = e3 <= f3 ? e3 : kNone;
+        int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+        sortPair(k0, c0, k1, c1);
+        sortPair(k2, c2, k3, c3);
+        sortPair(k0, c0, k2, c2);
+        sortPair(k1, c1, k3, c3);
+        sortPair(k1, c1, k2, c2);
+        if (k0 != kNone) {
+            if (k3 != kNone) push(stk, sp, c3);
+            if (k2 != kNone) push(stk, sp, c2);
+            if (k1 != kNone) push(stk, sp, c1);
+            node = c0;
+            return 0;
+        }
+    } else {
+        const int code = ~node;
+        const int start = code >> kLeafCountBits;
+        const int cnt = (code & (kMaxLeafTris - 1)) + 1;
+        const float4* tp = S.leafTris + 3 * (size_t)start;
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        float t, u, v;
+        if (hitTriangle(r, a, b, c, RT_EPS, tmax, t, u, v)) {
+            if (any) return 1;
+            const int id = __float_as_int(a.w);
+            if (t < tmax || bestTri < 0 || id < bestTri) {
+                tmax = t; bestTri = id; tO = t; uO = u; vO = v;
+            }
+        }
+        if (cnt > 1) {   // the rest of the leaf is the next step
+            node = ~(((start + 1) << kLeafCountBits) | (cnt - 2));
+            return 0;
+        }
+    }
+    if (sp == 0) return 2;
+    node = pop(stk, sp);
+    return 0;
+}
+
+template <int L, int W, bool BATCH = false>
+__global__ __launch_bounds__(kBlock, BTE_TAIL_WAVES) void k_tail_steps(DevScene sc, WaveBuffers wb, int bounce,
+                                                                        int maxDepth,
+                                                                        const DevScene* __restrict__ scenes) {
+    __shared__ int lds[L * kBlock];
+    __shared__ int sStat[kTailStatBounces * kTailStats];
+    for (int i = threadIdx.x; i < kTailStatBounces * kTailStats; i += kBlock) sStat[i] = 0;
+    __syncthreads();
+    const int cur = bounce & 1;
+    const int count = BATCH ? segSlots(wb, bounce, kSegRays) : wb.counters[(size_t)bounce * kCtrStride + kCtrRays];
+    const int gtid = blockIdx.x * kBlock + threadIdx.x;
+    const StackRef<L> stk{lds + threadIdx.x, wb.ovfStack + (size_t)gtid * kOvfStack};
+    const int grab = 64 * wb.grabRows;
+    int* wk = work_group(wb, bounce, kWorkTail);
+    RayPool pool = makePool(grab);
+    DevScene S = sc;
+    auto stat = [&](int b, int what) __attribute__((always_inline)) {
+        const int j = b - bounce;
+        if (j < kTailStatBounces) atomicAdd(&sStat[j * kTailStats + what], 1);
+        else atomicAdd(wb.counters + (size_t)b * kCtrStride + tailCounter(wb, what), 1);
+    };
+    // the lane's path: o, d = its next extension ray
+    bool hasPath = false, spec = false, survive = false;
+    V3 o = {0, 0, 0}, d = {0, 0, 0}, thr = {0, 0, 0}, rad = {0, 0, 0};
+    float brdfPdf = 0.0f, eta = 1.0f;
+    uint32_t rng = 0, pid = 0, aovBit = 0;
+    int b = bounce;
+    // the last hit, for its next-event estimation: stages [stage, stageEnd) =
+    // area light, spotlights 0..spotCount-1, sky
+    V3 hp = {0, 0, 0}, hn = {0, 0, 0}, dIn = {0, 0, 0}, thrIn = {0, 0, 0};
+    int hTri = 0, stage = 0, stageEnd = 0;
+    float hu = 0.0f, hv = 0.0f, a0 = 0.0f, a1 = 0.0f, e0 = 0.0f, e1 = 0.0f;
+    // the query in flight
+    bool running = false, shadow = false;
+    WtRay r = makeWtRay(V3{0, 0, 0}, V3{0, 0, 1});
+    float qmax = 0.0f, qt = 0.0f, qu = 0.0f, qv = 0.0f;
+    int node = 0, sp = 0, qTri = -1, qEnd = 0;
+    V3 pendC = {0, 0, 0};
+
+    // Values a later step cannot read are reset where they die, so that the
+    // compiler does not keep them in registers across the loop.
+    auto endQuery = [&]() __attribute__((always_inline)) {
+        r = makeWtRay(V3{0, 0, 0}, V3{0, 0, 1});
+        qmax = qt = qu = qv = 0.0f;
+        node = sp = 0;
+        qTri = -1;
+        pendC = V3{0, 0, 0};
+    };
+    auto endHit = [&]() __attribute__((always_inline)) {
+        hp = hn = dIn = thrIn = V3{0, 0, 0};
+        hTri = stage = stageEnd = 0;
+        hu = hv = a0 = a1 = e0 = e1 = 0.0f;
+    };
+    auto finish = [&]() __attribute__((always_inline)) {
+        wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
+        hasPath = false;
+        endQuery();
+        endHit();
+        o = d = thr = rad = V3{0, 0, 0};
+    };
+    auto miss = [&]() __attribute__((always_inline)) {                               // as shadeMiss
+        if (wb.hitCapture && b < wb.maxRec) wb.hitCapture[(size_t)pid * wb.maxRec + b] = -1;
+        envHit(S, r.d, spec, brdfPdf, thr, rad);
+        finish();
+    };
+    // Starts the closest query of ray (o, d); an untraceable ray is a miss at once.
+    auto startClosest = [&]() __attribute__((always_inline)) {
+        r = makeWtRay(o, d);
+        o = d = V3{0, 0, 0};                          // the query holds the ray now
+        if (!r.valid || S.triCount == 0) {
+            miss();
+            return;
+        }
+        qmax = 1e30f;
+        node = sp = 0;
+        qTri = -1;
+        qt = qu = qv = 0.0f;
+        shadow = false;
+        running = true;
+    };
+    // Next query of the path: the hit's remaining light stages, then the next bounce.
+    auto advance = [&]() __attribute__((always_inline)) {
+        while (stage < stageEnd) {
+            const int k = stage++;
+            Its its;
+            its.p = hp; its.n = hn; its.tri = hTri;
+            Bsdf bsdf = loadBsdf(S, hTri);
+            applyTextures(S, hTri, hu, hv, bsdf);
+            const Frame f = makeFrame(hn);
+            NeeRec rec;
+            rec.ok = false;
+            int what = kTsArea;
+            if (k == 0) {
+                rec = areaNEE<0>(S, dIn, thrIn, its, f, bsdf, a0, a1);
+            } else if (k <= S.spotCount) {
+                rec = spotNEE<0>(S, dIn, thrIn, its, f, bsdf, k - 1);
+                what = kTsSpot;
+            } else if (S.hasEnv) {
+                rec = envNEE<0>(S, dIn, thrIn, its, f, bsdf, e0, e1);
+                what = kTsEnv;
+            }
+            if (!rec.ok) continue;
+            stat(b, what);
+            r = makeWtRay(hp, rec.d);
+            if (!r.valid || S.triCount == 0) {       // nothing can block it
+                rad = add3(rad, rec.c);
+                continue;
+            }
+            qmax = rec.tMax;
+            node = sp = 0;
+            pendC = rec.c;
+            shadow = true;
+            running = true;
+            return;
+        }
+        if (!survive) {
+            finish();
+            return;
+        }
+        endHit();
+        ++b;
+        stat(b, kTsRays);
+        startClosest();
+    };
+    // Shading after a closest hit (a miss ends the path).
+    auto shade = [&]() __attribute__((always_inline)) {
+        const int tri = qTri;
+        if (tri < 0) {
+            miss();
+            return;
+        }
+        if (wb.hitCapture && b < wb.maxRec) wb.hitCapture[(size_t)pid * wb.maxRec + b] = tri;
+        stat(b, kTsHits);
+        const V3 ro = r.o, rd = r.d;
+        const Its its = makeIts(S, ro, rd, make_float4(__int_as_float(tri), qt, qu, qv));
+        Bsdf bsdf = loadBsdf(S, tri);
+        applyTextures(S, tri, qu, qv, bsdf);
+        if (S.triFlags && S.triFlags[tri] != 0) emitterHit(S, ro, rd, spec, brdfPdf, thr, its, rad);
+        const Frame frame = makeFrame(its.n);
+        stage = stageEnd = 0;
+        if (bsdfIsDiffuse(bsdf.type)) {
+            spec = false;
+            a0 = rngNext(rng);
+            a1 = rngNext(rng);
+            if (S.hasEnv) { e0 = rngNext(rng); e1 = rngNext(rng); }
+            if (wb.aovAlb && !aovBit) {
+                wb.aovAlb[pid] = make_float4(bsdf.p0.x, bsdf.p0.y, bsdf.p0.z, 0.0f);
+                wb.aovNrm[pid] = make_float4(its.n.x, its.n.y, its.n.z, 0.0f);
+                aovBit = kAovBit;
+            }
+            hp = its.p; hn = its.n; hTri = tri; hu = qu; hv = qv;
+            dIn = rd; thrIn = thr;
+            stageEnd = S.spotCount + 2;
+        } else {
+            spec = true;
+        }
+        d = rd;
+        scatter<0>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
+        survive = roulette(b, rng, thr, eta) && b + 1 < maxDepth;
+    };
+    auto start = [&](int slot) __attribute__((always_inline)) {
+        const float4 ro = wb.rayO[cur][slot], rd = wb.rayD[cur][slot], th = wb.thr[cur][slot], ra = wb.rad[cur][slot];
+        const uint2 st = wb.st[cur][slot];
+        o = xyz(ro); brdfPdf = ro.w;
+        d = xyz(rd); eta = rd.w;
+        thr = xyz(th);
+        rad = xyz(ra);
+        rng = st.x;
+        pid = st.y & kPidMask;
+        aovBit = st.y & kAovBit;
+        spec = (st.y & kSpecBit) != 0;
+        b = bounce;
+        hasPath = true;
+        running = false;
+        startClosest();
+        return hasPath;
+    };
+
+    for (;;) {
+        const unsigned long long busy = __ballot(running);
+        if (64 - __popcll(busy) >= kTailService || busy == 0ull) {
+            if (hasPath && !running) {
+                if (!shadow) shade();
+                else if (qEnd == 2) rad = add3(rad, pendC);   // an unblocked shadow ray
+                if (hasPath) advance();
+            }
+            if (__ballot(!hasPath) != 0ull) {
+                if (refill<BATCH>(pool, S, scenes, wb, bounce, kSegRays, wk, grab, count, hasPath, start) == ~0ull)
+                    break;
+            }
+        }
+        if (running) {
+            qEnd = queryStep<L, W>(S, r, shadow, qmax, node, sp, stk, qTri, qt, qu, qv);
+            if (qEnd != 0) running = false;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kTailStatBounces * kTailStats; i += kBlock) {
+        const int n = sStat[i];
+        if (n) atomicAdd(wb.counters + (size_t)(bounce + i / kTailStats) * kCtrStride + tailCounter(wb, i % kTailStats), n);
+    }
+}
+
 // Welford update of one pixel's running means with `frames` new samples: path
 // ids src + f * stride, output pixel `out`.
 BTE_DEV void presentOne(const WaveBuffers& wb, int frames, int frameBase, size_t src, size_t stride, size_t out,
@@ -2133,6 +2579,50 @@ void launch_bounce(const DevScene& sc, const WaveBuffers& wb, int bounce, int ma
     else launch_bounce_b<false>(sc, wb, bounce, maxDepth, ls, s, ev, nullptr, width);
 }
 
+template <int L, int W, bool B>
+static void launch_tail_l(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                          hipStream_t s, hipEvent_t* ev, const DevScene* scenes) {
+    if (ev) (void)hipEventRecord(ev[0], s);
+    if constexpr (W >= 4)
+        hipLaunchKernelGGL((k_tail_steps<L, W, B>), dim3(ls.tailBlocks), dim3(kBlock), 0, s, sc, wb, bounce, maxDepth,
+                           scenes);
+    else
+        hipLaunchKernelGGL((k_tail<L, 2, B>), dim3(ls.tailBlocks), dim3(kBlock), 0, s, sc, wb, bounce, maxDepth,
+                           scenes);
+    if (ev)
+        for (int i = 1; i < 4; ++i) (void)hipEventRecord(ev[i], s);
+}
+
+template <bool B, int L>
+static void launch_tail_w(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                          hipStream_t s, hipEvent_t* ev, const DevScene* scenes, int width) {
+    if (width == 8) launch_tail_l<L, 8, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
+    else if (width == 4) launch_tail_l<L, 4, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
+    else launch_tail_l<L, 2, B>(sc, wb, bounce, maxDepth, ls, s, ev, scenes);
+}
+
+template <bool B>
+static void launch_tail_b(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                          hipStream_t s, hipEvent_t* ev, const DevScene* scenes, int width) {
+    switch (ls.ldsStack) {
+        case 8: launch_tail_w<B, 8>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, width); break;
+        case 16: launch_tail_w<B, 16>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, width); break;
+        default: launch_tail_w<B, 32>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, width); break;
+    }
+}
+
+void launch_tail(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                 hipStream_t s, hipEvent_t* ev, const DevScene* scenes) {
+    // the host checks that every scene of a batch carries the chosen layout
+    int width = ls.bvhWidth;
+    if (!scenes) {
+        if (width == 8 && sc.nodes8 == nullptr) width = 4;
+        if (width == 4 && sc.nodes4 == nullptr) width = 2;
+    }
+    if (scenes) launch_tail_b<true>(sc, wb, bounce, maxDepth, ls, s, ev, scenes, width);
+    else launch_tail_b<false>(sc, wb, bounce, maxDepth, ls, s, ev, nullptr, width);
+}
+
 void launch_generate_batch(const DevScene* scenes, const WaveBuffers& wb, int maxScenePaths, long long totalSlots,
                            hipStream_t s) {
     if (maxScenePaths <= 0 || wb.sceneCount <= 0) return;
@@ -2250,9 +2740,18 @@ static int travOccupancy() {
     return std::max(1, std::min(a, b));
 }
 
+template <int L, int W>
+static int tailOccupancy() {
+    int a = 0;
+    if constexpr (W >= 4) (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_tail_steps<L, W, false>, kBlock, 0);
+    else (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_tail<L, 2, false>, kBlock, 0);
+    return std::max(1, a);
+}
+
 LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows, int bvhWidth) {
     static int cachedDev = -1;
     static int cus = 1, occS = 1, occMT = 1, occL[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};   // [width][stack]
+    static int occT[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};                                  // tail [width][stack]
     if (device != cachedDev) {
         hipDeviceProp_t prop;
         (void)hipGetDeviceProperties(&prop, device);
@@ -2273,6 +2772,15 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
         occL[2][0] = travOccupancy<8, 8>();
         occL[2][1] = travOccupancy<16, 8>();
         occL[2][2] = travOccupancy<32, 8>();
+        occT[0][0] = tailOccupancy<8, 2>();
+        occT[0][1] = tailOccupancy<16, 2>();
+        occT[0][2] = tailOccupancy<32, 2>();
+        occT[1][0] = tailOccupancy<8, 4>();
+        occT[1][1] = tailOccupancy<16, 4>();
+        occT[1][2] = tailOccupancy<32, 4>();
+        occT[2][0] = tailOccupancy<8, 8>();
+        occT[2][1] = tailOccupancy<16, 8>();
+        occT[2][2] = tailOccupancy<32, 8>();
         cachedDev = device;
     }
     LaunchShape ls{};
@@ -2285,11 +2793,12 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     ls.connectBlocks = cus * occ;
     ls.shadeBlocks = cus * occS;
     ls.auxBlocks = cus * occMT;
+    ls.tailBlocks = cus * occT[ls.bvhWidth == 2 ? 0 : (ls.bvhWidth == 4 ? 1 : 2)][ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
     return ls;
 }
 
 int persistent_threads(const LaunchShape& ls) {
-    return std::max(std::max(ls.extendBlocks, ls.connectBlocks), ls.auxBlocks) * kBlock;
+    return std::max(std::max(std::max(ls.extendBlocks, ls.connectBlocks), ls.auxBlocks), ls.tailBlocks) * kBlock;
 }
 
 int overflow_stack_ints(const LaunchShape& ls) { return persistent_threads(ls) * kOvfStack + 64; }

@@ -22,6 +22,8 @@ LIB_PATH = os.environ.get("BTE_LIB") or os.path.join(_HERE, "_lib", "libbte.so")
 
 BTE_MODE_MEGAKERNEL = 0
 BTE_MODE_WAVEFRONT = 1
+BTE_INSTR_KERNEL_TIMES = 1        # HIP events around every extend / shade / connect launch
+BTE_INSTR_TRAVERSAL_COUNTS = 2    # BVH node and triangle-test counts
 
 _p = ctypes.c_void_p
 _i = ctypes.c_int
@@ -43,7 +45,19 @@ class BteFloat3(ctypes.Structure):
 class BteConfig(ctypes.Structure):
     _fields_ = [("maxDepth", _i), ("megakernelSpp", _i), ("framesInFlight", _i), ("memoryBudgetMB", _i),
                 ("traversalStack", _i), ("graphs", _i), ("workChunk", _i), ("traversalBlocksPerCU", _i),
-                ("bvhWidth", _i)]
+                ("bvhWidth", _i), ("tailBounce", _i)]
+
+
+def default_config():
+    """The library's default configuration (bte_get_default_config) as a dict."""
+    cfg = BteConfig()
+    _call("bte_get_default_config", ctypes.byref(cfg))
+    return {k: getattr(cfg, k) for k, _ in cfg._fields_}
+
+
+def _tail_bounce(value):
+    """The tail-pass bounce to configure: `value`, or the library default for None."""
+    return int(value) if value is not None else default_config()["tailBounce"]
 
 
 class BteSceneInfo(ctypes.Structure):
@@ -76,7 +90,7 @@ class BteStats(ctypes.Structure):
                 ("extendMs", ctypes.c_double), ("shadeMs", ctypes.c_double), ("connectMs", ctypes.c_double),
                 ("extendLaunches", _i), ("connectLaunches", _i),
                 ("extendNodes", _ll), ("extendTris", _ll), ("connectNodes", _ll), ("connectTris", _ll),
-                ("graphLaunches", _i)]
+                ("graphLaunches", _i), ("tailMs", ctypes.c_double), ("tailLaunches", _i)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "raysPerBounce"}
@@ -110,6 +124,7 @@ API = {
     "bte_cleanup": [],
     "bte_set_config": [_p],
     "bte_get_config": [_p],
+    "bte_get_default_config": [_p],
     "bte_get_stats": [_p],
     "bte_set_instrumentation": [_i],
     "bte_last_error": [],
@@ -334,22 +349,24 @@ class Renderer:
     """Uploads a SceneData through the C ABI and renders into device buffers."""
 
     def __init__(self, scene, max_depth=None, frames_in_flight=0, memory_budget_mb=0, mode=BTE_MODE_WAVEFRONT,
-                 megakernel_spp=5, traversal_stack=0, graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=4):
+                 megakernel_spp=5, traversal_stack=0, graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=4,
+                 tail_bounce=None):
         self.scene = scene
         lib()
         cudaCleanup()
         _upload(scene)
         self.set_config(max_depth if max_depth is not None else scene.max_depth, frames_in_flight,
                         memory_budget_mb, megakernel_spp, traversal_stack, graphs, work_chunk,
-                        traversal_blocks_per_cu, bvh_width)
+                        traversal_blocks_per_cu, bvh_width, tail_bounce)
         cudaSetRenderMode(mode)
         cudaResetAccumulation(scene.width, scene.height)
 
     def set_config(self, max_depth, frames_in_flight=0, memory_budget_mb=0, megakernel_spp=5, traversal_stack=0,
-                   graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=4):
+                   graphs=1, work_chunk=0, traversal_blocks_per_cu=0, bvh_width=4, tail_bounce=None):
+        """tail_bounce None: the library default."""
         cfg = BteConfig(int(max_depth), int(megakernel_spp), int(frames_in_flight), int(memory_budget_mb),
                         int(traversal_stack), int(graphs), int(work_chunk), int(traversal_blocks_per_cu),
-                        int(bvh_width))
+                        int(bvh_width), _tail_bounce(tail_bounce))
         _call("bte_set_config", ctypes.byref(cfg))
 
     def config(self):
@@ -453,11 +470,12 @@ class Batch:
     accumulates exactly what a Renderer of that scene alone would."""
 
     def __init__(self, max_depth=10, frames_in_flight=0, memory_budget_mb=0, traversal_stack=0, graphs=1,
-                 work_chunk=0, traversal_blocks_per_cu=0, bvh_width=4, aov=False):
+                 work_chunk=0, traversal_blocks_per_cu=0, bvh_width=4, aov=False, tail_bounce=None):
         lib()
         cudaCleanup()
         cfg = BteConfig(int(max_depth), 5, int(frames_in_flight), int(memory_budget_mb), int(traversal_stack),
-                        int(graphs), int(work_chunk), int(traversal_blocks_per_cu), int(bvh_width))
+                        int(graphs), int(work_chunk), int(traversal_blocks_per_cu), int(bvh_width),
+                        _tail_bounce(tail_bounce))
         _call("bte_set_config", ctypes.byref(cfg))
         cudaSetRenderMode(BTE_MODE_WAVEFRONT)
         _call("bte_enable_aov", 1 if aov else 0)

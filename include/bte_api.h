@@ -89,6 +89,8 @@ typedef struct {
     int workChunk;           /* 64-slot queue rows a persistent wave takes per atomic: 1, 2, 4, 8 (0 = 4) */
     int traversalBlocksPerCU;/* resident traversal blocks per CU (0 = as many as occupancy allows) */
     int bvhWidth;            /* 2: BVH2, 4: BVH4 or 8: BVH8 (both collapsed from the BVH2) for extend / connect (default 4; 0 = 2) */
+    int tailBounce;          /* wavefront: from this bounce on one fused pass finishes every path
+                                (0 = per-bounce passes to the end; default 6) */
 } BteConfig;
 
 /* Counters of the last bte_render / bte_render_frames call. */
@@ -109,11 +111,14 @@ typedef struct {
     long long extendNodes, extendTris;     /* BVH nodes visited / triangles tested */
     long long connectNodes, connectTris;
     int graphLaunches;       /* captured bounce-loop segments replayed by this call */
+    double tailMs;           /* kernel time of the tail passes (instrumented; not in extendMs) */
+    int tailLaunches;
 } BteStats;
 
 enum BteInstrument {
     BTE_INSTR_KERNEL_TIMES = 1,        /* HIP events around every extend/shade/connect launch */
-    BTE_INSTR_TRAVERSAL_COUNTS = 2     /* count BVH nodes and triangle tests (slower kernels) */
+    BTE_INSTR_TRAVERSAL_COUNTS = 2     /* count BVH nodes and triangle tests (slower kernels; the
+                                          per-bounce passes then run to the last bounce, no tail pass) */
 };
 
 /* ---- scene upload: render_kernel.h:103-173 ------------------------------ */
@@ -197,6 +202,7 @@ int bte_cleanup(void);
 /* ---- headless extensions ------------------------------------------------ */
 int bte_set_config(const BteConfig* config);
 int bte_get_config(BteConfig* config);
+int bte_get_default_config(BteConfig* config);   /* the values a new context starts with */
 int bte_get_stats(BteStats* stats);
 int bte_set_instrumentation(int flags);
 const char* bte_last_error(void);

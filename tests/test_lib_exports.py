@@ -41,19 +41,22 @@ def test_config_validation_without_device():
     saved = renderer.BteConfig()
     assert L.bte_get_config(ctypes.byref(saved)) == 0
     assert (saved.graphs, saved.workChunk, saved.traversalBlocksPerCU, saved.bvhWidth) == (1, 4, 0, 4)
+    assert renderer.default_config()["tailBounce"] == 6
     for fields, word in [(dict(graphs=2), b"graphs"), (dict(workChunk=3), b"workChunk"),
                          (dict(traversalBlocksPerCU=-1), b"traversalBlocksPerCU"),
-                         (dict(traversalStack=12), b"traversalStack"), (dict(bvhWidth=3), b"bvhWidth")]:
+                         (dict(traversalStack=12), b"traversalStack"), (dict(bvhWidth=3), b"bvhWidth"),
+                         (dict(tailBounce=-1), b"tailBounce")]:
         cfg = renderer.BteConfig(12, 5, 0, 0, 16, 1, 4, 0, 2)
         for k, v in fields.items():
             setattr(cfg, k, v)
         assert L.bte_set_config(ctypes.byref(cfg)) == -1
         assert word in L.bte_last_error()
-    good = renderer.BteConfig(12, 5, 0, 0, 8, 0, 2, 3, 4)
+    good = renderer.BteConfig(12, 5, 0, 0, 8, 0, 2, 3, 4, 5)
     assert L.bte_set_config(ctypes.byref(good)) == 0
     back = renderer.BteConfig()
     L.bte_get_config(ctypes.byref(back))
-    assert (back.traversalStack, back.graphs, back.workChunk, back.traversalBlocksPerCU, back.bvhWidth) == (8, 0, 2, 3, 4)
+    assert (back.traversalStack, back.graphs, back.workChunk, back.traversalBlocksPerCU, back.bvhWidth,
+            back.tailBounce) == (8, 0, 2, 3, 4, 5)
     assert L.bte_set_config(ctypes.byref(saved)) == 0
 
 
