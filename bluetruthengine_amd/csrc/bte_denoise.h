@@ -48,8 +48,10 @@ void launch_conv3x3(const ConvArgs& a, hipStream_t s);
 void set_conv_variant(int v);
 int conv_variant();
 
-// dW[co][tap][ci] += sum_p g[p][co] * src(p + tap)[ci] (fp32 atomics), src as
-// in ConvArgs; dB[co] += sum_p g[p][co] when dB is non-null.
+// dW[co][tap][ci] += sum_p g[p][co] * src(p + tap)[ci] for co < cout, src as
+// in ConvArgs (c0 and c1 multiples of 32, w_ a multiple of 32); dB[co] +=
+// sum_p g[p][co] when dB is non-null.  Partial sums of pixel chunks go to
+// `part` and are added in chunk order (cout <= 128, c0 + c1 <= 256).
 struct WgradArgs {
     const bf16* g; int gC;         // output gradient, gC channels (multiple of 16 >= cout)
     const bf16* src0; int c0; int up0;
@@ -58,7 +60,13 @@ struct WgradArgs {
     int cout, coutPad;             // weight rows; coutPad multiple of 16
     float* dW;                     // [coutPad][9][c0 + c1]
     float* dB;                     // [cout] or null
+    float* part;                   // scratch for the chunk partial sums
 };
+// part holds kWgradScratchFloats: the chunk sums (at most kWgradPartFloats)
+// followed by kWgradGroups group sums of every weight and bias gradient.
+constexpr size_t kWgradPartFloats = (size_t)8 << 20;
+constexpr int kWgradGroups = 16;
+constexpr size_t kWgradScratchFloats = kWgradPartFloats + (size_t)kWgradGroups * (128 * 9 * 256 + 256);
 void launch_wgrad3x3(const WgradArgs& a, hipStream_t s);
 
 // 2x2 average pool: dst (n, h/2, w/2, c) from src (n, h, w, c).

@@ -695,103 +695,197 @@ void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// Weight gradient: for one tap, dW_tap (cout x cin) = G^T (cout x P) X_tap
-// (P x cin), K = pixels.  A wave owns a 16 x 16 (co, ci) tile of one tap and
-// a range of pixels (split K); partial sums are added to dW with fp32
-// atomics.  Lane (row r, group g) reads 8 consecutive pixels [8g, 8g + 8) of
-// a 32-pixel step for its output channel (A) and for its input channel (B),
-// so both operands enumerate K identically.  Rows are a multiple of 32
-// pixels wide, so a step never crosses an image row.
+// Weight gradient as a GEMM over pixels: dW_tap (cout x cin) = G^T X_tap with
+// K = pixels.  NHWC tensors keep a pixel's channels together, while an MFMA
+// operand wants 8 consecutive pixels of one channel per lane, so a block
+// stages each K step (32 pixels of one image row) transposed in LDS: the
+// gradient rows G^T (CO channels x 32 pixels) and, for each of the 9 taps, the
+// shifted input window X_tap (32 channels x 32 pixels, zero outside the image;
+// nearest upsampling and the channel concatenation resolved while staging).
+// The input rows y-1..y+1 are read once per step and written to the three
+// horizontal shifts.  Each wave then runs its (tap, m tile, n tile) 16 x 16
+// tiles from LDS, the A fragment of an m tile shared by its taps.  A block
+// covers CO = 16 MT output channels x 32 input channels x all 9 taps over a
+// chunk of K steps; its partial sums go to part[chunk] and a second kernel
+// adds the chunks in order (deterministic, no atomics).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_wgrad3x3(WgradArgs a, int chunk) {
+constexpr int kWgRow = 40;   // bf16 per staged LDS row (32 pixels used): 80-byte rows spread the 16-byte reads
+template <int MT>
+__global__ __launch_bounds__(kBlock) void k_wgrad3x3(WgradArgs a, int stepsPerChunk, long long steps) {
+    constexpr int CO = 16 * MT;
+    constexpr int TILES = 9 * MT * 2;
+    constexpr int PER_WAVE = (TILES + 3) / 4;
+    __shared__ __attribute__((aligned(16))) bf16 sG[CO * kWgRow];
+    __shared__ __attribute__((aligned(16))) bf16 sX[9 * 32 * kWgRow];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r16 = lane & 15;
     const int cin = a.c0 + a.c1;
-    const int ciTiles = cin / 16;
-    const int wz = blockIdx.z * 4 + wave;
-    if (wz >= 9 * ciTiles) return;
-    const int t = wz / ciTiles, cit = wz % ciTiles;
-    const int dy = t / 3 - 1, dx = t % 3 - 1;
-    const int co = blockIdx.y * 16 + r16;              // A row of this lane
-    const int ci = cit * 16 + r16;                     // B column of this lane
-    const bool fromS0 = ci < a.c0;
+    const int coBase = blockIdx.y * CO;
+    const int ciBase = blockIdx.z * 32;
+    const bool fromS0 = ciBase < a.c0;
     const int hh = a.h >> 1, wh = a.w_ >> 1;
-    const long long P = (long long)a.n * a.h * a.w_;
-    const long long p0 = (long long)blockIdx.x * chunk;
-    const long long p1 = p0 + chunk < P ? p0 + chunk : P;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (long long k0 = p0; k0 < p1; k0 += 32) {
-        const int x0 = (int)(k0 % a.w_);
-        const long long q = k0 / a.w_;
-        const int y = (int)(q % a.h), n = (int)(q / a.h);
-        bf16x8 A, B;
+    const int rowSteps = a.w_ / 32;
+    const long long st0 = (long long)blockIdx.x * stepsPerChunk;
+    const long long st1 = st0 + stepsPerChunk < steps ? st0 + stepsPerChunk : steps;
+    f32x4 acc[PER_WAVE];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int x = x0 + 8 * g + e;
-            const long long p = k0 + 8 * g + e;
-            const bool in = p < p1;
-            A[e] = in ? a.g[(size_t)p * a.gC + co] : (bf16)0.0f;
-            const int yy = y + dy, xx = x + dx;
-            bf16 b = (bf16)0.0f;
-            if (in && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_) {
+    for (int i = 0; i < PER_WAVE; ++i) acc[i] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // bias gradient (blocks of the first input-channel slice): thread v < 4 CO
+    // stages channels [8 (v % (CO / 8)), + 8) of pixel v / (CO / 8) every step
+    // and sums them
+    const bool biasSums = a.dB != nullptr && blockIdx.z == 0;
+    float bsum[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    for (long long st = st0; st < st1; ++st) {
+        const int x0 = (int)(st % rowSteps) * 32;
+        const long long q = st / rowSteps;
+        const int y = (int)(q % a.h), n = (int)(q / a.h);
+        const long long pRow = ((long long)n * a.h + y) * a.w_ + x0;
+        // G^T: CO channels x 32 pixels
+        for (int v = threadIdx.x; v < 32 * (CO / 8); v += kBlock) {
+            const int k = v / (CO / 8), c8 = v % (CO / 8);
+            const int co = coBase + 8 * c8;
+            bf16x8 x = zero8();
+            if (co < a.gC) x = *reinterpret_cast<const bf16x8*>(a.g + (size_t)(pRow + k) * a.gC + co);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sG[(8 * c8 + e) * kWgRow + k] = x[e];
+            if (biasSums) {   // 32 * CO / 8 <= kBlock: v == threadIdx.x, the same (k, c8) every step
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bsum[e] += (float)x[e];
+            }
+        }
+        // X: rows y-1..y+1, columns x0-1..x0+32, 32 channels, into the 9 tap windows
+        for (int v = threadIdx.x; v < 3 * 34 * 4; v += kBlock) {
+            const int c8 = v & 3, j = (v >> 2) % 34, dyi = (v >> 2) / 34;
+            const int yy = y + dyi - 1, xx = x0 + j - 1;
+            bf16x8 x = zero8();
+            if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_) {
                 if (fromS0) {
-                    b = a.up0 ? a.src0[(((size_t)n * hh + (yy >> 1)) * wh + (xx >> 1)) * a.c0 + ci]
-                              : a.src0[(((size_t)n * a.h + yy) * a.w_ + xx) * a.c0 + ci];
+                    const int c = ciBase + 8 * c8;
+                    x = a.up0 ? *reinterpret_cast<const bf16x8*>(
+                                    a.src0 + (((size_t)n * hh + (yy >> 1)) * wh + (xx >> 1)) * a.c0 + c)
+                              : *reinterpret_cast<const bf16x8*>(
+                                    a.src0 + (((size_t)n * a.h + yy) * a.w_ + xx) * a.c0 + c);
                 } else {
-                    b = a.src1[(((size_t)n * a.h + yy) * a.w_ + xx) * a.c1 + (ci - a.c0)];
+                    const int c = ciBase - a.c0 + 8 * c8;
+                    x = *reinterpret_cast<const bf16x8*>(a.src1 + (((size_t)n * a.h + yy) * a.w_ + xx) * a.c1 + c);
                 }
             }
-            B[e] = b;
+            // tap (dyi, dxi) at pixel k reads column k + dxi
+#pragma unroll
+            for (int dxi = 0; dxi < 3; ++dxi) {
+                const int k = j - dxi;
+                if (k < 0 || k >= 32) continue;
+                bf16* row = sX + ((dyi * 3 + dxi) * 32 + 8 * c8) * kWgRow + k;
+#pragma unroll
+                for (int e = 0; e < 8; ++e) row[e * kWgRow] = x[e];
+            }
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc, 0, 0, 0);
+        __syncthreads();
+        const bf16x8 A0 = *reinterpret_cast<const bf16x8*>(sG + r16 * kWgRow + 8 * g);
+        const bf16x8 A1 = *reinterpret_cast<const bf16x8*>(sG + ((MT - 1) * 16 + r16) * kWgRow + 8 * g);
+#pragma unroll
+        for (int i = 0; i < PER_WAVE; ++i) {
+            const int j = wave + 4 * i;
+            if (j < TILES) {
+                const int tap = j / (2 * MT), mt = (j >> 1) % MT, nt = j & 1;
+                const bf16x8 B = *reinterpret_cast<const bf16x8*>(sX + (tap * 32 + nt * 16 + r16) * kWgRow + 8 * g);
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(mt == 0 ? A0 : A1, B, acc[i], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D[4g + e][r16] of tile (tap, mt, nt): output channel coBase + 16 mt + 4g + e, input channel ciBase + 16 nt + r16
+    const size_t nw = (size_t)a.coutPad * 9 * cin;
+    float* part = a.part + (size_t)blockIdx.x * (nw + a.coutPad);
+    if (biasSums) {
+        // fold the 32 pixels of each channel group in a fixed order; sX is free now
+        float* red = reinterpret_cast<float*>(sX);
+        constexpr int V = 32 * (CO / 8);
+        if (threadIdx.x < V)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) red[e * V + threadIdx.x] = bsum[e];
+        __syncthreads();
+        if (threadIdx.x < CO) {
+            const int c8 = threadIdx.x / 8, e = threadIdx.x % 8;
+            float t = 0.0f;
+            for (int k = 0; k < 32; ++k) t += red[e * V + k * (CO / 8) + c8];
+            part[nw + coBase + threadIdx.x] = t;
+        }
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int row = blockIdx.y * 16 + 4 * g + i;
-        if (row < a.cout) atomicAdd(a.dW + ((size_t)row * 9 + t) * cin + ci, acc[i]);
+    for (int i = 0; i < PER_WAVE; ++i) {
+        const int j = wave + 4 * i;
+        if (j < TILES) {
+            const int tap = j / (2 * MT), mt = (j >> 1) % MT, nt = j & 1;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int co = coBase + mt * 16 + 4 * g + e;
+                part[((size_t)co * 9 + tap) * cin + ciBase + nt * 16 + r16] = acc[i][e];
+            }
+        }
     }
 }
 
-// dB[co] += sum_p g[p][co]: thread t reads channel t % gC of pixels
-// t / gC, t / gC + 256 / gC, ... (consecutive threads read consecutive
-// channels), then the block folds its rows in LDS and adds one value per
-// channel.  gC divides 256.
-__global__ __launch_bounds__(kBlock) void k_bias_grad(const bf16* g, int gC, long long P, int cout, int chunk,
-                                                      float* dB) {
-    __shared__ float part[kBlock];
-    const int c = threadIdx.x % gC, row = threadIdx.x / gC, rows = kBlock / gC;
-    const long long p0 = (long long)blockIdx.x * chunk;
-    const long long p1 = p0 + chunk < P ? p0 + chunk : P;
-    float sum = 0.0f;
-    for (long long p = p0 + row; p < p1; p += rows) sum += f(g[(size_t)p * gC + c]);
-    part[threadIdx.x] = sum;
-    __syncthreads();
-    if (threadIdx.x < gC && threadIdx.x < cout) {
-        float s = 0.0f;
-        for (int r = 0; r < rows; ++r) s += part[r * gC + threadIdx.x];
-        atomicAdd(dB + threadIdx.x, s);
-    }
+// Chunk sums in two passes with a fixed order: pass 1 adds the chunks of group
+// y (chunks [y cpg, (y + 1) cpg)) of element i into part2[y][i]; pass 2 adds
+// the groups into the weight (i < nW: element i of dW, rows below cout) or
+// bias (i - nW) gradient.  Element i lies at offset i (weights) or coutPad 9
+// cin + i - nW (biases) of a chunk's partial sums.
+__global__ __launch_bounds__(kBlock) void k_wgrad_reduce1(const float* part, int chunks, int cpg, long long stride,
+                                                          long long nW, long long biasOff, long long count,
+                                                          float* part2) {
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    const long long off = i < nW ? i : biasOff + (i - nW);
+    const int c0 = blockIdx.y * cpg, c1 = min(c0 + cpg, chunks);
+    float s = 0.0f;
+    for (int c = c0; c < c1; ++c) s += part[(size_t)c * stride + off];
+    part2[(size_t)blockIdx.y * count + i] = s;
+}
+__global__ __launch_bounds__(kBlock) void k_wgrad_reduce2(const float* part2, int groups, long long nW,
+                                                          long long count, float* dW, float* dB) {
+    const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= count) return;
+    float s = 0.0f;
+    for (int y = 0; y < groups; ++y) s += part2[(size_t)y * count + i];
+    if (i < nW) dW[i] += s;
+    else dB[i - nW] += s;
+}
+
+// K steps (32-pixel row segments) per chunk: enough blocks to fill the device
+// about twice, and at most kWgradPartFloats of partial sums.
+static void wgradChunks(const WgradArgs& a, int& chunks, int& stepsPerChunk) {
+    const long long steps = (long long)a.n * a.h * (a.w_ / 32);
+    const int mt = a.coutPad % 32 == 0 ? 2 : 1;
+    const long long tiles = (long long)(a.coutPad / (16 * mt)) * ((a.c0 + a.c1) / 32);
+    const long long perChunk = (long long)a.coutPad * 9 * (a.c0 + a.c1) + a.coutPad;
+    long long c = std::max(1ll, 2048 / tiles);
+    c = std::min(c, std::max(1ll, (long long)kWgradPartFloats / perChunk));
+    c = std::min(c, steps);
+    stepsPerChunk = (int)((steps + c - 1) / c);
+    chunks = (int)((steps + stepsPerChunk - 1) / stepsPerChunk);
 }
 
 void launch_wgrad3x3(const WgradArgs& a, hipStream_t s) {
-    const long long P = (long long)a.n * a.h * a.w_;
+    const long long steps = (long long)a.n * a.h * (a.w_ / 32);
     const int cin = a.c0 + a.c1;
-    const int waves = 9 * (cin / 16);
-    const int zBlocks = (waves + 3) / 4;
-    const int yBlocks = a.coutPad / 16;
-    // about 4096 blocks in all, chunks a multiple of 32 pixels
-    long long chunks = 4096 / ((long long)zBlocks * yBlocks);
-    if (chunks < 1) chunks = 1;
-    long long chunk = (P + chunks - 1) / chunks;
-    chunk = ((chunk + 31) / 32) * 32;
-    if (chunk < 32) chunk = 32;
-    const unsigned xBlocks = (unsigned)((P + chunk - 1) / chunk);
-    hipLaunchKernelGGL(k_wgrad3x3, dim3(xBlocks, yBlocks, zBlocks), dim3(kBlock), 0, s, a, (int)chunk);
-    if (a.dB) {
-        const long long bchunk = 256;
-        hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((P + bchunk - 1) / bchunk)), dim3(kBlock), 0, s, a.g, a.gC, P,
-                           a.cout, (int)bchunk, a.dB);
-    }
+    int chunks = 0, spc = 0;
+    wgradChunks(a, chunks, spc);
+    if (a.coutPad % 32 == 0)
+        hipLaunchKernelGGL(k_wgrad3x3<2>, dim3(chunks, a.coutPad / 32, cin / 32), dim3(kBlock), 0, s, a, spc, steps);
+    else
+        hipLaunchKernelGGL(k_wgrad3x3<1>, dim3(chunks, a.coutPad / 16, cin / 32), dim3(kBlock), 0, s, a, spc, steps);
+    const long long nW = (long long)a.cout * 9 * cin;
+    const long long count = nW + (a.dB ? a.cout : 0);
+    const long long biasOff = (long long)a.coutPad * 9 * cin;
+    const int groups = std::min(chunks, kWgradGroups);
+    const int cpg = (chunks + groups - 1) / groups;
+    const int used = (chunks + cpg - 1) / cpg;
+    float* part2 = a.part + kWgradPartFloats;
+    const unsigned xb = (unsigned)((count + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_wgrad_reduce1, dim3(xb, used), dim3(kBlock), 0, s, a.part, chunks, cpg, biasOff + a.coutPad,
+                       nW, biasOff, count, part2);
+    hipLaunchKernelGGL(k_wgrad_reduce2, dim3(xb), dim3(kBlock), 0, s, part2, used, nW, count, a.dW, a.dB);
 }
 
 // ---------------------------------------------------------------------------

@@ -80,6 +80,7 @@ struct Net {
     bool train = false;
     DBuf t[kTensors];
     DBuf o, target, loss;
+    DBuf wgPart;                 // weight-gradient chunk sums
     // dataset of training pairs
     int slots = 0, dsW = 0, dsH = 0;
     DBuf dsX, dsT;
@@ -201,6 +202,7 @@ int ensureWorkspace(Net& N, int n, int h, int w, bool train) {
     if (train) {
         CHECK_RC(N.target.ensure(P * 3 * sizeof(float)));
         CHECK_RC(N.loss.ensure(sizeof(float)));
+        CHECK_RC(N.wgPart.ensure(dn::kWgradScratchFloats * sizeof(float)));
     }
     N.n = n;
     N.h = h;
@@ -257,6 +259,7 @@ void wgrad(Net& N, int layer, const bf16* g, const Src& in, int level, hipStream
     a.cout = l.cout; a.coutPad = l.coutPad;
     a.dW = N.g.as<float>() + l.wOff;
     a.dB = N.g.as<float>() + l.bOff;
+    a.part = N.wgPart.as<float>();
     dn::launch_wgrad3x3(a, s);
 }
 
@@ -427,7 +430,7 @@ int denoiser_present() {
 void denoiser_release() {
     Net& N = net();
     for (DBuf& b : N.t) b.release();
-    N.o.release(); N.target.release(); N.loss.release();
+    N.o.release(); N.target.release(); N.loss.release(); N.wgPart.release();
     N.denoised.release();
     N.outW = N.outH = 0;
     N.enabled = false;
