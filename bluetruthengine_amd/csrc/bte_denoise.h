@@ -41,6 +41,8 @@ struct ConvArgs {
     const bf16* mask; int maskC;   // optional: times (mask[p][co] > 0)
     const bf16* resid; int residC; // optional: plus resid[p][co]
     bf16* dstB; float* dstF; int dstC;   // output (one of the two), channel stride dstC
+    bf16* pool;                    // optional (bf16 output, no mask or residual): the 2x2 average
+                                   // of dstB at (n, h/2, w/2), as launch_avgpool computes it
 };
 void launch_conv3x3(const ConvArgs& a, hipStream_t s);
 // 0: each wave reads weights from L1/L2; 1: weight tiles staged in LDS per
@@ -107,7 +109,11 @@ void launch_weight_copies(const float* w, int coutPad, int cin, int coutT, bf16*
 // for b < flips (1, 2 or 4): image b is mirrored left-right when bit 0 of b is
 // set, top-bottom when bit 1 is set (a flip ensemble at inference).
 void launch_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
-                     int flips, bf16* x, hipStream_t s);
+                     int flips, bf16* x, bf16* rgb4, hipStream_t s);
+// The residual input of the output layer: channels 0-2 of the network input x
+// (kInPad channels per pixel) and a zero, 4 channels per pixel (launch_features
+// writes it along with x when rgb4 is non-null).
+void launch_input_rgb(const bf16* x, long long pixels, bf16* rgb4, hipStream_t s);
 // Network output (log space, fp32 [flips][hp][wp][3]) -> the mean over the
 // mirrored-back images in log space -> linear float4 image and an optional
 // gamma-packed RGBA8 buffer (the present step of the denoised image).

@@ -12,6 +12,7 @@ namespace bte {
 namespace dn {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
@@ -347,13 +348,56 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX,
 }
 
 // Epilogue of a tile kernel: bias, residual, ReLU, mask and the store of one
-// wave's MT x 16 pixels x NT x 16 channels.  With EXTRA (a residual or a
+// wave's MT x 16 pixels x NT x 16 channels.  EPI 0: plain; 1 (EXTRA): a residual or a
 // mask), the residual / mask reads of an M tile are issued together before
 // any of them is used, so each M tile waits for global memory once; without
 // it the epilogue reads no global memory.
-template <int MT, int NT, int RW, bool EXTRA>
+template <int MT, int NT, int RW, int EPI>
 __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, f32x4 (&acc)[MT][NT], const float (&biasv)[NT],
                                              int n, int x0, int y0, int wave, int g, int r16, int coBase) {
+    constexpr bool EXTRA = EPI == 1;
+    if constexpr (EPI == 2) {
+        // outputs and their 2x2 averages (k_avgpool's sum order over the stored
+        // bf16 values): rows 2r, 2r + 1 of the wave are M tiles m0 = 4r + hf and
+        // m0 + 2, the column pairs (4g + i, 4g + i + 1) elements i, i + 1 of a lane
+        static_assert(RW % 2 == 0, "pooling needs row pairs within a wave");
+        const int h2 = a.h >> 1, w2 = a.w_ >> 1;
+#pragma unroll
+        for (int r = 0; r < RW / 2; ++r) {
+            const int y = y0 + wave * RW + 2 * r;
+#pragma unroll
+            for (int hf = 0; hf < 2; ++hf) {
+                const int m0 = 4 * r + hf, m1 = m0 + 2;
+                const int xb = x0 + hf * 16 + 4 * g;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int co = coBase + nt * 16 + r16;
+                    if (y >= a.h || xb >= a.w_ || co >= a.cout) continue;
+                    float top[4], bot[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        top[i] = bot[i] = 0.0f;
+                        if (xb + i >= a.w_) continue;   // w is even: pairs (i, i + 1) are both in or both out
+                        float v0 = acc[m0][nt][i], v1 = acc[m1][nt][i];
+                        if (a.bias) { v0 += biasv[nt]; v1 += biasv[nt]; }
+                        if (a.relu) { v0 = fmaxf(v0, 0.0f); v1 = fmaxf(v1, 0.0f); }
+                        const bf16 b0 = (bf16)v0, b1 = (bf16)v1;
+                        const size_t p = ((size_t)n * a.h + y) * a.w_ + xb + i;
+                        a.dstB[p * a.dstC + co] = b0;
+                        a.dstB[(p + a.w_) * a.dstC + co] = b1;
+                        top[i] = f(b0);
+                        bot[i] = f(b1);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; i += 2)
+                        if (xb + i < a.w_)
+                            a.pool[(((size_t)n * h2 + (y >> 1)) * w2 + ((xb + i) >> 1)) * a.dstC + co] =
+                                (bf16)(0.25f * (top[i] + top[i + 1] + bot[i] + bot[i + 1]));
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
         const int y = y0 + wave * RW + (mt >> 1);
@@ -395,7 +439,7 @@ __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, f32x4 (&acc)[MT]
 // of all 9 taps in LDS for their whole run and walk tiles blockIdx.x,
 // blockIdx.x + gridDim.x, ...; the next tile's input (with halo) is loaded
 // into registers while the current one is computed and stored.
-template <int NT, int RW, bool EXTRA>
+template <int NT, int RW, int EPI>
 __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX, int tilesY, int numTiles) {
     constexpr int TH = 4 * RW, MT = 2 * RW;
     constexpr int TR = TH + 2, TC = kTileW + 2;
@@ -491,7 +535,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
         }
         const int tx = tile % tilesX, rest = tile / tilesX;
         const int ty = rest % tilesY, n = rest / tilesY;
-        tileEpilogue<MT, NT, RW, EXTRA>(a, acc, biasv, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
+        tileEpilogue<MT, NT, RW, EPI>(a, acc, biasv, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
     }
 }
 
@@ -499,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
 // (tile, 32-channel slice) items in order; while one slice's 9 taps run from
 // LDS, the next item's input slice (with halo) and weight slice are loaded
 // into registers.  The output of a tile is written after its last slice.
-template <int NT, int RW, bool EXTRA>
+template <int NT, int RW, int EPI>
 __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX, int tilesY, int numTiles) {
     constexpr int TH = 4 * RW, MT = 2 * RW;
     constexpr int TR = TH + 2, TC = kTileW + 2;
@@ -593,7 +637,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
         if (last) {
             const int tx = tile % tilesX, rest = tile / tilesX;
             const int ty = rest % tilesY, n = rest / tilesY;
-            tileEpilogue<MT, NT, RW, EXTRA>(a, acc, biasv, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
+            tileEpilogue<MT, NT, RW, EPI>(a, acc, biasv, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -625,7 +669,8 @@ static int g_convVariant = 2;   // 0: direct loads, 1: LDS-staged weights, 2: LD
 void set_conv_variant(int v) { g_convVariant = v; }
 int conv_variant() { return g_convVariant; }
 
-void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
+// Launches the convolution; returns whether its kernel also wrote a.pool.
+static bool convKernels(const ConvArgs& a, hipStream_t s) {
     const long long P = (long long)a.n * a.h * a.w_;
     const int tiles = a.coutPad / 16;
     if (g_convVariant == 2) {
@@ -641,17 +686,22 @@ void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
         };
         if (a.c0 + a.c1 == 32) {
             const bool extra = a.resid || a.mask;
-#define BTE_TILE1(NT_, EX_)                                                                                   \
-    hipLaunchKernelGGL((k_conv3x3_tile1<NT_, 2, EX_>),                                                         \
-                       persistent(reinterpret_cast<const void*>(&k_conv3x3_tile1<NT_, 2, EX_>), tiles / NT_), \
+            const bool pool = !extra && a.pool && a.dstB;
+#define BTE_TILE1(NT_, EPI_)                                                                                   \
+    hipLaunchKernelGGL((k_conv3x3_tile1<NT_, 2, EPI_>),                                                         \
+                       persistent(reinterpret_cast<const void*>(&k_conv3x3_tile1<NT_, 2, EPI_>), tiles / NT_), \
                        dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles)
             switch (nt4) {
-                case 4: if (extra) BTE_TILE1(4, true); else BTE_TILE1(4, false); break;
-                case 2: if (extra) BTE_TILE1(2, true); else BTE_TILE1(2, false); break;
-                default: if (extra) BTE_TILE1(1, true); else BTE_TILE1(1, false); break;
+                case 4: if (extra) BTE_TILE1(4, 1); else BTE_TILE1(4, 0); break;
+                case 2:
+                    if (extra) BTE_TILE1(2, 1);
+                    else if (pool) BTE_TILE1(2, 2);
+                    else BTE_TILE1(2, 0);
+                    return pool;
+                default: if (extra) BTE_TILE1(1, 1); else BTE_TILE1(1, 0); break;
             }
 #undef BTE_TILE1
-            return;
+            return false;
         }
         if ((a.c0 + a.c1) >= 192 && nt4 == 4) {
             // many slices and >= 64 output channels: one pass over the input
@@ -659,19 +709,25 @@ void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
             // 32-channel groups and than 4-row tiles with 64 channels)
             const dim3 grid((unsigned)numTiles, (unsigned)(tiles / 4));
             hipLaunchKernelGGL((k_conv3x3_tile<4, 2>), grid, dim3(kBlock), 0, s, a, tilesX, tilesY);
-            return;
+            return false;
         }
         // several slices: at most 32 output channels per block (registers for the prefetch)
         const int nt2 = (tiles % 2 == 0) ? 2 : 1;
         const bool extra = a.resid || a.mask;
-#define BTE_TILEN(NT_, EX_)                                                                                   \
-    hipLaunchKernelGGL((k_conv3x3_tileN<NT_, 2, EX_>),                                                         \
-                       persistent(reinterpret_cast<const void*>(&k_conv3x3_tileN<NT_, 2, EX_>), tiles / NT_), \
+        const bool pool = !extra && a.pool && a.dstB;
+#define BTE_TILEN(NT_, EPI_)                                                                                   \
+    hipLaunchKernelGGL((k_conv3x3_tileN<NT_, 2, EPI_>),                                                         \
+                       persistent(reinterpret_cast<const void*>(&k_conv3x3_tileN<NT_, 2, EPI_>), tiles / NT_), \
                        dim3(kBlock), 0, s, a, tilesX, tilesY, (int)numTiles)
-        if (nt2 == 2) { if (extra) BTE_TILEN(2, true); else BTE_TILEN(2, false); }
-        else { if (extra) BTE_TILEN(1, true); else BTE_TILEN(1, false); }
+        if (nt2 == 2) {
+            if (extra) BTE_TILEN(2, 1);
+            else if (pool) BTE_TILEN(2, 2);
+            else BTE_TILEN(2, 0);
+            return pool;
+        }
+        if (extra) BTE_TILEN(1, 1); else BTE_TILEN(1, 0);
 #undef BTE_TILEN
-        return;
+        return false;
     }
     const int nt = (tiles % 8 == 0) ? 8 : (tiles % 4 == 0) ? 4 : (tiles % 2 == 0) ? 2 : 1;
     if (g_convVariant == 1) {
@@ -683,7 +739,7 @@ void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
             case 2: hipLaunchKernelGGL((k_conv3x3_lds<2, 4>), dim3((unsigned)((P + 255) / 256), gy), dim3(kBlock), 0, s, a); break;
             default: hipLaunchKernelGGL((k_conv3x3_lds<1, 4>), dim3((unsigned)((P + 255) / 256), gy), dim3(kBlock), 0, s, a); break;
         }
-        return;
+        return false;
     }
     const dim3 grid((unsigned)((P + 127) / 128), (unsigned)(tiles / nt));
     switch (nt) {
@@ -692,6 +748,11 @@ void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
         case 2: hipLaunchKernelGGL(k_conv3x3<2>, grid, dim3(kBlock), 0, s, a); break;
         default: hipLaunchKernelGGL(k_conv3x3<1>, grid, dim3(kBlock), 0, s, a); break;
     }
+    return false;
+}
+
+void launch_conv3x3(const ConvArgs& a, hipStream_t s) {
+    if (!convKernels(a, s) && a.pool && a.dstB) launch_avgpool(a.dstB, a.pool, a.n, a.h, a.w_, a.dstC, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -891,23 +952,31 @@ void launch_wgrad3x3(const WgradArgs& a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // pooling, upsampling gradients
 // ---------------------------------------------------------------------------
+// One thread per output pixel and 8 channels (16-byte loads and store); each
+// channel is (a + b + c + d) * 0.25 in that order.
 __global__ void k_avgpool(const bf16* src, bf16* dst, int n, int h, int w, int c) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int h2 = h >> 1, w2 = w >> 1;
-    const long long total = (long long)n * h2 * w2 * c;
+    const int h2 = h >> 1, w2 = w >> 1, c8 = c >> 3;
+    const long long total = (long long)n * h2 * w2 * c8;
     if (i >= total) return;
-    const int ch = (int)(i % c);
-    const long long q = i / c;
+    const int ch = (int)(i % c8) * 8;
+    const long long q = i / c8;
     const int x = (int)(q % w2);
     const long long r = q / w2;
     const int y = (int)(r % h2), nn = (int)(r / h2);
     const size_t b00 = (((size_t)nn * h + 2 * y) * w + 2 * x) * c + ch;
-    const float sum = f(src[b00]) + f(src[b00 + c]) + f(src[b00 + (size_t)w * c]) + f(src[b00 + (size_t)w * c + c]);
-    dst[i] = (bf16)(0.25f * sum);
+    const bf16x8 v00 = *reinterpret_cast<const bf16x8*>(src + b00);
+    const bf16x8 v01 = *reinterpret_cast<const bf16x8*>(src + b00 + c);
+    const bf16x8 v10 = *reinterpret_cast<const bf16x8*>(src + b00 + (size_t)w * c);
+    const bf16x8 v11 = *reinterpret_cast<const bf16x8*>(src + b00 + (size_t)w * c + c);
+    bf16x8 o;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) o[e] = (bf16)(0.25f * (f(v00[e]) + f(v01[e]) + f(v10[e]) + f(v11[e])));
+    *reinterpret_cast<bf16x8*>(dst + q * c + ch) = o;
 }
 
 void launch_avgpool(const bf16* src, bf16* dst, int n, int h, int w, int c, hipStream_t s) {
-    const long long total = (long long)n * (h / 2) * (w / 2) * c;
+    const long long total = (long long)n * (h / 2) * (w / 2) * (c / 8);
     hipLaunchKernelGGL(k_avgpool, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, src, dst, n, h, w, c);
 }
 
@@ -1059,7 +1128,7 @@ __device__ __forceinline__ float fireflyLum(float4 c) {
 // when bit 0 of b is set and top-bottom when bit 1 is set, in the top-left
 // w x h corner of its wp x hp tensor (zero padding right and below).
 __global__ void k_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
-                           int flips, bf16* x) {
+                           int flips, bf16* x, bf16* rgb4) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long plane = (long long)wp * hp;
     if (i >= plane * flips) return;
@@ -1112,13 +1181,30 @@ __global__ void k_features(const float4* rgb, const float4* alb, const float4* n
     bf16x8* dst = reinterpret_cast<bf16x8*>(x + (size_t)i * kInPad);
 #pragma unroll
     for (int k = 0; k < kInPad / 8; ++k) dst[k] = out[k];
+    if (rgb4) {
+        bf16x4 r;
+        r[0] = out[0][0]; r[1] = out[0][1]; r[2] = out[0][2]; r[3] = (bf16)0.0f;
+        *reinterpret_cast<bf16x4*>(rgb4 + (size_t)i * 4) = r;
+    }
 }
 
 void launch_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
-                     int flips, bf16* x, hipStream_t s) {
+                     int flips, bf16* x, bf16* rgb4, hipStream_t s) {
     const long long total = (long long)wp * hp * flips;
     hipLaunchKernelGGL(k_features, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, rgb, alb, nrm, w, h, wp,
-                       hp, flips, x);
+                       hp, flips, x, rgb4);
+}
+
+__global__ void k_input_rgb(const bf16* x, long long pixels, bf16* rgb4) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pixels) return;
+    const bf16* src = x + (size_t)i * kInPad;
+    bf16* r = rgb4 + (size_t)i * 4;
+    r[0] = src[0]; r[1] = src[1]; r[2] = src[2]; r[3] = (bf16)0.0f;
+}
+
+void launch_input_rgb(const bf16* x, long long pixels, bf16* rgb4, hipStream_t s) {
+    hipLaunchKernelGGL(k_input_rgb, dim3((unsigned)((pixels + 255) / 256)), dim3(256), 0, s, x, pixels, rgb4);
 }
 
 // Mean of the `flips` outputs in log space, each mirrored back (image b as in

@@ -81,6 +81,7 @@ struct Net {
     DBuf t[kTensors];
     DBuf o, target, loss;
     DBuf wgPart;                 // weight-gradient chunk sums
+    DBuf rgb4;                   // channels 0-2 of the input, 4 per pixel: the output layer's residual
     // dataset of training pairs
     int slots = 0, dsW = 0, dsH = 0;
     DBuf dsX, dsT;
@@ -199,6 +200,7 @@ int ensureWorkspace(Net& N, int n, int h, int w, bool train) {
     }
     const size_t P = (size_t)n * h * w;
     CHECK_RC(N.o.ensure(P * 3 * sizeof(float)));
+    CHECK_RC(N.rgb4.ensure(P * 4 * sizeof(bf16)));
     if (train) {
         CHECK_RC(N.target.ensure(P * 3 * sizeof(float)));
         CHECK_RC(N.loss.ensure(sizeof(float)));
@@ -218,7 +220,8 @@ struct Src {
     const bf16* s1; int c1;
 };
 
-void conv(Net& N, int layer, const Src& in, int level, bf16* dstB, float* dstF, const bf16* resid, hipStream_t s) {
+void conv(Net& N, int layer, const Src& in, int level, bf16* dstB, float* dstF, const bf16* resid, hipStream_t s,
+          bf16* pool = nullptr) {
     const Layer& l = N.L[layer];
     dn::ConvArgs a{};
     a.src0 = in.s0; a.c0 = in.c0; a.up0 = in.up0;
@@ -228,8 +231,9 @@ void conv(Net& N, int layer, const Src& in, int level, bf16* dstB, float* dstF, 
     a.n = N.n; a.h = N.h >> level; a.w_ = N.wdt >> level;
     a.cout = l.cout; a.coutPad = l.coutPad;
     a.relu = l.relu;
-    a.resid = resid; a.residC = dn::kInPad;
+    a.resid = resid; a.residC = 4;
     a.dstB = dstB; a.dstF = dstF; a.dstC = dstF ? dn::kOutChannels : l.cout;
+    a.pool = pool;
     dn::launch_conv3x3(a, s);
 }
 
@@ -267,16 +271,12 @@ Src one(const bf16* s0, int c0) { return Src{s0, c0, 0, nullptr, 0}; }
 Src cat(const bf16* up, int cu, const bf16* skip, int cs) { return Src{up, cu, 1, skip, cs}; }
 
 void forward(Net& N, hipStream_t s) {
-    const int n = N.n, h = N.h, w = N.wdt;
     conv(N, C0A, one(T(N, T_X), 32), 0, T(N, T_E0A), nullptr, nullptr, s);
-    conv(N, C0B, one(T(N, T_E0A), 32), 0, T(N, T_S0), nullptr, nullptr, s);
-    dn::launch_avgpool(T(N, T_S0), T(N, T_P0), n, h, w, 32, s);
+    conv(N, C0B, one(T(N, T_E0A), 32), 0, T(N, T_S0), nullptr, nullptr, s, T(N, T_P0));
     conv(N, C1A, one(T(N, T_P0), 32), 1, T(N, T_E1A), nullptr, nullptr, s);
-    conv(N, C1B, one(T(N, T_E1A), 64), 1, T(N, T_S1), nullptr, nullptr, s);
-    dn::launch_avgpool(T(N, T_S1), T(N, T_P1), n, h / 2, w / 2, 64, s);
+    conv(N, C1B, one(T(N, T_E1A), 64), 1, T(N, T_S1), nullptr, nullptr, s, T(N, T_P1));
     conv(N, C2A, one(T(N, T_P1), 64), 2, T(N, T_E2A), nullptr, nullptr, s);
-    conv(N, C2B, one(T(N, T_E2A), 128), 2, T(N, T_S2), nullptr, nullptr, s);
-    dn::launch_avgpool(T(N, T_S2), T(N, T_P2), n, h / 4, w / 4, 128, s);
+    conv(N, C2B, one(T(N, T_E2A), 128), 2, T(N, T_S2), nullptr, nullptr, s, T(N, T_P2));
     conv(N, C3A, one(T(N, T_P2), 128), 3, T(N, T_E3A), nullptr, nullptr, s);
     conv(N, C3B, one(T(N, T_E3A), 128), 3, T(N, T_B3), nullptr, nullptr, s);
     conv(N, D2A, cat(T(N, T_B3), 128, T(N, T_S2), 128), 2, T(N, T_D2A), nullptr, nullptr, s);
@@ -285,7 +285,7 @@ void forward(Net& N, hipStream_t s) {
     conv(N, D1B, one(T(N, T_D1A), 64), 1, T(N, T_D1B), nullptr, nullptr, s);
     conv(N, D0A, cat(T(N, T_D1B), 64, T(N, T_S0), 32), 0, T(N, T_D0A), nullptr, nullptr, s);
     conv(N, D0B, one(T(N, T_D0A), 32), 0, T(N, T_D0B), nullptr, nullptr, s);
-    conv(N, OUT, one(T(N, T_D0B), 32), 0, nullptr, N.o.as<float>(), T(N, T_X), s);
+    conv(N, OUT, one(T(N, T_D0B), 32), 0, nullptr, N.o.as<float>(), N.rgb4.as<bf16>(), s);
 }
 
 // Gradients of the mean L1 loss into N.g (cleared first); loss into N.loss.
@@ -377,6 +377,7 @@ int uploadInput(Net& N, const float* x9, hipStream_t s) {
     for (size_t p = 0; p < P; ++p)
         for (int c = 0; c < dn::kInChannels; ++c) buf[p * dn::kInPad + c] = toBf16(x9[p * dn::kInChannels + c]);
     HIPCHK(hipMemcpyAsync(N.t[T_X].p, buf.data(), buf.size() * 2, hipMemcpyHostToDevice, s));
+    dn::launch_input_rgb(T(N, T_X), (long long)P, N.rgb4.as<bf16>(), s);
     HIPCHK(hipStreamSynchronize(s));
     return BTE_OK;
 }
@@ -405,7 +406,8 @@ int denoise_view(const RenderView& v, float4* out) {
     const int hp = (v.height + 7) / 8 * 8, wp = (v.width + 7) / 8 * 8;
     // the flip ensemble runs as one batch of N.flips mirrored images
     CHECK_RC(ensureWorkspace(N, N.flips, hp, wp, false));
-    dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, wp, hp, N.flips, T(N, T_X), v.stream);
+    dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, wp, hp, N.flips, T(N, T_X),
+                        N.rgb4.as<bf16>(), v.stream);
     forward(N, v.stream);
     dn::launch_output(N.o.as<float>(), wp, hp, v.width, v.height, N.flips, out, v.fb, v.stream);
     return BTE_OK;
@@ -430,7 +432,7 @@ int denoiser_present() {
 void denoiser_release() {
     Net& N = net();
     for (DBuf& b : N.t) b.release();
-    N.o.release(); N.target.release(); N.loss.release(); N.wgPart.release();
+    N.o.release(); N.target.release(); N.loss.release(); N.wgPart.release(); N.rgb4.release();
     N.denoised.release();
     N.outW = N.outH = 0;
     N.enabled = false;
@@ -712,7 +714,7 @@ int bte_denoiser_capture(int slot, int which) {
     if (which == 0) {
         if (!v.aovAlb) return fail(BTE_ERR_STATE, "capture of inputs needs the AOVs (bte_enable_aov)");
         dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, v.width, v.height, 1,
-                            N.dsX.as<bf16>() + (size_t)slot * P * dn::kInPad, v.stream);
+                            N.dsX.as<bf16>() + (size_t)slot * P * dn::kInPad, nullptr, v.stream);
     } else {
         dn::launch_log_target(v.accum, (long long)P, N.dsT.as<float>() + (size_t)slot * P * 3, v.stream);
     }
@@ -744,6 +746,7 @@ int bte_denoiser_train(int steps, int batch, float lr, float weightDecay, unsign
             HIPCHK(hipMemcpyAsync(N.target.as<float>() + (size_t)b * P * 3, N.dsT.as<float>() + (size_t)slot * P * 3,
                                   P * 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
         }
+        dn::launch_input_rgb(T(N, T_X), (long long)batch * (long long)P, N.rgb4.as<bf16>(), s);
         forward(N, s);
         backward(N, s);
         adamStep(N, lr, weightDecay, s);
