@@ -58,23 +58,36 @@ def profile_value(name, key):
 
 
 def cpu_baseline(sd, seconds):
+    """The oracle (binned-SAH BVH2, nearer child first) on the host's cores, on
+    a bounded sample of the workload's pixel-samples."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")     # read when the OpenMP runtime starts
+    os.environ.setdefault("OMP_PLACES", "cores")
     import oracle
-    ref = oracle.OracleScene(sd, env_cdf=None)
-    threads = oracle.lib().ora_num_threads()
-    W, H = sd.width, sd.height
-    rs = np.random.default_rng(1234)
-    n = 65536
-    t0 = time.perf_counter()
-    ref.trace_samples(rs.integers(0, W * H, n).astype(np.int32), W, H, 0, sd.max_depth)
-    dt = max(time.perf_counter() - t0, 1e-3)
-    n2 = int(min(60_000_000, max(n, n * seconds / dt)))
-    pix = rs.integers(0, W * H, n2).astype(np.int32)
-    t0 = time.perf_counter()
-    ref.trace_samples(pix, W, H, 0, sd.max_depth)
-    dt2 = time.perf_counter() - t0
+    path, build = oracle.baseline_library()
+    info = oracle.cpu_info()
+    with oracle.using_library(path) as L:
+        ref = oracle.OracleScene(sd, env_cdf=None)
+        threads = L.ora_num_threads()
+        W, H = sd.width, sd.height
+        rs = np.random.default_rng(1234)
+        n = 65536
+        t0 = time.perf_counter()
+        ref.trace_samples(rs.integers(0, W * H, n).astype(np.int32), W, H, 0, sd.max_depth)
+        dt = max(time.perf_counter() - t0, 1e-3)
+        n2 = int(min(60_000_000, max(n, n * seconds / dt)))
+        pix = rs.integers(0, W * H, n2).astype(np.int32)
+        t0 = time.perf_counter()
+        ref.trace_samples(pix, W, H, 0, sd.max_depth)
+        dt2 = time.perf_counter() - t0
+        del ref
     return {"value": round(n2 / dt2 / 1e6, 4), "unit": "Msamples/s", "cores": int(threads), "kind": "port",
             "sample": f"{n2} random pixels of frame 0 (1 spp each) of the same 1920x1080 depth-12 workload, "
-                      f"{dt2:.1f} s on {threads} OpenMP threads ({os.cpu_count()} logical CPUs visible)"}
+                      f"{dt2:.1f} s on {threads} OpenMP threads",
+            "build": build, "bvh": "binned-SAH BVH2 (16 bins), nearer child first",
+            "cpu": {"model": info["model"], "logical_cpus": info["logical_cpus"],
+                    "physical_cores": info["physical_cores"],
+                    "smt": round(info["logical_cpus"] / max(1, info["physical_cores"]), 2),
+                    "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}}
 
 
 def relmse(x, ref):

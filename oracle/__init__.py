@@ -24,8 +24,48 @@ _p = ctypes.c_void_p
 _u32 = ctypes.c_uint32
 
 
+_V4_PATH = os.path.join(_HERE, "_build", "libbte_oracle_v4.so")
+_V4_FLAGS = {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl", "avx2", "bmi2", "fma", "movbe"}
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def cpu_info():
+    """CPU model, logical CPUs, physical cores and the flags of this host (/proc/cpuinfo)."""
+    model, flags, cores = "unknown", set(), set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name":
+                    model = v
+                elif k == "flags" and not flags:
+                    flags = set(v.split())
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    logical = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": logical, "physical_cores": len(cores) or logical, "flags": flags}
+
+
+def baseline_library():
+    """(path, build) of the oracle build the CPU baseline should time: the
+    x86-64-v4 build where this CPU has its features, else the portable one."""
+    if _V4_FLAGS <= cpu_info()["flags"] and os.path.exists(_V4_PATH):
+        return _V4_PATH, "g++ -O3 -march=x86-64-v4 -fopenmp -ffp-contract=off"
+    return _LIB_PATH, "g++ -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off"
 
 
 def lib():

@@ -677,7 +677,7 @@ static inline bool bsdfIsDiffuse(const OraBsdf& b) {
 }
 
 // ---------------------------------------------------------------------------
-// Scene, BVH (oracle's own median-split BVH2) and watertight intersection
+// Scene, BVH (oracle's own binned-SAH BVH2) and watertight intersection
 // ---------------------------------------------------------------------------
 struct Node { float lo[3], hi[3]; int left, right, start, count; };
 
@@ -690,55 +690,138 @@ struct Scene {
     float envRot = 0.0f;
 };
 
-static void buildNode(Scene& s, std::vector<float>& cent, int nodeIdx, int start, int count, int depth) {
-    Node& nd = s.nodes[nodeIdx];
-    for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
-    for (int i = start; i < start + count; ++i) {
-        const OraTriangle& t = s.d.triangles[s.triOrder[i]];
-        const F3 v[3] = {t.v0, t.v1, t.v2};
-        for (int j = 0; j < 3; ++j) {
-            const float c[3] = {v[j].x, v[j].y, v[j].z};
-            for (int k = 0; k < 3; ++k) { nd.lo[k] = std::min(nd.lo[k], c[k]); nd.hi[k] = std::max(nd.hi[k], c[k]); }
+// Binned surface-area-heuristic build (16 bins per axis on the centroid
+// bounds; cost 1 per node visit plus 1 per triangle test, weighted by child
+// area): a tree of the quality a production builder gives, so the oracle's
+// traversal is a fair CPU baseline.  Any tree gives the same hits (the
+// triangle test clamps to the triangle's own box), so the shape only affects
+// speed.  Centroids that coincide fall back to an object-median split.
+constexpr int kSahBins = 16;
+constexpr int kMaxLeaf = 8;
+
+static float boxArea(const float lo[3], const float hi[3]) {
+    const float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+static void makeLeaf(Node& nd, int start, int count) {
+    nd.left = nd.right = -1;
+    nd.start = start;
+    nd.count = count;
+}
+
+static void buildNode(Scene& s, const std::vector<float>& cent, const std::vector<float>& tbox, int nodeIdx,
+                      int start, int count, int depth) {
+    {
+        Node& nd = s.nodes[nodeIdx];
+        for (int k = 0; k < 3; ++k) { nd.lo[k] = INFINITY; nd.hi[k] = -INFINITY; }
+        for (int i = start; i < start + count; ++i) {
+            const float* b = &tbox[6 * (size_t)s.triOrder[i]];
+            for (int k = 0; k < 3; ++k) { nd.lo[k] = std::min(nd.lo[k], b[k]); nd.hi[k] = std::max(nd.hi[k], b[3 + k]); }
         }
-    }
-    if (count <= 4 || depth > 60) {
-        nd.left = nd.right = -1; nd.start = start; nd.count = count;
-        return;
+        if (count <= 2 || depth > 60) {
+            makeLeaf(nd, start, count);
+            return;
+        }
     }
     float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int i = start; i < start + count; ++i)
-        for (int k = 0; k < 3; ++k) { clo[k] = std::min(clo[k], cent[3 * s.triOrder[i] + k]); chi[k] = std::max(chi[k], cent[3 * s.triOrder[i] + k]); }
-    int axis = 0;
-    for (int k = 1; k < 3; ++k) if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
-    const int mid = start + count / 2;
-    std::nth_element(s.triOrder.begin() + start, s.triOrder.begin() + mid, s.triOrder.begin() + start + count,
-                     [&](int a, int b) {
-                         const float ca = cent[3 * a + axis], cb = cent[3 * b + axis];
-                         return ca < cb || (ca == cb && a < b);
-                     });
+        for (int k = 0; k < 3; ++k) {
+            clo[k] = std::min(clo[k], cent[3 * (size_t)s.triOrder[i] + k]);
+            chi[k] = std::max(chi[k], cent[3 * (size_t)s.triOrder[i] + k]);
+        }
+    const float parentArea = boxArea(s.nodes[nodeIdx].lo, s.nodes[nodeIdx].hi);
+    float bestCost = INFINITY;
+    int bestAxis = -1, bestSplit = 0;
+    for (int k = 0; k < 3; ++k) {
+        const float ext = chi[k] - clo[k];
+        if (!(ext > 0.0f)) continue;
+        const float scale = (float)kSahBins / ext;
+        int cnt[kSahBins] = {};
+        float blo[kSahBins][3], bhi[kSahBins][3];
+        for (int b = 0; b < kSahBins; ++b)
+            for (int j = 0; j < 3; ++j) { blo[b][j] = INFINITY; bhi[b][j] = -INFINITY; }
+        for (int i = start; i < start + count; ++i) {
+            const int id = s.triOrder[i];
+            const int b = std::min(kSahBins - 1, (int)((cent[3 * (size_t)id + k] - clo[k]) * scale));
+            ++cnt[b];
+            const float* tb = &tbox[6 * (size_t)id];
+            for (int j = 0; j < 3; ++j) { blo[b][j] = std::min(blo[b][j], tb[j]); bhi[b][j] = std::max(bhi[b][j], tb[3 + j]); }
+        }
+        float rArea[kSahBins];
+        int rCnt[kSahBins];
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        int n = 0;
+        for (int b = kSahBins - 1; b > 0; --b) {
+            n += cnt[b];
+            for (int j = 0; j < 3; ++j) { lo[j] = std::min(lo[j], blo[b][j]); hi[j] = std::max(hi[j], bhi[b][j]); }
+            rCnt[b] = n;
+            rArea[b] = n ? boxArea(lo, hi) : 0.0f;
+        }
+        for (int j = 0; j < 3; ++j) { lo[j] = INFINITY; hi[j] = -INFINITY; }
+        n = 0;
+        for (int b = 0; b < kSahBins - 1; ++b) {      // split between bins b and b + 1
+            n += cnt[b];
+            for (int j = 0; j < 3; ++j) { lo[j] = std::min(lo[j], blo[b][j]); hi[j] = std::max(hi[j], bhi[b][j]); }
+            if (n == 0 || rCnt[b + 1] == 0) continue;
+            const float cost = 1.0f + (boxArea(lo, hi) * (float)n + rArea[b + 1] * (float)rCnt[b + 1]) / parentArea;
+            if (cost < bestCost) { bestCost = cost; bestAxis = k; bestSplit = b + 1; }
+        }
+    }
+    int mid;
+    if (bestAxis >= 0) {
+        if (count <= kMaxLeaf && (float)count <= bestCost) {
+            makeLeaf(s.nodes[nodeIdx], start, count);
+            return;
+        }
+        const float scale = (float)kSahBins / (chi[bestAxis] - clo[bestAxis]);
+        const float c0 = clo[bestAxis];
+        mid = (int)(std::stable_partition(s.triOrder.begin() + start, s.triOrder.begin() + start + count,
+                                          [&](int id) {
+                                              return std::min(kSahBins - 1,
+                                                              (int)((cent[3 * (size_t)id + bestAxis] - c0) * scale)) <
+                                                     bestSplit;
+                                          }) -
+                    s.triOrder.begin());
+    } else {
+        // coincident centroids: object median on index order
+        if (count <= kMaxLeaf) {
+            makeLeaf(s.nodes[nodeIdx], start, count);
+            return;
+        }
+        mid = start + count / 2;
+    }
     const int l = (int)s.nodes.size();
     s.nodes.push_back(Node{});
     s.nodes.push_back(Node{});
     s.nodes[nodeIdx].left = l; s.nodes[nodeIdx].right = l + 1; s.nodes[nodeIdx].start = 0; s.nodes[nodeIdx].count = 0;
-    buildNode(s, cent, l, start, mid - start, depth + 1);
-    buildNode(s, cent, l + 1, mid, start + count - mid, depth + 1);
+    buildNode(s, cent, tbox, l, start, mid - start, depth + 1);
+    buildNode(s, cent, tbox, l + 1, mid, start + count - mid, depth + 1);
 }
 
 static void buildBvh(Scene& s) {
     const int T = s.d.triangleCount;
     s.triOrder.resize(T);
     for (int i = 0; i < T; ++i) s.triOrder[i] = i;
-    std::vector<float> cent(3 * (size_t)T);
+    std::vector<float> cent(3 * (size_t)T), tbox(6 * (size_t)T);
     for (int i = 0; i < T; ++i) {
         const OraTriangle& t = s.d.triangles[i];
         cent[3 * i + 0] = (t.v0.x + t.v1.x + t.v2.x) / 3.0f;
         cent[3 * i + 1] = (t.v0.y + t.v1.y + t.v2.y) / 3.0f;
         cent[3 * i + 2] = (t.v0.z + t.v1.z + t.v2.z) / 3.0f;
+        const F3 v[3] = {t.v0, t.v1, t.v2};
+        float* b = &tbox[6 * (size_t)i];
+        b[0] = b[1] = b[2] = INFINITY;
+        b[3] = b[4] = b[5] = -INFINITY;
+        for (int j = 0; j < 3; ++j) {
+            const float c[3] = {v[j].x, v[j].y, v[j].z};
+            for (int k = 0; k < 3; ++k) { b[k] = std::min(b[k], c[k]); b[3 + k] = std::max(b[3 + k], c[k]); }
+        }
     }
     s.nodes.clear();
     s.nodes.reserve(2 * (size_t)T + 1);
     s.nodes.push_back(Node{});
-    if (T > 0) buildNode(s, cent, 0, 0, T, 0);
+    if (T > 0) buildNode(s, cent, tbox, 0, 0, T, 0);
     else { s.nodes[0].left = s.nodes[0].right = -1; s.nodes[0].start = 0; s.nodes[0].count = 0;
            for (int k = 0; k < 3; ++k) { s.nodes[0].lo[k] = INFINITY; s.nodes[0].hi[k] = -INFINITY; } }
 }
@@ -840,17 +923,40 @@ static inline bool hitBox(const WtRay& r, const Node& n, float tmin, float tmax)
 
 struct Hit { int tri = -1; float t = 0, u = 0, v = 0; };
 
+// Slab test of a node's box; the entry distance in t0.
+static inline bool boxEntry(const WtRay& r, const Node& n, float tmin, float tmax, float& t0) {
+    float t1 = tmax;
+    t0 = tmin;
+    const float o[3] = {r.o.x, r.o.y, r.o.z};
+    for (int k = 0; k < 3; ++k) {
+        const float a = (n.lo[k] - o[k]) * r.inv[k];
+        const float b = (n.hi[k] - o[k]) * r.inv[k];
+        const float lo = std::fmin(a, b), hi = std::fmax(a, b);
+        t0 = std::fmax(t0, lo);
+        t1 = std::fmin(t1, hi);
+    }
+    return t0 <= t1;
+}
+
+// Children are visited nearer entry first; a postponed child is skipped when a
+// hit found meanwhile lies before its entry, which is hitBox against the
+// current best distance (its slab interval is already known to be non-empty).
 static Hit closestHit(const Scene& s, const F3& o, const F3& d, float tmin, float tmax) {
     Hit best;
     const WtRay r = makeWtRay(o, d);
     if (!r.valid || s.d.triangleCount == 0) return best;
     float bestT = tmax;
     int stack[128];
+    float entry[128];
     int sp = 0;
-    stack[sp++] = 0;
+    float e0;
+    if (!boxEntry(r, s.nodes[0], tmin, bestT, e0)) return best;
+    stack[sp] = 0;
+    entry[sp++] = e0;
     while (sp) {
-        const Node& n = s.nodes[stack[--sp]];
-        if (!hitBox(r, n, tmin, bestT)) continue;
+        --sp;
+        if (entry[sp] > bestT) continue;
+        const Node& n = s.nodes[stack[sp]];
         if (n.left < 0) {
             for (int i = n.start; i < n.start + n.count; ++i) {
                 const int id = s.triOrder[i];
@@ -861,9 +967,23 @@ static Hit closestHit(const Scene& s, const F3& o, const F3& d, float tmin, floa
                     }
                 }
             }
-        } else {
-            stack[sp++] = n.left;
-            stack[sp++] = n.right;
+            continue;
+        }
+        float el, er;
+        const bool hl = boxEntry(r, s.nodes[n.left], tmin, bestT, el);
+        const bool hr = boxEntry(r, s.nodes[n.right], tmin, bestT, er);
+        if (hl && hr) {
+            const bool leftFirst = el <= er;
+            stack[sp] = leftFirst ? n.right : n.left;
+            entry[sp++] = leftFirst ? er : el;
+            stack[sp] = leftFirst ? n.left : n.right;
+            entry[sp++] = leftFirst ? el : er;
+        } else if (hl) {
+            stack[sp] = n.left;
+            entry[sp++] = el;
+        } else if (hr) {
+            stack[sp] = n.right;
+            entry[sp++] = er;
         }
     }
     return best;
