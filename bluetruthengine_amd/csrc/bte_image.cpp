@@ -5,8 +5,13 @@
 // tables and the DECODE / RECEIVE / EXTEND procedures (Annex C, F.2.2),
 // dequantization, the inverse DCT of A.3.3 evaluated in binary64 and rounded,
 // the level shift, and the JFIF YCbCr -> RGB conversion.  Chroma planes
-// sampled below full resolution are replicated (nearest sample).  Decoded
-// pixels can differ by a unit from a decoder that uses a fixed-point IDCT.
+// sampled at half resolution are upsampled the way stb_image does it
+// (stb_image.h:3465-3527, chosen at :3909-3918, rows at :3927-3938): a triangle filter
+// weighting the nearer sample 3/4 and the farther 1/4, rounded as
+// (3 near + far + 2) >> 2 along one axis and (3 t0 + t1 + 8) >> 4 over the
+// vertical sums t of both, edges replicated; other factors repeat the sample.
+// Decoded pixels can differ by a unit from a decoder that uses a fixed-point
+// IDCT.
 #include "bte_image.h"
 
 #include <algorithm>
@@ -121,6 +126,75 @@ struct Component {
     int pred = 0;             // DC predictor
     int planeW = 0, planeH = 0;
     std::vector<uint8_t> plane;
+};
+
+// Produces a component's samples at full resolution, one output row at a
+// time.  hs x vs = hmax / h x vmax / v; the component holds ceil(W / hs)
+// x ceil(H v / vmax) samples.  The two source rows of output row y: the
+// nearer one, row floor(y / vs), and the farther one, the row above for the
+// upper half of a pair and the row below for the lower half (clamped to the
+// plane), advanced as a two-row window per vs output rows.
+struct Upsampler {
+    int hs = 1, vs = 1, wLo = 0, rowsLo = 0;
+    int ystep = 0, ypos = 0, line0 = 0, line1 = 0;
+    std::vector<uint8_t> row;
+    void init(const Component& c, int W, int H, int hmax, int vmax) {
+        hs = std::max(1, hmax / c.h);
+        vs = std::max(1, vmax / c.v);
+        wLo = (W + hs - 1) / hs;
+        rowsLo = (H * c.v + vmax - 1) / vmax;
+        ystep = vs >> 1;
+        ypos = 0;
+        line0 = line1 = 0;
+        row.assign((size_t)wLo * hs + 1, 0);
+    }
+    void nextRow(const Component& c) {
+        const bool bottom = ystep >= (vs >> 1);
+        const uint8_t* nearRow = &c.plane[(size_t)(bottom ? line1 : line0) * c.planeW];
+        const uint8_t* farRow = &c.plane[(size_t)(bottom ? line0 : line1) * c.planeW];
+        uint8_t* out = row.data();
+        if (hs == 1 && vs == 1) {
+            std::memcpy(out, nearRow, (size_t)wLo);
+        } else if (hs == 1 && vs == 2) {
+            for (int i = 0; i < wLo; ++i) out[i] = (uint8_t)((3 * nearRow[i] + farRow[i] + 2) >> 2);
+        } else if (hs == 2 && vs == 1) {
+            if (wLo == 1) {
+                out[0] = out[1] = nearRow[0];
+            } else {
+                out[0] = nearRow[0];
+                out[1] = (uint8_t)((nearRow[0] * 3 + nearRow[1] + 2) >> 2);
+                for (int i = 1; i < wLo - 1; ++i) {
+                    const int n = 3 * nearRow[i] + 2;
+                    out[2 * i] = (uint8_t)((n + nearRow[i - 1]) >> 2);
+                    out[2 * i + 1] = (uint8_t)((n + nearRow[i + 1]) >> 2);
+                }
+                out[2 * (wLo - 1)] = (uint8_t)((nearRow[wLo - 2] * 3 + nearRow[wLo - 1] + 2) >> 2);
+                out[2 * (wLo - 1) + 1] = nearRow[wLo - 1];
+            }
+        } else if (hs == 2 && vs == 2) {
+            int t1 = 3 * nearRow[0] + farRow[0];
+            if (wLo == 1) {
+                out[0] = out[1] = (uint8_t)((t1 + 2) >> 2);
+            } else {
+                out[0] = (uint8_t)((t1 + 2) >> 2);
+                for (int i = 1; i < wLo; ++i) {
+                    const int t0 = t1;
+                    t1 = 3 * nearRow[i] + farRow[i];
+                    out[2 * i - 1] = (uint8_t)((3 * t0 + t1 + 8) >> 4);
+                    out[2 * i] = (uint8_t)((3 * t1 + t0 + 8) >> 4);
+                }
+                out[2 * wLo - 1] = (uint8_t)((t1 + 2) >> 2);
+            }
+        } else {
+            for (int i = 0; i < wLo; ++i)
+                for (int j = 0; j < hs; ++j) out[(size_t)i * hs + j] = nearRow[i];
+        }
+        if (++ystep >= vs) {
+            ystep = 0;
+            line0 = line1;
+            if (++ypos < rowsLo) ++line1;
+        }
+    }
 };
 
 // kIdct[x][u] = C(u) / 2 * cos((2x + 1) u pi / 16), C(0) = 1 / sqrt(2).
@@ -335,15 +409,15 @@ bool decode_jpeg(const uint8_t* data, size_t size, int& width, int& height, std:
     width = W;
     height = H;
     rgba.assign((size_t)W * (size_t)H * 4, 255);
-    for (int y = 0; y < H; ++y)
+    // per component: its full-resolution row for the current output row
+    std::vector<Upsampler> up(comps.size());
+    for (size_t c = 0; c < comps.size(); ++c) up[c].init(comps[c], W, H, hmax, vmax);
+    for (int y = 0; y < H; ++y) {
+        for (size_t c = 0; c < comps.size(); ++c) up[c].nextRow(comps[c]);
         for (int x = 0; x < W; ++x) {
             uint8_t* o = &rgba[((size_t)y * W + x) * 4];
             double s[3] = {0.0, 0.0, 0.0};
-            for (size_t c = 0; c < comps.size(); ++c) {
-                const Component& cp = comps[c];
-                const int sx = x * cp.h / hmax, sy = y * cp.v / vmax;
-                s[c] = cp.plane[(size_t)sy * cp.planeW + sx];
-            }
+            for (size_t c = 0; c < comps.size(); ++c) s[c] = up[c].row[(size_t)x];
             if (comps.size() == 1) {
                 o[0] = o[1] = o[2] = (uint8_t)s[0];
             } else if (adobeRgb) {
@@ -357,6 +431,7 @@ bool decode_jpeg(const uint8_t* data, size_t size, int& width, int& height, std:
                 o[2] = clampByte(yy + 1.772 * cb);
             }
         }
+    }
     return true;
 }
 

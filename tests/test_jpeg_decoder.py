@@ -4,9 +4,11 @@ written here from ITU-T T.81: forward DCT, quantization, zig-zag, DC
 prediction, run/size coding with canonical Huffman codes, byte stuffing,
 restart markers.  The decoder must return exactly the encoder's own
 reconstruction (dequantized coefficients -> inverse DCT -> level shift ->
-JFIF YCbCr -> RGB, all in binary64), allowing one unit where a value sits on
-a rounding boundary.  Cases: 4:4:4 colour, 4:2:0 colour with a size that is
-not a multiple of the MCU, grayscale, restart intervals."""
+JFIF YCbCr -> RGB, all in binary64), with chroma planes upsampled as
+stb_image does it (the reference's loader), allowing one unit where a value
+sits on a rounding boundary.  Cases: 4:4:4 colour, 4:2:0 colour with a size
+that is not a multiple of the MCU, grayscale, restart intervals, and a
+hand-computed 4:2:0 seam."""
 import os
 import subprocess
 
@@ -152,10 +154,53 @@ def encode(planes, sampling, quant, restart=0):
     return bytes(out), recon
 
 
+def _upsample(plane, hs, vs, H, W):
+    """A component plane at full resolution as stb_image produces it
+    (stb_image.h:3465-3527 and its row loop at :3927-3938): output row y takes
+    the nearer source row y // vs and the farther one (above for the upper
+    row of a pair, below for the lower, clamped); 2x factors use the 3:1
+    triangle filter with stb's integer rounding, others repeat the sample."""
+    wlo = -(-W // hs)
+    rows = -(-H // vs)
+    out = np.zeros((H, W), np.int64)
+    for y in range(H):
+        m = y // vs
+        if vs == 2:
+            far = max(m - 1, 0) if y % 2 == 0 else min(m + 1, rows - 1)
+        else:
+            far = m
+        near_row = plane[m, :wlo].astype(np.int64)
+        far_row = plane[far, :wlo].astype(np.int64)
+        if hs == 2:
+            t = 3 * near_row + far_row if vs == 2 else 4 * near_row
+            row = np.zeros(2 * wlo, np.int64)
+            if vs == 2:
+                row[0], row[-1] = (t[0] + 2) >> 2, (t[-1] + 2) >> 2
+            else:
+                row[0], row[-1] = near_row[0], near_row[-1]
+                if wlo > 1:
+                    row[1] = (3 * near_row[0] + near_row[1] + 2) >> 2
+                    row[-2] = (near_row[-2] * 3 + near_row[-1] + 2) >> 2
+            for i in range(1, wlo):
+                if vs == 2:
+                    row[2 * i - 1] = (3 * t[i - 1] + t[i] + 8) >> 4
+                    row[2 * i] = (3 * t[i] + t[i - 1] + 8) >> 4
+                elif 0 < i < wlo - 1:
+                    row[2 * i] = (3 * near_row[i] + 2 + near_row[i - 1]) >> 2
+                    row[2 * i + 1] = (3 * near_row[i] + 2 + near_row[i + 1]) >> 2
+            if wlo == 1:
+                row[:] = (t[0] + 2) >> 2 if vs == 2 else near_row[0]
+        elif vs == 2:
+            row = (3 * near_row + far_row + 2) >> 2
+        else:
+            row = np.repeat(near_row, hs)
+        out[y] = row[:W]
+    return out
+
+
 def _to_rgb(recon, sampling, H, W):
     hmax, vmax = max(s[0] for s in sampling), max(s[1] for s in sampling)
-    ys, xs = np.mgrid[0:H, 0:W]
-    s = [r[ys * v // vmax, xs * h // hmax] for r, (h, v) in zip(recon, sampling)]
+    s = [_upsample(r, hmax // h, vmax // v, H, W) for r, (h, v) in zip(recon, sampling)]
     if len(s) == 1:
         return np.repeat(s[0][..., None], 3, -1)
     y, cb, cr = s[0], s[1] - 128.0, s[2] - 128.0
@@ -223,6 +268,33 @@ def test_decoder_reproduces_the_encoders_reconstruction(decoder, tmp_path, case)
     # the chroma planes; a grayscale file keeps only the luma)
     src = np.repeat(y[..., None], 3, -1) if case == "gray" else rgb
     assert np.abs(got[..., :3].astype(np.float64) - src).mean() < (12.0 if case == "420" else 8.0)
+
+
+def test_chroma_upsampling_hand_computed(decoder, tmp_path):
+    # 32 x 32 4:2:0 picture, Y = 128 everywhere; the 16 x 16 chroma planes are
+    # constant per 8 x 8 block, with values whose DC coefficient the
+    # quantizer keeps exactly (8 (c - 128) a multiple of QUANT[0] = 9): Cb
+    # 101 | 173 left | right, Cr 92 / 164 top / bottom.  By hand, the 3:1
+    # filter gives at the seams Cb (3 * 404 + 692 + 8) >> 4 = 119 and
+    # (3 * 692 + 404 + 8) >> 4 = 155 (t = 4 c), Cr (4 * 440 + 8) >> 4 = 110 and
+    # (4 * 584 + 8) >> 4 = 146 (t = 3 * 92 + 164, 3 * 164 + 92); nearest
+    # sampling would jump 101 -> 173 and 92 -> 164.
+    H = W = 32
+    y = np.full((H, W), 128.0)
+    cb = np.full((16, 16), 101.0)
+    cb[:, 8:] = 173.0
+    cr = np.full((16, 16), 92.0)
+    cr[8:, :] = 164.0
+    data, recon = encode([y, cb, cr], [(2, 2), (1, 1), (1, 1)], QUANT)
+    assert np.array_equal(recon[1][:16, :16], cb) and np.array_equal(recon[2][:16, :16], cr)
+    cb_row = np.array([101] * 15 + [119, 155] + [173] * 15, np.float64)
+    cr_col = np.array([92] * 15 + [110, 146] + [164] * 15, np.float64)
+    CB, CR = np.meshgrid(cb_row - 128.0, cr_col - 128.0)
+    want = np.clip(np.floor(np.stack([128 + 1.402 * CR, 128 - 0.344136 * CB - 0.714136 * CR, 128 + 1.772 * CB], -1)
+                            + 0.5), 0, 255)
+    got = _decode(decoder, data, tmp_path)[..., :3]
+    assert np.array_equal(got, want)
+    assert np.array_equal(_to_rgb(recon, [(2, 2), (1, 1), (1, 1)], H, W), want)
 
 
 def test_decoder_rejects_what_it_does_not_decode(decoder, tmp_path):
