@@ -624,6 +624,7 @@ def main():
                     "request_note": f"bytes the traversal asks the memory system for ({node_bytes} B per node visit, "
                                     f"{TRI_BYTES} B per triangle test, {EXTEND_RAY_BYTES} B ray + hit); the L2 and "
                                     "MALL serve all but the traffic above",
+                    "l2_hit_rate": profile_value("extend_c3_hbm_traffic.txt", "extend_l2_hit_rate"),
                     "valu_utilization_pct": profile_value("c3_occupancy_divergence.md", "extend_valu_utilization"),
                     "waves_per_cu": profile_value("c3_occupancy_divergence.md", "extend_waves_per_cu"),
                     "limiter": "vector-ALU issue of divergent traversal waves (lanes busy per VALU instruction "
