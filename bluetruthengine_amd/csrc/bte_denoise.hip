@@ -435,6 +435,65 @@ __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, f32x4 (&acc)[MT]
     }
 }
 
+// Loads of one 32-channel slice of a tile with its halo, without branches:
+// piece p of the tile (pixel p / 4 of the TR x TC window, 8 channels p % 4)
+// reads from a clamped, in-image address and keeps zero outside the image, so
+// every lane issues its loads back to back.  The slice lies in src0 (at half
+// resolution when up0) or in src1.
+struct SliceSrc {
+    const bf16* base;   // first channel of the slice
+    int stride;         // channels per pixel of that tensor
+    int sh;             // 1: read through the nearest 2x upsample
+};
+__device__ __forceinline__ SliceSrc sliceSrc(const ConvArgs& a, int c) {
+    SliceSrc r;
+    const bool s0 = c < a.c0;
+    r.base = s0 ? a.src0 + c : a.src1 + (c - a.c0);
+    r.stride = s0 ? a.c0 : a.c1;
+    r.sh = (s0 && a.up0) ? 1 : 0;
+    return r;
+}
+template <int TC, int kPieces, int kPer>
+__device__ __forceinline__ void loadSlice(const ConvArgs& a, const SliceSrc& src, int n, int x0, int y0,
+                                          bf16x8 (&va)[kPer]) {
+    const int hS = a.h >> src.sh, wS = a.w_ >> src.sh;
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) {
+        const int piece = threadIdx.x + j * kBlock;
+        const int pc = piece < kPieces ? piece : kPieces - 1;
+        const int sidx = pc >> 2, q = pc & 3;
+        const int yy = y0 - 1 + sidx / TC, xx = x0 - 1 + sidx % TC;
+        const bool in = piece < kPieces && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_;
+        const int yc = min(max(yy, 0), a.h - 1), xc = min(max(xx, 0), a.w_ - 1);
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(
+            src.base + (((size_t)n * hS + (yc >> src.sh)) * wS + (xc >> src.sh)) * src.stride + 8 * q);
+        va[j] = in ? v : zero8();
+    }
+}
+
+// The 9 taps of one staged slice: per tap the weight fragments of the lane's
+// output channels, then per M tile its pixel fragment and NT MFMAs.
+template <int MT, int NT, int RW, int TC>
+__device__ __forceinline__ void tapsFromLds(const bf16* sA, const bf16* sW, int wave, int g, int r16,
+                                            f32x4 (&acc)[MT][NT]) {
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        const int oy = t / 3, ox = t % 3;   // tap offset + 1: tile row / column shift
+        bf16x8 B[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+            B[nt] = *reinterpret_cast<const bf16x8*>(sW + ((t * NT + nt) * 16 + r16) * kLdsRow + 8 * g);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            const int ry = wave * RW + (mt >> 1), col = (mt & 1) * 16 + r16;
+            const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[nt], acc[mt][nt], 0, 0, 0);
+        }
+    }
+}
+
 // Single-slice layers (32 input channels): persistent blocks keep the weights
 // of all 9 taps in LDS for their whole run and walk tiles blockIdx.x,
 // blockIdx.x + gridDim.x, ...; the next tile's input (with halo) is loaded
@@ -450,7 +509,6 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r16 = lane & 15;
     const int coBase = blockIdx.y * NT * 16;
-    const int hh = a.h >> 1, wh = a.w_ >> 1;
     {   // weights of the one 32-channel slice, all taps
         bf16x8 vw[kPerW];
 #pragma unroll
@@ -477,28 +535,11 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
         biasv[nt] = (a.bias && co < a.cout) ? a.bias[co] : 0.0f;
     }
     bf16x8 va[kPerA];
+    const SliceSrc src = sliceSrc(a, 0);
     auto loadTile = [&](int tile) {
         const int tx = tile % tilesX, rest = tile / tilesX;
         const int ty = rest % tilesY, n = rest / tilesY;
-        const int x0 = tx * kTileW, y0 = ty * TH;
-#pragma unroll
-        for (int j = 0; j < kPerA; ++j) {
-            const int piece = threadIdx.x + j * kBlock;
-            va[j] = zero8();
-            if (piece < kPiecesA) {
-                const int sidx = piece >> 2, q = piece & 3;
-                const int yy = y0 - 1 + sidx / TC, xx = x0 - 1 + sidx % TC;
-                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_) {
-                    const bf16* src;
-                    if (a.c0 > 0)
-                        src = a.up0 ? a.src0 + (((size_t)n * hh + (yy >> 1)) * wh + (xx >> 1)) * 32
-                                    : a.src0 + (((size_t)n * a.h + yy) * a.w_ + xx) * 32;
-                    else
-                        src = a.src1 + (((size_t)n * a.h + yy) * a.w_ + xx) * 32;
-                    va[j] = *reinterpret_cast<const bf16x8*>(src + 8 * q);
-                }
-            }
-        }
+        loadSlice<TC, kPiecesA, kPerA>(a, src, n, tx * kTileW, ty * TH, va);
     };
     int tile = blockIdx.x;
     if (tile < numTiles) loadTile(tile);
@@ -517,22 +558,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int oy = t / 3, ox = t % 3;
-            bf16x8 B[NT];
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                B[nt] = *reinterpret_cast<const bf16x8*>(sW + ((t * NT + nt) * 16 + r16) * kLdsRow + 8 * g);
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const int ry = wave * RW + (mt >> 1), col = (mt & 1) * 16 + r16;
-                const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[nt], acc[mt][nt], 0, 0, 0);
-            }
-        }
+        tapsFromLds<MT, NT, RW, TC>(sA, sW, wave, g, r16, acc);
         const int tx = tile % tilesX, rest = tile / tilesX;
         const int ty = rest % tilesY, n = rest / tilesY;
         tileEpilogue<MT, NT, RW, EPI>(a, acc, biasv, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
@@ -555,7 +581,6 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
     const int g = lane >> 4, r16 = lane & 15;
     const int coBase = blockIdx.y * NT * 16;
     const int cin = a.c0 + a.c1, slices = cin / 32;
-    const int hh = a.h >> 1, wh = a.w_ >> 1;
     float biasv[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
@@ -566,33 +591,15 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
     auto load = [&](int tile, int slice) {
         const int tx = tile % tilesX, rest = tile / tilesX;
         const int ty = rest % tilesY, n = rest / tilesY;
-        const int x0 = tx * kTileW, y0 = ty * TH, c = slice * 32;
-#pragma unroll
-        for (int j = 0; j < kPerA; ++j) {
-            const int piece = threadIdx.x + j * kBlock;
-            va[j] = zero8();
-            if (piece < kPiecesA) {
-                const int sidx = piece >> 2, q = piece & 3;
-                const int yy = y0 - 1 + sidx / TC, xx = x0 - 1 + sidx % TC;
-                if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_) {
-                    const bf16* src;
-                    if (c < a.c0)
-                        src = a.up0 ? a.src0 + (((size_t)n * hh + (yy >> 1)) * wh + (xx >> 1)) * a.c0 + c
-                                    : a.src0 + (((size_t)n * a.h + yy) * a.w_ + xx) * a.c0 + c;
-                    else
-                        src = a.src1 + (((size_t)n * a.h + yy) * a.w_ + xx) * a.c1 + (c - a.c0);
-                    va[j] = *reinterpret_cast<const bf16x8*>(src + 8 * q);
-                }
-            }
-        }
+        const int c = slice * 32;
+        loadSlice<TC, kPiecesA, kPerA>(a, sliceSrc(a, c), n, tx * kTileW, ty * TH, va);
 #pragma unroll
         for (int j = 0; j < kPerW; ++j) {
             const int piece = threadIdx.x + j * kBlock;
-            if (piece < kPiecesW) {
-                const int row = piece >> 2, q = piece & 3;
-                const int t = row / (NT * 16), r = row % (NT * 16);
-                vw[j] = *reinterpret_cast<const bf16x8*>(a.w + ((size_t)(coBase + r) * 9 + t) * cin + c + 8 * q);
-            }
+            const int pc = piece < kPiecesW ? piece : kPiecesW - 1;
+            const int row = pc >> 2, q = pc & 3;
+            const int t = row / (NT * 16), r = row % (NT * 16);
+            vw[j] = *reinterpret_cast<const bf16x8*>(a.w + ((size_t)(coBase + r) * 9 + t) * cin + c + 8 * q);
         }
     };
     f32x4 acc[MT][NT];
@@ -618,22 +625,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
         const bool last = slice + 1 == slices;
         const int nTile = last ? tile + (int)gridDim.x : tile, nSlice = last ? 0 : slice + 1;
         if (nTile < numTiles) load(nTile, nSlice);   // in flight while this slice is computed
-#pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int oy = t / 3, ox = t % 3;
-            bf16x8 B[NT];
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt)
-                B[nt] = *reinterpret_cast<const bf16x8*>(sW + ((t * NT + nt) * 16 + r16) * kLdsRow + 8 * g);
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const int ry = wave * RW + (mt >> 1), col = (mt & 1) * 16 + r16;
-                const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[nt], acc[mt][nt], 0, 0, 0);
-            }
-        }
+        tapsFromLds<MT, NT, RW, TC>(sA, sW, wave, g, r16, acc);
         if (last) {
             const int tx = tile % tilesX, rest = tile / tilesX;
             const int ty = rest % tilesY, n = rest / tilesY;
