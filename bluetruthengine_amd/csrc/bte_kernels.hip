@@ -945,11 +945,24 @@ BTE_DEV void generateOne(const DevScene& sc, const WaveBuffers& wb, long long sl
     }
 }
 
+// Queue order of a frame's paths: 8 x 8 pixel tiles (a wave's 64 slots are one
+// tile) when both image sides are multiples of 8, else rows.  Path ids stay
+// frame * pixels + pixel; only the queue slots (which paths share a wave) move.
+BTE_DEV int tilePixel(int q, int W, int H) {
+    if ((W | H) & 7) return q;
+    const int tile = q >> 6, within = q & 63;
+    const int tilesX = W >> 3;
+    const int tx = tile % tilesX, ty = tile / tilesX;
+    return (ty * 8 + (within >> 3)) * W + tx * 8 + (within & 7);
+}
+
 __global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb, long long paths, int pixels) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= paths) return;
     const int frame0 = *wb.frameBase;
-    generateOne(sc, wb, i, (uint32_t)i, (int)(i % pixels), frame0 + (int)(i / pixels));
+    const int f = (int)(i / pixels);
+    const int pixel = tilePixel((int)(i % pixels), sc.width, sc.height);
+    generateOne(sc, wb, i, (uint32_t)((long long)f * pixels + pixel), pixel, frame0 + f);
     if (i == 0) wb.counters[kCtrRays] = (int)paths;
 }
 
@@ -964,8 +977,10 @@ __global__ __launch_bounds__(kBlock) void k_generate_batch(const DevScene* __res
     if (i >= b.paths) return;
     const DevScene sc = fetchScene(scenes, s);
     const int frame0 = *wb.frameBase;
-    generateOne(sc, wb, b.pathBase + i, (uint32_t)(b.pathBase + i), (int)(i % b.pixels),
-                frame0 + (int)(i / b.pixels));
+    const int f = (int)(i / b.pixels);
+    const int pixel = tilePixel((int)(i % b.pixels), sc.width, sc.height);
+    generateOne(sc, wb, b.pathBase + i, (uint32_t)(b.pathBase + (long long)f * b.pixels + pixel), pixel,
+                frame0 + f);
     if (i == 0) segCounts(wb, 0, kSegRays)[s] = b.paths;
     if (s == 0 && i == 0) wb.counters[kCtrRays] = (int)totalSlots;
 }
