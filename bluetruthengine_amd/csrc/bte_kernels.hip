@@ -945,23 +945,38 @@ BTE_DEV void generateOne(const DevScene& sc, const WaveBuffers& wb, long long sl
     }
 }
 
-// Queue order of a frame's paths: 8 x 8 pixel tiles (a wave's 64 slots are one
-// tile) when both image sides are multiples of 8, else rows.  Path ids stay
-// frame * pixels + pixel; only the queue slots (which paths share a wave) move.
-BTE_DEV int tilePixel(int q, int W, int H) {
-    if ((W | H) & 7) return q;
-    const int tile = q >> 6, within = q & 63;
-    const int tilesX = W >> 3;
-    const int tx = tile % tilesX, ty = tile / tilesX;
-    return (ty * 8 + (within >> 3)) * W + tx * 8 + (within & 7);
+// Queue order of a pass's paths: a wave's 64 slots hold Fg frames (16, 8, 4,
+// 2 or 1: the largest that divides the pass's frame count) of a tile of
+// 64 / Fg pixels (2x2 up to 8x8), so its primary rays start nearly alike;
+// images whose sides are not multiples of the tile keep plain row order.
+// Path ids stay frame * pixels + pixel; only the queue slots move.
+BTE_DEV void slotPath(long long i, int pixels, int frames, int W, int H, int& frame, int& pixel) {
+    const int fg = (frames % 16 == 0) ? 16 : (frames % 8 == 0) ? 8 : (frames % 4 == 0) ? 4 : (frames % 2 == 0) ? 2 : 1;
+    const int tpx = 64 / fg;                                // pixels per tile: 8x8, 8x4, 4x4, 4x2, 2x2
+    const int tw = (tpx >= 32) ? 8 : (tpx >= 8 ? 4 : 2), th = tpx / tw;
+    if (W % tw || H % th) {
+        frame = (int)(i / pixels);
+        pixel = (int)(i % pixels);
+        return;
+    }
+    const long long chunk = (long long)fg * pixels;
+    const int c = (int)(i / chunk);
+    const long long r = i - (long long)c * chunk;
+    const long long group = r >> 6;
+    const int within = (int)(r & 63);
+    frame = c * fg + within % fg;
+    const int inTile = within / fg;
+    const int tilesX = W / tw;
+    const int tx = (int)(group % tilesX), ty = (int)(group / tilesX);
+    pixel = (ty * th + inTile / tw) * W + tx * tw + inTile % tw;
 }
 
 __global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb, long long paths, int pixels) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= paths) return;
     const int frame0 = *wb.frameBase;
-    const int f = (int)(i / pixels);
-    const int pixel = tilePixel((int)(i % pixels), sc.width, sc.height);
+    int f, pixel;
+    slotPath(i, pixels, (int)(paths / pixels), sc.width, sc.height, f, pixel);
     generateOne(sc, wb, i, (uint32_t)((long long)f * pixels + pixel), pixel, frame0 + f);
     if (i == 0) wb.counters[kCtrRays] = (int)paths;
 }
@@ -977,8 +992,8 @@ __global__ __launch_bounds__(kBlock) void k_generate_batch(const DevScene* __res
     if (i >= b.paths) return;
     const DevScene sc = fetchScene(scenes, s);
     const int frame0 = *wb.frameBase;
-    const int f = (int)(i / b.pixels);
-    const int pixel = tilePixel((int)(i % b.pixels), sc.width, sc.height);
+    int f, pixel;
+    slotPath(i, b.pixels, b.paths / b.pixels, sc.width, sc.height, f, pixel);
     generateOne(sc, wb, b.pathBase + i, (uint32_t)(b.pathBase + (long long)f * b.pixels + pixel), pixel,
                 frame0 + f);
     if (i == 0) segCounts(wb, 0, kSegRays)[s] = b.paths;
