@@ -597,25 +597,31 @@ def main():
     if not args.no_instrumented:
         r.set_instrumentation(1)
         step()
-        s1 = r.stats()
+        s1 = r.stats()                 # kernel split of the configuration measured above
+        tail = r.set_tail_bounce(0)
+        step()
+        s0 = r.stats()                 # every bounce a per-bounce launch: the extension kernel's roofline
+        r.set_tail_bounce(tail)
         r.set_instrumentation(2)
         step()
-        s2 = r.stats()
+        s2 = r.stats()                 # node / triangle counts (per-bounce passes)
         r.set_instrumentation(0)
-        launches = max(1, s1.extendLaunches)
-        avg_ms = s1.extendMs / launches
+        launches = max(1, s0.extendLaunches)
+        avg_ms = s0.extendMs / launches
         node_bytes = NODE_BYTES * (cfg["bvhWidth"] // 2 if cfg["bvhWidth"] in (4, 8) else 1)
         request_total = EXTEND_RAY_BYTES * s2.extensionRays + node_bytes * s2.extendNodes + TRI_BYTES * s2.extendTris
         request_per_launch = request_total / launches
-        # bytes that cross HBM per launch: rocprofv3 FETCH_SIZE + WRITE_SIZE of
-        # this kernel on this workload, calibrated with a copy of known size
-        traffic = profile_value("extend_c3_hbm_traffic.txt", "bytes_per_launch")
+        # bytes that cross HBM: rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel
+        # on this workload, bounce by bounce, calibrated with a copy of known
+        # size, over the same launches (every bounce, no tail pass)
+        per_bounce = [profile_value("extend_c3_hbm_traffic.txt", f"extend_bytes_bounce_{b}") for b in range(launches)]
+        traffic = sum(per_bounce) / launches if all(v is not None for v in per_bounce) else None
         achieved = traffic / (avg_ms * 1e-3) / 1e9 if traffic else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None, "peak": HBM_PEAK_GBPS,
                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4) if achieved else None,
                     "traffic": int(traffic) if traffic else None,
                     "traffic_source": "profiles/extend_c3_hbm_traffic.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE per "
-                                      "launch, copy-calibrated)",
+                                      "launch, copy-calibrated; every bounce a launch, no tail pass)",
                     "kernel": (f"k_extend_pool (BVH{cfg['bvhWidth']} closest hit)" if cfg["bvhWidth"] in (4, 8)
                                else "k_extend (BVH2 closest hit)"), "launches": launches,
                     "avg_launch_ms": round(avg_ms, 4),

@@ -374,6 +374,15 @@ class Renderer:
         _call("bte_get_config", ctypes.byref(cfg))
         return {k: getattr(cfg, k) for k, _ in cfg._fields_}
 
+    def set_tail_bounce(self, bounce):
+        """Sets BteConfig.tailBounce (0 = per-bounce passes to the end), the rest unchanged; returns the old value."""
+        cfg = BteConfig()
+        _call("bte_get_config", ctypes.byref(cfg))
+        old = cfg.tailBounce
+        cfg.tailBounce = int(bounce)
+        _call("bte_set_config", ctypes.byref(cfg))
+        return old
+
     def autotune(self, frames):
         """Picks the fastest launch shape for `frames`-frame passes at the scene's resolution (resets the
         accumulation); returns (best ms, config dict)."""
