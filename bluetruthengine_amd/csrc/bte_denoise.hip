@@ -1118,19 +1118,19 @@ __device__ __forceinline__ float fireflyLum(float4 c) {
 
 // Image b of the `flips` images (1, 2 or 4) is the frame mirrored left-right
 // when bit 0 of b is set and top-bottom when bit 1 is set, in the top-left
-// w x h corner of its wp x hp tensor (zero padding right and below).
+// w x h corner of its wp x hp tensor (zero padding right and below).  One
+// thread per tensor position: it computes the features of the frame's pixel
+// there once and writes them to that pixel's mirrored position in every image.
 __global__ void k_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
                            int flips, bf16* x, bf16* rgb4) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long plane = (long long)wp * hp;
-    if (i >= plane * flips) return;
-    const int b = (int)(i / plane);
-    const long long j = i - (long long)b * plane;
+    if (j >= plane) return;
     const int px = (int)(j % wp), py = (int)(j / wp);
     float v[kInChannels] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (px < w && py < h) {
-        const int sx = (b & 1) ? w - 1 - px : px, sy = (b & 2) ? h - 1 - py : py;
-        const size_t q = (size_t)sy * w + sx;
+    const bool in = px < w && py < h;
+    if (in) {
+        const size_t q = (size_t)py * w + px;
         float4 c = rgb[q];
         c.x = fmaxf(c.x, 0.0f);
         c.y = fmaxf(c.y, 0.0f);
@@ -1141,7 +1141,7 @@ __global__ void k_features(const float4* rgb, const float4* alb, const float4* n
         float nb = 0.0f;
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
-                const int nx = sx + dx, ny = sy + dy;
+                const int nx = px + dx, ny = py + dy;
                 if ((dx | dy) == 0 || nx < 0 || ny < 0 || nx >= w || ny >= h) continue;
                 float4 cn = rgb[(size_t)ny * w + nx];
                 cn.x = fmaxf(cn.x, 0.0f);
@@ -1170,20 +1170,23 @@ __global__ void k_features(const float4* rgb, const float4* alb, const float4* n
     for (int k = 0; k < kInPad / 8; ++k) out[k] = zero8();
 #pragma unroll
     for (int c = 0; c < kInChannels; ++c) out[c / 8][c % 8] = (bf16)v[c];
-    bf16x8* dst = reinterpret_cast<bf16x8*>(x + (size_t)i * kInPad);
+    bf16x4 r4;
+    r4[0] = out[0][0]; r4[1] = out[0][1]; r4[2] = out[0][2]; r4[3] = (bf16)0.0f;
+    for (int b = 0; b < flips; ++b) {
+        // padding stays in place; frame pixels go to their mirrored position
+        const int ox = (in && (b & 1)) ? w - 1 - px : px, oy = (in && (b & 2)) ? h - 1 - py : py;
+        const size_t o = (size_t)b * (size_t)plane + (size_t)oy * wp + ox;
+        bf16x8* dst = reinterpret_cast<bf16x8*>(x + o * kInPad);
 #pragma unroll
-    for (int k = 0; k < kInPad / 8; ++k) dst[k] = out[k];
-    if (rgb4) {
-        bf16x4 r;
-        r[0] = out[0][0]; r[1] = out[0][1]; r[2] = out[0][2]; r[3] = (bf16)0.0f;
-        *reinterpret_cast<bf16x4*>(rgb4 + (size_t)i * 4) = r;
+        for (int k = 0; k < kInPad / 8; ++k) dst[k] = out[k];
+        if (rgb4) *reinterpret_cast<bf16x4*>(rgb4 + o * 4) = r4;
     }
 }
 
 void launch_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
                      int flips, bf16* x, bf16* rgb4, hipStream_t s) {
-    const long long total = (long long)wp * hp * flips;
-    hipLaunchKernelGGL(k_features, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, rgb, alb, nrm, w, h, wp,
+    const long long plane = (long long)wp * hp;
+    hipLaunchKernelGGL(k_features, dim3((unsigned)((plane + 255) / 256)), dim3(256), 0, s, rgb, alb, nrm, w, h, wp,
                        hp, flips, x, rgb4);
 }
 
