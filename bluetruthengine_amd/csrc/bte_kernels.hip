@@ -48,6 +48,33 @@ BTE_DEV int wave_append(int* ctr, bool pred) {
     return base + prefix;
 }
 
+// Block-aggregated append: the waves' counts meet in LDS (sh: 2 ints per
+// wave) and one lane adds the block's total, so a counter that every wave of
+// the grid appends to sees one atomic per block instead of one per wave
+// (same-address device atomics serialise).  Every wave of the block must call
+// it the same number of times; consecutive calls may share sh.
+BTE_DEV int block_append(int* ctr, bool pred, int* sh) {
+    constexpr int kWaves = kBlock / 64;
+    const unsigned long long mask = __ballot(pred);
+    const int w = (int)(threadIdx.x >> 6);
+    if (lane_id() == 0) sh[w] = __popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = 0;
+        for (int i = 0; i < kWaves; ++i) {
+            const int c = sh[i];
+            sh[kWaves + i] = total;
+            total += c;
+        }
+        const int base = total ? atomicAdd(ctr, total) : 0;
+        for (int i = 0; i < kWaves; ++i) sh[kWaves + i] += base;
+    }
+    __syncthreads();
+    const unsigned lo = (unsigned)(mask & 0xffffffffull), hi = (unsigned)(mask >> 32);
+    const int prefix = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+    return pred ? sh[kWaves + w] + prefix : -1;
+}
+
 BTE_DEV void wave_add_u64(unsigned long long* ctr, uint32_t v) {
     unsigned long long x = v;
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
@@ -1462,10 +1489,11 @@ struct ShadeSinks {
 
 // Shades one 64-slot row of a category queue: lanes with `active` set hold
 // the hits at list[li].  Every lane of the wave must call it (wave-wide
-// appends).
+// appends); with blk (LDS, 2 ints per wave) the appends are block-wide and
+// every wave of the block must call it the same number of times.
 template <int T>
 BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, int li, bool active,
-                      const ShadeSinks& out) {
+                      const ShadeSinks& out, int* blk = nullptr) {
     const int cur = bounce & 1, nxt = cur ^ 1;
     const int* list = wb.catQ + (size_t)T * (size_t)wb.cap;
     constexpr bool kDiffuseType = (T == BTE_BSDF_DIFFUSE || T == BTE_BSDF_MICROFACET || T == BTE_BSDF_DISNEY);
@@ -1520,7 +1548,7 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
         scatter<T>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
         survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
     }
-    const int slot = out.survBase + wave_append(out.survCtr, survive);
+    const int slot = out.survBase + (blk ? block_append(out.survCtr, survive, blk) : wave_append(out.survCtr, survive));
     if (survive) {
         wb.rayO[nxt][slot] = make_float4(o.x, o.y, o.z, brdfPdf);
         wb.rayD[nxt][slot] = make_float4(d.x, d.y, d.z, eta);
@@ -1549,7 +1577,8 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
                 else if (k == wb.kindEnv) { if (sc.hasEnv) rec = envNEE<T>(sc, dIn, thrIn, its, frame, bsdf, e0, e1); }
                 else if (spots) rec.ok = sc.spotCount > 0;     // a request: the spot pass evaluates every spotlight
             }
-            const int sl = out.shBase + wave_append(out.shCtr + k * out.shStride, rec.ok);
+            int* const ctr = out.shCtr + k * out.shStride;
+            const int sl = out.shBase + (blk ? block_append(ctr, rec.ok, blk) : wave_append(ctr, rec.ok));
             if (rec.ok) {
                 const size_t idx = (size_t)k * (size_t)wb.cap + (size_t)sl;
                 if (spots) {
@@ -1576,24 +1605,33 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
     const int count = BATCH ? segSlots(wb, bounce, kSegCat + T) : ctr[kCtrCat + T];
     const int grab = 64 * wb.grabRows;
     int* wk = work_group(wb, bounce, kWorkShade + T);
-    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(wk, grab, count)) {
-        if constexpr (!BATCH) {
-            const ShadeSinks out{ctrNext + kCtrRays, 0, ctr + kCtrShadow, 1, 0};
+    if constexpr (!BATCH) {
+        // the block's waves shade their rows in step and append block-wide:
+        // one atomic per block and row on the shared queue counters
+        __shared__ int blk[2 * (kBlock / 64)];
+        const ShadeSinks out{ctrNext + kCtrRays, 0, ctr + kCtrShadow, 1, 0};
+        for (int chunk = first_chunk(grab);;) {
+            const bool has = chunk < count;
+            if (!__syncthreads_or(has)) break;
             for (int row = 0; row < wb.grabRows; ++row) {
                 const int li = chunk + row * 64 + lane_id();
-                shadeRow<T>(sc, wb, bounce, maxDepth, li, li < count, out);
+                shadeRow<T>(sc, wb, bounce, maxDepth, li, has && li < count, out, blk);
             }
-        } else {
-            const SegChunk k = segChunk(wb, bounce, kSegCat + T, chunk);
-            if (k.left <= 0) continue;
-            const int begin = (int)wb.slots[k.s].pathBase;
-            const DevScene S = fetchScene(scenes, k.s);
-            const ShadeSinks out{segCounts(wb, bounce + 1, kSegRays) + k.s, begin,
-                                 segCounts(wb, bounce, kSegShadow) + k.s, wb.sceneCount, begin};
-            for (int row = 0; row < wb.grabRows; ++row) {
-                const int e = row * 64 + lane_id();
-                shadeRow<T>(S, wb, bounce, maxDepth, k.first + e, e < k.left, out);
-            }
+            if (has) chunk = next_chunk(wk, grab, count);
+        }
+        return;
+    }
+    // a batch appends per wave: its counters are per scene
+    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(wk, grab, count)) {
+        const SegChunk k = segChunk(wb, bounce, kSegCat + T, chunk);
+        if (k.left <= 0) continue;
+        const int begin = (int)wb.slots[k.s].pathBase;
+        const DevScene S = fetchScene(scenes, k.s);
+        const ShadeSinks out{segCounts(wb, bounce + 1, kSegRays) + k.s, begin,
+                             segCounts(wb, bounce, kSegShadow) + k.s, wb.sceneCount, begin};
+        for (int row = 0; row < wb.grabRows; ++row) {
+            const int e = row * 64 + lane_id();
+            shadeRow<T>(S, wb, bounce, maxDepth, k.first + e, e < k.left, out);
         }
     }
 }
