@@ -602,7 +602,8 @@ int renderWavefront(int nframes) {
         for (int i = 0; i < b; ++i) {
             const int* cb = ctrHost.data() + (size_t)i * kCtrStride;
             c.stats.extensionRays += cb[kCtrRays];
-            c.stats.hits += cb[kCtrHits];
+            c.stats.hits += cb[kCtrHits];                                     // tail pass
+            for (int k = 1; k < kNumCats; ++k) c.stats.hits += cb[kCtrCat + k];   // per-bounce passes
             for (int k = 0; k < kinds; ++k) c.stats.shadowRays += (k == kindSpot) ? cb[kCtrSpotRays] : cb[kCtrShadow + k];
             if (i < 64) c.stats.raysPerBounce[i] += cb[kCtrRays];
         }

@@ -1459,7 +1459,9 @@ __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers 
                 }
             int* catCtr = BATCH ? segCounts(wb, bounce, kSegCat + k) + s : ctr + kCtrCat + k;
             sBase[k] = run ? begin + atomicAdd(catCtr, run) : 0;
-            if (run) atomicAdd(ctr + kCtrHits, run);
+            // a single scene's hits are the sum of its category counts (added
+            // up on the host): no second atomic on a shared counter
+            if (BATCH && run) atomicAdd(ctr + kCtrHits, run);
         }
         __syncthreads();
         for (int j = 0; j < kCatSub; ++j) {
@@ -2568,23 +2570,23 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
                            scenes);
     }
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(k_categorize<B>, dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+    hipLaunchKernelGGL(k_categorize<B>, dim3(ls.shadeCatBlocks[kCatMiss]), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     if (B) launch_segment_rows(wb, bounce, kSegCat, kNumCats, s);
     // a BSDF category is shaded only when the scene uses that type
     if (wb.catMask & (1 << BTE_BSDF_DIFFUSE))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIFFUSE, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIFFUSE, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_DIFFUSE]), dim3(kBlock), 0, s, sc, wb, bounce,
                            maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_DIELECTRIC))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIELECTRIC, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIELECTRIC, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_DIELECTRIC]), dim3(kBlock), 0, s, sc, wb,
                            bounce, maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_MIRROR))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_MIRROR, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_MIRROR, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_MIRROR]), dim3(kBlock), 0, s, sc, wb, bounce,
                            maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_MICROFACET))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_MICROFACET, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_MICROFACET, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_MICROFACET]), dim3(kBlock), 0, s, sc, wb,
                            bounce, maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_DISNEY))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_DISNEY, B>), dim3(ls.shadeBlocks), dim3(kBlock), 0, s, sc, wb, bounce,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_DISNEY, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_DISNEY]), dim3(kBlock), 0, s, sc, wb, bounce,
                            maxDepth, scenes);
     if (B) {
         launch_segment_rows(wb, bounce + 1, kSegRays, 1, s);
@@ -2794,6 +2796,16 @@ void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, d
     hipLaunchKernelGGL(k_env_guides, dim3(H + 1), dim3(256), 0, s, marg, cond, W, H, margGuide, condGuide, monotone);
 }
 
+// Resident blocks per CU of one category's shading kernel (single scene and
+// batch instantiations).
+template <int T>
+static int shadeOccupancy() {
+    int a = 0, b = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_shade<T, false>, kBlock, 0);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shade<T, true>, kBlock, 0);
+    return std::max(1, std::min(a, b));
+}
+
 // Resident blocks per CU of the traversal passes for one stack size and BVH width.
 template <int L, int W>
 static int travOccupancy() {
@@ -2820,6 +2832,7 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     static int cachedDev = -1;
     static int cus = 1, occS = 1, occMT = 1, occL[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};   // [width][stack]
     static int occT[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};                                  // tail [width][stack]
+    static int occCat[kNumCats] = {1, 1, 1, 1, 1, 1};                                           // shading per category
     if (device != cachedDev) {
         hipDeviceProp_t prop;
         (void)hipGetDeviceProperties(&prop, device);
@@ -2827,6 +2840,17 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
         int o = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_shade<BTE_BSDF_DISNEY>, kBlock, 0);
         occS = std::max(1, o);
+        {   // the categoriser shades the misses
+            int a = 0, b = 0;
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_categorize<false>, kBlock, 0);
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_categorize<true>, kBlock, 0);
+            occCat[kCatMiss] = std::max(1, std::min(a, b));
+        }
+        occCat[BTE_BSDF_DIFFUSE] = shadeOccupancy<BTE_BSDF_DIFFUSE>();
+        occCat[BTE_BSDF_DIELECTRIC] = shadeOccupancy<BTE_BSDF_DIELECTRIC>();
+        occCat[BTE_BSDF_MIRROR] = shadeOccupancy<BTE_BSDF_MIRROR>();
+        occCat[BTE_BSDF_MICROFACET] = shadeOccupancy<BTE_BSDF_MICROFACET>();
+        occCat[BTE_BSDF_DISNEY] = shadeOccupancy<BTE_BSDF_DISNEY>();
         int m = 0, t = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_megakernel, kBlock, 0);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&t, k_trace_rays<kOvfStack>, kBlock, 0);
@@ -2860,6 +2884,7 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     ls.extendBlocks = cus * occ;
     ls.connectBlocks = cus * occ;
     ls.shadeBlocks = cus * occS;
+    for (int k = 0; k < kNumCats; ++k) ls.shadeCatBlocks[k] = cus * occCat[k];
     ls.auxBlocks = cus * occMT;
     ls.tailBlocks = cus * occT[ls.bvhWidth == 2 ? 0 : (ls.bvhWidth == 4 ? 1 : 2)][ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
     return ls;
