@@ -223,12 +223,12 @@ BTE_DEV WtRay makeWtRay(V3 o, V3 d) {
     const float dz = comp(d, kz);
     if (dz < 0.0f) { const int t = kx; kx = ky; ky = t; }
     r.kx = kx; r.ky = ky; r.kz = kz;
-    r.Sx = comp(d, kx) / dz;
-    r.Sy = comp(d, ky) / dz;
-    r.Sz = 1.0f / dz;
     r.inv[0] = 1.0f / d.x;
     r.inv[1] = 1.0f / d.y;
     r.inv[2] = 1.0f / d.z;
+    r.Sx = comp(d, kx) / dz;
+    r.Sy = comp(d, ky) / dz;
+    r.Sz = kz == 0 ? r.inv[0] : (kz == 1 ? r.inv[1] : r.inv[2]);   // = 1 / dz, one division fewer
     return r;
 }
 
@@ -316,7 +316,13 @@ BTE_DEV void push(const StackRef<L>& s, int& sp, int v) {
 template <int L>
 BTE_DEV int pop(const StackRef<L>& s, int& sp) {
     --sp;
-    return (sp < L) ? s.lds[sp * kBlock] : s.ovf[sp - L];
+    // an LDS read for every lane (clamped slot), the overflow slice only
+    // where the stack is that deep (a volatile read, so that the two loads
+    // are not merged into one flat load, whose wait covers every
+    // outstanding memory access of the wave)
+    int v = s.lds[min(sp, L - 1) * kBlock];
+    if (sp >= L) v = *(const volatile int*)(s.ovf + (sp - L));
+    return v;
 }
 
 // ANY = false: closest hit, returns triangle id (or -1) with t,u,v.
@@ -372,10 +378,35 @@ BTE_DEV int traverse(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, con
     return ANY ? 0 : bestTri;
 }
 
+// Compare-exchange by key, written as selects: a branch per pair costs the
+// wave an exec-mask save/restore and the moves of both arms.
 BTE_DEV void sortPair(float& ka, int& ca, float& kb, int& cb) {
-    if (kb < ka) {
-        const float tk = ka; ka = kb; kb = tk;
-        const int tc = ca; ca = cb; cb = tc;
+    const bool sw = kb < ka;
+    const float lo = sw ? kb : ka, hi = sw ? ka : kb;
+    const int clo = sw ? cb : ca, chi = sw ? ca : cb;
+    ka = lo; kb = hi;
+    ca = clo; cb = chi;
+}
+
+// Pushes c3, c2, c1 in this order (c1 on top), each only when its flag is
+// set.  While all three slots lie in LDS the stores are unconditional: the
+// slot of an unset child is overwritten by the next store or left above the
+// top.  Same stack contents as three conditional pushes.
+template <int L>
+BTE_DEV void push3(const StackRef<L>& s, int& sp, int c1, bool v1, int c2, bool v2, int c3, bool v3) {
+    if (sp + 2 < L) {
+        int p = sp;
+        s.lds[p * kBlock] = c3;
+        p += v3 ? 1 : 0;
+        s.lds[p * kBlock] = c2;
+        p += v2 ? 1 : 0;
+        s.lds[p * kBlock] = c1;
+        p += v1 ? 1 : 0;
+        sp = p;
+    } else {
+        if (v3) push(s, sp, c3);
+        if (v2) push(s, sp, c2);
+        if (v1) push(s, sp, c1);
     }
 }
 
@@ -421,9 +452,7 @@ BTE_DEV int traverse4(const DevScene& sc, V3 o, V3 d, float tmin, float tmax, co
                 }
             }
             if (k0 != kNone) {
-                if (k3 != kNone) push(stk, sp, c3);
-                if (k2 != kNone) push(stk, sp, c2);
-                if (k1 != kNone) push(stk, sp, c1);
+                push3(stk, sp, c1, k1 != kNone, c2, k2 != kNone, c3, k3 != kNone);
                 node = c0;
                 continue;
             }
@@ -1168,9 +1197,7 @@ BTE_DEV bool closestStep(const DevScene& S, const WtRay& r, int& node, int& sp, 
         sortPair(k1, c1, k3, c3);
         sortPair(k1, c1, k2, c2);
         if (k0 != kNone) {
-            if (k3 != kNone) push(stk, sp, c3);
-            if (k2 != kNone) push(stk, sp, c2);
-            if (k1 != kNone) push(stk, sp, c1);
+            push3(stk, sp, c1, k1 != kNone, c2, k2 != kNone, c3, k3 != kNone);
             node = c0;
             return false;
         }
@@ -1244,9 +1271,7 @@ BTE_DEV int anyStep(const DevScene& S, const WtRay& r, float tmax, int& node, in
             else if (k3 != kNone) { k0 = k3; c0 = c3; k3 = kNone; }
         }
         if (k0 != kNone) {
-            if (k3 != kNone) push(stk, sp, c3);
-            if (k2 != kNone) push(stk, sp, c2);
-            if (k1 != kNone) push(stk, sp, c1);
+            push3(stk, sp, c1, k1 != kNone, c2, k2 != kNone, c3, k3 != kNone);
             node = c0;
             return 0;
         }
@@ -2043,9 +2068,7 @@ BTE_DEV int queryStep(const DevScene& S, const WtRay& r, bool any, float& tmax, 
         sortPair(k1, c1, k3, c3);
         sortPair(k1, c1, k2, c2);
         if (k0 != kNone) {
-            if (k3 != kNone) push(stk, sp, c3);
-            if (k2 != kNone) push(stk, sp, c2);
-            if (k1 != kNone) push(stk, sp, c1);
+            push3(stk, sp, c1, k1 != kNone, c2, k2 != kNone, c3, k3 != kNone);
             node = c0;
             return 0;
         }
