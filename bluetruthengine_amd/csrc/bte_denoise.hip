@@ -15,6 +15,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 
 constexpr int kBlock = 256;
 
@@ -224,6 +225,138 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_lds(ConvArgs a) {
     }
 }
 
+// Epilogue of a tile kernel: bias, residual, ReLU, mask and the store of one
+// wave's MT x 16 pixels x NT x 16 channels.  The tap loops take the weights as
+// the MFMA's A operand and the pixels as B, so the accumulators are
+// transposed: element i of lane (g, r16) in acc[mt][nt] is output channel
+// coBase + nt * 16 + 4 g + i of pixel (wave row mt >> 1, column (mt & 1) * 16
+// + r16).  A lane therefore stores 4 consecutive channels of one pixel at once
+// (8 bytes of bf16) and reads a residual or mask the same way.  EPI 0: plain;
+// 1 (EXTRA): a residual or a mask (never both), read beforehand by epiLoad;
+// 2: with the 2x2 average pooling of the stored outputs.
+__device__ __forceinline__ bf16x4 zero4() { return __builtin_bit_cast(bf16x4, u32x2{0u, 0u}); }
+
+// Residual or mask values of a tile's outputs (4 channels per lane and
+// (mt, nt)); issued before the next tile's loads, so that waiting for them
+// does not wait for those.
+template <int MT, int NT, int RW>
+__device__ __forceinline__ void epiLoad(const ConvArgs& a, int n, int x0, int y0, int wave, int g, int r16,
+                                        int coBase, bf16x4 (&ext)[MT][NT]) {
+    const bf16* src = a.resid ? a.resid : a.mask;
+    const int srcC = a.resid ? a.residC : a.maskC;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int y = y0 + wave * RW + (mt >> 1), x = x0 + (mt & 1) * 16 + r16;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = coBase + nt * 16 + 4 * g;
+            const bool in = src != nullptr && y < a.h && x < a.w_ && co < a.cout;
+            const size_t p = in ? ((size_t)n * a.h + y) * a.w_ + x : 0;
+            ext[mt][nt] = in ? *reinterpret_cast<const bf16x4*>(src + p * srcC + co) : zero4();
+        }
+    }
+}
+
+// Biases of the lane's output channels (4 per N tile).
+template <int NT>
+__device__ __forceinline__ void laneBiases(const ConvArgs& a, int coBase, int g, float (&biasv)[NT][4]) {
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int co = coBase + nt * 16 + 4 * g + i;
+            biasv[nt][i] = (a.bias && co < a.cout) ? a.bias[co] : 0.0f;
+        }
+}
+
+template <int MT, int NT, int RW, int EPI>
+__device__ __forceinline__ void tileEpilogue(const ConvArgs& a, f32x4 (&acc)[MT][NT], const float (&biasv)[NT][4],
+                                             const bf16x4 (&ext)[MT][NT], int n, int x0, int y0, int wave, int g,
+                                             int r16, int coBase) {
+    constexpr bool EXTRA = EPI == 1;
+    if constexpr (EPI == 2) {
+        // outputs and their 2x2 averages (k_avgpool's sum order over the stored
+        // bf16 values): rows 2r, 2r + 1 of the wave are M tiles m0 = 4r + hf and
+        // m0 + 2; the horizontal partner of a pixel is the neighbouring lane
+        static_assert(RW % 2 == 0, "pooling needs row pairs within a wave");
+        const int h2 = a.h >> 1, w2 = a.w_ >> 1;
+#pragma unroll
+        for (int r = 0; r < RW / 2; ++r) {
+            const int y = y0 + wave * RW + 2 * r;
+#pragma unroll
+            for (int hf = 0; hf < 2; ++hf) {
+                const int m0 = 4 * r + hf, m1 = m0 + 2;
+                const int x = x0 + hf * 16 + r16;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int co = coBase + nt * 16 + 4 * g;
+                    const bool in = y < a.h && x < a.w_ && co < a.cout;   // h, w even: row and column pairs
+                    bf16x4 bt, bb;
+                    float top[4], bot[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        float v0 = acc[m0][nt][i], v1 = acc[m1][nt][i];
+                        if (a.bias) { v0 += biasv[nt][i]; v1 += biasv[nt][i]; }
+                        if (a.relu) { v0 = fmaxf(v0, 0.0f); v1 = fmaxf(v1, 0.0f); }
+                        bt[i] = (bf16)v0;
+                        bb[i] = (bf16)v1;
+                        top[i] = f(bt[i]);
+                        bot[i] = f(bb[i]);
+                    }
+                    const size_t p = in ? ((size_t)n * a.h + y) * a.w_ + x : 0;
+                    if (in) {
+                        *reinterpret_cast<bf16x4*>(a.dstB + p * a.dstC + co) = bt;
+                        *reinterpret_cast<bf16x4*>(a.dstB + (p + a.w_) * a.dstC + co) = bb;
+                    }
+                    float topN[4], botN[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        topN[i] = __shfl_xor(top[i], 1, 64);
+                        botN[i] = __shfl_xor(bot[i], 1, 64);
+                    }
+                    if (in && (r16 & 1) == 0) {
+                        bf16x4 pv;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) pv[i] = (bf16)(0.25f * (top[i] + topN[i] + bot[i] + botN[i]));
+                        *reinterpret_cast<bf16x4*>(a.pool + (((size_t)n * h2 + (y >> 1)) * w2 + (x >> 1)) * a.dstC +
+                                                   co) = pv;
+                    }
+                }
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        const int y = y0 + wave * RW + (mt >> 1), x = x0 + (mt & 1) * 16 + r16;
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int co = coBase + nt * 16 + 4 * g;
+            if (y >= a.h || x >= a.w_ || co >= a.cout) continue;
+            const size_t p = ((size_t)n * a.h + y) * a.w_ + x;
+            float v[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                v[i] = acc[mt][nt][i];
+                if (a.bias) v[i] += biasv[nt][i];
+                if (EXTRA && a.resid) v[i] += f(ext[mt][nt][i]);
+                if (a.relu) v[i] = fmaxf(v[i], 0.0f);
+                if (EXTRA && a.mask && !(f(ext[mt][nt][i]) > 0.0f)) v[i] = 0.0f;
+            }
+            if (a.dstF) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (co + i < a.cout) a.dstF[p * a.dstC + co + i] = v[i];
+            } else {
+                bf16x4 o;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) o[i] = (bf16)v[i];
+                *reinterpret_cast<bf16x4*>(a.dstB + p * a.dstC + co) = o;
+            }
+        }
+    }
+}
+
 // The convolution as a spatial tile per block: 4 * RW image rows x 32 pixels
 // (each wave RW rows, i.e. 2 * RW M tiles of 16 pixels) x NT * 16 output
 // channels.  For every 32-channel slice of the input, the block stages in LDS
@@ -318,120 +451,18 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX,
                 const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[nt], acc[mt][nt], 0, 0, 0);
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[nt], A, acc[mt][nt], 0, 0, 0);
             }
         }
     }
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const int y = y0 + wave * RW + (mt >> 1);
-        if (y >= a.h) continue;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int x = x0 + (mt & 1) * 16 + 4 * g + i;
-            if (x >= a.w_) continue;
-            const size_t p = ((size_t)n * a.h + y) * a.w_ + x;
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const int co = coBase + nt * 16 + r16;
-                if (co >= a.cout) continue;
-                float v = acc[mt][nt][i];
-                if (a.bias) v += a.bias[co];
-                if (a.resid) v += f(a.resid[p * a.residC + co]);
-                if (a.relu) v = fmaxf(v, 0.0f);
-                if (a.mask && !(f(a.mask[p * a.maskC + co]) > 0.0f)) v = 0.0f;
-                if (a.dstF) a.dstF[p * a.dstC + co] = v;
-                else a.dstB[p * a.dstC + co] = (bf16)v;
-            }
-        }
-    }
-}
-
-// Epilogue of a tile kernel: bias, residual, ReLU, mask and the store of one
-// wave's MT x 16 pixels x NT x 16 channels.  EPI 0: plain; 1 (EXTRA): a residual or a
-// mask), the residual / mask reads of an M tile are issued together before
-// any of them is used, so each M tile waits for global memory once; without
-// it the epilogue reads no global memory.
-template <int MT, int NT, int RW, int EPI>
-__device__ __forceinline__ void tileEpilogue(const ConvArgs& a, f32x4 (&acc)[MT][NT], const float (&biasv)[NT],
-                                             int n, int x0, int y0, int wave, int g, int r16, int coBase) {
-    constexpr bool EXTRA = EPI == 1;
-    if constexpr (EPI == 2) {
-        // outputs and their 2x2 averages (k_avgpool's sum order over the stored
-        // bf16 values): rows 2r, 2r + 1 of the wave are M tiles m0 = 4r + hf and
-        // m0 + 2, the column pairs (4g + i, 4g + i + 1) elements i, i + 1 of a lane
-        static_assert(RW % 2 == 0, "pooling needs row pairs within a wave");
-        const int h2 = a.h >> 1, w2 = a.w_ >> 1;
-#pragma unroll
-        for (int r = 0; r < RW / 2; ++r) {
-            const int y = y0 + wave * RW + 2 * r;
-#pragma unroll
-            for (int hf = 0; hf < 2; ++hf) {
-                const int m0 = 4 * r + hf, m1 = m0 + 2;
-                const int xb = x0 + hf * 16 + 4 * g;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int co = coBase + nt * 16 + r16;
-                    if (y >= a.h || xb >= a.w_ || co >= a.cout) continue;
-                    float top[4], bot[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        top[i] = bot[i] = 0.0f;
-                        if (xb + i >= a.w_) continue;   // w is even: pairs (i, i + 1) are both in or both out
-                        float v0 = acc[m0][nt][i], v1 = acc[m1][nt][i];
-                        if (a.bias) { v0 += biasv[nt]; v1 += biasv[nt]; }
-                        if (a.relu) { v0 = fmaxf(v0, 0.0f); v1 = fmaxf(v1, 0.0f); }
-                        const bf16 b0 = (bf16)v0, b1 = (bf16)v1;
-                        const size_t p = ((size_t)n * a.h + y) * a.w_ + xb + i;
-                        a.dstB[p * a.dstC + co] = b0;
-                        a.dstB[(p + a.w_) * a.dstC + co] = b1;
-                        top[i] = f(b0);
-                        bot[i] = f(b1);
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; i += 2)
-                        if (xb + i < a.w_)
-                            a.pool[(((size_t)n * h2 + (y >> 1)) * w2 + ((xb + i) >> 1)) * a.dstC + co] =
-                                (bf16)(0.25f * (top[i] + top[i + 1] + bot[i] + bot[i + 1]));
-                }
-            }
-        }
-        return;
-    }
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-        const int y = y0 + wave * RW + (mt >> 1);
-        float rv[4][NT];
-        bool keep[4][NT];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int x = x0 + (mt & 1) * 16 + 4 * g + i;
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const int co = coBase + nt * 16 + r16;
-                const bool in = y < a.h && x < a.w_ && co < a.cout;
-                const size_t p = in ? ((size_t)n * a.h + y) * a.w_ + x : 0;
-                rv[i][nt] = (EXTRA && a.resid && in) ? f(a.resid[p * a.residC + co]) : 0.0f;
-                keep[i][nt] = !(EXTRA && a.mask && in) || f(a.mask[p * a.maskC + co]) > 0.0f;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int x = x0 + (mt & 1) * 16 + 4 * g + i;
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const int co = coBase + nt * 16 + r16;
-                if (y >= a.h || x >= a.w_ || co >= a.cout) continue;
-                const size_t p = ((size_t)n * a.h + y) * a.w_ + x;
-                float v = acc[mt][nt][i];
-                if (a.bias) v += biasv[nt];
-                if (EXTRA && a.resid) v += rv[i][nt];
-                if (a.relu) v = fmaxf(v, 0.0f);
-                if (EXTRA && !keep[i][nt]) v = 0.0f;
-                if (a.dstF) a.dstF[p * a.dstC + co] = v;
-                else a.dstB[p * a.dstC + co] = (bf16)v;
-            }
-        }
+    float biasv[NT][4];
+    laneBiases<NT>(a, coBase, g, biasv);
+    bf16x4 ext[MT][NT];
+    if (a.resid || a.mask) {
+        epiLoad<MT, NT, RW>(a, n, x0, y0, wave, g, r16, coBase, ext);
+        tileEpilogue<MT, NT, RW, 1>(a, acc, biasv, ext, n, x0, y0, wave, g, r16, coBase);
+    } else {
+        tileEpilogue<MT, NT, RW, 0>(a, acc, biasv, ext, n, x0, y0, wave, g, r16, coBase);
     }
 }
 
@@ -472,7 +503,8 @@ __device__ __forceinline__ void loadSlice(const ConvArgs& a, const SliceSrc& src
 }
 
 // The 9 taps of one staged slice: per tap the weight fragments of the lane's
-// output channels, then per M tile its pixel fragment and NT MFMAs.
+// output channels, then per M tile its pixel fragment and NT MFMAs (weights
+// as the A operand: transposed accumulators, see tileEpilogue).
 template <int MT, int NT, int RW, int TC>
 __device__ __forceinline__ void tapsFromLds(const bf16* sA, const bf16* sW, int wave, int g, int r16,
                                             f32x4 (&acc)[MT][NT]) {
@@ -489,7 +521,7 @@ __device__ __forceinline__ void tapsFromLds(const bf16* sA, const bf16* sW, int 
             const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
-                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[nt], acc[mt][nt], 0, 0, 0);
+                acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[nt], A, acc[mt][nt], 0, 0, 0);
         }
     }
 }
@@ -528,12 +560,8 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
     }
     // per-lane output channels' biases, read once (no global loads in the
     // epilogue of bias-only layers, so the next tile's loads stay in flight)
-    float biasv[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        const int co = coBase + nt * 16 + r16;
-        biasv[nt] = (a.bias && co < a.cout) ? a.bias[co] : 0.0f;
-    }
+    float biasv[NT][4];
+    laneBiases<NT>(a, coBase, g, biasv);
     bf16x8 va[kPerA];
     const SliceSrc src = sliceSrc(a, 0);
     auto loadTile = [&](int tile) {
@@ -551,6 +579,10 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
             if (piece < kPiecesA) *reinterpret_cast<bf16x8*>(sA + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = va[j];
         }
         __syncthreads();
+        const int tx = tile % tilesX, rest = tile / tilesX;
+        const int ty = rest % tilesY, n = rest / tilesY;
+        bf16x4 ext[MT][NT];
+        if constexpr (EPI == 1) epiLoad<MT, NT, RW>(a, n, tx * kTileW, ty * TH, wave, g, r16, coBase, ext);
         const int next = tile + (int)gridDim.x;
         if (next < numTiles) loadTile(next);   // in flight while this tile is computed
         f32x4 acc[MT][NT];
@@ -559,9 +591,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile1(ConvArgs a, int tilesX
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
         tapsFromLds<MT, NT, RW, TC>(sA, sW, wave, g, r16, acc);
-        const int tx = tile % tilesX, rest = tile / tilesX;
-        const int ty = rest % tilesY, n = rest / tilesY;
-        tileEpilogue<MT, NT, RW, EPI>(a, acc, biasv, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
+        tileEpilogue<MT, NT, RW, EPI>(a, acc, biasv, ext, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
     }
 }
 
@@ -581,12 +611,8 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
     const int g = lane >> 4, r16 = lane & 15;
     const int coBase = blockIdx.y * NT * 16;
     const int cin = a.c0 + a.c1, slices = cin / 32;
-    float biasv[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-        const int co = coBase + nt * 16 + r16;
-        biasv[nt] = (a.bias && co < a.cout) ? a.bias[co] : 0.0f;
-    }
+    float biasv[NT][4];
+    laneBiases<NT>(a, coBase, g, biasv);
     bf16x8 va[kPerA], vw[kPerW];
     auto load = [&](int tile, int slice) {
         const int tx = tile % tilesX, rest = tile / tilesX;
@@ -624,12 +650,15 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
         __syncthreads();
         const bool last = slice + 1 == slices;
         const int nTile = last ? tile + (int)gridDim.x : tile, nSlice = last ? 0 : slice + 1;
+        const int tx = tile % tilesX, rest = tile / tilesX;
+        const int ty = rest % tilesY, n = rest / tilesY;
+        bf16x4 ext[MT][NT];
+        if constexpr (EPI == 1)
+            if (last) epiLoad<MT, NT, RW>(a, n, tx * kTileW, ty * TH, wave, g, r16, coBase, ext);
         if (nTile < numTiles) load(nTile, nSlice);   // in flight while this slice is computed
         tapsFromLds<MT, NT, RW, TC>(sA, sW, wave, g, r16, acc);
         if (last) {
-            const int tx = tile % tilesX, rest = tile / tilesX;
-            const int ty = rest % tilesY, n = rest / tilesY;
-            tileEpilogue<MT, NT, RW, EPI>(a, acc, biasv, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
+            tileEpilogue<MT, NT, RW, EPI>(a, acc, biasv, ext, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
