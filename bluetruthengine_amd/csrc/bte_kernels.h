@@ -149,7 +149,7 @@ struct WaveBuffers {
 
 struct LaunchShape {
     int extendBlocks, shadeBlocks, connectBlocks, auxBlocks;
-    int shadeCatBlocks[kNumCats];   // resident grid of each category's shading kernel
+    int shadeCatBlocks[2][kNumCats];   // resident grid of each category's shading kernel [batch][category]
     int tailBlocks;       // resident grid of the tail pass (launch_tail)
     int ldsStack;         // traversal LDS stack entries per lane: 8, 16 or 32
     int grabRows;         // work chunk of the persistent kernels, in 64-slot rows

@@ -33,47 +33,65 @@ constexpr uint32_t kFinalBit = 0x80000000u;
 // ---------------------------------------------------------------------------
 BTE_DEV int lane_id() { return (int)(threadIdx.x & 63u); }
 
-// Wave-aggregated append: one atomic per wave, returns this lane's slot when
-// pred is true.  Must be reached by every active lane of the wave.
-BTE_DEV int wave_append(int* ctr, bool pred) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return -1;
-    const int total = __popcll(mask);
-    const int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane_id() == leader) base = atomicAdd(ctr, total);
-    base = __shfl(base, leader, 64);
-    const unsigned lo = (unsigned)(mask & 0xffffffffull), hi = (unsigned)(mask >> 32);
-    const int prefix = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-    return base + prefix;
-}
-
-// Block-aggregated append: the waves' counts meet in LDS (sh: 2 ints per
-// wave) and one lane adds the block's total, so a counter that every wave of
-// the grid appends to sees one atomic per block instead of one per wave
-// (same-address device atomics serialise).  Every wave of the block must call
-// it the same number of times; consecutive calls may share sh.
-BTE_DEV int block_append(int* ctr, bool pred, int* sh) {
+// Block-aggregated append: the waves' counts meet in LDS and one lane adds
+// the counter's total, so a counter that every wave of the grid appends to
+// sees one atomic per block instead of one per wave (same-address device
+// atomics serialise).  MULTI: waves may pass different counters (the
+// per-scene queues of a batch), and waves with the same counter share one
+// atomic.  sh holds 2 ints per wave, ctrs one pointer per wave (MULTI only).
+// Every wave of the block must call it the same number of times;
+// consecutive calls may share sh and ctrs.
+template <bool MULTI>
+BTE_DEV int block_append(int* ctr, bool pred, int* sh, int** ctrs) {
     constexpr int kWaves = kBlock / 64;
     const unsigned long long mask = __ballot(pred);
     const int w = (int)(threadIdx.x >> 6);
-    if (lane_id() == 0) sh[w] = __popcll(mask);
+    if (lane_id() == 0) {
+        sh[w] = __popcll(mask);
+        if (MULTI) ctrs[w] = ctr;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int total = 0;
-        for (int i = 0; i < kWaves; ++i) {
-            const int c = sh[i];
-            sh[kWaves + i] = total;
-            total += c;
+        if constexpr (!MULTI) {
+            int total = 0;
+            for (int i = 0; i < kWaves; ++i) {
+                const int c = sh[i];
+                sh[kWaves + i] = total;
+                total += c;
+            }
+            const int base = total ? atomicAdd(ctr, total) : 0;
+            for (int i = 0; i < kWaves; ++i) sh[kWaves + i] += base;
+        } else {
+            // waves in order; each counter's total goes to the first wave using it
+            int first[kWaves];
+            for (int i = 0; i < kWaves; ++i) {
+                first[i] = i;
+                for (int j = 0; j < i; ++j)
+                    if (first[i] == i && ctrs[j] == ctrs[i]) first[i] = j;
+            }
+            int run[kWaves];
+            for (int i = 0; i < kWaves; ++i) run[i] = 0;
+            for (int i = 0; i < kWaves; ++i) {
+                const int c = sh[i];
+                sh[kWaves + i] = run[first[i]];
+                run[first[i]] += c;
+            }
+            for (int i = 0; i < kWaves; ++i)
+                if (first[i] == i) run[i] = run[i] ? atomicAdd(ctrs[i], run[i]) : 0;
+            for (int i = 0; i < kWaves; ++i) sh[kWaves + i] += run[first[i]];
         }
-        const int base = total ? atomicAdd(ctr, total) : 0;
-        for (int i = 0; i < kWaves; ++i) sh[kWaves + i] += base;
     }
     __syncthreads();
     const unsigned lo = (unsigned)(mask & 0xffffffffull), hi = (unsigned)(mask >> 32);
     const int prefix = (int)__builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
     return pred ? sh[kWaves + w] + prefix : -1;
 }
+
+// LDS of a block's appends (block_append)
+struct BlockAppendLds {
+    int n[2 * (kBlock / 64)];
+    int* ctrs[kBlock / 64];
+};
 
 BTE_DEV void wave_add_u64(unsigned long long* ctr, uint32_t v) {
     unsigned long long x = v;
@@ -1515,12 +1533,11 @@ struct ShadeSinks {
 };
 
 // Shades one 64-slot row of a category queue: lanes with `active` set hold
-// the hits at list[li].  Every lane of the wave must call it (wave-wide
-// appends); with blk (LDS, 2 ints per wave) the appends are block-wide and
-// every wave of the block must call it the same number of times.
-template <int T>
+// the hits at list[li].  Every lane of the block must call it the same number
+// of times (block-wide appends).
+template <int T, bool BATCH>
 BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, int li, bool active,
-                      const ShadeSinks& out, int* blk = nullptr) {
+                      const ShadeSinks& out, BlockAppendLds& blk) {
     const int cur = bounce & 1, nxt = cur ^ 1;
     const int* list = wb.catQ + (size_t)T * (size_t)wb.cap;
     constexpr bool kDiffuseType = (T == BTE_BSDF_DIFFUSE || T == BTE_BSDF_MICROFACET || T == BTE_BSDF_DISNEY);
@@ -1575,7 +1592,7 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
         scatter<T>(its, frame, bsdf, rng, o, d, thr, eta, brdfPdf);
         survive = roulette(bounce, rng, thr, eta) && (bounce + 1 < maxDepth);
     }
-    const int slot = out.survBase + (blk ? block_append(out.survCtr, survive, blk) : wave_append(out.survCtr, survive));
+    const int slot = out.survBase + block_append<BATCH>(out.survCtr, survive, blk.n, blk.ctrs);
     if (survive) {
         wb.rayO[nxt][slot] = make_float4(o.x, o.y, o.z, brdfPdf);
         wb.rayD[nxt][slot] = make_float4(d.x, d.y, d.z, eta);
@@ -1604,8 +1621,7 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
                 else if (k == wb.kindEnv) { if (sc.hasEnv) rec = envNEE<T>(sc, dIn, thrIn, its, frame, bsdf, e0, e1); }
                 else if (spots) rec.ok = sc.spotCount > 0;     // a request: the spot pass evaluates every spotlight
             }
-            int* const ctr = out.shCtr + k * out.shStride;
-            const int sl = out.shBase + (blk ? block_append(ctr, rec.ok, blk) : wave_append(ctr, rec.ok));
+            const int sl = out.shBase + block_append<BATCH>(out.shCtr + k * out.shStride, rec.ok, blk.n, blk.ctrs);
             if (rec.ok) {
                 const size_t idx = (size_t)k * (size_t)wb.cap + (size_t)sl;
                 if (spots) {
@@ -1632,34 +1648,34 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, i
     const int count = BATCH ? segSlots(wb, bounce, kSegCat + T) : ctr[kCtrCat + T];
     const int grab = 64 * wb.grabRows;
     int* wk = work_group(wb, bounce, kWorkShade + T);
-    if constexpr (!BATCH) {
-        // the block's waves shade their rows in step and append block-wide:
-        // one atomic per block and row on the shared queue counters
-        __shared__ int blk[2 * (kBlock / 64)];
-        const ShadeSinks out{ctrNext + kCtrRays, 0, ctr + kCtrShadow, 1, 0};
-        for (int chunk = first_chunk(grab);;) {
-            const bool has = chunk < count;
-            if (!__syncthreads_or(has)) break;
+    // the block's waves shade their rows in step and append block-wide: one
+    // atomic per block, row and counter (a batch's counters are per scene;
+    // waves on the same scene share one)
+    __shared__ BlockAppendLds blk;
+    for (int chunk = first_chunk(grab);;) {
+        const bool has = chunk < count;
+        if (!__syncthreads_or(has)) break;
+        if constexpr (!BATCH) {
+            const ShadeSinks out{ctrNext + kCtrRays, 0, ctr + kCtrShadow, 1, 0};
             for (int row = 0; row < wb.grabRows; ++row) {
                 const int li = chunk + row * 64 + lane_id();
-                shadeRow<T>(sc, wb, bounce, maxDepth, li, has && li < count, out, blk);
+                shadeRow<T, false>(sc, wb, bounce, maxDepth, li, has && li < count, out, blk);
             }
-            if (has) chunk = next_chunk(wk, grab, count);
+        } else {
+            SegChunk k{0, 0, 0};   // a wave without a chunk shades nothing (scene 0 stands in)
+            if (has) k = segChunk(wb, bounce, kSegCat + T, chunk);
+            // wave-uniform (the scene descriptor stays in scalar registers)
+            const int sIdx = __builtin_amdgcn_readfirstlane(k.s);
+            const int begin = (int)wb.slots[sIdx].pathBase;
+            const DevScene S = fetchScene(scenes, sIdx);
+            const ShadeSinks out{segCounts(wb, bounce + 1, kSegRays) + sIdx, begin,
+                                 segCounts(wb, bounce, kSegShadow) + sIdx, wb.sceneCount, begin};
+            for (int row = 0; row < wb.grabRows; ++row) {
+                const int e = row * 64 + lane_id();
+                shadeRow<T, true>(S, wb, bounce, maxDepth, k.first + e, has && e < k.left, out, blk);
+            }
         }
-        return;
-    }
-    // a batch appends per wave: its counters are per scene
-    for (int chunk = first_chunk(grab); chunk < count; chunk = next_chunk(wk, grab, count)) {
-        const SegChunk k = segChunk(wb, bounce, kSegCat + T, chunk);
-        if (k.left <= 0) continue;
-        const int begin = (int)wb.slots[k.s].pathBase;
-        const DevScene S = fetchScene(scenes, k.s);
-        const ShadeSinks out{segCounts(wb, bounce + 1, kSegRays) + k.s, begin,
-                             segCounts(wb, bounce, kSegShadow) + k.s, wb.sceneCount, begin};
-        for (int row = 0; row < wb.grabRows; ++row) {
-            const int e = row * 64 + lane_id();
-            shadeRow<T>(S, wb, bounce, maxDepth, k.first + e, e < k.left, out);
-        }
+        if (has) chunk = next_chunk(wk, grab, count);
     }
 }
 
@@ -2593,23 +2609,23 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
                            scenes);
     }
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(k_categorize<B>, dim3(ls.shadeCatBlocks[kCatMiss]), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+    hipLaunchKernelGGL(k_categorize<B>, dim3(ls.shadeCatBlocks[B][kCatMiss]), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     if (B) launch_segment_rows(wb, bounce, kSegCat, kNumCats, s);
     // a BSDF category is shaded only when the scene uses that type
     if (wb.catMask & (1 << BTE_BSDF_DIFFUSE))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIFFUSE, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_DIFFUSE]), dim3(kBlock), 0, s, sc, wb, bounce,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIFFUSE, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_DIFFUSE]), dim3(kBlock), 0, s, sc, wb, bounce,
                            maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_DIELECTRIC))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIELECTRIC, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_DIELECTRIC]), dim3(kBlock), 0, s, sc, wb,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIELECTRIC, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_DIELECTRIC]), dim3(kBlock), 0, s, sc, wb,
                            bounce, maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_MIRROR))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_MIRROR, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_MIRROR]), dim3(kBlock), 0, s, sc, wb, bounce,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_MIRROR, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_MIRROR]), dim3(kBlock), 0, s, sc, wb, bounce,
                            maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_MICROFACET))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_MICROFACET, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_MICROFACET]), dim3(kBlock), 0, s, sc, wb,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_MICROFACET, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_MICROFACET]), dim3(kBlock), 0, s, sc, wb,
                            bounce, maxDepth, scenes);
     if (wb.catMask & (1 << BTE_BSDF_DISNEY))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_DISNEY, B>), dim3(ls.shadeCatBlocks[BTE_BSDF_DISNEY]), dim3(kBlock), 0, s, sc, wb, bounce,
+        hipLaunchKernelGGL((k_shade<BTE_BSDF_DISNEY, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_DISNEY]), dim3(kBlock), 0, s, sc, wb, bounce,
                            maxDepth, scenes);
     if (B) {
         launch_segment_rows(wb, bounce + 1, kSegRays, 1, s);
@@ -2819,14 +2835,12 @@ void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, d
     hipLaunchKernelGGL(k_env_guides, dim3(H + 1), dim3(256), 0, s, marg, cond, W, H, margGuide, condGuide, monotone);
 }
 
-// Resident blocks per CU of one category's shading kernel (single scene and
-// batch instantiations).
-template <int T>
+// Resident blocks per CU of one category's shading kernel.
+template <int T, bool B>
 static int shadeOccupancy() {
-    int a = 0, b = 0;
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_shade<T, false>, kBlock, 0);
-    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shade<T, true>, kBlock, 0);
-    return std::max(1, std::min(a, b));
+    int a = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_shade<T, B>, kBlock, 0);
+    return std::max(1, a);
 }
 
 // Resident blocks per CU of the traversal passes for one stack size and BVH width.
@@ -2855,7 +2869,7 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     static int cachedDev = -1;
     static int cus = 1, occS = 1, occMT = 1, occL[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};   // [width][stack]
     static int occT[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};                                  // tail [width][stack]
-    static int occCat[kNumCats] = {1, 1, 1, 1, 1, 1};                                           // shading per category
+    static int occCat[2][kNumCats] = {{1, 1, 1, 1, 1, 1}, {1, 1, 1, 1, 1, 1}};                  // shading [batch][category]
     if (device != cachedDev) {
         hipDeviceProp_t prop;
         (void)hipGetDeviceProperties(&prop, device);
@@ -2867,17 +2881,19 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
             int a = 0, b = 0;
             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_categorize<false>, kBlock, 0);
             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_categorize<true>, kBlock, 0);
-            occCat[kCatMiss] = std::max(1, std::min(a, b));
+            occCat[0][kCatMiss] = std::max(1, a);
+            occCat[1][kCatMiss] = std::max(1, b);
         }
-        occCat[BTE_BSDF_DIFFUSE] = shadeOccupancy<BTE_BSDF_DIFFUSE>();
-        occCat[BTE_BSDF_DIELECTRIC] = shadeOccupancy<BTE_BSDF_DIELECTRIC>();
-        occCat[BTE_BSDF_MIRROR] = shadeOccupancy<BTE_BSDF_MIRROR>();
-        occCat[BTE_BSDF_MICROFACET] = shadeOccupancy<BTE_BSDF_MICROFACET>();
-        occCat[BTE_BSDF_DISNEY] = shadeOccupancy<BTE_BSDF_DISNEY>();
-        int m = 0, t = 0;
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&m, k_megakernel, kBlock, 0);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&t, k_trace_rays<kOvfStack>, kBlock, 0);
-        occMT = std::max(1, std::min(m, t));
+        occCat[0][BTE_BSDF_DIFFUSE] = shadeOccupancy<BTE_BSDF_DIFFUSE, false>();
+        occCat[0][BTE_BSDF_DIELECTRIC] = shadeOccupancy<BTE_BSDF_DIELECTRIC, false>();
+        occCat[0][BTE_BSDF_MIRROR] = shadeOccupancy<BTE_BSDF_MIRROR, false>();
+        occCat[0][BTE_BSDF_MICROFACET] = shadeOccupancy<BTE_BSDF_MICROFACET, false>();
+        occCat[0][BTE_BSDF_DISNEY] = shadeOccupancy<BTE_BSDF_DISNEY, false>();
+        occCat[1][BTE_BSDF_DIFFUSE] = shadeOccupancy<BTE_BSDF_DIFFUSE, true>();
+        occCat[1][BTE_BSDF_DIELECTRIC] = shadeOccupancy<BTE_BSDF_DIELECTRIC, true>();
+        occCat[1][BTE_BSDF_MIRROR] = shadeOccupancy<BTE_BSDF_MIRROR, true>();
+        occCat[1][BTE_BSDF_MICROFACET] = shadeOccupancy<BTE_BSDF_MICROFACET, true>();
+        occCat[1][BTE_BSDF_DISNEY] = shadeOccupancy<BTE_BSDF_DISNEY, true>();
         occL[0][0] = travOccupancy<8, 2>();
         occL[0][1] = travOccupancy<16, 2>();
         occL[0][2] = travOccupancy<32, 2>();
@@ -2907,7 +2923,8 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     ls.extendBlocks = cus * occ;
     ls.connectBlocks = cus * occ;
     ls.shadeBlocks = cus * occS;
-    for (int k = 0; k < kNumCats; ++k) ls.shadeCatBlocks[k] = cus * occCat[k];
+    for (int b = 0; b < 2; ++b)
+        for (int k = 0; k < kNumCats; ++k) ls.shadeCatBlocks[b][k] = cus * occCat[b][k];
     ls.auxBlocks = cus * occMT;
     ls.tailBlocks = cus * occT[ls.bvhWidth == 2 ? 0 : (ls.bvhWidth == 4 ? 1 : 2)][ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
     return ls;
