@@ -151,6 +151,7 @@ struct LaunchShape {
     int extendBlocks, shadeBlocks, connectBlocks, auxBlocks;
     int shadeCatBlocks[2][kNumCats];   // resident grid of each category's shading kernel [batch][category]
     int tailBlocks;       // resident grid of the tail pass (launch_tail)
+    int packetBlocks;     // resident grid of the camera-ray packet pass (0: per-ray traversal on bounce 0)
     int ldsStack;         // traversal LDS stack entries per lane: 8, 16 or 32
     int grabRows;         // work chunk of the persistent kernels, in 64-slot rows
     int bvhWidth;         // 2 or 4: BVH traversed by extend / connect

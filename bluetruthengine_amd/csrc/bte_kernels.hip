@@ -1433,6 +1433,101 @@ __global__ __launch_bounds__(kBlock) void k_extend_pool(DevScene sc, WaveBuffers
     }
 }
 
+// Camera rays as packets.  A wave's 64 queue slots on the first bounce are 16
+// frames of one 2x2 pixel tile: nearly the same ray.  The wave traverses the
+// BVH4 once for all of them: a child is entered when any lane's ray enters it
+// before that lane's closest hit so far, children go in order of the entry
+// distance of their first entering lane, the stack is the wave's own (LDS,
+// uniform) and node and triangle loads are wave-uniform (scalar).  Every node
+// that a lane's own traversal would have to visit to find its closest hit is
+// visited, and each lane runs its own triangle tests, so the hits are those
+// of the per-ray traversal (the closest hit does not depend on the set or
+// order of the visited nodes).  No lane diverges from the wave's step.
+template <class T>
+BTE_DEV const T* constPtr(const T* p) {
+    return (const T*)(const __attribute__((address_space(4))) T*)(uintptr_t)p;
+}
+BTE_DEV float uniformLane(float v, unsigned long long mask) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)__ffsll((long long)mask) - 1));
+}
+
+__global__ __launch_bounds__(kBlock) void k_extend_packet(DevScene sc, WaveBuffers wb, int bounce) {
+    constexpr float kNone = __builtin_huge_valf();
+    __shared__ int stacks[kBlock / 64][kStackSize];
+    int* stk = stacks[threadIdx.x >> 6];
+    const int cur = bounce & 1;
+    const int count = wb.counters[(size_t)bounce * kCtrStride + kCtrRays];
+    int* wk = work_group(wb, bounce, kWorkExtend);
+    const float4* nodes = constPtr(sc.nodes4);
+    const float4* tris = constPtr(sc.leafTris);
+    for (int chunk = first_chunk(64); chunk < count; chunk = next_chunk(wk, 64, count)) {
+        const int qi = chunk + lane_id();
+        const bool valid = qi < count;
+        V3 o = {0, 0, 0}, d = {0, 0, 1};
+        if (valid) {
+            o = xyz(wb.rayO[cur][qi]);
+            d = xyz(wb.rayD[cur][qi]);
+        }
+        const WtRay r = makeWtRay(o, d);
+        const bool act = valid && r.valid && sc.triCount > 0;
+        float best = 1e30f, tO = 0.0f, uO = 0.0f, vO = 0.0f;
+        int bestTri = -1;
+        int node = 0, sp = 0;
+        bool more = __ballot(act) != 0ull;
+        while (more) {
+            if (node >= 0) {
+                const float4* np = nodes + 8 * (size_t)node;
+                const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+                const int4 ch = *reinterpret_cast<const int4*>(np + 6);
+                float f0, f1, f2, f3;
+                const float e0 = boxEnter(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, RT_EPS, best, f0);
+                const float e1 = boxEnter(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, RT_EPS, best, f1);
+                const float e2 = boxEnter(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, RT_EPS, best, f2);
+                const float e3 = boxEnter(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, RT_EPS, best, f3);
+                const unsigned long long m0 = __ballot(act && e0 <= f0), m1 = __ballot(act && e1 <= f1);
+                const unsigned long long m2 = __ballot(act && e2 <= f2), m3 = __ballot(act && e3 <= f3);
+                float k0 = m0 ? uniformLane(e0, m0) : kNone, k1 = m1 ? uniformLane(e1, m1) : kNone;
+                float k2 = m2 ? uniformLane(e2, m2) : kNone, k3 = m3 ? uniformLane(e3, m3) : kNone;
+                int c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+                sortPair(k0, c0, k1, c1);
+                sortPair(k2, c2, k3, c3);
+                sortPair(k0, c0, k2, c2);
+                sortPair(k1, c1, k3, c3);
+                sortPair(k1, c1, k2, c2);
+                if (k0 != kNone) {
+                    // farthest first, so the nearer children are popped first
+                    if (k3 != kNone) stk[sp++] = c3;
+                    if (k2 != kNone) stk[sp++] = c2;
+                    if (k1 != kNone) stk[sp++] = c1;
+                    node = c0;
+                    continue;
+                }
+            } else {
+                const int code = ~node;
+                const int start = code >> kLeafCountBits;
+                const int cnt = (code & (kMaxLeafTris - 1)) + 1;
+                for (int j = 0; j < cnt; ++j) {
+                    const float4* tp = tris + 3 * (size_t)(start + j);
+                    const float4 a = tp[0], b = tp[1], c = tp[2];
+                    float t, u, v;
+                    if (act && hitTriangle(r, a, b, c, RT_EPS, best, t, u, v)) {
+                        const int id = __float_as_int(a.w);
+                        if (t < best || bestTri < 0 || id < bestTri) {
+                            best = t; bestTri = id; tO = t; uO = u; vO = v;
+                        }
+                    }
+                }
+            }
+            if (sp == 0) break;
+            node = __builtin_amdgcn_readfirstlane(stk[--sp]);
+        }
+        if (valid) {
+            wb.hit[qi] = make_float4(__int_as_float(bestTri), tO, uO, vO);
+            wb.cat[qi] = shadeCategory(sc, bestTri);
+        }
+    }
+}
+
 // Groups the queue slots of one bounce by shading category, keeping queue
 // order within each category.  A block handles 1024 consecutive slots per
 // iteration: ballot ranks within each wave, an LDS scan over the block, and one
@@ -2601,6 +2696,8 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     if (ev) (void)hipEventRecord(ev[0], s);
     if constexpr (W >= 4) {
         if (count) hipLaunchKernelGGL((k_extend_pool<true, L, false, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+        else if (!B && W == 4 && bounce == 0 && ls.packetBlocks > 0)
+            hipLaunchKernelGGL(k_extend_packet, dim3(ls.packetBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
         else hipLaunchKernelGGL((k_extend_pool<false, L, B, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     } else if (count) {
         hipLaunchKernelGGL((k_extend<true, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
@@ -2867,7 +2964,7 @@ static int tailOccupancy() {
 
 LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows, int bvhWidth) {
     static int cachedDev = -1;
-    static int cus = 1, occS = 1, occMT = 1, occL[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};   // [width][stack]
+    static int cus = 1, occS = 1, occP = 1, occMT = 1, occL[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};   // [width][stack]
     static int occT[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};                                  // tail [width][stack]
     static int occCat[2][kNumCats] = {{1, 1, 1, 1, 1, 1}, {1, 1, 1, 1, 1, 1}};                  // shading [batch][category]
     if (device != cachedDev) {
@@ -2877,6 +2974,8 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
         int o = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_shade<BTE_BSDF_DISNEY>, kBlock, 0);
         occS = std::max(1, o);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_extend_packet, kBlock, 0);
+        occP = std::max(1, o);
         {   // the categoriser shades the misses
             int a = 0, b = 0;
             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_categorize<false>, kBlock, 0);
@@ -2926,6 +3025,7 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < kNumCats; ++k) ls.shadeCatBlocks[b][k] = cus * occCat[b][k];
     ls.auxBlocks = cus * occMT;
+    ls.packetBlocks = cus * occP;
     ls.tailBlocks = cus * occT[ls.bvhWidth == 2 ? 0 : (ls.bvhWidth == 4 ? 1 : 2)][ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
     return ls;
 }
