@@ -1451,18 +1451,29 @@ BTE_DEV float uniformLane(float v, unsigned long long mask) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)__ffsll((long long)mask) - 1));
 }
 
-__global__ __launch_bounds__(kBlock) void k_extend_packet(DevScene sc, WaveBuffers wb, int bounce) {
+// In a batch a chunk (one compact row) lies in one scene.
+template <bool BATCH>
+__global__ __launch_bounds__(kBlock) void k_extend_packet(DevScene sc, WaveBuffers wb, int bounce,
+                                                           const DevScene* __restrict__ scenes) {
     constexpr float kNone = __builtin_huge_valf();
     __shared__ int stacks[kBlock / 64][kStackSize];
     int* stk = stacks[threadIdx.x >> 6];
     const int cur = bounce & 1;
-    const int count = wb.counters[(size_t)bounce * kCtrStride + kCtrRays];
+    const int count = BATCH ? segSlots(wb, bounce, kSegRays) : wb.counters[(size_t)bounce * kCtrStride + kCtrRays];
     int* wk = work_group(wb, bounce, kWorkExtend);
-    const float4* nodes = constPtr(sc.nodes4);
-    const float4* tris = constPtr(sc.leafTris);
     for (int chunk = first_chunk(64); chunk < count; chunk = next_chunk(wk, 64, count)) {
-        const int qi = chunk + lane_id();
-        const bool valid = qi < count;
+        int first = chunk, left = count - chunk;
+        if constexpr (BATCH) {
+            const SegChunk k = segChunk(wb, bounce, kSegRays, chunk);
+            if (k.left <= 0) continue;
+            first = k.first;
+            left = k.left;
+            sc = fetchScene(scenes, k.s);
+        }
+        const float4* nodes = constPtr(sc.nodes4);
+        const float4* tris = constPtr(sc.leafTris);
+        const int qi = first + lane_id();
+        const bool valid = lane_id() < left;
         V3 o = {0, 0, 0}, d = {0, 0, 1};
         if (valid) {
             o = xyz(wb.rayO[cur][qi]);
@@ -2696,8 +2707,8 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     if (ev) (void)hipEventRecord(ev[0], s);
     if constexpr (W >= 4) {
         if (count) hipLaunchKernelGGL((k_extend_pool<true, L, false, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
-        else if (!B && W == 4 && bounce == 0 && ls.packetBlocks > 0)
-            hipLaunchKernelGGL(k_extend_packet, dim3(ls.packetBlocks), dim3(kBlock), 0, s, sc, wb, bounce);
+        else if (W == 4 && bounce == 0 && ls.packetBlocks > 0)
+            hipLaunchKernelGGL(k_extend_packet<B>, dim3(ls.packetBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
         else hipLaunchKernelGGL((k_extend_pool<false, L, B, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     } else if (count) {
         hipLaunchKernelGGL((k_extend<true, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
@@ -2974,8 +2985,10 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
         int o = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_shade<BTE_BSDF_DISNEY>, kBlock, 0);
         occS = std::max(1, o);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_extend_packet, kBlock, 0);
-        occP = std::max(1, o);
+        int o2 = 0;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_extend_packet<false>, kBlock, 0);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o2, k_extend_packet<true>, kBlock, 0);
+        occP = std::max(1, std::min(o, o2));
         {   // the categoriser shades the misses
             int a = 0, b = 0;
             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_categorize<false>, kBlock, 0);
