@@ -1746,8 +1746,12 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
 }
 
 // Shade one category of hits (T = BsdfType; misses are shaded by k_categorize).
+// Registers held to 4 waves per SIMD (128): the rows wait on chains of
+// gathers, and the Disney kernel (142 registers unforced, 3 waves) runs
+// faster with the extra wave than without its few spills.
 template <int T, bool BATCH = false>
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, WaveBuffers wb, int bounce, int maxDepth,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(DevScene sc, WaveBuffers wb,
+                                                                                          int bounce, int maxDepth,
                                                    const DevScene* __restrict__ scenes) {
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
     int* ctrNext = wb.counters + (size_t)(bounce + 1) * kCtrStride;
