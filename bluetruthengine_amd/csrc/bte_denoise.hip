@@ -1153,14 +1153,20 @@ __device__ __forceinline__ float fireflyLum(float4 c) {
 // w x h corner of its wp x hp tensor (zero padding right and below).  One
 // thread per tensor position: it computes the features of the frame's pixel
 // there once and writes them to that pixel's mirrored position in every image.
-__global__ void k_features(const float4* rgb, const float4* alb, const float4* nrm, int w, int h, int wp, int hp,
-                           int flips, bf16* x, bf16* rgb4) {
+// One thread per pixel of the (padded) plane computes the features; the 64-B
+// records of a wave's 64 pixels are then written through LDS so that each
+// store instruction covers 16 whole records (1 KB) instead of a 16-B piece
+// of 64 records.
+__global__ __launch_bounds__(256) void k_features(const float4* rgb, const float4* alb, const float4* nrm, int w,
+                                                  int h, int wp, int hp, int flips, bf16* x, bf16* rgb4) {
+    static_assert(kInPad == 32 && kInChannels <= 16, "records of 4 16-byte pieces, the last two zero");
+    __shared__ bf16x8 sRec[256 / 64][64][2];
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long plane = (long long)wp * hp;
-    if (j >= plane) return;
-    const int px = (int)(j % wp), py = (int)(j / wp);
+    const bool valid = j < plane;
+    const int px = valid ? (int)(j % wp) : 0, py = valid ? (int)(j / wp) : 0;
     float v[kInChannels] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const bool in = px < w && py < h;
+    const bool in = valid && px < w && py < h;
     if (in) {
         const size_t q = (size_t)py * w + px;
         float4 c = rgb[q];
@@ -1204,14 +1210,30 @@ __global__ void k_features(const float4* rgb, const float4* alb, const float4* n
     for (int c = 0; c < kInChannels; ++c) out[c / 8][c % 8] = (bf16)v[c];
     bf16x4 r4;
     r4[0] = out[0][0]; r4[1] = out[0][1]; r4[2] = out[0][2]; r4[3] = (bf16)0.0f;
+    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
+    sRec[wv][lane][0] = out[0];
+    sRec[wv][lane][1] = out[1];
+    __syncthreads();
+    const long long jw = j - lane;   // the wave's first pixel
+    const int q = lane & 3;          // this lane's 16-byte piece of a record
     for (int b = 0; b < flips; ++b) {
         // padding stays in place; frame pixels go to their mirrored position
-        const int ox = (in && (b & 1)) ? w - 1 - px : px, oy = (in && (b & 2)) ? h - 1 - py : py;
-        const size_t o = (size_t)b * (size_t)plane + (size_t)oy * wp + ox;
-        bf16x8* dst = reinterpret_cast<bf16x8*>(x + o * kInPad);
+        if (rgb4 && valid) {
+            const int ox = (in && (b & 1)) ? w - 1 - px : px, oy = (in && (b & 2)) ? h - 1 - py : py;
+            const size_t o = (size_t)b * (size_t)plane + (size_t)oy * wp + ox;
+            *reinterpret_cast<bf16x4*>(rgb4 + o * 4) = r4;
+        }
 #pragma unroll
-        for (int k = 0; k < kInPad / 8; ++k) dst[k] = out[k];
-        if (rgb4) *reinterpret_cast<bf16x4*>(rgb4 + o * 4) = r4;
+        for (int k = 0; k < 4; ++k) {
+            const int p = (lane >> 2) + 16 * k;   // record p of the wave, piece q
+            const long long jp = jw + p;
+            if (jp >= plane) continue;
+            const int ppx = (int)(jp % wp), ppy = (int)(jp / wp);
+            const bool pin = ppx < w && ppy < h;
+            const int ox = (pin && (b & 1)) ? w - 1 - ppx : ppx, oy = (pin && (b & 2)) ? h - 1 - ppy : ppy;
+            const size_t o = (size_t)b * (size_t)plane + (size_t)oy * wp + ox;
+            *reinterpret_cast<bf16x8*>(x + o * kInPad + 8 * q) = q < 2 ? sRec[wv][p][q] : zero8();
+        }
     }
 }
 
