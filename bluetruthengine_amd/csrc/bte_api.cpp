@@ -1023,6 +1023,12 @@ int batchSlot(const Context& c, int slot) {
 int ensure_device() { return ensureDevice(); }
 hipStream_t main_stream() { return ctx().stream; }
 
+int run_captured(const std::vector<unsigned char>& key, const std::function<void()>& emit) {
+    Context& c = ctx();
+    return runSegment(c.cfg.graphs != 0 && c.instr == 0, key, emit);
+}
+
+
 int render_view(RenderView& v) {
     CHECK_RC(ensureDevice());
     Context& c = ctx();

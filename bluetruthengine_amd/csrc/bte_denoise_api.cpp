@@ -399,18 +399,37 @@ uint64_t splitmix(uint64_t& s) {
 
 bool denoiser_active() { return net().enabled && net().ready; }
 
-int denoise_view(const RenderView& v, float4* out) {
+int denoise_view(const RenderView& v, float4* out, bool captured) {
     Net& N = net();
     if (!N.ready) return fail(BTE_ERR_STATE, "denoiser weights not loaded");
     if (!v.aovAlb) return fail(BTE_ERR_STATE, "the denoiser needs the albedo/normal AOVs (bte_enable_aov)");
     const int hp = (v.height + 7) / 8 * 8, wp = (v.width + 7) / 8 * 8;
     // the flip ensemble runs as one batch of N.flips mirrored images
     CHECK_RC(ensureWorkspace(N, N.flips, hp, wp, false));
-    dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, wp, hp, N.flips, T(N, T_X),
-                        N.rgb4.as<bf16>(), v.stream);
-    forward(N, v.stream);
-    dn::launch_output(N.o.as<float>(), wp, hp, v.width, v.height, N.flips, out, v.fb, v.stream);
-    return BTE_OK;
+    auto emit = [&]() {
+        dn::launch_features(v.accum, v.aovAlb, v.aovNrm, v.width, v.height, wp, hp, N.flips, T(N, T_X),
+                            N.rgb4.as<bf16>(), v.stream);
+        forward(N, v.stream);
+        dn::launch_output(N.o.as<float>(), wp, hp, v.width, v.height, N.flips, out, v.fb, v.stream);
+    };
+    if (!captured || v.stream != main_stream()) {
+        emit();
+        return BTE_OK;
+    }
+    // a graph of the ~18 launches: every buffer and size they use is in the key
+    // (weights are read from their buffers at replay, so new values need no
+    // new graph)
+    std::vector<unsigned char> key;
+    auto add = [&](const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        key.insert(key.end(), b, b + n);
+    };
+    const long long sizes[] = {0x64656e6f697365ll, N.flips, wp, hp, v.width, v.height, dn::conv_variant()};
+    add(sizes, sizeof sizes);
+    const void* ptrs[] = {v.accum, v.aovAlb, v.aovNrm, out, v.fb, N.o.p, N.rgb4.p, N.w16.p, N.w.p};
+    add(ptrs, sizeof ptrs);
+    for (const DBuf& b : N.t) add(&b.p, sizeof b.p);
+    return run_captured(key, emit);
 }
 
 int denoiser_present() {
@@ -418,7 +437,7 @@ int denoiser_present() {
     RenderView v;
     CHECK_RC(render_view(v));
     CHECK_RC(N.denoised.ensure((size_t)v.width * v.height * sizeof(float4)));
-    CHECK_RC(denoise_view(v, N.denoised.as<float4>()));
+    CHECK_RC(denoise_view(v, N.denoised.as<float4>(), true));
     HIPCHK(hipStreamSynchronize(v.stream));
     HIPCHK(hipGetLastError());
     N.outW = v.width;
@@ -660,7 +679,7 @@ int bte_denoise_image(const float* rgb, const float* albedo, const float* normal
     v.accum = reinterpret_cast<const float4*>(rgb);
     v.aovAlb = reinterpret_cast<const float4*>(albedo);
     v.aovNrm = reinterpret_cast<const float4*>(normal);
-    CHECK_RC(denoise_view(v, reinterpret_cast<float4*>(out)));
+    CHECK_RC(denoise_view(v, reinterpret_cast<float4*>(out)));   // caller buffers vary: no graph per call
     HIPCHK(hipStreamSynchronize(v.stream));
     HIPCHK(hipGetLastError());
     return BTE_OK;

@@ -7,6 +7,8 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
+#include <vector>
 
 #include "../../include/bte_api.h"
 
@@ -97,7 +99,13 @@ bool denoiser_active();
 int denoiser_present();                // denoise the accumulation into the display buffer
 // Denoise one view into `out` (float4 per pixel, linear) and the view's display
 // buffer; asynchronous on v.stream.
-int denoise_view(const RenderView& v, float4* out);
+// captured: replay the launches as a HIP graph (cached by the buffers and
+// sizes they use) when graphs are on; single frames only, a batch's many
+// views would churn the graph cache.
+int denoise_view(const RenderView& v, float4* out, bool captured = false);
+// Runs emit's launches on the library's stream, captured once per key into a
+// HIP graph and replayed (plain launches when graphs are off).
+int run_captured(const std::vector<unsigned char>& key, const std::function<void()>& emit);
 void denoiser_release();               // bte_cleanup
 
 }  // namespace bte

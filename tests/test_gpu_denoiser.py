@@ -221,6 +221,39 @@ def test_render_then_denoise_runs_and_matches_oracle():
     assert dn.toggle() is False
 
 
+def test_denoise_graph_replays_match_plain_launches():
+    _say("case: Cornell 64x64 with AOVs, random weights, bte_denoise captured as a graph: replays with host "
+         "reads between, new frames, new weights, then plain launches (graphs off)")
+    sd = scenes.cornell_c1(width=64, height=64, spp=4, max_depth=8)
+    r = renderer.Renderer(sd)
+    r.enable_aov(True)
+    _, flat = _params(7)
+    dn.set_weights(flat)
+    dn.set_flips(4)
+    r.render(4)
+    dn.denoise()                       # captured and launched
+    first = dn.denoised(64, 64)
+    for _ in range(3):                 # replays
+        dn.denoise()
+        assert np.array_equal(dn.denoised(64, 64), first)
+    r.render(2)                        # new accumulation, same buffers: the same graph
+    dn.denoise()
+    more = dn.denoised(64, 64)
+    assert not np.array_equal(more, first)
+    _, flat2 = _params(8)
+    dn.set_weights(flat2)              # new weights in the same buffers
+    dn.denoise()
+    other = dn.denoised(64, 64)
+    r.set_config(8, graphs=0)          # plain launches
+    dn.set_weights(flat)
+    dn.denoise()
+    assert np.array_equal(dn.denoised(64, 64), more)
+    dn.set_weights(flat2)
+    dn.denoise()
+    assert np.array_equal(dn.denoised(64, 64), other)
+    r.set_config(8, graphs=1)
+
+
 def test_dataset_capture_and_training_reduce_loss():
     _say("case: 2 training pairs from Cornell 256x32 (2 and 64 frames), 30 AdamW steps, batch 2")
     sd = scenes.cornell_c1(width=256, height=32, spp=2, max_depth=8)
