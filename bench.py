@@ -622,7 +622,9 @@ def main():
                     "traffic": int(traffic) if traffic else None,
                     "traffic_source": "profiles/extend_c3_hbm_traffic.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE per "
                                       "launch, copy-calibrated; every bounce a launch, no tail pass)",
-                    "kernel": (f"k_extend_pool (BVH{cfg['bvhWidth']} closest hit)" if cfg["bvhWidth"] in (4, 8)
+                    "kernel": ("k_extend_pool (BVH4 closest hit; camera rays of bounce 0 by k_extend_packet)"
+                               if cfg["bvhWidth"] == 4 else
+                               f"k_extend_pool (BVH{cfg['bvhWidth']} closest hit)" if cfg["bvhWidth"] == 8
                                else "k_extend (BVH2 closest hit)"), "launches": launches,
                     "avg_launch_ms": round(avg_ms, 4),
                     "request_bytes_per_launch": int(request_per_launch),
