@@ -368,16 +368,56 @@ __device__ __forceinline__ void tileEpilogue(const ConvArgs& a, f32x4 (&acc)[MT]
 // one output channel's 32 channels) are padded to 80 bytes to spread the
 // 16-byte fragment reads over the banks.
 constexpr int kTileW = 32;   // output pixels per tile row
+
+// LDS layout of a staged input tile or weight slice: one row (a pixel's or
+// an output channel's 32 bf16 of the slice) is 64 bytes, and its 16-byte
+// chunk q sits at chunk q ^ ((row >> 1) & 3).  Measured on this LDS, both
+// access patterns of the tile kernels then run without bank conflicts: the
+// staging stores (4 consecutive lanes write the 4 chunks of one row, a wave
+// 16 consecutive rows) and the MFMA fragment reads (16 consecutive rows from
+// any first row, one chunk per 16 lanes), where rows of 80 bytes (32 + 8
+// bf16) took twice the cycles per fragment read and 4/3 per store.
+constexpr int kTileRow = 32;   // bf16 per staged row
+__device__ __forceinline__ int ldsAt(int row, int q) { return row * kTileRow + 8 * (q ^ ((row >> 1) & 3)); }
+// Staging order: piece p is chunk p & 3 of row p >> 2.
+constexpr int stagedPieces(int rows) { return rows * 4; }
+__device__ __forceinline__ int pieceRow(int piece) { return piece >> 2; }
+__device__ __forceinline__ int pieceQ(int piece) { return piece & 3; }
+
+// Per-lane element offsets of the MFMA fragment reads from a staged TR x TC
+// tile.  Lane (g, r16) of a wave reads chunk g of row
+// (wave * RW + m + oy) * TC + h * 16 + r16 + ox (m: M-tile row, h: M-tile
+// half, ox / oy: tap offset + 1).  With TC and kTileW / 2 even multiples
+// as asserted, the row's swizzle key is ((wave * RW + m + oy) * TC / 2 +
+// ((r16 + ox) >> 1)) & 3, so a read is
+//   (((m + oy) * TC + h * 16) * kTileRow) + aOff[ox][(m + oy) & 3]
+// with the lane part precomputed here (12 registers, no address arithmetic
+// in the tap loops).  Weight rows of a slice start at multiples of 16, so
+// their key is (r16 >> 1) & 3 and bOff = ldsAt(r16, g).
+template <int RW, int TC>
+__device__ __forceinline__ void fragOffsets(int wave, int g, int r16, int (&aOff)[3][4], int& bOff) {
+    static_assert(TC % 2 == 0 && (kTileW / 2) % 4 == 0, "swizzle key of a fragment row");
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int key = (((wave * RW + c) * (TC / 2)) + ((r16 + ox) >> 1)) & 3;
+            aOff[ox][c] = (wave * RW * TC + r16 + ox) * kTileRow + 8 * (g ^ key);
+        }
+    bOff = ldsAt(r16, g);
+}
 template <int NT, int RW>
 __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX, int tilesY) {
     constexpr int TH = 4 * RW, MT = 2 * RW;
     constexpr int TR = TH + 2, TC = kTileW + 2;
-    constexpr int kPiecesA = TR * TC * 4, kPerA = (kPiecesA + kBlock - 1) / kBlock;
-    constexpr int kPiecesW = 9 * NT * 16 * 4, kPerW = (kPiecesW + kBlock - 1) / kBlock;
-    __shared__ __attribute__((aligned(16))) bf16 sA[TR * TC * kLdsRow];
-    __shared__ __attribute__((aligned(16))) bf16 sW[9 * NT * 16 * kLdsRow];
+    constexpr int kRowsA = TR * TC, kPiecesA = stagedPieces(kRowsA), kPerA = (kPiecesA + kBlock - 1) / kBlock;
+    constexpr int kPiecesW = stagedPieces(9 * NT * 16), kPerW = (kPiecesW + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) bf16 sA[TR * TC * kTileRow];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * NT * 16 * kTileRow];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r16 = lane & 15;
+    int aOff[3][4], bOff;
+    fragOffsets<RW, TC>(wave, g, r16, aOff, bOff);
     int bid = blockIdx.x;
     const int tx = bid % tilesX;
     bid /= tilesX;
@@ -400,8 +440,8 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX,
             for (int j = 0; j < kPerA; ++j) {
                 const int piece = threadIdx.x + j * kBlock;
                 va[j] = zero8();
-                if (piece < kPiecesA) {
-                    const int sidx = piece >> 2, q = piece & 3;
+                const int sidx = pieceRow(piece), q = pieceQ(piece);
+                if (piece < kPiecesA && sidx < kRowsA) {
                     const int yy = y0 - 1 + sidx / TC, xx = x0 - 1 + sidx % TC;
                     if (yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_) {
                         const bf16* src;
@@ -417,7 +457,9 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX,
 #pragma unroll
             for (int j = 0; j < kPerA; ++j) {
                 const int piece = threadIdx.x + j * kBlock;
-                if (piece < kPiecesA) *reinterpret_cast<bf16x8*>(sA + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = va[j];
+                const int row = pieceRow(piece);
+                if (piece < kPiecesA && row < kRowsA)
+                    *reinterpret_cast<bf16x8*>(sA + ldsAt(row, pieceQ(piece))) = va[j];
             }
         }
         {   // weight slice of all 9 taps: global -> registers -> LDS
@@ -426,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX,
             for (int j = 0; j < kPerW; ++j) {
                 const int piece = threadIdx.x + j * kBlock;
                 if (piece < kPiecesW) {
-                    const int row = piece >> 2, q = piece & 3;
+                    const int row = pieceRow(piece), q = pieceQ(piece);
                     const int t = row / (NT * 16), r = row % (NT * 16);
                     vw[j] = *reinterpret_cast<const bf16x8*>(a.w + ((size_t)(coBase + r) * 9 + t) * cin + c + 8 * q);
                 }
@@ -434,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX,
 #pragma unroll
             for (int j = 0; j < kPerW; ++j) {
                 const int piece = threadIdx.x + j * kBlock;
-                if (piece < kPiecesW) *reinterpret_cast<bf16x8*>(sW + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = vw[j];
+                if (piece < kPiecesW) *reinterpret_cast<bf16x8*>(sW + ldsAt(pieceRow(piece), pieceQ(piece))) = vw[j];
             }
         }
         __syncthreads();
@@ -444,11 +486,11 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX,
             bf16x8 B[NT];
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
-                B[nt] = *reinterpret_cast<const bf16x8*>(sW + ((t * NT + nt) * 16 + r16) * kLdsRow + 8 * g);
+                B[nt] = *reinterpret_cast<const bf16x8*>(sW + (t * NT + nt) * 16 * kTileRow + bOff);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                const int ry = wave * RW + (mt >> 1), col = (mt & 1) * 16 + r16;
-                const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
+                const int m = (mt >> 1) + oy;
+                const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + (m * TC + (mt & 1) * 16) * kTileRow + aOff[ox][m & 3]);
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
                     acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[nt], A, acc[mt][nt], 0, 0, 0);
@@ -467,8 +509,8 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tile(ConvArgs a, int tilesX,
 }
 
 // Loads of one 32-channel slice of a tile with its halo, without branches:
-// piece p of the tile (pixel p / 4 of the TR x TC window, 8 channels p % 4)
-// reads from a clamped, in-image address and keeps zero outside the image, so
+// piece p of the tile (pixel pieceRow(p) of the TR x TC window, 8 channels
+// pieceQ(p)) reads from a clamped, in-image address and keeps zero outside the image, so
 // every lane issues its loads back to back.  The slice lies in src0 (at half
 // resolution when up0) or in src1.
 struct SliceSrc {
@@ -484,17 +526,18 @@ __device__ __forceinline__ SliceSrc sliceSrc(const ConvArgs& a, int c) {
     r.sh = (s0 && a.up0) ? 1 : 0;
     return r;
 }
-template <int TC, int kPieces, int kPer>
+template <int TC, int kRows, int kPer>
 __device__ __forceinline__ void loadSlice(const ConvArgs& a, const SliceSrc& src, int n, int x0, int y0,
                                           bf16x8 (&va)[kPer]) {
     const int hS = a.h >> src.sh, wS = a.w_ >> src.sh;
 #pragma unroll
     for (int j = 0; j < kPer; ++j) {
         const int piece = threadIdx.x + j * kBlock;
-        const int pc = piece < kPieces ? piece : kPieces - 1;
-        const int sidx = pc >> 2, q = pc & 3;
+        const int row = pieceRow(piece), q = pieceQ(piece);
+        const bool valid = piece < stagedPieces(kRows) && row < kRows;
+        const int sidx = valid ? row : kRows - 1;
         const int yy = y0 - 1 + sidx / TC, xx = x0 - 1 + sidx % TC;
-        const bool in = piece < kPieces && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_;
+        const bool in = valid && yy >= 0 && yy < a.h && xx >= 0 && xx < a.w_;
         const int yc = min(max(yy, 0), a.h - 1), xc = min(max(xx, 0), a.w_ - 1);
         const bf16x8 v = *reinterpret_cast<const bf16x8*>(
             src.base + (((size_t)n * hS + (yc >> src.sh)) * wS + (xc >> src.sh)) * src.stride + 8 * q);
@@ -506,7 +549,7 @@ __device__ __forceinline__ void loadSlice(const ConvArgs& a, const SliceSrc& src
 // output channels, then per M tile its pixel fragment and NT MFMAs (weights
 // as the A operand: transposed accumulators, see tileEpilogue).
 template <int MT, int NT, int RW, int TC>
-__device__ __forceinline__ void tapsFromLds(const bf16* sA, const bf16* sW, int wave, int g, int r16,
+__device__ __forceinline__ void tapsFromLds(const bf16* sA, const bf16* sW, const int (&aOff)[3][4], int bOff,
                                             f32x4 (&acc)[MT][NT]) {
 #pragma unroll
     for (int t = 0; t < 9; ++t) {
@@ -514,11 +557,11 @@ __device__ __forceinline__ void tapsFromLds(const bf16* sA, const bf16* sW, int 
         bf16x8 B[NT];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
-            B[nt] = *reinterpret_cast<const bf16x8*>(sW + ((t * NT + nt) * 16 + r16) * kLdsRow + 8 * g);
+            B[nt] = *reinterpret_cast<const bf16x8*>(sW + (t * NT + nt) * 16 * kTileRow + bOff);
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
-            const int ry = wave * RW + (mt >> 1), col = (mt & 1) * 16 + r16;
-            const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + ((ry + oy) * TC + col + ox) * kLdsRow + 8 * g);
+            const int m = (mt >> 1) + oy;
+            const bf16x8 A = *reinterpret_cast<const bf16x8*>(sA + (m * TC + (mt & 1) * 16) * kTileRow + aOff[ox][m & 3]);
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
                 acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[nt], A, acc[mt][nt], 0, 0, 0);
@@ -537,12 +580,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                                                                                              int tilesY, int numTiles) {
     constexpr int TH = 4 * RW, MT = 2 * RW;
     constexpr int TR = TH + 2, TC = kTileW + 2;
-    constexpr int kPiecesA = TR * TC * 4, kPerA = (kPiecesA + kBlock - 1) / kBlock;
-    constexpr int kPiecesW = 9 * NT * 16 * 4, kPerW = (kPiecesW + kBlock - 1) / kBlock;
-    __shared__ __attribute__((aligned(16))) bf16 sA[TR * TC * kLdsRow];
-    __shared__ __attribute__((aligned(16))) bf16 sW[9 * NT * 16 * kLdsRow];
+    constexpr int kRowsA = TR * TC, kPiecesA = stagedPieces(kRowsA), kPerA = (kPiecesA + kBlock - 1) / kBlock;
+    constexpr int kPiecesW = stagedPieces(9 * NT * 16), kPerW = (kPiecesW + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) bf16 sA[TR * TC * kTileRow];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * NT * 16 * kTileRow];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r16 = lane & 15;
+    int aOff[3][4], bOff;
+    fragOffsets<RW, TC>(wave, g, r16, aOff, bOff);
     const int coBase = blockIdx.y * NT * 16;
     {   // weights of the one 32-channel slice, all taps
         bf16x8 vw[kPerW];
@@ -550,7 +595,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         for (int j = 0; j < kPerW; ++j) {
             const int piece = threadIdx.x + j * kBlock;
             if (piece < kPiecesW) {
-                const int row = piece >> 2, q = piece & 3;
+                const int row = pieceRow(piece), q = pieceQ(piece);
                 const int t = row / (NT * 16), r = row % (NT * 16);
                 vw[j] = *reinterpret_cast<const bf16x8*>(a.w + ((size_t)(coBase + r) * 9 + t) * 32 + 8 * q);
             }
@@ -558,7 +603,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 #pragma unroll
         for (int j = 0; j < kPerW; ++j) {
             const int piece = threadIdx.x + j * kBlock;
-            if (piece < kPiecesW) *reinterpret_cast<bf16x8*>(sW + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = vw[j];
+            if (piece < kPiecesW) *reinterpret_cast<bf16x8*>(sW + ldsAt(pieceRow(piece), pieceQ(piece))) = vw[j];
         }
     }
     // per-lane output channels' biases, read once (no global loads in the
@@ -570,7 +615,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     auto loadTile = [&](int tile) {
         const int tx = tile % tilesX, rest = tile / tilesX;
         const int ty = rest % tilesY, n = rest / tilesY;
-        loadSlice<TC, kPiecesA, kPerA>(a, src, n, tx * kTileW, ty * TH, va);
+        loadSlice<TC, kRowsA, kPerA>(a, src, n, tx * kTileW, ty * TH, va);
     };
     int tile = blockIdx.x;
     if (tile < numTiles) loadTile(tile);
@@ -579,7 +624,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 #pragma unroll
         for (int j = 0; j < kPerA; ++j) {
             const int piece = threadIdx.x + j * kBlock;
-            if (piece < kPiecesA) *reinterpret_cast<bf16x8*>(sA + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = va[j];
+            const int row = pieceRow(piece);
+            if (piece < kPiecesA && row < kRowsA) *reinterpret_cast<bf16x8*>(sA + ldsAt(row, pieceQ(piece))) = va[j];
         }
         __syncthreads();
         const int tx = tile % tilesX, rest = tile / tilesX;
@@ -593,7 +639,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        tapsFromLds<MT, NT, RW, TC>(sA, sW, wave, g, r16, acc);
+        tapsFromLds<MT, NT, RW, TC>(sA, sW, aOff, bOff, acc);
         tileEpilogue<MT, NT, RW, EPI>(a, acc, biasv, ext, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
     }
 }
@@ -606,12 +652,14 @@ template <int NT, int RW, int EPI>
 __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX, int tilesY, int numTiles) {
     constexpr int TH = 4 * RW, MT = 2 * RW;
     constexpr int TR = TH + 2, TC = kTileW + 2;
-    constexpr int kPiecesA = TR * TC * 4, kPerA = (kPiecesA + kBlock - 1) / kBlock;
-    constexpr int kPiecesW = 9 * NT * 16 * 4, kPerW = (kPiecesW + kBlock - 1) / kBlock;
-    __shared__ __attribute__((aligned(16))) bf16 sA[TR * TC * kLdsRow];
-    __shared__ __attribute__((aligned(16))) bf16 sW[9 * NT * 16 * kLdsRow];
+    constexpr int kRowsA = TR * TC, kPiecesA = stagedPieces(kRowsA), kPerA = (kPiecesA + kBlock - 1) / kBlock;
+    constexpr int kPiecesW = stagedPieces(9 * NT * 16), kPerW = (kPiecesW + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) bf16 sA[TR * TC * kTileRow];
+    __shared__ __attribute__((aligned(16))) bf16 sW[9 * NT * 16 * kTileRow];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane >> 4, r16 = lane & 15;
+    int aOff[3][4], bOff;
+    fragOffsets<RW, TC>(wave, g, r16, aOff, bOff);
     const int coBase = blockIdx.y * NT * 16;
     const int cin = a.c0 + a.c1, slices = cin / 32;
     float biasv[NT][4];
@@ -621,12 +669,12 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
         const int tx = tile % tilesX, rest = tile / tilesX;
         const int ty = rest % tilesY, n = rest / tilesY;
         const int c = slice * 32;
-        loadSlice<TC, kPiecesA, kPerA>(a, sliceSrc(a, c), n, tx * kTileW, ty * TH, va);
+        loadSlice<TC, kRowsA, kPerA>(a, sliceSrc(a, c), n, tx * kTileW, ty * TH, va);
 #pragma unroll
         for (int j = 0; j < kPerW; ++j) {
             const int piece = threadIdx.x + j * kBlock;
             const int pc = piece < kPiecesW ? piece : kPiecesW - 1;
-            const int row = pc >> 2, q = pc & 3;
+            const int row = pieceRow(pc), q = pieceQ(pc);
             const int t = row / (NT * 16), r = row % (NT * 16);
             vw[j] = *reinterpret_cast<const bf16x8*>(a.w + ((size_t)(coBase + r) * 9 + t) * cin + c + 8 * q);
         }
@@ -643,12 +691,13 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
 #pragma unroll
         for (int j = 0; j < kPerA; ++j) {
             const int piece = threadIdx.x + j * kBlock;
-            if (piece < kPiecesA) *reinterpret_cast<bf16x8*>(sA + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = va[j];
+            const int row = pieceRow(piece);
+            if (piece < kPiecesA && row < kRowsA) *reinterpret_cast<bf16x8*>(sA + ldsAt(row, pieceQ(piece))) = va[j];
         }
 #pragma unroll
         for (int j = 0; j < kPerW; ++j) {
             const int piece = threadIdx.x + j * kBlock;
-            if (piece < kPiecesW) *reinterpret_cast<bf16x8*>(sW + (piece >> 2) * kLdsRow + 8 * (piece & 3)) = vw[j];
+            if (piece < kPiecesW) *reinterpret_cast<bf16x8*>(sW + ldsAt(pieceRow(piece), pieceQ(piece))) = vw[j];
         }
         __syncthreads();
         const bool last = slice + 1 == slices;
@@ -659,7 +708,7 @@ __global__ __launch_bounds__(kBlock) void k_conv3x3_tileN(ConvArgs a, int tilesX
         if constexpr (EPI == 1)
             if (last) epiLoad<MT, NT, RW>(a, n, tx * kTileW, ty * TH, wave, g, r16, coBase, ext);
         if (nTile < numTiles) load(nTile, nSlice);   // in flight while this slice is computed
-        tapsFromLds<MT, NT, RW, TC>(sA, sW, wave, g, r16, acc);
+        tapsFromLds<MT, NT, RW, TC>(sA, sW, aOff, bOff, acc);
         if (last) {
             tileEpilogue<MT, NT, RW, EPI>(a, acc, biasv, ext, n, tx * kTileW, ty * TH, wave, g, r16, coBase);
 #pragma unroll
