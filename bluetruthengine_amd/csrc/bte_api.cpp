@@ -65,6 +65,7 @@ struct SceneData {
     DBuf dNodes, dNodes4, dNodes8, dLeafTris;
     int bvh4Depth = 0, bvh8Depth = 0;
     DBuf dEnv, dEnvMarg, dEnvCond, dEnvRowSums, dEnvTotal, dEnvMargGuide, dEnvCondGuide, dEnvMono;
+    DBuf dEnvRec;   // footprint records of the bilinear lookup and its pdf (launch_env_records)
     bool envGuides = false;
     bool haveEmission = false, haveFlags = false;
     bool bvhDirty = true;
@@ -271,6 +272,7 @@ DevScene makeScene(const SceneData& c, int width, int height) {
     s.envCond = c.dEnvCond.as<float>();
     s.envMargGuide = c.envGuides ? c.dEnvMargGuide.as<int>() : nullptr;
     s.envCondGuide = c.envGuides ? c.dEnvCondGuide.as<int>() : nullptr;
+    s.envRec = c.dEnvRec.as<float4>();
     if (c.texCount > 0) {
         s.triMat = c.dTriMat.as<int>();
         s.texels = c.dTexels.as<uint32_t>();
@@ -1234,7 +1236,7 @@ int bte_init_env_map(const float* hdriPixelsRGBA, int width, int height, float i
     Context& c = ctx();
     c.hasEnv = false;
     c.dEnv.release(); c.dEnvMarg.release(); c.dEnvCond.release(); c.dEnvRowSums.release(); c.dEnvTotal.release();
-    c.dEnvMargGuide.release(); c.dEnvCondGuide.release(); c.dEnvMono.release();
+    c.dEnvMargGuide.release(); c.dEnvCondGuide.release(); c.dEnvMono.release(); c.dEnvRec.release();
     c.envGuides = false;
     c.envW = c.envH = 0;
     c.envTotal = 0.0f;
@@ -1257,6 +1259,9 @@ int bte_init_env_map(const float* hdriPixelsRGBA, int width, int height, float i
     HIPCHK(hipMemcpyAsync(&monotone, c.dEnvMono.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     HIPCHK(hipStreamSynchronize(c.stream));
     c.envGuides = monotone != 0;
+    CHECK_RC(c.dEnvRec.ensure(env_record_bytes(width, height)));
+    launch_env_records(c.dEnv.as<float4>(), c.dEnvMarg.as<float>(), c.dEnvCond.as<float>(), width, height, total,
+                       c.dEnvRec.as<float4>(), c.stream);
     HIPCHK(hipGetLastError());
     c.envW = width;
     c.envH = height;
@@ -1314,7 +1319,7 @@ int bte_cleanup(void) {
     DBuf* bufs[] = {&c.dTris, &c.dMaterials, &c.dTriMat, &c.dTriBsdf, &c.dBsdfs, &c.dEmission, &c.dFlags,
                     &c.dEmitterOfTri, &c.dEmitters, &c.dEmTriIdx, &c.dEmTriCdf, &c.dSceneCdf, &c.dSpots, &c.dNodes, &c.dNodes4, &c.dNodes8,
                     &c.dLeafTris, &c.dEnv, &c.dEnvMarg, &c.dEnvCond, &c.dEnvRowSums, &c.dEnvTotal, &c.dEnvMargGuide,
-                    &c.dEnvCondGuide, &c.dEnvMono, &c.dAccum,
+                    &c.dEnvCondGuide, &c.dEnvMono, &c.dEnvRec, &c.dAccum,
                     &c.dFb, &c.dSum64, &c.dHit, &c.dCatQ, &c.dCat, &c.dFinal, &c.dShO, &c.dShD, &c.dShC, &c.dShX, &c.dCounters, &c.dHitCap,
                     &c.dOvf};
     for (DBuf* b : bufs) b->release();

@@ -30,6 +30,7 @@ struct DevScene {
     const float* envCond;
     const int* envMargGuide;      // guide tables of the CDF inversion, null = plain binary search
     const int* envCondGuide;
+    const float4* envRec;         // (envH + 1) * envW footprint records of 4 float4 (launch_env_records)
     const int* triMat;            // material id per triangle (texture index); null without textures
     const uint32_t* texels;       // RGBA8 texels of every albedo texture, back to back
     const int4* texInfo;          // [texCount]: texel offset, width, height (width 0 = no texture)
@@ -202,6 +203,11 @@ size_t noalloc_scratch_bytes();
 int noalloc_max_bvh_depth();
 void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, double* rowSums, float* total,
                     int* margGuide, int* condGuide, int* monotone, hipStream_t s);
+// Footprint records of the bilinear environment lookup and its pdf, built
+// from the texels and the CDFs of launch_env_cdf (total: their power sum).
+size_t env_record_bytes(int W, int H);
+void launch_env_records(const float4* env, const float* marg, const float* cond, int W, int H, float total,
+                        float4* rec, hipStream_t s);
 // traversalBlocksPerCU 0 = as many as occupancy allows; grabRows 1, 2, 4 or 8 (else 4).
 LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows, int bvhWidth);
 int persistent_threads(const LaunchShape& ls);

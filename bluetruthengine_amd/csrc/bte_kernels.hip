@@ -601,25 +601,48 @@ BTE_DEV V3 envTexel(const DevScene& sc, int ix, int iy) {
     const float4 t = sc.env[(size_t)iy * (size_t)sc.envW + (size_t)ix];
     return {t.x, t.y, t.z};
 }
-BTE_DEV V3 envBilinear(const DevScene& sc, float u, float v) {
+// The 2x2 texels of a bilinear environment lookup at (u, v): corners ix0 / ix1
+// (wrapped) by iy0 / iy1 (clamped), weights a / b, and the footprint record
+// (iyb + 1) * W + ix0 that holds them (iyb = floor(v H - 0.5) in [-1, H-1]).
+struct EnvFoot {
+    int ix0, ix1, iy0, iy1;
+    float a, b;
+    size_t rec;
+};
+BTE_DEV EnvFoot envFoot(const DevScene& sc, float u, float v) {
     const int W = sc.envW, H = sc.envH;
     float x = u * (float)W - 0.5f;
     float y = v * (float)H - 0.5f;
     if (!(x == x)) x = 0.0f;
     if (!(y == y)) y = 0.0f;
     const float fx = floorf(x), fy = floorf(y);
-    const float a = x - fx, b = y - fy;
+    EnvFoot f;
+    f.a = x - fx;
+    f.b = y - fy;
     const int ix = (int)fx, iy = (int)fy;
     int ix0 = ix % W; if (ix0 < 0) ix0 += W;
     int ix1 = ix0 + 1; if (ix1 == W) ix1 = 0;
-    const int iy0 = min(max(iy, 0), H - 1);
-    const int iy1 = min(max(iy + 1, 0), H - 1);
+    f.ix0 = ix0;
+    f.ix1 = ix1;
+    f.iy0 = min(max(iy, 0), H - 1);
+    f.iy1 = min(max(iy + 1, 0), H - 1);
+    const int iyb = min(max(iy, -1), H - 1);   // iy0 and iy1 depend on iy only through this
+    f.rec = (size_t)(iyb + 1) * (size_t)W + (size_t)ix0;
+    return f;
+}
+// Bilinear texel mix of a footprint record (r0..r2: the corners' RGB).
+BTE_DEV V3 envMix(const EnvFoot& f, const float4& r0, const float4& r1, const float4& r2) {
+    const float a = f.a, b = f.b;
     const float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
-    const V3 t00 = envTexel(sc, ix0, iy0), t10 = envTexel(sc, ix1, iy0);
-    const V3 t01 = envTexel(sc, ix0, iy1), t11 = envTexel(sc, ix1, iy1);
+    const V3 t00 = {r0.x, r0.y, r0.z}, t10 = {r0.w, r1.x, r1.y}, t01 = {r1.z, r1.w, r2.x}, t11 = {r2.y, r2.z, r2.w};
     return {w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
             w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
             w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z};
+}
+BTE_DEV V3 envBilinear(const DevScene& sc, float u, float v) {
+    const EnvFoot f = envFoot(sc, u, v);
+    const float4* r = sc.envRec + 4 * f.rec;
+    return envMix(f, r[0], r[1], r[2]);
 }
 BTE_DEV void dirToEquirect(V3 dir, float rot, float& u, float& v) {
     float phi = datan2(dir.z, dir.x) + rot;
@@ -662,25 +685,38 @@ BTE_DEV int cdfSearchGuided(const float* cdf, int count, float x, const int* gui
     }
     return lo;
 }
-BTE_DEV float envPdfUV(const DevScene& sc, float u, float v) {
-    if (sc.envTotal <= 0.0f) return 0.0f;
-    const int W = sc.envW, H = sc.envH;
-    const float fx = u * (float)W;
-    const float fy = v * (float)H;
-    const int ix = min(max((int)fminf(fmaxf(fx, 0.0f), (float)W), 0), W - 1);
-    const int iy = min(max((int)fminf(fmaxf(fy, 0.0f), (float)H), 0), H - 1);
-    const float ms = sc.envMarg[H];
-    const float rw = sc.envMarg[iy + 1] - sc.envMarg[iy];
+// The environment pdf of texel (ix, iy) before its division by
+// 2 pi^2 sin(theta): ((pdfRow * pdfCol) * W) * H, or 0 where the pdf is 0.
+BTE_DEV float envPdfFactor(const float* marg, const float* cond, int W, int H, float total, int ix, int iy) {
+    if (total <= 0.0f) return 0.0f;
+    const float ms = marg[H];
+    const float rw = marg[iy + 1] - marg[iy];
     if (ms <= 0.0f || rw <= 0.0f) return 0.0f;
     const float pdfRow = rw / ms;
-    const float* row = sc.envCond + (size_t)iy * (size_t)(W + 1);
+    const float* row = cond + (size_t)iy * (size_t)(W + 1);
     const float rs = row[W];
     const float cw = row[ix + 1] - row[ix];
     if (rs <= 0.0f || cw <= 0.0f) return 0.0f;
     const float pdfCol = cw / rs;
+    return pdfRow * pdfCol * (float)W * (float)H;
+}
+// The pdf texel of (u, v), nearest rather than bilinear.
+BTE_DEV void envPdfTexel(const DevScene& sc, float u, float v, int& ix, int& iy) {
+    const int W = sc.envW, H = sc.envH;
+    const float fx = u * (float)W;
+    const float fy = v * (float)H;
+    ix = min(max((int)fminf(fmaxf(fx, 0.0f), (float)W), 0), W - 1);
+    iy = min(max((int)fminf(fmaxf(fy, 0.0f), (float)H), 0), H - 1);
+}
+BTE_DEV float envPdfFromFactor(float q, float v) {
     const float theta = v * m::PI;
     const float st = fmaxf(dsin(theta), 1e-6f);
-    return (pdfRow * pdfCol * (float)W * (float)H) / (2.0f * m::PI * m::PI * st);
+    return q / (2.0f * m::PI * m::PI * st);
+}
+BTE_DEV float envPdfUV(const DevScene& sc, float u, float v) {
+    int ix, iy;
+    envPdfTexel(sc, u, v, ix, iy);
+    return envPdfFromFactor(envPdfFactor(sc.envMarg, sc.envCond, sc.envW, sc.envH, sc.envTotal, ix, iy), v);
 }
 
 // ---------------------------------------------------------------------------
@@ -940,11 +976,25 @@ BTE_DEV void envHit(const DevScene& sc, V3 d, bool spec, float brdfPdf, V3 thr, 
     if (!sc.hasEnv) return;
     float u, v;
     dirToEquirect(d, sc.envRot, u, v);       // shared by eval and pdf (same inputs, same result)
-    const V3 tx = envBilinear(sc, u, v);
+    // one footprint record: the bilinear texels and the pdf factors of the
+    // same four texels, one of which is the pdf's nearest texel
+    const EnvFoot f = envFoot(sc, u, v);
+    const float4* r = sc.envRec + 4 * f.rec;
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    const V3 tx = envMix(f, r0, r1, r2);
     const V3 Le = {tx.x * sc.envIntensity, tx.y * sc.envIntensity, tx.z * sc.envIntensity};
     float w = 1.0f;
     if (!spec) {
-        const float ep = envPdfUV(sc, u, v);
+        int ix, iy;
+        envPdfTexel(sc, u, v, ix, iy);
+        float q;
+        if ((ix == f.ix0 || ix == f.ix1) && (iy == f.iy0 || iy == f.iy1)) {
+            const float qy0 = (ix == f.ix0) ? r3.x : r3.y, qy1 = (ix == f.ix0) ? r3.z : r3.w;
+            q = (iy == f.iy0) ? qy0 : qy1;
+        } else {   // not reached for finite (u, v); the CDFs give the same value
+            q = envPdfFactor(sc.envMarg, sc.envCond, sc.envW, sc.envH, sc.envTotal, ix, iy);
+        }
+        const float ep = envPdfFromFactor(q, v);
         const float sum = brdfPdf + ep;
         w = (sum > 0.0f) ? brdfPdf / sum : 0.0f;
     }
@@ -2945,6 +2995,36 @@ void launch_env_cdf(const float4* env, int W, int H, float* marg, float* cond, d
     hipLaunchKernelGGL(k_env_marginal, dim3(1), dim3(kScanBlock), 0, s, rowSums, H, marg, total);
     launch_fill_i32(monotone, 1, 1, s);
     hipLaunchKernelGGL(k_env_guides, dim3(H + 1), dim3(256), 0, s, marg, cond, W, H, margGuide, condGuide, monotone);
+}
+
+// Footprint records (see envFoot): record (iyb + 1) * W + ix0 holds the RGB of
+// texels (ix0, iy0), (ix1, iy0), (ix0, iy1), (ix1, iy1) in 12 floats and
+// their pdf factors in 4, so that a lookup reads one aligned 64-byte record
+// instead of two texel rows and four CDF entries.
+__global__ void k_env_records(const float4* env, const float* marg, const float* cond, int W, int H, float total,
+                              float4* rec) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)(H + 1) * W) return;
+    const int ix0 = (int)(i % W), iyb = (int)(i / W) - 1;
+    const int ix1 = (ix0 + 1 == W) ? 0 : ix0 + 1;
+    const int iy0 = min(max(iyb, 0), H - 1), iy1 = min(max(iyb + 1, 0), H - 1);
+    const float4 t00 = env[(size_t)iy0 * W + ix0], t10 = env[(size_t)iy0 * W + ix1];
+    const float4 t01 = env[(size_t)iy1 * W + ix0], t11 = env[(size_t)iy1 * W + ix1];
+    float4* r = rec + 4 * i;
+    r[0] = make_float4(t00.x, t00.y, t00.z, t10.x);
+    r[1] = make_float4(t10.y, t10.z, t01.x, t01.y);
+    r[2] = make_float4(t01.z, t11.x, t11.y, t11.z);
+    r[3] = make_float4(envPdfFactor(marg, cond, W, H, total, ix0, iy0), envPdfFactor(marg, cond, W, H, total, ix1, iy0),
+                       envPdfFactor(marg, cond, W, H, total, ix0, iy1), envPdfFactor(marg, cond, W, H, total, ix1, iy1));
+}
+
+size_t env_record_bytes(int W, int H) { return (size_t)(H + 1) * (size_t)W * 4 * sizeof(float4); }
+
+void launch_env_records(const float4* env, const float* marg, const float* cond, int W, int H, float total,
+                        float4* rec, hipStream_t s) {
+    const long long n = (long long)(H + 1) * W;
+    hipLaunchKernelGGL(k_env_records, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, env, marg, cond, W, H, total,
+                       rec);
 }
 
 // Resident blocks per CU of one category's shading kernel.
