@@ -549,6 +549,12 @@ int renderWavefront(int nframes) {
     while (done < nframes) {
         const int frames = std::min(F, nframes - done);
         const long long paths = (long long)N * frames;
+        // the bounce-0 passes compute each slot's camera path instead of
+        // loading it (k_generate stores none)
+        wb.camPaths = paths;
+        wb.camPixels = N;
+        // rays too when the packet pass traces bounce 0 (it computes them)
+        wb.camRays = (ls.bvhWidth == 4 && ls.packetBlocks > 0 && !wb.travCtr) ? 1 : 0;
         // the batch's first frame index lives in device memory, so captured
         // segments replay unchanged from frame to frame
         launch_fill_i32(c.dFrameBase.as<int>(), 1, (int)c.frameIndex, c.stream);

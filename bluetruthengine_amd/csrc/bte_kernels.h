@@ -138,6 +138,15 @@ struct WaveBuffers {
     float4* aovAccAlb;    // per pixel running means of the two AOVs (with aovAlb)
     float4* aovAccNrm;
     const int* frameBase; // device int: frame index of the batch's first frame (set before each batch)
+    // Single scenes: camPaths > 0 means the bounce-0 path state of the pass's
+    // camPaths slots over camPixels pixels is not stored; the bounce-0
+    // categoriser and shading kernels (their CAM instantiations) compute each
+    // slot's camera path (cameraPath) where they would load it, and with
+    // camRays set the bounce-0 rays are not stored either (the packet pass
+    // computes them).
+    long long camPaths;
+    int camPixels;
+    int camRays;
     // multi-scene batches only (null / 0 for a single scene)
     const uint8_t* sceneOfChunk;   // scene of each kBatchChunk-aligned block of slots / path ids
     const BatchSlot* slots;        // [sceneCount]
@@ -150,7 +159,8 @@ struct WaveBuffers {
 
 struct LaunchShape {
     int extendBlocks, shadeBlocks, connectBlocks, auxBlocks;
-    int shadeCatBlocks[2][kNumCats];   // resident grid of each category's shading kernel [batch][category]
+    int shadeCatBlocks[2][kNumCats];
+    int shadeCamBlocks[kNumCats];       // grids of the bounce-0 (camera path) categoriser and shading kernels   // resident grid of each category's shading kernel [batch][category]
     int tailBlocks;       // resident grid of the tail pass (launch_tail)
     int packetBlocks;     // resident grid of the camera-ray packet pass (0: per-ray traversal on bounce 0)
     int ldsStack;         // traversal LDS stack entries per lane: 8, 16 or 32

@@ -1050,14 +1050,21 @@ __global__ void k_fill_i32(int* p, long long n, int v) {
     if (i < n) p[i] = v;
 }
 
+// Primary direction of a camera path and its random state after the pixel
+// jitter draws (pixel / frame seed it).
+BTE_DEV V3 cameraRay(const DevScene& sc, int pixel, int frame, uint32_t& rng) {
+    const int px = pixel % sc.width, py = pixel / sc.width;
+    rng = seedRng(hashUint((uint32_t)pixel ^ ((uint32_t)frame * 0x9e3779b9u)));
+    const float jx = rngNext(rng) - 0.5f;
+    const float jy = rngNext(rng) - 0.5f;
+    return primaryDir(sc.cam, px, py, sc.width, sc.height, jx, jy);
+}
+
 // One camera path: queue slot `slot`, path id `pid`, pixel / frame seed it.
 BTE_DEV void generateOne(const DevScene& sc, const WaveBuffers& wb, long long slot, uint32_t pid, int pixel,
                          int frame) {
-    const int px = pixel % sc.width, py = pixel / sc.width;
-    uint32_t rng = seedRng(hashUint((uint32_t)pixel ^ ((uint32_t)frame * 0x9e3779b9u)));
-    const float jx = rngNext(rng) - 0.5f;
-    const float jy = rngNext(rng) - 0.5f;
-    const V3 d = primaryDir(sc.cam, px, py, sc.width, sc.height, jx, jy);
+    uint32_t rng;
+    const V3 d = cameraRay(sc, pixel, frame, rng);
     wb.rayO[0][slot] = make_float4(sc.cam.origin.x, sc.cam.origin.y, sc.cam.origin.z, 0.0f);
     wb.rayD[0][slot] = make_float4(d.x, d.y, d.z, 1.0f);
     wb.thr[0][slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
@@ -1095,14 +1102,98 @@ BTE_DEV void slotPath(long long i, int pixels, int frames, int W, int H, int& fr
     pixel = (ty * th + inTile / tw) * W + tx * tw + inTile % tw;
 }
 
+// slotPath for a pass below 2^31 slots (every single-scene pass: its counters
+// are 32-bit), in 32-bit unsigned arithmetic: the same frame and pixel.
+BTE_DEV void slotPath32(uint32_t i, uint32_t pixels, uint32_t frames, uint32_t W, uint32_t H, int& frame, int& pixel) {
+    const uint32_t fg = (frames % 16u == 0) ? 16u : (frames % 8u == 0) ? 8u : (frames % 4u == 0) ? 4u : (frames % 2u == 0) ? 2u : 1u;
+    const uint32_t tpx = 64u / fg;
+    const uint32_t tw = (tpx >= 32u) ? 8u : (tpx >= 8u ? 4u : 2u), th = tpx / tw;
+    if (W % tw || H % th) {
+        frame = (int)(i / pixels);
+        pixel = (int)(i % pixels);
+        return;
+    }
+    const uint32_t chunk = fg * pixels;
+    const uint32_t c = i / chunk;
+    const uint32_t r = i - c * chunk;
+    const uint32_t group = r >> 6, within = r & 63u;
+    frame = (int)(c * fg + within % fg);
+    const uint32_t inTile = within / fg;
+    const uint32_t tilesX = W / tw;
+    const uint32_t tx = group % tilesX, ty = group / tilesX;
+    pixel = (int)((ty * th + inTile / tw) * W + tx * tw + inTile % tw);
+}
+
 __global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb, long long paths, int pixels) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= paths) return;
     const int frame0 = *wb.frameBase;
     int f, pixel;
     slotPath(i, pixels, (int)(paths / pixels), sc.width, sc.height, f, pixel);
-    generateOne(sc, wb, i, (uint32_t)((long long)f * pixels + pixel), pixel, frame0 + f);
+    const uint32_t pid = (uint32_t)((long long)f * pixels + pixel);
+    if (wb.camPaths == 0) {
+        generateOne(sc, wb, i, pid, pixel, frame0 + f);
+    } else {   // the state is computed where it is used; the rays only for per-ray traversal
+        if (!wb.camRays) {
+            uint32_t rng;
+            const V3 d = cameraRay(sc, pixel, frame0 + f, rng);
+            wb.rayO[0][i] = make_float4(sc.cam.origin.x, sc.cam.origin.y, sc.cam.origin.z, 0.0f);
+            wb.rayD[0][i] = make_float4(d.x, d.y, d.z, 1.0f);
+        }
+        if (wb.aovAlb) {
+            wb.aovAlb[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            wb.aovNrm[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
     if (i == 0) wb.counters[kCtrRays] = (int)paths;
+}
+
+// Bounce-0 state of queue slot `slot` when it is not stored (wb.camPaths >
+// 0): the values generateOne writes for that slot.
+BTE_DEV void cameraPath(const DevScene& sc, const WaveBuffers& wb, long long slot, float4& ro, float4& rd, float4& th,
+                        float4& ra, uint2& st) {
+    int f, pixel;
+    slotPath32((uint32_t)slot, (uint32_t)wb.camPixels, (uint32_t)(wb.camPaths / wb.camPixels), (uint32_t)sc.width,
+               (uint32_t)sc.height, f, pixel);
+    uint32_t rng;
+    const V3 d = cameraRay(sc, pixel, *wb.frameBase + f, rng);
+    ro = make_float4(sc.cam.origin.x, sc.cam.origin.y, sc.cam.origin.z, 0.0f);
+    rd = make_float4(d.x, d.y, d.z, 1.0f);
+    th = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    st = make_uint2(rng, (uint32_t)((long long)f * wb.camPixels + pixel) | kSpecBit);
+}
+// Path state of queue slot qi entering `bounce` (CAM: the bounce-0 camera path,
+// computed).
+template <bool CAM>
+BTE_DEV void loadPath(const DevScene& sc, const WaveBuffers& wb, int bounce, int qi, float4& ro, float4& rd,
+                      float4& th, float4& ra, uint2& st) {
+    if constexpr (CAM) {
+        cameraPath(sc, wb, qi, ro, rd, th, ra, st);
+    } else {
+        const int cur = bounce & 1;
+        ro = wb.rayO[cur][qi];
+        rd = wb.rayD[cur][qi];
+        th = wb.thr[cur][qi];
+        ra = wb.rad[cur][qi];
+        st = wb.st[cur][qi];
+    }
+}
+// The ray of queue slot qi entering bounce 0 (the packet pass): computed when
+// it is not stored.
+BTE_DEV void loadCameraRay(const DevScene& sc, const WaveBuffers& wb, int qi, float4& ro, float4& rd) {
+    if (wb.camPaths > 0 && wb.camRays) {
+        int f, pixel;
+        slotPath32((uint32_t)qi, (uint32_t)wb.camPixels, (uint32_t)(wb.camPaths / wb.camPixels), (uint32_t)sc.width,
+                   (uint32_t)sc.height, f, pixel);
+        uint32_t rng;
+        const V3 d = cameraRay(sc, pixel, *wb.frameBase + f, rng);
+        ro = make_float4(sc.cam.origin.x, sc.cam.origin.y, sc.cam.origin.z, 0.0f);
+        rd = make_float4(d.x, d.y, d.z, 1.0f);
+        return;
+    }
+    ro = wb.rayO[0][qi];
+    rd = wb.rayD[0][qi];
 }
 
 // Batch: blockIdx.y = scene.  Scene s's paths take the path ids and queue
@@ -1137,10 +1228,10 @@ BTE_DEV uint32_t shadeCategory(const DevScene& sc, int tri) {
 // take 64 consecutive queue slots per atomic: consecutive slots hold rays that
 // were adjacent before compaction, so a wave's lanes share BVH cache lines.
 template <bool COUNT, int L, int W>
-BTE_DEV void extendOne(const DevScene& sc, const WaveBuffers& wb, int cur, int qi, const StackRef<L>& stk,
+BTE_DEV void extendOne(const DevScene& sc, const WaveBuffers& wb, int bounce, int qi, const StackRef<L>& stk,
                        uint32_t* nodes, uint32_t* tris) {
-    const float4 o = wb.rayO[cur][qi];
-    const float4 d = wb.rayD[cur][qi];
+    const float4 o = wb.rayO[bounce & 1][qi];
+    const float4 d = wb.rayD[bounce & 1][qi];
     float t = 0.0f, u = 0.0f, v = 0.0f;
     const int tri = traverseW<W, false, COUNT, L>(sc, xyz(o), xyz(d), RT_EPS, 1e30f, stk, t, u, v, nodes, tris);
     wb.hit[qi] = make_float4(__int_as_float(tri), t, u, v);
@@ -1151,7 +1242,6 @@ template <bool COUNT, int L, int W, bool BATCH = false>
 __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, int bounce,
                                                     const DevScene* __restrict__ scenes) {
     __shared__ int lds[L * kBlock];
-    const int cur = bounce & 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
     const int count = BATCH ? segSlots(wb, bounce, kSegRays) : ctr[kCtrRays];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -1163,7 +1253,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
         if constexpr (!BATCH) {
             for (int row = 0; row < wb.grabRows; ++row) {
                 const int qi = chunk + row * 64 + lane_id();
-                if (qi < count) extendOne<COUNT, L, W>(sc, wb, cur, qi, stk, &nodes, &tris);
+                if (qi < count) extendOne<COUNT, L, W>(sc, wb, bounce, qi, stk, &nodes, &tris);
             }
         } else {
             const SegChunk k = segChunk(wb, bounce, kSegRays, chunk);
@@ -1171,7 +1261,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(DevScene sc, WaveBuffers wb, 
             const DevScene S = fetchScene(scenes, k.s);
             for (int row = 0; row < wb.grabRows; ++row) {
                 const int e = row * 64 + lane_id();
-                if (e < k.left) extendOne<COUNT, L, W>(S, wb, cur, k.first + e, stk, &nodes, &tris);
+                if (e < k.left) extendOne<COUNT, L, W>(S, wb, bounce, k.first + e, stk, &nodes, &tris);
             }
         }
     }
@@ -1437,7 +1527,6 @@ template <bool COUNT, int L, bool BATCH, int W = 4>
 __global__ __launch_bounds__(kBlock) void k_extend_pool(DevScene sc, WaveBuffers wb, int bounce,
                                                          const DevScene* __restrict__ scenes) {
     __shared__ int lds[L * kBlock];
-    const int cur = bounce & 1;
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
     const int count = BATCH ? segSlots(wb, bounce, kSegRays) : ctr[kCtrRays];
     const int gtid = blockIdx.x * kBlock + threadIdx.x;
@@ -1453,7 +1542,7 @@ __global__ __launch_bounds__(kBlock) void k_extend_pool(DevScene sc, WaveBuffers
     WtRay r;
     auto start = [&](int slot) {
         qi = slot;
-        const float4 o = wb.rayO[cur][qi], d = wb.rayD[cur][qi];
+        const float4 o = wb.rayO[bounce & 1][qi], d = wb.rayD[bounce & 1][qi];
         r = makeWtRay(xyz(o), xyz(d));
         best = 1e30f;
         bestTri = -1;
@@ -1508,7 +1597,6 @@ __global__ __launch_bounds__(kBlock) void k_extend_packet(DevScene sc, WaveBuffe
     constexpr float kNone = __builtin_huge_valf();
     __shared__ int stacks[kBlock / 64][kStackSize];
     int* stk = stacks[threadIdx.x >> 6];
-    const int cur = bounce & 1;
     const int count = BATCH ? segSlots(wb, bounce, kSegRays) : wb.counters[(size_t)bounce * kCtrStride + kCtrRays];
     int* wk = work_group(wb, bounce, kWorkExtend);
     for (int chunk = first_chunk(64); chunk < count; chunk = next_chunk(wk, 64, count)) {
@@ -1526,8 +1614,10 @@ __global__ __launch_bounds__(kBlock) void k_extend_packet(DevScene sc, WaveBuffe
         const bool valid = lane_id() < left;
         V3 o = {0, 0, 0}, d = {0, 0, 1};
         if (valid) {
-            o = xyz(wb.rayO[cur][qi]);
-            d = xyz(wb.rayD[cur][qi]);
+            float4 ro, rd;
+            loadCameraRay(sc, wb, qi, ro, rd);
+            o = xyz(ro);
+            d = xyz(rd);
         }
         const WtRay r = makeWtRay(o, d);
         const bool act = valid && r.valid && sc.triCount > 0;
@@ -1596,10 +1686,11 @@ __global__ __launch_bounds__(kBlock) void k_extend_packet(DevScene sc, WaveBuffe
 // serialise, so they are amortised over the block).
 // A missed ray ends its path: env-map emission with MIS (A13) added to its
 // radiance, which becomes the path's final radiance.
+template <bool CAM>
 BTE_DEV void shadeMiss(const DevScene& sc, const WaveBuffers& wb, int qi, int bounce) {
-    const int cur = bounce & 1;
-    const float4 ro = wb.rayO[cur][qi], rd = wb.rayD[cur][qi], th = wb.thr[cur][qi], ra = wb.rad[cur][qi];
-    const uint2 st = wb.st[cur][qi];
+    float4 ro, rd, th, ra;
+    uint2 st;
+    loadPath<CAM>(sc, wb, bounce, qi, ro, rd, th, ra, st);
     const uint32_t pid = st.y & kPidMask;
     const bool spec = (st.y & kSpecBit) != 0;
     if (wb.hitCapture && bounce < wb.maxRec)
@@ -1611,7 +1702,7 @@ BTE_DEV void shadeMiss(const DevScene& sc, const WaveBuffers& wb, int qi, int bo
 
 constexpr int kCatSub = 4;   // 64-slot rows per wave per iteration
 // Misses are shaded here, in queue order, and are not queued.
-template <bool BATCH = false>
+template <bool BATCH = false, bool CAM = false>
 __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers wb, int bounce,
                                                         const DevScene* __restrict__ scenes) {
     __shared__ int sCnt[kCatSub][kBlock / 64][kNumCats];
@@ -1637,8 +1728,8 @@ __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers 
             const int i = first + e;
             c[j] = (e < left) ? (int)(wb.cat[i] & 7u) : -1;
             if (c[j] == kCatMiss) {
-                if constexpr (BATCH) shadeMiss(fetchScene(scenes, s), wb, i, bounce);
-                else shadeMiss(sc, wb, i, bounce);
+                if constexpr (BATCH) shadeMiss<false>(fetchScene(scenes, s), wb, i, bounce);
+                else shadeMiss<CAM>(sc, wb, i, bounce);
                 c[j] = -1;
             }
             for (int k = 1; k < kNumCats; ++k) {
@@ -1691,7 +1782,7 @@ struct ShadeSinks {
 // Shades one 64-slot row of a category queue: lanes with `active` set hold
 // the hits at list[li].  Every lane of the block must call it the same number
 // of times (block-wide appends).
-template <int T, bool BATCH>
+template <int T, bool BATCH, bool CAM>
 BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, int li, bool active,
                       const ShadeSinks& out, BlockAppendLds& blk) {
     const int cur = bounce & 1, nxt = cur ^ 1;
@@ -1705,8 +1796,9 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
     int bsdfId = 0;
     if (active) {
         const int qi = list[li];
-        const float4 ro = wb.rayO[cur][qi], rd = wb.rayD[cur][qi], th = wb.thr[cur][qi], ra = wb.rad[cur][qi];
-        const uint2 st = wb.st[cur][qi];
+        float4 ro, rd, th, ra;
+        uint2 st;
+        loadPath<CAM>(sc, wb, bounce, qi, ro, rd, th, ra, st);
         h = wb.hit[qi];
         bsdfId = (int)(wb.cat[qi] >> 3);
         o = xyz(ro); brdfPdf = ro.w;
@@ -1799,7 +1891,7 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
 // Registers held to 4 waves per SIMD (128): the rows wait on chains of
 // gathers, and the Disney kernel (142 registers unforced, 3 waves) runs
 // faster with the extra wave than without its few spills.
-template <int T, bool BATCH = false>
+template <int T, bool BATCH = false, bool CAM = false>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_shade(DevScene sc, WaveBuffers wb,
                                                                                           int bounce, int maxDepth,
                                                    const DevScene* __restrict__ scenes) {
@@ -1819,7 +1911,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             const ShadeSinks out{ctrNext + kCtrRays, 0, ctr + kCtrShadow, 1, 0};
             for (int row = 0; row < wb.grabRows; ++row) {
                 const int li = chunk + row * 64 + lane_id();
-                shadeRow<T, false>(sc, wb, bounce, maxDepth, li, has && li < count, out, blk);
+                shadeRow<T, false, CAM>(sc, wb, bounce, maxDepth, li, has && li < count, out, blk);
             }
         } else {
             SegChunk k{0, 0, 0};   // a wave without a chunk shades nothing (scene 0 stands in)
@@ -1832,7 +1924,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
                                  segCounts(wb, bounce, kSegShadow) + sIdx, wb.sceneCount, begin};
             for (int row = 0; row < wb.grabRows; ++row) {
                 const int e = row * 64 + lane_id();
-                shadeRow<T, true>(S, wb, bounce, maxDepth, k.first + e, has && e < k.left, out, blk);
+                shadeRow<T, true, false>(S, wb, bounce, maxDepth, k.first + e, has && e < k.left, out, blk);
             }
         }
         if (has) chunk = next_chunk(wk, grab, count);
@@ -2750,7 +2842,27 @@ void launch_fill_i32(int* p, long long n, int value, hipStream_t s) {
 }
 
 void launch_generate(const DevScene& sc, const WaveBuffers& wb, long long paths, int pixels, hipStream_t s) {
+    if (wb.camPaths > 0 && wb.camRays && !wb.aovAlb) {   // nothing to store: the pass's ray count only
+        launch_fill_i32(wb.counters + kCtrRays, 1, (int)paths, s);
+        return;
+    }
     hipLaunchKernelGGL(k_generate, dim3(blocksFor(paths, kBlock)), dim3(kBlock), 0, s, sc, wb, paths, pixels);
+}
+
+// One category's shading kernel, if the scene uses that BSDF type.
+template <int T, bool B>
+static void launch_shade_cat(const DevScene& sc, const WaveBuffers& wb, int bounce, int maxDepth, const LaunchShape& ls,
+                             hipStream_t s, const DevScene* scenes, bool cam) {
+    if (!(wb.catMask & (1 << T))) return;
+    if constexpr (!B) {
+        if (cam) {
+            hipLaunchKernelGGL((k_shade<T, false, true>), dim3(ls.shadeCamBlocks[T]), dim3(kBlock), 0, s, sc, wb, bounce,
+                               maxDepth, scenes);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_shade<T, B>), dim3(ls.shadeCatBlocks[B][T]), dim3(kBlock), 0, s, sc, wb, bounce, maxDepth,
+                       scenes);
 }
 
 template <int L, int W, bool B>
@@ -2771,24 +2883,19 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
                            scenes);
     }
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(k_categorize<B>, dim3(ls.shadeCatBlocks[B][kCatMiss]), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+    // bounce 0 of a single scene: the camera-path instantiations (the state is computed, not loaded)
+    const bool cam = !B && bounce == 0 && wb.camPaths > 0;
+    if constexpr (!B) {
+        if (cam) hipLaunchKernelGGL((k_categorize<false, true>), dim3(ls.shadeCamBlocks[kCatMiss]), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+    }
+    if (!cam) hipLaunchKernelGGL(k_categorize<B>, dim3(ls.shadeCatBlocks[B][kCatMiss]), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     if (B) launch_segment_rows(wb, bounce, kSegCat, kNumCats, s);
     // a BSDF category is shaded only when the scene uses that type
-    if (wb.catMask & (1 << BTE_BSDF_DIFFUSE))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIFFUSE, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_DIFFUSE]), dim3(kBlock), 0, s, sc, wb, bounce,
-                           maxDepth, scenes);
-    if (wb.catMask & (1 << BTE_BSDF_DIELECTRIC))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_DIELECTRIC, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_DIELECTRIC]), dim3(kBlock), 0, s, sc, wb,
-                           bounce, maxDepth, scenes);
-    if (wb.catMask & (1 << BTE_BSDF_MIRROR))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_MIRROR, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_MIRROR]), dim3(kBlock), 0, s, sc, wb, bounce,
-                           maxDepth, scenes);
-    if (wb.catMask & (1 << BTE_BSDF_MICROFACET))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_MICROFACET, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_MICROFACET]), dim3(kBlock), 0, s, sc, wb,
-                           bounce, maxDepth, scenes);
-    if (wb.catMask & (1 << BTE_BSDF_DISNEY))
-        hipLaunchKernelGGL((k_shade<BTE_BSDF_DISNEY, B>), dim3(ls.shadeCatBlocks[B][BTE_BSDF_DISNEY]), dim3(kBlock), 0, s, sc, wb, bounce,
-                           maxDepth, scenes);
+    launch_shade_cat<BTE_BSDF_DIFFUSE, B>(sc, wb, bounce, maxDepth, ls, s, scenes, cam);
+    launch_shade_cat<BTE_BSDF_DIELECTRIC, B>(sc, wb, bounce, maxDepth, ls, s, scenes, cam);
+    launch_shade_cat<BTE_BSDF_MIRROR, B>(sc, wb, bounce, maxDepth, ls, s, scenes, cam);
+    launch_shade_cat<BTE_BSDF_MICROFACET, B>(sc, wb, bounce, maxDepth, ls, s, scenes, cam);
+    launch_shade_cat<BTE_BSDF_DISNEY, B>(sc, wb, bounce, maxDepth, ls, s, scenes, cam);
     if (B) {
         launch_segment_rows(wb, bounce + 1, kSegRays, 1, s);
         launch_segment_rows(wb, bounce, kSegShadow, wb.kinds, s);
@@ -3034,6 +3141,12 @@ static int shadeOccupancy() {
     (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_shade<T, B>, kBlock, 0);
     return std::max(1, a);
 }
+template <int T>
+static int shadeCamOccupancy() {
+    int a = 0;
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_shade<T, false, true>, kBlock, 0);
+    return std::max(1, a);
+}
 
 // Resident blocks per CU of the traversal passes for one stack size and BVH width.
 template <int L, int W>
@@ -3062,6 +3175,7 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     static int cus = 1, occS = 1, occP = 1, occMT = 1, occL[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};   // [width][stack]
     static int occT[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};                                  // tail [width][stack]
     static int occCat[2][kNumCats] = {{1, 1, 1, 1, 1, 1}, {1, 1, 1, 1, 1, 1}};                  // shading [batch][category]
+    static int occCam[kNumCats] = {1, 1, 1, 1, 1, 1};                                          // bounce-0 camera paths
     if (device != cachedDev) {
         hipDeviceProp_t prop;
         (void)hipGetDeviceProperties(&prop, device);
@@ -3080,6 +3194,16 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
             occCat[0][kCatMiss] = std::max(1, a);
             occCat[1][kCatMiss] = std::max(1, b);
         }
+        {
+            int a = 0;
+            (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_categorize<false, true>, kBlock, 0);
+            occCam[kCatMiss] = std::max(1, a);
+        }
+        occCam[BTE_BSDF_DIFFUSE] = shadeCamOccupancy<BTE_BSDF_DIFFUSE>();
+        occCam[BTE_BSDF_DIELECTRIC] = shadeCamOccupancy<BTE_BSDF_DIELECTRIC>();
+        occCam[BTE_BSDF_MIRROR] = shadeCamOccupancy<BTE_BSDF_MIRROR>();
+        occCam[BTE_BSDF_MICROFACET] = shadeCamOccupancy<BTE_BSDF_MICROFACET>();
+        occCam[BTE_BSDF_DISNEY] = shadeCamOccupancy<BTE_BSDF_DISNEY>();
         occCat[0][BTE_BSDF_DIFFUSE] = shadeOccupancy<BTE_BSDF_DIFFUSE, false>();
         occCat[0][BTE_BSDF_DIELECTRIC] = shadeOccupancy<BTE_BSDF_DIELECTRIC, false>();
         occCat[0][BTE_BSDF_MIRROR] = shadeOccupancy<BTE_BSDF_MIRROR, false>();
@@ -3121,6 +3245,7 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     ls.shadeBlocks = cus * occS;
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < kNumCats; ++k) ls.shadeCatBlocks[b][k] = cus * occCat[b][k];
+    for (int k = 0; k < kNumCats; ++k) ls.shadeCamBlocks[k] = cus * occCam[k];
     ls.auxBlocks = cus * occMT;
     ls.packetBlocks = cus * occP;
     ls.tailBlocks = cus * occT[ls.bvhWidth == 2 ? 0 : (ls.bvhWidth == 4 ? 1 : 2)][ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
