@@ -115,7 +115,7 @@ struct Context : SceneData {
     int mode = BTE_MODE_WAVEFRONT;
 
     // wavefront buffers
-    DBuf dRayO[2], dRayD[2], dThr[2], dRad[2], dSt[2], dHit, dCatQ, dCat, dFinal, dShO, dShD, dShC, dShX, dCounters,
+    DBuf dRayO[2], dRayD[2], dAux[2], dHit, dCatQ, dCat, dFinal, dShO, dShD, dShC, dShX, dCounters,
         dHitCap, dOvf;
     DBuf dFrameBase, dWork;
     bool aov = false;                 // albedo / normal AOVs of the first non-delta hit
@@ -354,7 +354,7 @@ int kindsInUse(int& kindArea, int& kindSpot, int& kindEnv) {
 }
 
 size_t bytesPerPath(int kinds, bool spots) {
-    return 2 * (4 * sizeof(float4) + sizeof(uint2)) + 2 * sizeof(float4) + kNumCats * sizeof(int) + sizeof(uint32_t) +
+    return 2 * (2 * sizeof(float4) + 2 * sizeof(float4)) + 2 * sizeof(float4) + kNumCats * sizeof(int) + sizeof(uint32_t) +
            (size_t)kinds * 3 * sizeof(float4) + (spots ? sizeof(float4) : 0) + (ctx().aov ? 2 * sizeof(float4) : 0);
 }
 
@@ -364,9 +364,7 @@ int ensureWaveBuffers(long long cap, int kinds, bool spots, int maxDepth, WaveBu
     for (int i = 0; i < 2; ++i) {
         CHECK_RC(c.dRayO[i].ensure(P * sizeof(float4)));
         CHECK_RC(c.dRayD[i].ensure(P * sizeof(float4)));
-        CHECK_RC(c.dThr[i].ensure(P * sizeof(float4)));
-        CHECK_RC(c.dRad[i].ensure(P * sizeof(float4)));
-        CHECK_RC(c.dSt[i].ensure(P * sizeof(uint2)));
+        CHECK_RC(c.dAux[i].ensure(2 * P * sizeof(float4)));
     }
     CHECK_RC(c.dHit.ensure(P * sizeof(float4)));
     CHECK_RC(c.dCatQ.ensure((size_t)kNumCats * P * sizeof(int)));
@@ -391,9 +389,7 @@ int ensureWaveBuffers(long long cap, int kinds, bool spots, int maxDepth, WaveBu
     for (int i = 0; i < 2; ++i) {
         wb.rayO[i] = c.dRayO[i].as<float4>();
         wb.rayD[i] = c.dRayD[i].as<float4>();
-        wb.thr[i] = c.dThr[i].as<float4>();
-        wb.rad[i] = c.dRad[i].as<float4>();
-        wb.st[i] = c.dSt[i].as<uint2>();
+        wb.aux[i] = c.dAux[i].as<float4>();
     }
     wb.hit = c.dHit.as<float4>();
     wb.catQ = c.dCatQ.as<int>();
@@ -1331,7 +1327,7 @@ int bte_cleanup(void) {
     for (DBuf* b : bufs) b->release();
     c.dTrav.release();
     for (int i = 0; i < 2; ++i) {
-        c.dRayO[i].release(); c.dRayD[i].release(); c.dThr[i].release(); c.dRad[i].release(); c.dSt[i].release();
+        c.dRayO[i].release(); c.dRayD[i].release(); c.dAux[i].release();
     }
     c.tris.clear(); c.materials.clear(); c.triMaterialIds.clear(); c.bsdfsHost.clear(); c.triBsdfHost.clear();
     c.emittersHost.clear(); c.emTriIdxHost.clear();

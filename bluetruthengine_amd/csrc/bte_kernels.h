@@ -108,9 +108,9 @@ struct BatchSlot {
 struct WaveBuffers {
     float4* rayO[2];      // origin xyz, w = brdfPdf of the last scatter
     float4* rayD[2];      // direction xyz, w = eta
-    float4* thr[2];       // throughput xyz
-    float4* rad[2];       // radiance xyz
-    uint2* st[2];         // x = rng state, y = path id | specular << 31
+    // 2 float4 per slot, one aligned 32-byte record: (radiance xyz, rng state
+    // bits), (throughput xyz, path id | flag bits)
+    float4* aux[2];
     float4* hit;          // (triangle id bits, t, u, v) per queue slot
     int* catQ;            // [kNumCats * cap]: queue slots grouped by shading category
     uint32_t* cat;        // [cap]: shading category (bits 0-2) | BSDF table index << 3, per queue slot

@@ -1067,9 +1067,8 @@ BTE_DEV void generateOne(const DevScene& sc, const WaveBuffers& wb, long long sl
     const V3 d = cameraRay(sc, pixel, frame, rng);
     wb.rayO[0][slot] = make_float4(sc.cam.origin.x, sc.cam.origin.y, sc.cam.origin.z, 0.0f);
     wb.rayD[0][slot] = make_float4(d.x, d.y, d.z, 1.0f);
-    wb.thr[0][slot] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-    wb.rad[0][slot] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    wb.st[0][slot] = make_uint2(rng, pid | kSpecBit);
+    wb.aux[0][2 * slot] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(rng));
+    wb.aux[0][2 * slot + 1] = make_float4(1.0f, 1.0f, 1.0f, __uint_as_float(pid | kSpecBit));
     if (wb.aovAlb) {
         wb.aovAlb[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         wb.aovNrm[pid] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1148,6 +1147,15 @@ __global__ __launch_bounds__(kBlock) void k_generate(DevScene sc, WaveBuffers wb
     if (i == 0) wb.counters[kCtrRays] = (int)paths;
 }
 
+// Throughput, radiance and random state / path id of queue slot `slot` (its
+// 32-byte record in wb.aux[cur]).
+BTE_DEV void loadAux(const WaveBuffers& wb, int cur, long long slot, float4& th, float4& ra, uint2& st) {
+    const float4 a0 = wb.aux[cur][2 * slot], a1 = wb.aux[cur][2 * slot + 1];
+    ra = make_float4(a0.x, a0.y, a0.z, 0.0f);
+    th = make_float4(a1.x, a1.y, a1.z, 0.0f);
+    st = make_uint2(__float_as_uint(a0.w), __float_as_uint(a1.w));
+}
+
 // Bounce-0 state of queue slot `slot` when it is not stored (wb.camPaths >
 // 0): the values generateOne writes for that slot.
 BTE_DEV void cameraPath(const DevScene& sc, const WaveBuffers& wb, long long slot, float4& ro, float4& rd, float4& th,
@@ -1174,9 +1182,7 @@ BTE_DEV void loadPath(const DevScene& sc, const WaveBuffers& wb, int bounce, int
         const int cur = bounce & 1;
         ro = wb.rayO[cur][qi];
         rd = wb.rayD[cur][qi];
-        th = wb.thr[cur][qi];
-        ra = wb.rad[cur][qi];
-        st = wb.st[cur][qi];
+        loadAux(wb, cur, qi, th, ra, st);
     }
 }
 // The ray of queue slot qi entering bounce 0 (the packet pass): computed when
@@ -1844,9 +1850,9 @@ BTE_DEV void shadeRow(const DevScene& sc, const WaveBuffers& wb, int bounce, int
     if (survive) {
         wb.rayO[nxt][slot] = make_float4(o.x, o.y, o.z, brdfPdf);
         wb.rayD[nxt][slot] = make_float4(d.x, d.y, d.z, eta);
-        wb.thr[nxt][slot] = make_float4(thr.x, thr.y, thr.z, 0.0f);
-        wb.rad[nxt][slot] = make_float4(rad.x, rad.y, rad.z, 0.0f);
-        wb.st[nxt][slot] = make_uint2(rng, pid | aovBit | (spec ? kSpecBit : 0u));
+        wb.aux[nxt][2 * (size_t)slot] = make_float4(rad.x, rad.y, rad.z, __uint_as_float(rng));
+        wb.aux[nxt][2 * (size_t)slot + 1] =
+            make_float4(thr.x, thr.y, thr.z, __uint_as_float(pid | aovBit | (spec ? kSpecBit : 0u)));
     } else if (active) {
         wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
     }
@@ -1941,7 +1947,7 @@ BTE_DEV void connectOne(const DevScene& sc, const WaveBuffers& wb, int nxt, size
     if (!occ) {
         const float4 c = wb.shC[si];
         const uint32_t dest = __float_as_uint(sd.w);
-        float4* r = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.rad[nxt] + dest);
+        float4* r = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.aux[nxt] + 2 * (size_t)dest);
         float4 v4 = *r;
         v4.x = v4.x + c.x;
         v4.y = v4.y + c.y;
@@ -1976,7 +1982,7 @@ __global__ __launch_bounds__(kBlock) void k_connect_pool(DevScene sc, WaveBuffer
     auto unoccluded = [&](size_t slot) {
         const float4 c = wb.shC[slot];
         const uint32_t dest = __float_as_uint(wb.shD[slot].w);
-        float4* rp = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.rad[nxt] + dest);
+        float4* rp = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.aux[nxt] + 2 * (size_t)dest);
         float4 v4 = *rp;
         v4.x = v4.x + c.x;
         v4.y = v4.y + c.y;
@@ -2063,7 +2069,7 @@ BTE_DEV int spotOne(const DevScene& sc, const WaveBuffers& wb, int nxt, size_t s
     applyTextures(sc, its.tri, c.w, x.w, bsdf);
     const Frame f = makeFrame(its.n);
     const uint32_t dest = __float_as_uint(a.w);
-    float4* rp = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.rad[nxt] + dest);
+    float4* rp = (dest & kFinalBit) ? (wb.finalRad + (dest & ~kFinalBit)) : (wb.aux[nxt] + 2 * (size_t)dest);
     int rays = 0;
     for (int s = 0; s < sc.spotCount; ++s) {
         const NeeRec rec = spotNEE<0>(sc, xyz(c), xyz(x), its, f, bsdf, s);
@@ -2193,8 +2199,10 @@ __global__ __launch_bounds__(kBlock, BTE_TAIL_WAVES) void k_tail(DevScene sc, Wa
     bool spec = false;
     int b = bounce;
     auto start = [&](int slot) {
-        const float4 ro = wb.rayO[cur][slot], rd = wb.rayD[cur][slot], th = wb.thr[cur][slot], ra = wb.rad[cur][slot];
-        const uint2 st = wb.st[cur][slot];
+        const float4 ro = wb.rayO[cur][slot], rd = wb.rayD[cur][slot];
+        float4 th, ra;
+        uint2 st;
+        loadAux(wb, cur, slot, th, ra, st);
         o = xyz(ro); brdfPdf = ro.w;
         d = xyz(rd); eta = rd.w;
         thr = xyz(th);
@@ -2525,8 +2533,10 @@ __global__ __launch_bounds__(kBlock, BTE_TAIL_WAVES) void k_tail_steps(DevScene 
         survive = roulette(b, rng, thr, eta) && b + 1 < maxDepth;
     };
     auto start = [&](int slot) __attribute__((always_inline)) {
-        const float4 ro = wb.rayO[cur][slot], rd = wb.rayD[cur][slot], th = wb.thr[cur][slot], ra = wb.rad[cur][slot];
-        const uint2 st = wb.st[cur][slot];
+        const float4 ro = wb.rayO[cur][slot], rd = wb.rayD[cur][slot];
+        float4 th, ra;
+        uint2 st;
+        loadAux(wb, cur, slot, th, ra, st);
         o = xyz(ro); brdfPdf = ro.w;
         d = xyz(rd); eta = rd.w;
         thr = xyz(th);
