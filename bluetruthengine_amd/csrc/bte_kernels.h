@@ -163,6 +163,7 @@ struct LaunchShape {
     int shadeCamBlocks[kNumCats];       // grids of the bounce-0 (camera path) categoriser and shading kernels   // resident grid of each category's shading kernel [batch][category]
     int tailBlocks;       // resident grid of the tail pass (launch_tail)
     int packetBlocks;     // resident grid of the camera-ray packet pass (0: per-ray traversal on bounce 0)
+    int packetBlocksBatch;               // the packet pass of multi-scene batches (its own occupancy)
     int ldsStack;         // traversal LDS stack entries per lane: 8, 16 or 32
     int grabRows;         // work chunk of the persistent kernels, in 64-slot rows
     int bvhWidth;         // 2 or 4: BVH traversed by extend / connect

@@ -2884,7 +2884,8 @@ static void launch_bounce_l(const DevScene& sc, const WaveBuffers& wb, int bounc
     if constexpr (W >= 4) {
         if (count) hipLaunchKernelGGL((k_extend_pool<true, L, false, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
         else if (W == 4 && bounce == 0 && ls.packetBlocks > 0)
-            hipLaunchKernelGGL(k_extend_packet<B>, dim3(ls.packetBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
+            hipLaunchKernelGGL(k_extend_packet<B>, dim3(B ? ls.packetBlocksBatch : ls.packetBlocks), dim3(kBlock), 0, s, sc, wb,
+                               bounce, scenes);
         else hipLaunchKernelGGL((k_extend_pool<false, L, B, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
     } else if (count) {
         hipLaunchKernelGGL((k_extend<true, L, W>), dim3(ls.extendBlocks), dim3(kBlock), 0, s, sc, wb, bounce, scenes);
@@ -3182,6 +3183,7 @@ static int tailOccupancy() {
 
 LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerCU, int grabRows, int bvhWidth) {
     static int cachedDev = -1;
+    static int occPB = 1;   // packet pass of batches
     static int cus = 1, occS = 1, occP = 1, occMT = 1, occL[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};   // [width][stack]
     static int occT[3][3] = {{1, 1, 1}, {1, 1, 1}, {1, 1, 1}};                                  // tail [width][stack]
     static int occCat[2][kNumCats] = {{1, 1, 1, 1, 1, 1}, {1, 1, 1, 1, 1, 1}};                  // shading [batch][category]
@@ -3196,7 +3198,8 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
         int o2 = 0;
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_extend_packet<false>, kBlock, 0);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&o2, k_extend_packet<true>, kBlock, 0);
-        occP = std::max(1, std::min(o, o2));
+        occP = std::max(1, o);
+        occPB = std::max(1, o2);
         {   // the categoriser shades the misses
             int a = 0, b = 0;
             (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_categorize<false>, kBlock, 0);
@@ -3258,6 +3261,7 @@ LaunchShape query_launch_shape(int device, int ldsStack, int traversalBlocksPerC
     for (int k = 0; k < kNumCats; ++k) ls.shadeCamBlocks[k] = cus * occCam[k];
     ls.auxBlocks = cus * occMT;
     ls.packetBlocks = cus * occP;
+    ls.packetBlocksBatch = cus * occPB;
     ls.tailBlocks = cus * occT[ls.bvhWidth == 2 ? 0 : (ls.bvhWidth == 4 ? 1 : 2)][ls.ldsStack == 8 ? 0 : (ls.ldsStack == 16 ? 1 : 2)];
     return ls;
 }
