@@ -1706,11 +1706,17 @@ BTE_DEV void shadeMiss(const DevScene& sc, const WaveBuffers& wb, int qi, int bo
     wb.finalRad[pid] = make_float4(rad.x, rad.y, rad.z, 0.0f);
 }
 
-constexpr int kCatSub = 4;   // 64-slot rows per wave per iteration
+// 64-slot rows per wave per iteration: 8 for a single scene; 4 in a batch,
+// whose per-scene queue segments are aligned so that a block's 4 x 256
+// entries lie in one scene
+template <bool BATCH>
+constexpr int catSub() { return BATCH ? 4 : 8; }
+static_assert(catSub<true>() * kBlock <= kSegRowAlign * 64, "a batch categoriser block must lie in one scene segment");
 // Misses are shaded here, in queue order, and are not queued.
 template <bool BATCH = false, bool CAM = false>
 __global__ __launch_bounds__(kBlock) void k_categorize(DevScene sc, WaveBuffers wb, int bounce,
                                                         const DevScene* __restrict__ scenes) {
+    constexpr int kCatSub = catSub<BATCH>();
     __shared__ int sCnt[kCatSub][kBlock / 64][kNumCats];
     __shared__ int sBase[kNumCats];
     int* ctr = wb.counters + (size_t)bounce * kCtrStride;
